@@ -1,0 +1,391 @@
+"""asuna_amd -- MI355X-native render path with Asuna's scene format and estimator.
+
+Thin ctypes layer over ``asuna_amd/lib/libasuna.so`` (C ABI in
+``include/asuna_hip.h`` and ``include/asuna_host.h``).  The library holds
+the gfx950 HIP kernels and the C++ host code (loader, scene, offline loop);
+this module only marshals arguments.  It raises when the library is missing:
+there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Optional
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "lib", "libasuna.so")
+
+ASH_OK = 0
+ASH_NUM_OUTPUT_IMAGES = 9
+ERRORS = {-1: "invalid argument", -2: "HIP runtime failure", -3: "call out of order",
+          -4: "device allocation failed", -5: "i/o failure", -6: "malformed scene file",
+          -7: "unsupported feature"}
+
+
+# ----------------------------------------------------------------- PODs
+class GpuVertex(C.Structure):
+    _fields_ = [("pos", C.c_float * 3), ("uv", C.c_float * 2), ("normal", C.c_float * 3),
+                ("tangent", C.c_float * 3)]
+
+
+class GpuMaterial(C.Structure):
+    _fields_ = [("diffuse", C.c_float * 3), ("rhoSpec", C.c_float * 3), ("anisoAlpha", C.c_float * 2),
+                ("ior", C.c_float), ("roughness", C.c_float), ("subsurface", C.c_float),
+                ("specular", C.c_float), ("specularTint", C.c_float), ("anisotropic", C.c_float),
+                ("sheen", C.c_float), ("sheenTint", C.c_float), ("clearcoat", C.c_float),
+                ("clearcoatGloss", C.c_float), ("radiance", C.c_float * 3), ("metalness", C.c_float),
+                ("radianceFactor", C.c_float * 3), ("diffuseTextureId", C.c_int32),
+                ("roughnessTextureId", C.c_int32), ("metalnessTextureId", C.c_int32),
+                ("radianceTextureId", C.c_int32), ("normalTextureId", C.c_int32),
+                ("tangentTextureId", C.c_int32), ("opacityTextureId", C.c_int32), ("type", C.c_uint32)]
+
+
+class GpuLight(C.Structure):
+    _fields_ = [("type", C.c_int32), ("position", C.c_float * 3), ("direction", C.c_float * 3),
+                ("radiance", C.c_float * 3), ("u", C.c_float * 3), ("v", C.c_float * 3),
+                ("radius", C.c_float), ("area", C.c_float), ("doubleSide", C.c_uint32)]
+
+
+class GpuCamera(C.Structure):
+    _fields_ = [("rasterToCamera", C.c_float * 16), ("cameraToWorld", C.c_float * 16),
+                ("envTransform", C.c_float * 16), ("fxfycxcy", C.c_float * 4), ("type", C.c_uint32),
+                ("aperture", C.c_float), ("focalDistance", C.c_float), ("padding", C.c_float)]
+
+
+class GpuPushConstantRaytrace(C.Structure):
+    _fields_ = [("spp", C.c_int32), ("curFrame", C.c_int32), ("maxPathDepth", C.c_int32),
+                ("numLights", C.c_int32), ("bgColor", C.c_float * 3), ("useFaceNormal", C.c_uint32),
+                ("ignoreEmissive", C.c_uint32), ("hasEnvMap", C.c_uint32),
+                ("envMapResolution", C.c_float * 2), ("envMapIntensity", C.c_float),
+                ("nMultiChannel", C.c_uint32), ("diffuseOutChannel", C.c_int32),
+                ("specularOutChannel", C.c_int32), ("roughnessOutChannel", C.c_int32),
+                ("normalOutChannel", C.c_int32), ("positionOutChannel", C.c_int32),
+                ("tangentOutChannel", C.c_int32), ("uvOutChannel", C.c_int32)]
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("num_vertices", C.c_uint32), ("num_indices", C.c_uint32),
+                ("vertices", C.POINTER(GpuVertex)), ("indices", C.POINTER(C.c_uint32))]
+
+
+class InstanceDesc(C.Structure):
+    _fields_ = [("transform", C.c_float * 16), ("mesh_id", C.c_uint32), ("material_id", C.c_uint32),
+                ("light_id", C.c_int32), ("pad", C.c_uint32)]
+
+
+class ImageDesc(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba", C.POINTER(C.c_float))]
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("num_meshes", C.c_uint32), ("meshes", C.POINTER(MeshDesc)),
+                ("num_instances", C.c_uint32), ("instances", C.POINTER(InstanceDesc)),
+                ("num_materials", C.c_uint32), ("materials", C.POINTER(GpuMaterial)),
+                ("num_lights", C.c_uint32), ("lights", C.POINTER(GpuLight)),
+                ("num_textures", C.c_uint32), ("textures", C.POINTER(ImageDesc)),
+                ("envmap", ImageDesc)]
+
+
+class Hit(C.Structure):
+    _fields_ = [("t", C.c_float), ("u", C.c_float), ("v", C.c_float), ("inst_id", C.c_int32),
+                ("prim_id", C.c_int32), ("normal", C.c_float * 3), ("pos", C.c_float * 3)]
+
+
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("inst_id", "<i4"), ("prim_id", "<i4"),
+                      ("normal", "<f4", 3), ("pos", "<f4", 3)])
+assert HIT_DTYPE.itemsize == C.sizeof(Hit)
+
+
+class Stats(C.Structure):
+    _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
+                ("dielectric_shadow_rays", C.c_uint64), ("render_ms", C.c_double), ("extend_ms", C.c_double),
+                ("shadow_ms", C.c_double), ("shade_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class BuildOpts(C.Structure):
+    _fields_ = [("leaf_max_tris", C.c_uint32), ("quality", C.c_uint32)]
+
+
+class BuildInfo(C.Structure):
+    _fields_ = [("num_triangles", C.c_uint64), ("num_nodes", C.c_uint64), ("max_depth", C.c_uint32),
+                ("pad", C.c_uint32), ("build_ms", C.c_double), ("sah_cost", C.c_double),
+                ("device_bytes", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class RenderOpts(C.Structure):
+    _fields_ = [("queue_paths", C.c_uint32), ("block_size", C.c_uint32), ("timing", C.c_uint32),
+                ("depthOutChannel", C.c_int32), ("max_bounce_iters", C.c_uint32), ("pad", C.c_uint32)]
+
+
+class SceneState(C.Structure):
+    _fields_ = [("film_width", C.c_uint32), ("film_height", C.c_uint32), ("num_shots", C.c_uint32),
+                ("tone_mapping", C.c_uint32), ("output_hdr", C.c_uint32),
+                ("output_render_result", C.c_uint32), ("channel_output_ldr", C.c_uint32),
+                ("depth_out_channel", C.c_int32), ("num_triangles", C.c_uint64)]
+
+
+class OfflineOpts(C.Structure):
+    _fields_ = [("gpu_id", C.c_int32), ("output_scanline", C.c_int32), ("spp_override", C.c_int32),
+                ("quiet", C.c_int32), ("render", RenderOpts)]
+
+
+class AsunaError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """The loaded libasuna.so (raises AsunaError when it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise AsunaError(f"{LIB_PATH} is missing: build it with __graft_entry__.build() "
+                         "(make -C asuna_amd/csrc)")
+    L = C.CDLL(LIB_PATH)
+    P = C.POINTER
+    vp = C.c_void_p
+    sigs = {
+        "ash_create": (C.c_int, [C.c_int, P(vp)]),
+        "ash_destroy": (None, [vp]),
+        "ash_last_error": (C.c_char_p, [vp]),
+        "ash_upload_scene": (C.c_int, [vp, P(SceneDesc)]),
+        "ash_build_accel": (C.c_int, [vp, P(BuildOpts), P(BuildInfo)]),
+        "ash_set_film": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
+        "ash_render": (C.c_int, [vp, P(GpuCamera), P(GpuPushConstantRaytrace), C.c_int32, C.c_int32,
+                                 P(RenderOpts), P(Stats)]),
+        "ash_render_shots": (C.c_int, [vp, C.c_uint32, P(GpuCamera), P(GpuPushConstantRaytrace), C.c_int32,
+                                       P(RenderOpts), P(Stats)]),
+        "ash_read_shot_output": (C.c_int, [vp, C.c_uint32, C.c_int32, P(C.c_float)]),
+        "ash_read_output": (C.c_int, [vp, C.c_int32, P(C.c_float)]),
+        "ash_output_device_ptr": (C.c_int, [vp, C.c_int32, P(vp)]),
+        "ash_first_hit": (C.c_int, [vp, P(GpuCamera), vp]),
+        "ash_env_tables": (C.c_int, [vp, P(C.c_float), P(C.c_float)]),
+        "ash_pod_layout": (None, [P(C.c_uint32)]),
+        "ash_version": (C.c_char_p, []),
+        "ash_scene_load": (C.c_int, [C.c_char_p, P(vp), C.c_char_p, C.c_size_t]),
+        "ash_scene_free": (None, [vp]),
+        "ash_scene_get_desc": (C.c_int, [vp, P(SceneDesc)]),
+        "ash_scene_get_state": (C.c_int, [vp, P(SceneState)]),
+        "ash_scene_shot": (C.c_int, [vp, C.c_uint32, P(GpuCamera), P(GpuPushConstantRaytrace)]),
+        "ash_camera_lookat": (None, [C.c_uint32, C.c_uint32, C.c_float, P(C.c_float), P(C.c_float),
+                                     P(C.c_float), C.c_float, C.c_float, P(C.c_float), P(GpuCamera)]),
+        "ash_run_offline": (C.c_int, [C.c_char_p, C.c_char_p, P(OfflineOpts), P(Stats), C.c_char_p,
+                                      C.c_size_t]),
+        "ash_write_image": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, P(C.c_float)]),
+        "ash_read_image": (C.c_int, [C.c_char_p, C.c_float, P(C.c_uint32), P(C.c_uint32),
+                                     P(P(C.c_float))]),
+        "ash_free_image": (None, [P(C.c_float)]),
+        "ash_tonemap": (None, [C.c_uint32, C.c_uint32, P(C.c_float)]),
+        "ash_generate_scene": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_char_p, C.c_char_p,
+                                         C.c_size_t]),
+    }
+    for name, (res, args) in sigs.items():
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+# ------------------------------------------------------------ host scene
+class HostScene:
+    """A scene file loaded by the library's Loader (JSON + OBJ + images)."""
+
+    def __init__(self, path: str):
+        L = lib()
+        self._h = C.c_void_p()
+        err = C.create_string_buffer(4096)
+        rc = L.ash_scene_load(path.encode(), C.byref(self._h), err, len(err))
+        if rc != ASH_OK:
+            self._h = None
+            raise AsunaError(f"scene load failed ({ERRORS.get(rc, rc)}): {err.value.decode()}")
+        self.path = path
+        self.desc = SceneDesc()
+        L.ash_scene_get_desc(self._h, C.byref(self.desc))
+        self.state = SceneState()
+        L.ash_scene_get_state(self._h, C.byref(self.state))
+
+    def close(self):
+        if self._h:
+            lib().ash_scene_free(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def film(self):
+        return self.state.film_width, self.state.film_height
+
+    def shot(self, i: int = 0):
+        cam, pc = GpuCamera(), GpuPushConstantRaytrace()
+        rc = lib().ash_scene_shot(self._h, i, C.byref(cam), C.byref(pc))
+        if rc != ASH_OK:
+            raise AsunaError(f"bad shot index {i}")
+        return cam, pc
+
+    def meshes(self):
+        out = []
+        for m in range(self.desc.num_meshes):
+            md = self.desc.meshes[m]
+            v = np.ctypeslib.as_array(C.cast(md.vertices, C.POINTER(C.c_float)), (md.num_vertices * 11,))
+            i = np.ctypeslib.as_array(md.indices, (md.num_indices,))
+            out.append((v.reshape(-1, 11).copy(), i.copy()))
+        return out
+
+
+def camera_lookat(width, height, fov, eye, lookat, up=(0, 1, 0), aperture=0.0, focal=0.1, env=None):
+    cam = GpuCamera()
+    e = (C.c_float * 3)(*eye)
+    l_ = (C.c_float * 3)(*lookat)
+    u = (C.c_float * 3)(*up)
+    envp = (C.c_float * 16)(*env) if env is not None else None
+    lib().ash_camera_lookat(width, height, fov, e, l_, u, aperture, focal, envp, C.byref(cam))
+    return cam
+
+
+def generate_scene(config: int, directory: str, seed: int = 0xA5A52026, scale: int = 0) -> str:
+    os.makedirs(directory, exist_ok=True)
+    buf = C.create_string_buffer(4096)
+    rc = lib().ash_generate_scene(config, seed, scale, directory.encode(), buf, len(buf))
+    if rc != ASH_OK:
+        raise AsunaError(f"scene generation failed: {ERRORS.get(rc, rc)}")
+    return buf.value.decode()
+
+
+def write_image(path: str, rgba: np.ndarray):
+    rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+    h, w = rgba.shape[:2]
+    if lib().ash_write_image(path.encode(), w, h, fptr(rgba)) != ASH_OK:
+        raise AsunaError(f"cannot write {path}")
+
+
+def read_image(path: str, gamma: float = 1.0) -> np.ndarray:
+    w, h = C.c_uint32(), C.c_uint32()
+    p = C.POINTER(C.c_float)()
+    if lib().ash_read_image(path.encode(), gamma, C.byref(w), C.byref(h), C.byref(p)) != ASH_OK:
+        raise AsunaError(f"cannot read {path}")
+    a = np.ctypeslib.as_array(p, (h.value, w.value, 4)).copy()
+    lib().ash_free_image(p)
+    return a
+
+
+def pod_layout():
+    out = (C.c_uint32 * 10)()
+    lib().ash_pod_layout(out)
+    return list(out)
+
+
+# ----------------------------------------------------------- render ctx
+class Renderer:
+    """A device context of the render library (one GPU)."""
+
+    def __init__(self, device: int = 0):
+        L = lib()
+        self._c = C.c_void_p()
+        rc = L.ash_create(device, C.byref(self._c))
+        if rc != ASH_OK:
+            self._c = None
+            raise AsunaError(f"ash_create({device}) failed: {ERRORS.get(rc, rc)}")
+        self.width = self.height = 0
+
+    def _check(self, rc, what):
+        if rc != ASH_OK:
+            msg = lib().ash_last_error(self._c).decode()
+            raise AsunaError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}")
+
+    def close(self):
+        if self._c:
+            lib().ash_destroy(self._c)
+            self._c = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scene: HostScene):
+        self._check(lib().ash_upload_scene(self._c, C.byref(scene.desc)), "upload_scene")
+
+    def upload_desc(self, desc: SceneDesc):
+        self._check(lib().ash_upload_scene(self._c, C.byref(desc)), "upload_scene")
+
+    def build(self, leaf_max_tris=4, quality=1) -> dict:
+        info = BuildInfo()
+        self._check(lib().ash_build_accel(self._c, C.byref(BuildOpts(leaf_max_tris, quality)), C.byref(info)),
+                    "build_accel")
+        return info.as_dict()
+
+    def set_film(self, w, h):
+        self._check(lib().ash_set_film(self._c, w, h), "set_film")
+        self.width, self.height = w, h
+
+    def render(self, cam, pc, first_frame=0, n_frames=1, queue_paths=0, timing=False, depth_channel=-1,
+               max_bounce_iters=0) -> dict:
+        opts = RenderOpts(queue_paths, 0, 1 if timing else 0, depth_channel, max_bounce_iters, 0)
+        st = Stats()
+        self._check(lib().ash_render(self._c, C.byref(cam), C.byref(pc), first_frame, n_frames, C.byref(opts),
+                                     C.byref(st)), "render")
+        return st.as_dict()
+
+    def render_shots(self, cams, pcs, n_frames, queue_paths=0, timing=False) -> dict:
+        n = len(cams)
+        ca = (GpuCamera * n)(*cams)
+        pa = (GpuPushConstantRaytrace * n)(*pcs)
+        opts = RenderOpts(queue_paths, 0, 1 if timing else 0, -1, 0, 0)
+        st = Stats()
+        self._check(lib().ash_render_shots(self._c, n, ca, pa, n_frames, C.byref(opts), C.byref(st)),
+                    "render_shots")
+        return st.as_dict()
+
+    def read_shot(self, shot, image) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        self._check(lib().ash_read_shot_output(self._c, shot, image, fptr(out)), "read_shot_output")
+        return out
+
+    def read(self, image: int) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 4), np.float32)
+        self._check(lib().ash_read_output(self._c, image, fptr(out)), "read_output")
+        return out
+
+    def first_hit(self, cam) -> np.ndarray:
+        out = np.zeros(self.width * self.height, HIT_DTYPE)
+        self._check(lib().ash_first_hit(self._c, C.byref(cam), out.ctypes.data_as(C.c_void_p)), "first_hit")
+        return out.reshape(self.height, self.width)
+
+    def env_tables(self, w, h):
+        m = np.zeros((h, 2), np.float32)
+        c = np.zeros((h, w, 2), np.float32)
+        self._check(lib().ash_env_tables(self._c, fptr(m), fptr(c)), "env_tables")
+        return m, c
+
+
+def run_offline(scene: str, out: str, gpu_id: int = 0, output_scanline: bool = False, spp_override: int = 0,
+                quiet: bool = True) -> dict:
+    o = OfflineOpts(gpu_id, int(output_scanline), spp_override, int(quiet), RenderOpts(0, 0, 0, -1, 0, 0))
+    st = Stats()
+    err = C.create_string_buffer(4096)
+    rc = lib().ash_run_offline(scene.encode(), out.encode(), C.byref(o), C.byref(st), err, len(err))
+    if rc != ASH_OK:
+        raise AsunaError(f"run_offline failed ({ERRORS.get(rc, rc)}): {err.value.decode()}")
+    return st.as_dict()

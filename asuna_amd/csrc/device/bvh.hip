@@ -1,0 +1,479 @@
+// GPU BVH construction for gfx950 (replaces the driver BLAS/TLAS build,
+// pipeline_raytrace.cpp:107-147).
+//
+// One flat BVH over world-space triangles of all instances:
+//   1. world triangles + boxes + centroids (one thread per triangle)
+//   2. centroid bounds (block reduction)
+//   3. 63-bit Morton keys (21 bits/axis), radix sort (hipCUB)
+//   4. binary radix tree, Karras 2012 "Maximizing Parallelism in the
+//      Construction of BVHs, Octrees, and k-d Trees" (equal keys ordered by
+//      index)
+//   5. bottom-up refit with arrival counters: boxes, triangle counts and the
+//      SAH cost; a subtree with <= leafMax triangles whose leaf cost does not
+//      exceed its split cost is collapsed into one leaf
+//   6. compaction of the surviving internal nodes (prefix sum) and emission
+//      of 64-byte BVH2 nodes that hold both child boxes; triangles are
+//      reordered into leaf order.
+#include <hipcub/hipcub.hpp>
+
+#include <cfloat>
+#include <cstdio>
+#include <vector>
+
+#include "devscene.h"
+
+namespace ashd {
+
+namespace {
+
+constexpr int kBlock = 256;
+constexpr float kCostNode = 1.0f;
+constexpr float kCostTri = 1.0f;
+
+#define HIPB(x)                                                                    \
+  do {                                                                             \
+    hipError_t e_ = (x);                                                           \
+    if (e_ != hipSuccess) {                                                        \
+      std::snprintf(err, errLen, "bvh: %s failed: %s", #x, hipGetErrorString(e_)); \
+      return ASH_E_HIP;                                                            \
+    }                                                                              \
+  } while (0)
+
+__device__ __forceinline__ float3 xform(const float* m, float4 p) {
+  // ((m0*x + m1*y) + m2*z) + m3 per row
+  return make_float3(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+                     m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11]);
+}
+
+__global__ void kWorldTris(BuildArgs a, float4* wtri, float4* boxLo, float4* boxHi) {
+  uint32_t g = blockIdx.x * blockDim.x + threadIdx.x;
+  if (g >= a.numTris) return;
+  // instance by binary search over the triangle-count prefix
+  uint32_t lo = 0, hi = a.numInst;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if (a.instTriOffset[mid] <= g) lo = mid;
+    else hi = mid;
+  }
+  const DevInstance& in = a.inst[lo];
+  uint32_t prim = g - a.instTriOffset[lo];
+  float3 p[3];
+  for (int k = 0; k < 3; ++k) {
+    uint32_t vi = in.vbase + a.indices[in.ibase + prim * 3 + k];
+    p[k] = xform(in.o2w, a.vertices[2 * size_t(vi)]);
+  }
+  wtri[3 * size_t(g) + 0] = make_float4(p[0].x, p[0].y, p[0].z, __int_as_float(int(lo)));
+  wtri[3 * size_t(g) + 1] = make_float4(p[1].x, p[1].y, p[1].z, __int_as_float(int(prim)));
+  wtri[3 * size_t(g) + 2] = make_float4(p[2].x, p[2].y, p[2].z, __int_as_float(int(g)));
+  boxLo[g] = make_float4(fminf(p[0].x, fminf(p[1].x, p[2].x)), fminf(p[0].y, fminf(p[1].y, p[2].y)),
+                         fminf(p[0].z, fminf(p[1].z, p[2].z)), 0.f);
+  boxHi[g] = make_float4(fmaxf(p[0].x, fmaxf(p[1].x, p[2].x)), fmaxf(p[0].y, fmaxf(p[1].y, p[2].y)),
+                         fmaxf(p[0].z, fmaxf(p[1].z, p[2].z)), 0.f);
+}
+
+__global__ void kCentroidBounds(const float4* boxLo, const float4* boxHi, uint32_t n, float4* partLo, float4* partHi) {
+  __shared__ float sl[3][kBlock], sh[3][kBlock];
+  float l[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, h[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    float4 a = boxLo[i], b = boxHi[i];
+    float c[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+    for (int k = 0; k < 3; ++k) l[k] = fminf(l[k], c[k]), h[k] = fmaxf(h[k], c[k]);
+  }
+  for (int k = 0; k < 3; ++k) sl[k][threadIdx.x] = l[k], sh[k][threadIdx.x] = h[k];
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s)
+      for (int k = 0; k < 3; ++k) {
+        sl[k][threadIdx.x] = fminf(sl[k][threadIdx.x], sl[k][threadIdx.x + s]);
+        sh[k][threadIdx.x] = fmaxf(sh[k][threadIdx.x], sh[k][threadIdx.x + s]);
+      }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    partLo[blockIdx.x] = make_float4(sl[0][0], sl[1][0], sl[2][0], 0.f);
+    partHi[blockIdx.x] = make_float4(sh[0][0], sh[1][0], sh[2][0], 0.f);
+  }
+}
+
+__device__ __forceinline__ uint64_t spread21(uint32_t v) {
+  uint64_t x = v & 0x1fffffu;
+  x = (x | x << 32) & 0x1f00000000ffffull;
+  x = (x | x << 16) & 0x1f0000ff0000ffull;
+  x = (x | x << 8) & 0x100f00f00f00f00full;
+  x = (x | x << 4) & 0x10c30c30c30c30c3ull;
+  x = (x | x << 2) & 0x1249249249249249ull;
+  return x;
+}
+
+__global__ void kMorton(const float4* boxLo, const float4* boxHi, uint32_t n, const float4* partLo,
+                        const float4* partHi, uint32_t nPart, uint64_t* keys, uint32_t* ids) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float l[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, h[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
+  for (uint32_t p = 0; p < nPart; ++p) {
+    float4 a = partLo[p], b = partHi[p];
+    l[0] = fminf(l[0], a.x), l[1] = fminf(l[1], a.y), l[2] = fminf(l[2], a.z);
+    h[0] = fmaxf(h[0], b.x), h[1] = fmaxf(h[1], b.y), h[2] = fmaxf(h[2], b.z);
+  }
+  float4 a = boxLo[i], b = boxHi[i];
+  float c[3] = {0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z)};
+  uint64_t key = 0;
+  for (int k = 0; k < 3; ++k) {
+    float ext = h[k] - l[k];
+    float t = ext > 0.f ? (c[k] - l[k]) / ext : 0.5f;
+    t = fminf(fmaxf(t, 0.f), 1.f);
+    uint32_t q = uint32_t(fminf(t * 2097152.0f, 2097151.0f));
+    key |= spread21(q) << (2 - k);
+  }
+  keys[i] = key;
+  ids[i] = i;
+}
+
+__device__ __forceinline__ int deltaK(const uint64_t* k, int n, int i, int j) {
+  if (j < 0 || j >= n) return -1;
+  uint64_t a = k[i], b = k[j];
+  if (a == b) return 64 + __clz(uint32_t(i ^ j));
+  return __clzll(a ^ b);
+}
+
+// Internal node i in [0, n-2]; children: c >= 0 internal, c < 0 leaf ~c.
+__global__ void kKarras(const uint64_t* keys, int n, int* left, int* right, int* parent, int* leafParent,
+                        int2* range) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n - 1) return;
+  int d = (deltaK(keys, n, i, i + 1) - deltaK(keys, n, i, i - 1)) >= 0 ? 1 : -1;
+  int dmin = deltaK(keys, n, i, i - d);
+  int lmax = 2;
+  while (deltaK(keys, n, i, i + lmax * d) > dmin) lmax <<= 1;
+  int l = 0;
+  for (int t = lmax >> 1; t >= 1; t >>= 1)
+    if (deltaK(keys, n, i, i + (l + t) * d) > dmin) l += t;
+  int j = i + l * d;
+  int dnode = deltaK(keys, n, i, j);
+  int s = 0, step = l;
+  do {
+    step = (step + 1) >> 1;
+    int ns = s + step;
+    if (ns < l && deltaK(keys, n, i, i + ns * d) > dnode) s = ns;
+  } while (step > 1);
+  int gamma = i + s * d + min(d, 0);
+  int lo = min(i, j), hi = max(i, j);
+  int lc = (lo == gamma) ? ~gamma : gamma;
+  int rc = (hi == gamma + 1) ? ~(gamma + 1) : gamma + 1;
+  left[i] = lc;
+  right[i] = rc;
+  range[i] = make_int2(lo, hi);
+  if (lc >= 0) parent[lc] = i;
+  else leafParent[~lc] = i;
+  if (rc >= 0) parent[rc] = i;
+  else leafParent[~rc] = i;
+  if (i == 0) parent[0] = -1;
+}
+
+__device__ __forceinline__ float halfArea(float4 lo, float4 hi) {
+  float dx = hi.x - lo.x, dy = hi.y - lo.y, dz = hi.z - lo.z;
+  return dx * dy + dy * dz + dz * dx;
+}
+
+// Bottom-up pass: the second child to arrive at a node computes it.
+__global__ void kRefit(int n, const uint32_t* sortedIds, const float4* boxLo, const float4* boxHi, const int* left,
+                       const int* right, const int* parent, const int* leafParent, const int2* range,
+                       unsigned* visit, float4* nodeLo, float4* nodeHi, float* cost, uint8_t* collapsed,
+                       uint32_t leafMax, uint32_t quality) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  int node = leafParent[k];
+  while (node >= 0) {
+    __threadfence();
+    if (atomicAdd(&visit[node], 1u) == 0u) return;  // first arrival stops
+    __threadfence();
+    int c[2] = {left[node], right[node]};
+    float4 lo = make_float4(FLT_MAX, FLT_MAX, FLT_MAX, 0.f), hi = make_float4(-FLT_MAX, -FLT_MAX, -FLT_MAX, 0.f);
+    float childCost = 0.f;
+    for (int q = 0; q < 2; ++q) {
+      float4 a, b;
+      float cc;
+      if (c[q] < 0) {
+        uint32_t t = sortedIds[~c[q]];
+        a = boxLo[t];
+        b = boxHi[t];
+        cc = kCostTri * halfArea(a, b);
+      } else {
+        const volatile float4* vl = nodeLo + c[q];
+        const volatile float4* vh = nodeHi + c[q];
+        a = make_float4(vl->x, vl->y, vl->z, 0.f);
+        b = make_float4(vh->x, vh->y, vh->z, 0.f);
+        cc = ((const volatile float*)cost)[c[q]];
+      }
+      lo = make_float4(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), 0.f);
+      hi = make_float4(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), 0.f);
+      childCost += cc;
+    }
+    float area = halfArea(lo, hi);
+    int2 rg = range[node];
+    uint32_t cnt = uint32_t(rg.y - rg.x + 1);
+    float splitCost = kCostNode * area + childCost;
+    float leafCost = kCostTri * area * float(cnt);
+    bool col = quality > 0 && cnt <= leafMax && leafCost <= splitCost;
+    nodeLo[node] = lo;
+    nodeHi[node] = hi;
+    cost[node] = col ? leafCost : splitCost;
+    collapsed[node] = col ? 1 : 0;
+    node = parent[node];
+  }
+}
+
+// emitted[i] = internal node i survives (neither it nor an ancestor collapsed).
+__global__ void kEmitFlags(int nInternal, const int* parent, const uint8_t* collapsed, uint32_t* emitted,
+                           unsigned* maxDepth) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nInternal) return;
+  bool e = !collapsed[i];
+  unsigned depth = 1;
+  for (int p = parent[i]; p >= 0 && e; p = parent[p]) {
+    if (collapsed[p]) e = false;
+    ++depth;
+  }
+  emitted[i] = e ? 1u : 0u;
+  if (e) atomicMax(maxDepth, depth);
+}
+
+__device__ __forceinline__ int32_t childRef(int c, const int2* range, const uint8_t* collapsed, const uint32_t* newIdx) {
+  if (c < 0) return encodeLeaf(uint32_t(~c), 1u);
+  if (collapsed[c]) {
+    int2 rg = range[c];
+    return encodeLeaf(uint32_t(rg.x), uint32_t(rg.y - rg.x + 1));
+  }
+  return int32_t(newIdx[c]);
+}
+
+__global__ void kEmitNodes(int nInternal, const uint32_t* emitted, const uint32_t* newIdx, const int* left,
+                           const int* right, const int2* range, const uint8_t* collapsed, const uint32_t* sortedIds,
+                           const float4* boxLo, const float4* boxHi, const float4* nodeLo, const float4* nodeHi,
+                           float4* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nInternal || !emitted[i]) return;
+  int c[2] = {left[i], right[i]};
+  float4 lo[2], hi[2];
+  for (int q = 0; q < 2; ++q) {
+    if (c[q] < 0) {
+      uint32_t t = sortedIds[~c[q]];
+      lo[q] = boxLo[t];
+      hi[q] = boxHi[t];
+    } else {
+      lo[q] = nodeLo[c[q]];
+      hi[q] = nodeHi[c[q]];
+    }
+  }
+  size_t o = size_t(newIdx[i]) * 4;
+  out[o + 0] = make_float4(lo[0].x, hi[0].x, lo[0].y, hi[0].y);
+  out[o + 1] = make_float4(lo[1].x, hi[1].x, lo[1].y, hi[1].y);
+  out[o + 2] = make_float4(lo[0].z, hi[0].z, lo[1].z, hi[1].z);
+  out[o + 3] = make_float4(__int_as_float(childRef(c[0], range, collapsed, newIdx)),
+                           __int_as_float(childRef(c[1], range, collapsed, newIdx)), 0.f, 0.f);
+}
+
+__global__ void kReorderTris(uint32_t n, const uint32_t* sortedIds, const float4* wtri, float4* tris) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint32_t t = sortedIds[j];
+  for (int k = 0; k < 3; ++k) tris[3 * size_t(j) + k] = wtri[3 * size_t(t) + k];
+}
+
+// Root for scenes the radix tree cannot express (0 or 1 triangle, or the
+// whole tree collapsed into one leaf): child 0 = one leaf, child 1 = empty.
+__global__ void kSingleLeafRoot(uint32_t n, const float4* boxLo, const float4* boxHi, const uint32_t* sortedIds,
+                                float4* out) {
+  float4 lo = make_float4(FLT_MAX, FLT_MAX, FLT_MAX, 0.f), hi = make_float4(-FLT_MAX, -FLT_MAX, -FLT_MAX, 0.f);
+  for (uint32_t j = 0; j < n; ++j) {
+    uint32_t t = sortedIds[j];
+    float4 a = boxLo[t], b = boxHi[t];
+    lo = make_float4(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), 0.f);
+    hi = make_float4(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), 0.f);
+  }
+  const float far = 3.0e38f;
+  int32_t r0 = n ? encodeLeaf(0, n) : kEmptyRef;
+  if (!n) lo = hi = make_float4(far, far, far, 0.f);
+  out[0] = make_float4(lo.x, hi.x, lo.y, hi.y);
+  out[1] = make_float4(far, far, far, far);
+  out[2] = make_float4(lo.z, hi.z, far, far);
+  out[3] = make_float4(__int_as_float(r0), __int_as_float(kEmptyRef), 0.f, 0.f);
+}
+
+// Structural check of the emitted BVH before any traversal touches it:
+// every reference is in range, leaves address valid triangles, and every
+// node except the root has exactly one parent (so traversal from the root
+// visits a tree and terminates).  bad[0] counts violations.
+__global__ void kValidate(const float4* nodes, uint32_t numNodes, uint32_t numTris, unsigned* refcnt, unsigned* bad) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= numNodes) return;
+  float4 n3 = nodes[size_t(i) * 4 + 3];
+  int32_t refs[2] = {__float_as_int(n3.x), __float_as_int(n3.y)};
+  for (int q = 0; q < 2; ++q) {
+    int32_t r = refs[q];
+    if (r == kEmptyRef) continue;
+    if (r >= 0) {
+      if (uint32_t(r) >= numNodes || r == 0) atomicAdd(bad, 1u);
+      else atomicAdd(&refcnt[r], 1u);
+    } else {
+      uint32_t first, cnt;
+      decodeLeaf(r, first, cnt);
+      if (uint64_t(first) + cnt > numTris) atomicAdd(bad, 1u);
+    }
+  }
+}
+__global__ void kValidateParents(const unsigned* refcnt, uint32_t numNodes, unsigned* bad) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= numNodes) return;
+  if (refcnt[i] != (i == 0 ? 0u : 1u)) atomicAdd(bad, 1u);
+}
+
+template <class T> struct DevBuf {
+  T* p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+}  // namespace
+
+int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_t errLen) {
+  hipEvent_t e0, e1;
+  HIPB(hipEventCreate(&e0));
+  HIPB(hipEventCreate(&e1));
+  HIPB(hipEventRecord(e0, s));
+  const uint32_t n = a.numTris;
+  if (a.leafMax < 1 || a.leafMax > uint32_t(kLeafMax)) {
+    std::snprintf(err, errLen, "bvh: leaf_max_tris must be in [1, %d]", kLeafMax);
+    return ASH_E_INVALID;
+  }
+  if (n >= (1u << 28)) {
+    std::snprintf(err, errLen, "bvh: %u triangles exceed the 2^28 leaf addressing limit", n);
+    return ASH_E_UNSUPPORTED;
+  }
+  DevBuf<float4> wtri, boxLo, boxHi, partLo, partHi, nodeLo, nodeHi;
+  DevBuf<uint64_t> keys, keys2;
+  DevBuf<uint32_t> ids, ids2, emitted, newIdx;
+  DevBuf<int> left, right, parent, leafParent;
+  DevBuf<int2> range;
+  DevBuf<unsigned> visit, maxDepth;
+  DevBuf<float> cost;
+  DevBuf<uint8_t> collapsed;
+  DevBuf<char> tmp;
+  size_t nn = n ? n : 1;
+  HIPB(hipMalloc(&wtri.p, nn * 3 * sizeof(float4)));
+  HIPB(hipMalloc(&boxLo.p, nn * sizeof(float4)));
+  HIPB(hipMalloc(&boxHi.p, nn * sizeof(float4)));
+  HIPB(hipMalloc(&keys.p, nn * sizeof(uint64_t)));
+  HIPB(hipMalloc(&keys2.p, nn * sizeof(uint64_t)));
+  HIPB(hipMalloc(&ids.p, nn * sizeof(uint32_t)));
+  HIPB(hipMalloc(&ids2.p, nn * sizeof(uint32_t)));
+  const uint32_t nPart = 512;
+  HIPB(hipMalloc(&partLo.p, nPart * sizeof(float4)));
+  HIPB(hipMalloc(&partHi.p, nPart * sizeof(float4)));
+  HIPB(hipMalloc(&maxDepth.p, sizeof(unsigned)));
+  HIPB(hipMemsetAsync(maxDepth.p, 0, sizeof(unsigned), s));
+  unsigned grid = (n + kBlock - 1) / kBlock;
+  if (n) {
+    kWorldTris<<<grid, kBlock, 0, s>>>(a, wtri.p, boxLo.p, boxHi.p);
+    kCentroidBounds<<<nPart, kBlock, 0, s>>>(boxLo.p, boxHi.p, n, partLo.p, partHi.p);
+    kMorton<<<grid, kBlock, 0, s>>>(boxLo.p, boxHi.p, n, partLo.p, partHi.p, nPart, keys.p, ids.p);
+    size_t tb = 0;
+    HIPB(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys.p, keys2.p, ids.p, ids2.p, int(n), 0, 63, s));
+    HIPB(hipMalloc(&tmp.p, tb));
+    HIPB(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, keys.p, keys2.p, ids.p, ids2.p, int(n), 0, 63, s));
+  }
+  const uint32_t* sorted = ids2.p;
+  uint32_t nInternal = n > 1 ? n - 1 : 0;
+  uint32_t numNodes = 1;
+  uint32_t depth = 1;
+  double sah = 0.0;
+  bool singleLeaf = n <= 1;
+  if (nInternal) {
+    size_t ni = nInternal;
+    HIPB(hipMalloc(&left.p, ni * sizeof(int)));
+    HIPB(hipMalloc(&right.p, ni * sizeof(int)));
+    HIPB(hipMalloc(&parent.p, ni * sizeof(int)));
+    HIPB(hipMalloc(&leafParent.p, nn * sizeof(int)));
+    HIPB(hipMalloc(&range.p, ni * sizeof(int2)));
+    HIPB(hipMalloc(&visit.p, ni * sizeof(unsigned)));
+    HIPB(hipMalloc(&nodeLo.p, ni * sizeof(float4)));
+    HIPB(hipMalloc(&nodeHi.p, ni * sizeof(float4)));
+    HIPB(hipMalloc(&cost.p, ni * sizeof(float)));
+    HIPB(hipMalloc(&collapsed.p, ni));
+    HIPB(hipMalloc(&emitted.p, ni * sizeof(uint32_t)));
+    HIPB(hipMalloc(&newIdx.p, ni * sizeof(uint32_t)));
+    HIPB(hipMemsetAsync(visit.p, 0, ni * sizeof(unsigned), s));
+    unsigned gi = (nInternal + kBlock - 1) / kBlock;
+    kKarras<<<gi, kBlock, 0, s>>>(keys2.p, int(n), left.p, right.p, parent.p, leafParent.p, range.p);
+    kRefit<<<grid, kBlock, 0, s>>>(int(n), sorted, boxLo.p, boxHi.p, left.p, right.p, parent.p, leafParent.p,
+                                   range.p, visit.p, nodeLo.p, nodeHi.p, cost.p, collapsed.p, a.leafMax, a.quality);
+    kEmitFlags<<<gi, kBlock, 0, s>>>(int(nInternal), parent.p, collapsed.p, emitted.p, maxDepth.p);
+    size_t tb = 0;
+    HIPB(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, emitted.p, newIdx.p, int(nInternal), s));
+    DevBuf<char> tmp2;
+    HIPB(hipMalloc(&tmp2.p, tb));
+    HIPB(hipcub::DeviceScan::ExclusiveSum(tmp2.p, tb, emitted.p, newIdx.p, int(nInternal), s));
+    uint32_t lastIdx = 0, lastEm = 0;
+    uint8_t rootCollapsed = 0;
+    float rootCost = 0.f;
+    float4 rl, rh;
+    HIPB(hipMemcpyAsync(&lastIdx, newIdx.p + nInternal - 1, 4, hipMemcpyDeviceToHost, s));
+    HIPB(hipMemcpyAsync(&lastEm, emitted.p + nInternal - 1, 4, hipMemcpyDeviceToHost, s));
+    HIPB(hipMemcpyAsync(&rootCollapsed, collapsed.p, 1, hipMemcpyDeviceToHost, s));
+    HIPB(hipMemcpyAsync(&rootCost, cost.p, 4, hipMemcpyDeviceToHost, s));
+    HIPB(hipMemcpyAsync(&rl, nodeLo.p, 16, hipMemcpyDeviceToHost, s));
+    HIPB(hipMemcpyAsync(&rh, nodeHi.p, 16, hipMemcpyDeviceToHost, s));
+    HIPB(hipMemcpyAsync(&depth, maxDepth.p, 4, hipMemcpyDeviceToHost, s));
+    HIPB(hipStreamSynchronize(s));
+    float ra = (rh.x - rl.x) * (rh.y - rl.y) + (rh.y - rl.y) * (rh.z - rl.z) + (rh.z - rl.z) * (rh.x - rl.x);
+    sah = ra > 0 ? double(rootCost) / double(ra) : 0.0;
+    if (rootCollapsed) {
+      singleLeaf = true;
+    } else {
+      numNodes = lastIdx + lastEm;
+      HIPB(hipMalloc(&r.nodes, size_t(numNodes) * 4 * sizeof(float4)));
+      kEmitNodes<<<gi, kBlock, 0, s>>>(int(nInternal), emitted.p, newIdx.p, left.p, right.p, range.p, collapsed.p,
+                                       sorted, boxLo.p, boxHi.p, nodeLo.p, nodeHi.p, r.nodes);
+    }
+  }
+  if (singleLeaf) {
+    numNodes = 1;
+    depth = 1;
+    HIPB(hipMalloc(&r.nodes, 4 * sizeof(float4)));
+    kSingleLeafRoot<<<1, 1, 0, s>>>(n, boxLo.p, boxHi.p, sorted, r.nodes);
+  }
+  HIPB(hipMalloc(&r.tris, nn * 3 * sizeof(float4)));
+  if (n) kReorderTris<<<grid, kBlock, 0, s>>>(n, sorted, wtri.p, r.tris);
+  HIPB(hipGetLastError());
+  {
+    DevBuf<unsigned> refcnt, bad;
+    HIPB(hipMalloc(&refcnt.p, size_t(numNodes) * sizeof(unsigned)));
+    HIPB(hipMalloc(&bad.p, sizeof(unsigned)));
+    HIPB(hipMemsetAsync(refcnt.p, 0, size_t(numNodes) * sizeof(unsigned), s));
+    HIPB(hipMemsetAsync(bad.p, 0, sizeof(unsigned), s));
+    unsigned gv = (numNodes + kBlock - 1) / kBlock;
+    kValidate<<<gv, kBlock, 0, s>>>(r.nodes, numNodes, n, refcnt.p, bad.p);
+    kValidateParents<<<gv, kBlock, 0, s>>>(refcnt.p, numNodes, bad.p);
+    unsigned nbad = 0;
+    HIPB(hipMemcpyAsync(&nbad, bad.p, sizeof nbad, hipMemcpyDeviceToHost, s));
+    HIPB(hipStreamSynchronize(s));
+    if (nbad) {
+      std::snprintf(err, errLen, "bvh: structural validation failed (%u violations)", nbad);
+      return ASH_E_INVALID;
+    }
+  }
+  HIPB(hipEventRecord(e1, s));
+  HIPB(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPB(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  r.numNodes = numNodes;
+  r.maxDepth = depth;
+  r.sahCost = sah;
+  r.ms = ms;
+  return ASH_OK;
+}
+
+}  // namespace ashd

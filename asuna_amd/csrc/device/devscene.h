@@ -1,0 +1,68 @@
+// Device-resident scene layout (HBM):
+//   instances  DevInstance[numInst]        128 B each (object<->world 3x4, mesh bases, ids)
+//   vertices   float4[2*numVert]           (pos.xyz, uv.x), (normal.xyz, uv.y): 32 B/vertex
+//   indices    uint32[numIdx]              per-mesh vertex indices, 3 per triangle
+//   nodes      float4[4*numNodes]          BVH2, child boxes in the parent (64 B/node)
+//   tris       float4[3*numTris]           world-space v0,v1,v2 in leaf order (48 B/tri);
+//                                           .w of v0/v1/v2 = instance, primitive, global id
+//   materials  GpuMaterial[], lights GpuLight[], textures (RGBA32F), env + tables
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../../include/asuna_hip.h"
+
+namespace ashd {
+
+struct DevInstance {
+  float o2w[12];  // row-major 3x4 (gl_ObjectToWorldEXT)
+  float w2o[12];  // row-major 3x4 (gl_WorldToObjectEXT)
+  uint32_t vbase;     // first vertex of the mesh in `vertices`
+  uint32_t ibase;     // first index of the mesh in `indices`
+  uint32_t material;  // material id (unused for emitters)
+  int32_t light;      // light id, -1 if not an emitter
+  uint32_t triBase;   // global id of the instance's first triangle
+  uint32_t ntri;
+  uint32_t pad[2];
+};
+static_assert(sizeof(DevInstance) == 128, "DevInstance layout");
+
+// Leaf reference: ~((first << 3) | (count - 1)), count in [1, 8].
+constexpr int kLeafMax = 8;
+__host__ __device__ inline int32_t encodeLeaf(uint32_t first, uint32_t count) {
+  return ~int32_t((first << 3) | (count - 1));
+}
+__host__ __device__ inline void decodeLeaf(int32_t ref, uint32_t& first, uint32_t& count) {
+  uint32_t v = uint32_t(~ref);
+  first = v >> 3;
+  count = (v & 7u) + 1u;
+}
+// A child slot that holds nothing: far-away degenerate box and this ref.
+constexpr int32_t kEmptyRef = int32_t(0x7fffffff);
+
+struct BuildArgs {
+  const DevInstance* inst;
+  uint32_t numInst;
+  const uint32_t* instTriOffset;  // numInst + 1 prefix of triangle counts
+  const float4* vertices;         // 2 float4 per vertex
+  const uint32_t* indices;
+  uint32_t numTris;
+  uint32_t leafMax;   // 1..8
+  uint32_t quality;   // 0 = LBVH (one triangle per leaf), 1 = SAH leaf collapse
+};
+
+struct BuildResult {
+  float4* nodes = nullptr;  // device, 4 float4 per node
+  float4* tris = nullptr;   // device, 3 float4 per triangle
+  uint32_t numNodes = 0;
+  uint32_t maxDepth = 0;
+  double sahCost = 0;
+  double ms = 0;
+};
+
+// Builds the BVH on `stream`; allocations in result are owned by the caller.
+// Returns 0 or a negative ASH_E_* code with a message in err.
+int buildBvh(const BuildArgs& a, hipStream_t stream, BuildResult& r, char* err, size_t errLen);
+
+}  // namespace ashd

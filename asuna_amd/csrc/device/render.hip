@@ -1,0 +1,1073 @@
+// Wavefront path tracing kernels for gfx950 (replaces raytrace.projective.rgen,
+// the closest-hit shaders of the plugin set, the miss shaders and the
+// hardware traversal of the Vulkan pipeline).
+//
+// One render batch = shots x frames x pixels paths.  Per bounce:
+//   extend  : closest hit of every live path (persistent waves fetching 64
+//             rays at a time; BVH2 traversal with a per-thread stack of 16
+//             entries in LDS and 48 spilled to global memory)
+//   shade   : miss / light hit / material closest-hit logic; next rays and
+//             shadow rays are appended to compacted queues with one atomic per
+//             wave (ballot + popcount)
+//   shadow  : any hit of the NEE rays; unoccluded paths add their pending Ld
+// then accumulate folds the frames of each pixel in frame order into the
+// running weighted mean exactly as the reference's per-frame image update.
+#include "render.h"
+
+namespace ashd {
+
+namespace {
+
+// ------------------------------------------------------------ wave utils
+__device__ __forceinline__ int waveAppend(uint32_t* counter, bool pred) {
+  uint64_t mask = __ballot(pred);
+  if (mask == 0) return -1;
+  int lane = __lane_id();
+  int leader = __ffsll((unsigned long long)mask) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, uint32_t(__popcll(mask)));
+  base = __shfl(base, leader);
+  uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
+  return pred ? int(base + uint32_t(__popcll(below))) : -1;
+}
+__device__ __forceinline__ void waveCount(uint32_t* counter, bool pred) {
+  uint64_t mask = __ballot(pred);
+  if (mask == 0) return;
+  int lane = __lane_id();
+  int leader = __ffsll((unsigned long long)mask) - 1;
+  if (lane == leader) atomicAdd(counter, uint32_t(__popcll(mask)));
+}
+
+// --------------------------------------------------------------- traversal
+struct TRay {
+  f3 o, d, inv;
+  int kx, ky, kz;
+  float Sx, Sy, Sz;
+};
+
+__device__ __forceinline__ float comp(f3 v, int k) { return k == 0 ? v.x : (k == 1 ? v.y : v.z); }
+
+__device__ __forceinline__ void setupRay(TRay& r, f3 o, f3 d) {
+  r.o = o;
+  r.d = d;
+  r.inv = f3{1.0f / d.x, 1.0f / d.y, 1.0f / d.z};
+  float ax = fabsf(d.x), ay = fabsf(d.y), az = fabsf(d.z);
+  r.kz = (ax > ay) ? (ax > az ? 0 : 2) : (ay > az ? 1 : 2);
+  r.kx = r.kz + 1 == 3 ? 0 : r.kz + 1;
+  r.ky = r.kx + 1 == 3 ? 0 : r.kx + 1;
+  if (comp(d, r.kz) < 0.0f) {
+    int t = r.kx;
+    r.kx = r.ky;
+    r.ky = t;
+  }
+  float dz = comp(d, r.kz);
+  r.Sx = comp(d, r.kx) / dz;
+  r.Sy = comp(d, r.ky) / dz;
+  r.Sz = 1.0f / dz;
+}
+
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013).
+__device__ __forceinline__ bool hitTri(const TRay& r, float4 p0, float4 p1, float4 p2, float tmin, float tmax,
+                                       float& t, float& u, float& v) {
+  f3 A = f3{p0.x, p0.y, p0.z} - r.o, B = f3{p1.x, p1.y, p1.z} - r.o, C = f3{p2.x, p2.y, p2.z} - r.o;
+  float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
+  float Ax = comp(A, r.kx) - r.Sx * Akz, Ay = comp(A, r.ky) - r.Sy * Akz;
+  float Bx = comp(B, r.kx) - r.Sx * Bkz, By = comp(B, r.ky) - r.Sy * Bkz;
+  float Cx = comp(C, r.kx) - r.Sx * Ckz, Cy = comp(C, r.ky) - r.Sy * Ckz;
+  float U = Cx * By - Cy * Bx;
+  float V = Ax * Cy - Ay * Cx;
+  float W = Bx * Ay - By * Ax;
+  if (U == 0.0f || V == 0.0f || W == 0.0f) {
+    U = float(double(Cx) * double(By) - double(Cy) * double(Bx));
+    V = float(double(Ax) * double(Cy) - double(Ay) * double(Cx));
+    W = float(double(Bx) * double(Ay) - double(By) * double(Ax));
+  }
+  if ((U < 0.0f || V < 0.0f || W < 0.0f) && (U > 0.0f || V > 0.0f || W > 0.0f)) return false;
+  float det = U + V + W;
+  if (det == 0.0f) return false;
+  float Az = r.Sz * Akz, Bz = r.Sz * Bkz, Cz = r.Sz * Ckz;
+  float T = U * Az + V * Bz + W * Cz;
+  float rcp = 1.0f / det;
+  float tt = T * rcp;
+  if (!(tt >= tmin && tt <= tmax)) return false;
+  t = tt;
+  u = V * rcp;
+  v = W * rcp;
+  return true;
+}
+
+// Conservative slab test (per-endpoint NaN ignored; far bound scaled by
+// 1 + 2*gamma_3, Ize 2013 "Robust BVH Ray Traversal").
+__device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, float lz, float hz, const TRay& r,
+                                     float tmin, float tmax, float& tn) {
+  float x0 = (lx - r.o.x) * r.inv.x, x1 = (hx - r.o.x) * r.inv.x;
+  float y0 = (ly - r.o.y) * r.inv.y, y1 = (hy - r.o.y) * r.inv.y;
+  float z0 = (lz - r.o.z) * r.inv.z, z1 = (hz - r.o.z) * r.inv.z;
+  float nx = x0 > x1 ? x1 : x0, fx = x0 > x1 ? x0 : x1;
+  float ny = y0 > y1 ? y1 : y0, fy = y0 > y1 ? y0 : y1;
+  float nz = z0 > z1 ? z1 : z0, fz = z0 > z1 ? z0 : z1;
+  tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
+  float tf = fminf(fminf(fx, fy), fminf(fz, tmax)) * 1.0000004f;
+  return tn <= tf;
+}
+
+struct Stack {
+  int* lds;
+  int* spill;
+  uint32_t stride;
+  int sp;
+  __device__ __forceinline__ void push(int v) {
+    if (sp < kStackLds) lds[sp * kTraceBlock] = v;
+    else if (sp < kStackLds + kSpill) spill[size_t(sp - kStackLds) * stride] = v;
+    ++sp;
+  }
+  __device__ __forceinline__ int pop() {
+    --sp;
+    if (sp < kStackLds) return lds[sp * kTraceBlock];
+    if (sp < kStackLds + kSpill) return spill[size_t(sp - kStackLds) * stride];
+    return kEmptyRef;  // overflowed entry (never happens when depth <= 64)
+  }
+};
+
+// Closest hit in [tmin, tmax]; ties at equal t go to the smaller global id.
+// Returns the leaf position of the hit triangle or -1.
+__device__ int traceClosest(const SceneView& sc, const TRay& r, float tmin, float tmax, Stack& st, float& bt,
+                            float& bu, float& bv) {
+  int best = -1, bestGid = 0x7fffffff;
+  bt = tmax;
+  st.sp = 0;
+  int node = 0;
+  for (;;) {
+    if (node >= 0) {
+      if (node != kEmptyRef) {
+        const float4* np = sc.nodes + size_t(node) * 4;
+        float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+        float t0, t1;
+        float lim = best >= 0 ? bt : tmax;
+        bool h0 = slab(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, r, tmin, lim, t0);
+        bool h1 = slab(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, r, tmin, lim, t1);
+        int c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
+        if (h0 && h1) {
+          if (t1 < t0) {
+            int tmp = c0;
+            c0 = c1;
+            c1 = tmp;
+          }
+          st.push(c1);
+          node = c0;
+          continue;
+        }
+        if (h0) {
+          node = c0;
+          continue;
+        }
+        if (h1) {
+          node = c1;
+          continue;
+        }
+      }
+    } else {
+      uint32_t first, cnt;
+      decodeLeaf(node, first, cnt);
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        const float4* tp = sc.tris + size_t(k) * 3;
+        float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+        float t, u, v;
+        if (hitTri(r, p0, p1, p2, tmin, best >= 0 ? bt : tmax, t, u, v)) {
+          int gid = __float_as_int(p2.w);
+          if (best < 0 || t < bt || (t == bt && gid < bestGid)) {
+            bt = t;
+            bu = u;
+            bv = v;
+            best = int(k);
+            bestGid = gid;
+          }
+        }
+      }
+    }
+    if (st.sp == 0) break;
+    node = st.pop();
+  }
+  return best;
+}
+
+__device__ bool traceAny(const SceneView& sc, const TRay& r, float tmin, float tmax, Stack& st) {
+  st.sp = 0;
+  int node = 0;
+  for (;;) {
+    if (node >= 0) {
+      if (node != kEmptyRef) {
+        const float4* np = sc.nodes + size_t(node) * 4;
+        float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+        float t0, t1;
+        bool h0 = slab(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, r, tmin, tmax, t0);
+        bool h1 = slab(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, r, tmin, tmax, t1);
+        int c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
+        if (h0 && h1) {
+          if (t1 < t0) {
+            int tmp = c0;
+            c0 = c1;
+            c1 = tmp;
+          }
+          st.push(c1);
+          node = c0;
+          continue;
+        }
+        if (h0) {
+          node = c0;
+          continue;
+        }
+        if (h1) {
+          node = c1;
+          continue;
+        }
+      }
+    } else {
+      uint32_t first, cnt;
+      decodeLeaf(node, first, cnt);
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        const float4* tp = sc.tris + size_t(k) * 3;
+        float t, u, v;
+        if (hitTri(r, tp[0], tp[1], tp[2], tmin, tmax, t, u, v)) return true;
+      }
+    }
+    if (st.sp == 0) break;
+    node = st.pop();
+  }
+  return false;
+}
+
+// -------------------------------------------------------------- raygen
+// raytrace.projective.rgen:26-90 for one (shot, frame, pixel).
+__device__ __forceinline__ void cameraRay(const GpuCamera& cam, float px, float py, uint32_t& seed, f3& ro,
+                                          f3& rd) {
+  f3 origin = transformPoint(cam.cameraToWorld, mk1(0.f));
+  if (cam.type == CameraTypePerspective) {
+    ro = origin;
+    f3 pCamera = transformPoint(cam.rasterToCamera, f3{px, py, 0.f});
+    f3 r = makeNormal(pCamera);
+    if (cam.aperture > 0.f) {
+      float l0 = rnd(seed), l1 = rnd(seed);
+      float dx, dy;
+      concentricDisk(l0, l1, dx, dy);
+      f3 pLens = f3{cam.aperture * dx, cam.aperture * dy, 0.f};
+      float ft = cam.focalDistance / r.z;
+      f3 pFocus = ft * r;
+      ro = transformPoint(cam.cameraToWorld, pLens);
+      r = pFocus - pLens;
+    }
+    rd = transformDirection(cam.cameraToWorld, r);
+  } else {
+    ro = origin;
+    float rx = (px - cam.fxfycxcy[2]) / cam.fxfycxcy[0];
+    float ry = (py - cam.fxfycxcy[3]) / cam.fxfycxcy[1];
+    rd = transformDirection(cam.cameraToWorld, f3{rx, ry, 1.f});
+  }
+}
+
+__global__ __launch_bounds__(256) void kRaygen(BatchDesc b, Wavefront w) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= b.n) return;
+  uint32_t pix = p % b.P, rest = p / b.P;
+  uint32_t fslot = rest % b.frames, shot = rest / b.frames;
+  int32_t frame = b.firstFrame + int32_t(fslot);
+  const ShotParams& sp = b.shotTab[shot];
+  uint32_t x = pix % sp.width, y = pix / sp.width;
+  uint32_t seed = xxhash32Seed(x, y, uint32_t(frame));
+  float jx, jy;
+  if (frame == 0) {
+    jx = jy = 0.5f;
+  } else {
+    jx = rnd(seed);
+    jy = rnd(seed);
+  }
+  f3 ro, rd;
+  cameraRay(sp.cam, float(x) + jx, float(y) + jy, seed, ro, rd);
+  // Gaussian filter weight, rgen:134-145 (exp(-8) folded to its correctly
+  // rounded float value).
+  const float expXY = 0.00033546262f;
+  float ox = jx - 0.5f, oy = jy - 0.5f;
+  float fw = fmaxf(0.0f, expf(-2.0f * ox * ox) - expXY) * fmaxf(0.0f, expf(-2.0f * oy * oy) - expXY);
+  w.ro[p] = f4(ro, 0.f);
+  w.rd[p] = f4(rd, 0.f);
+  w.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+  w.rad[p] = make_float4(0.f, 0.f, 0.f, fw);
+  w.info[p] = make_uint4(seed, 1u, pix, uint32_t(fslot) | (shot << 16));
+  w.queue[0][p] = int(p);
+  if (frame == 0) {
+    size_t plane = size_t(sp.width) * sp.height;
+    for (uint32_t c = 0; c < sp.pc.nMultiChannel; ++c) sp.images[(c + 1) * plane + pix] = make_float4(0.f, 0.f, 0.f, 1.f);
+  }
+  if (p == 0) w.ctr[0] = b.n;
+}
+
+// ---------------------------------------------------------------- extend
+__global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w, int cur) {
+  __shared__ int lds[kStackLds * kTraceBlock];
+  Stack st;
+  st.lds = lds + threadIdx.x;
+  uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
+  st.spill = w.spill + gt;
+  st.stride = w.spillThreads;
+  const uint32_t count = w.ctr[cur];
+  const int* q = w.queue[cur];
+  int lane = __lane_id();
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&w.ctr[3], 64u);
+    base = __shfl(base, 0);
+    if (base >= count) break;
+    uint32_t i = base + uint32_t(lane);
+    if (i < count) {
+      int pid = q[i];
+      TRay r;
+      setupRay(r, xyz(w.ro[pid]), xyz(w.rd[pid]));
+      float t = 0.f, u = 0.f, v = 0.f;
+      int pos = traceClosest(sc, r, kMinimum, kInf, st, t, u, v);
+      if (pos >= 0) {
+        const float4* tp = sc.tris + size_t(pos) * 3;
+        w.hitF[pid] = make_float4(t, u, v, 0.f);
+        w.hitI[pid] = make_int4(__float_as_int(tp[0].w), __float_as_int(tp[1].w), __float_as_int(tp[2].w), pos);
+      } else {
+        w.hitI[pid] = make_int4(-1, -1, -1, -1);
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------- shadow
+__global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w) {
+  __shared__ int lds[kStackLds * kTraceBlock];
+  Stack st;
+  st.lds = lds + threadIdx.x;
+  uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
+  st.spill = w.spill + gt;
+  st.stride = w.spillThreads;
+  const uint32_t count = w.ctr[2];
+  int lane = __lane_id();
+  for (;;) {
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(&w.ctr[4], 64u);
+    base = __shfl(base, 0);
+    if (base >= count) break;
+    uint32_t i = base + uint32_t(lane);
+    if (i < count) {
+      float4 so = w.shO[i], sd = w.shD[i];
+      TRay r;
+      setupRay(r, xyz(so), xyz(sd));
+      int pid = __float_as_int(sd.w);
+      if (!traceAny(sc, r, 0.0f, so.w, st)) {
+        float4 L = w.rad[pid], d = w.ld[pid];
+        w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
+      }
+    }
+  }
+}
+
+// ----------------------------------------------------------------- shade
+struct Hit {
+  int lightId;
+  float uvx, uvy;
+  f3 pos, V, N, ffN, X, Y;
+  uint32_t mat;
+};
+
+// getHitState + configureShadingFrame, rchit_layouts.glsl:61-95.
+__device__ __forceinline__ void hitState(const SceneView& sc, int inst, int prim, float bu, float bv, f3 rayDir,
+                                         uint32_t useFaceNormal, Hit& h) {
+  const DevInstance& in = sc.inst[inst];
+  const uint32_t* ix = sc.idx + in.ibase + uint32_t(prim) * 3;
+  uint32_t i0 = in.vbase + ix[0], i1 = in.vbase + ix[1], i2 = in.vbase + ix[2];
+  float4 a0 = sc.verts[2 * size_t(i0)], b0 = sc.verts[2 * size_t(i0) + 1];
+  float4 a1 = sc.verts[2 * size_t(i1)], b1 = sc.verts[2 * size_t(i1) + 1];
+  float4 a2 = sc.verts[2 * size_t(i2)], b2 = sc.verts[2 * size_t(i2) + 1];
+  f3 ba = f3{1.0f - bu - bv, bu, bv};
+  h.lightId = in.light;
+  h.uvx = a0.w * ba.x + a1.w * ba.y + a2.w * ba.z;
+  h.uvy = b0.w * ba.x + b1.w * ba.y + b2.w * ba.z;
+  f3 op = xyz(a0) * ba.x + xyz(a1) * ba.y + xyz(a2) * ba.z;
+  const float* m = in.o2w;
+  h.pos = f3{m[0] * op.x + m[1] * op.y + m[2] * op.z + m[3], m[4] * op.x + m[5] * op.y + m[6] * op.z + m[7],
+             m[8] * op.x + m[9] * op.y + m[10] * op.z + m[11]};
+  f3 n = xyz(b0) * ba.x + xyz(b1) * ba.y + xyz(b2) * ba.z;
+  const float* q = in.w2o;
+  f3 nw = f3{n.x * q[0] + n.y * q[4] + n.z * q[8], n.x * q[1] + n.y * q[5] + n.z * q[9],
+             n.x * q[2] + n.y * q[6] + n.z * q[10]};
+  h.N = makeNormal(nw);
+  h.V = makeNormal(-rayDir);
+  h.mat = in.material;
+  if (useFaceNormal == 1) h.N = mk1(0.f);  // geoN is never assigned (rchit_layouts.glsl:62)
+  basis(h.N, h.X, h.Y);
+  h.ffN = dot(h.N, h.V) > 0.0f ? h.N : -h.N;
+}
+
+__device__ __forceinline__ void normalMap(const SceneView& sc, Hit& h, int texId, uint32_t useFaceNormal) {
+  float4 c = tex2D(sc.textures[texId], h.uvx, h.uvy);
+  f3 n = 2.0f * xyz(c) - mk1(1.f);
+  h.N = toWorld(h.X, h.Y, h.N, n);
+  h.N = makeNormal(h.N);
+  if (useFaceNormal == 1) h.N = mk1(0.f);
+  basis(h.N, h.X, h.Y);
+  h.ffN = dot(h.N, h.V) > 0.0f ? h.N : -h.N;
+}
+
+struct LSample {
+  f3 d, n;
+  float dist, pdf;
+  uint32_t flags;
+};
+
+// sampleEnvmap, sample_light.glsl:106-129
+__device__ __forceinline__ f3 sampleEnv(const SceneView& sc, float r1, float r2, const float* envT, float resx,
+                                        float resy, float intensity, f3& L, float& pdf) {
+  uint32_t W = sc.env.w, H = sc.env.h;
+  float uvy = texMarginal(sc.marg, W, H, r1).x;
+  float uvx = texCond(sc.cond, W, H, r2, uvy).x;
+  pdf = texCond(sc.cond, W, H, uvx, uvy).y * texMarginal(sc.marg, W, H, uvy).y;
+  float phi = uvx * kTwoPi;
+  float theta = uvy * kPi;
+  if (sinf(theta) == 0.0f) pdf = 0.0f;
+  pdf = (pdf * resx * resy) / (kTwoPi * kPi * sinf(theta));
+  L = f3{-sinf(theta) * cosf(phi), cosf(theta), -sinf(theta) * sinf(phi)};
+  L = transformDirection(envT, L);
+  float4 e = tex2D(sc.env, uvx, uvy);
+  return intensity * xyz(e);
+}
+__device__ __forceinline__ void envUV(const float* envT, f3 L, float& u, float& v, float& theta) {
+  L = makeNormal(transformVectorT(envT, L));
+  theta = acosf(clampf(L.y, -1.0f, 1.0f));
+  u = (kPi + atan2f(L.z, L.x)) * kInv2Pi;
+  v = theta * kInvPi;
+}
+// pdfEnvmap, sample_light.glsl:84-95
+__device__ __forceinline__ float pdfEnv(const SceneView& sc, const float* envT, float resx, float resy, f3 L) {
+  float u, v, theta;
+  envUV(envT, L, u, v, theta);
+  float pdf = texCond(sc.cond, sc.env.w, sc.env.h, u, v).y * texMarginal(sc.marg, sc.env.w, sc.env.h, v).y;
+  float s = sinf(theta);
+  if (s == 0.0f) return 0.0f;
+  return (pdf * resx * resy) / (kTwoPi * kPi * s);
+}
+// evalEnvmap, sample_light.glsl:97-104
+__device__ __forceinline__ f3 evalEnv(const SceneView& sc, const float* envT, float intensity, f3 L) {
+  float u, v, theta;
+  envUV(envT, L, u, v, theta);
+  return intensity * xyz(tex2D(sc.env, u, v));
+}
+
+// sampleRectLight / sampleTriangleLight / samplePointLight /
+// sampleDistantLight, sample_light.glsl:10-82
+__device__ __forceinline__ f3 sampleOneLight(float r0, float r1, const GpuLight& L, f3 p, LSample& s) {
+  int type = L.type;
+  f3 lu = ld3(L.u), lv = ld3(L.v);
+  if (type == LightTypeRect || type == LightTypeTriangle) {
+    float b = type == LightTypeRect ? r1 : (1.0f - r0) * r1;
+    f3 lp = ld3(L.position) + lu * r0 + lv * b;
+    s.d = lp - p;
+    s.dist = length(s.d);
+    float distSq = s.dist * s.dist;
+    s.d = s.d / s.dist;
+    s.n = makeNormal(cross(lu, lv));
+    s.pdf = distSq / (L.area * fabsf(dot(s.n, s.d)) + kEps);
+    s.flags = kLightArea;
+    return ld3(L.radiance);
+  }
+  if (type == LightTypeDirectional) {
+    s.d = makeNormal(ld3(L.direction));
+    s.n = -s.d;
+    s.dist = kInf;
+    s.pdf = 1.0f;
+    s.flags = kLightDelta;
+    return ld3(L.radiance);
+  }
+  if (type == LightTypePoint) {
+    s.d = ld3(L.position) - p;
+    s.n = -s.d;
+    s.dist = length(s.d);
+    float distSq = s.dist * s.dist;
+    s.d = s.d / (s.dist + kEps);
+    s.pdf = 1.0f;
+    s.flags = kLightDelta;
+    return ld3(L.radiance) / (distSq + kEps);
+  }
+  return mk1(0.f);
+}
+
+struct PathCtx {
+  const SceneView* sc;
+  const ShotParams* sp;
+  uint32_t seed;
+  f3 ro, rd, T, L;
+  float bpdf;
+  uint32_t bflags, depth;
+  bool stop;
+  // NEE result
+  bool shadow;
+  f3 Ld, shO, shD;
+  float shDist;
+  bool fromDielectric;
+};
+
+// sampleLights, rchit_layouts.glsl:118-167
+__device__ f3 sampleLights(PathCtx& c, f3 pos, f3 nrm, bool& visible, LSample& l) {
+  const GpuPushConstantRaytrace& pc = c.sp->pc;
+  bool allowDoubleSide = false;
+  f3 radiance = mk1(0.f);
+  bool hasEnv = pc.hasEnvMap == 1, hasLight = pc.numLights > 0;
+  float envSel, anaSel;
+  if (hasEnv && hasLight) envSel = anaSel = 0.5f;
+  else if (hasEnv) envSel = 1.f, anaSel = 0.f;
+  else if (hasLight) envSel = 0.f, anaSel = 1.f;
+  else envSel = anaSel = 0.f;
+  l.d = l.n = mk1(0.f);
+  l.dist = l.pdf = 0.f;
+  l.flags = 0;
+  float sel = rnd(c.seed);
+  if (sel < envSel) {
+    l.flags = kLightArea;
+    l.dist = kInf;
+    l.n = -makeNormal(c.rd);
+    float r0 = rnd(c.seed), r1 = rnd(c.seed);
+    f3 Ldir;
+    float pdf;
+    f3 rad = sampleEnv(*c.sc, r0, r1, c.sp->cam.envTransform, pc.envMapResolution[0], pc.envMapResolution[1],
+                       pc.envMapIntensity, Ldir, pdf);
+    l.d = Ldir;
+    l.pdf = pdf;
+    radiance = rad / envSel;
+    allowDoubleSide = true;
+  } else if (sel < envSel + anaSel) {
+    int li = min(1 + int(rnd(c.seed) * float(pc.numLights)), pc.numLights);
+    const GpuLight& light = c.sc->lights[li];
+    float r0 = rnd(c.seed), r1 = rnd(c.seed);
+    radiance = sampleOneLight(r0, r1, light, pos, l) * float(pc.numLights) / anaSel;
+    allowDoubleSide = (light.doubleSide == 1);
+  }
+  c.shO = offsetRay(pos, nrm);
+  c.shD = l.d;
+  c.shDist = l.dist;
+  visible = (dot(l.d, nrm) > 0.0f && l.pdf > 0.0f);
+  visible = visible && (dot(l.n, l.d) < 0.0f || allowDoubleSide);
+  return radiance;
+}
+
+__device__ __forceinline__ void writeAov(const PathCtx& c, int ch, f3 v, uint32_t pix) {
+  if (ch < 0) return;
+  size_t plane = size_t(c.sp->width) * c.sp->height;
+  c.sp->images[size_t(ch + 1) * plane + pix] = f4(v, 1.f);
+}
+
+// hitLight, brdf_lambertian.rchit:44-68
+__device__ void hitLight(PathCtx& c, int lightId, f3 hitPos) {
+  const GpuLight& light = c.sc->lights[lightId];
+  f3 dir = makeNormal(c.rd);
+  f3 ln = makeNormal(cross(ld3(light.u), ld3(light.v)));
+  float proj = dot(ln, dir);
+  c.stop = true;
+  if (proj > 0.0f && light.doubleSide == 0) return;
+  float mis = 1.0f;
+  if (isNonSpecular(c.bflags) && c.depth != 1) {
+    float lightDist = length(hitPos - c.ro);
+    float distSquare = lightDist * lightDist;
+    float lightPdf = distSquare / (light.area * fabsf(proj) + kEps);
+    mis = powerHeuristic(c.bpdf, lightPdf);
+  }
+  c.L = c.L + c.T * ld3(light.radiance) * mis;
+}
+
+__device__ __forceinline__ void nextRay(PathCtx& c, f3 pos, f3 offN, f3 d, f3 weightOverPdf, float pdf,
+                                        uint32_t flags) {
+  c.bpdf = pdf;
+  c.bflags = flags;
+  c.ro = offsetRay(pos, offN);
+  c.rd = d;
+  c.T = c.T * weightOverPdf;
+}
+
+// brdf_lambertian.rchit:70-151 (non-emitter part)
+__device__ void shadeLambertian(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t pix, bool aov) {
+  const GpuPushConstantRaytrace& pc = c.sp->pc;
+  f3 kd = ld3(m.diffuse);
+  if (m.diffuseTextureId >= 0) kd = xyz(tex2D(c.sc->textures[m.diffuseTextureId], h.uvx, h.uvy));
+  if (m.normalTextureId >= 0) normalMap(*c.sc, h, m.normalTextureId, pc.useFaceNormal);
+  if (aov) {
+    writeAov(c, pc.diffuseOutChannel, kd, pix);
+    writeAov(c, pc.normalOutChannel, h.N, pix);
+    writeAov(c, pc.specularOutChannel, mk1(0.f), pix);
+    writeAov(c, pc.tangentOutChannel, h.X, pix);
+    writeAov(c, pc.roughnessOutChannel, f3{1.f, 1.f, 0.f}, pix);
+    writeAov(c, pc.positionOutChannel, h.pos, pix);
+    writeAov(c, pc.uvOutChannel, f3{h.uvx, h.uvy, 1.f}, pix);
+  }
+  {
+    bool visible;
+    LSample l;
+    f3 radiance = sampleLights(c, h.pos, h.ffN, visible, l);
+    f3 Ld = mk1(0.f);
+    if (visible) {
+      f3 bw = mk1(0.f);
+      float NdotL = dot(h.ffN, l.d), NdotV = dot(h.ffN, h.V);
+      if (!(NdotL < 0.0f || NdotV < 0.0f)) bw = kd * kInvPi * NdotL;
+      float bsdfPdf = (l.flags & kLightArea) ? cosinePdf(dot(h.ffN, l.d)) : 0.0f;
+      float mis = powerHeuristic(l.pdf, bsdfPdf);
+      Ld = mis * bw * radiance * c.T / (l.pdf + kEps);
+    }
+    c.Ld = Ld;
+    c.shadow = visible;
+  }
+  float u0 = rnd(c.seed), u1 = rnd(c.seed);
+  f3 wi = cosineHemisphere(u0, u1);
+  float pdf = cosinePdf(wi.z);
+  f3 d = toWorld(h.X, h.Y, h.ffN, wi);
+  f3 w = kd * kInvPi * fabsf(wi.z);
+  if (pdf <= 0.0f || length(w) == 0.0f) {
+    c.stop = true;
+    return;
+  }
+  nextRay(c, h.pos, h.ffN, d, w / pdf, pdf, kDiffuseRefl);
+}
+
+// brdf_pbr_metalness_roughness.rchit:57-128
+__device__ f3 pbrEval(f3 L, f3 V, f3 N, f3 X, f3 Y, f3 albedo, float ax, float ay, float metal, float eta) {
+  f3 weight = mk1(0.f);
+  float NdotL = dot(N, L), NdotV = dot(N, V);
+  if (NdotL <= 0.0f || NdotV <= 0.0f) return weight;
+  f3 H = makeNormal(L + V);
+  float HdotN = dot(H, N), HdotX = dot(H, X), HdotY = dot(H, Y), HdotV = dot(H, V);
+  float F0 = sqr((eta - 1.0f) / (eta + 1.0f));
+  f3 f0 = mk1(F0) * (1.0f - metal) + albedo * metal;
+  float pw = powf(clampf(1.0f - HdotV, 0.0f, 1.0f), 5.0f);
+  f3 Fs = f0 + (mk1(1.f) - f0) * pw;
+  float Ds = dAnisoGGX(HdotN, HdotX, HdotY, ax, ay);
+  float Gs = g1GGX(NdotV, dot(V, X), dot(V, Y), ax, ay);
+  Gs *= g1GGX(NdotL, dot(L, X), dot(L, Y), ax, ay);
+  f3 diffuse = albedo * (1.0f - metal) * kInvPi;
+  f3 specular = Fs * Ds * Gs;
+  weight = weight + diffuse;
+  weight = weight + specular;
+  weight = weight * NdotL;
+  return weight;
+}
+
+__device__ void shadePbr(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t pix, bool aov) {
+  const GpuPushConstantRaytrace& pc = c.sp->pc;
+  const SceneView& sc = *c.sc;
+  f3 kd = ld3(m.diffuse);
+  float metal = m.metalness, rough = m.roughness;
+  if (m.diffuseTextureId >= 0) kd = xyz(tex2D(sc.textures[m.diffuseTextureId], h.uvx, h.uvy));
+  if (m.metalnessTextureId >= 0) metal = tex2D(sc.textures[m.metalnessTextureId], h.uvx, h.uvy).x;
+  if (m.roughnessTextureId >= 0) rough = tex2D(sc.textures[m.roughnessTextureId], h.uvx, h.uvy).x;
+  if (m.normalTextureId >= 0) normalMap(sc, h, m.normalTextureId, pc.useFaceNormal);
+  float opacity = m.specular;
+  if (m.opacityTextureId >= 0) opacity = tex2D(sc.textures[m.opacityTextureId], h.uvx, h.uvy).x;
+  if (rnd(c.seed) < opacity) {
+    // pass through: same direction, same depth (pbr:156-159)
+    c.ro = offsetRay(h.pos, -h.ffN);
+    c.depth--;
+    return;
+  }
+  float ax = fmaxf(sqr(rough), 0.001f), ay = ax;
+  float eta = m.ior;
+  if (aov) {
+    writeAov(c, pc.diffuseOutChannel, kd, pix);
+    writeAov(c, pc.normalOutChannel, h.ffN, pix);
+  }
+  {
+    bool visible;
+    LSample l;
+    f3 radiance = sampleLights(c, h.pos, h.ffN, visible, l);
+    f3 Ld = mk1(0.f);
+    if (visible) {
+      f3 bw = pbrEval(l.d, h.V, h.ffN, h.X, h.Y, kd, ax, ay, metal, eta);
+      float bsdfPdf = 0.0f;
+      if (l.flags & kLightArea) {
+        float NdotL = dot(h.ffN, l.d), NdotV = dot(h.ffN, h.V);
+        if (!(NdotL <= 0.0f || NdotV <= 0.0f)) {
+          f3 H = makeNormal(l.d + h.V);
+          f3 wh = makeNormal(toLocal(h.X, h.Y, h.ffN, H));
+          f3 wo = makeNormal(toLocal(h.X, h.Y, h.ffN, h.V));
+          f3 wi = makeNormal(toLocal(h.X, h.Y, h.ffN, l.d));
+          bsdfPdf += 0.2f * cosinePdf(wi.z);
+          bsdfPdf += (1.0f - 0.2f) * pdfGGX(wh, wo, ax, ay);
+        }
+      }
+      float mis = powerHeuristic(l.pdf, bsdfPdf);
+      Ld = mis * bw * radiance * c.T / (l.pdf + kEps);
+    }
+    c.Ld = Ld;
+    c.shadow = visible;
+  }
+  float u0 = rnd(c.seed), u1 = rnd(c.seed);
+  f3 wo = makeNormal(toLocal(h.X, h.Y, h.ffN, h.V));
+  f3 wi;
+  float pdf = 0.0f;
+  uint32_t flags;
+  if (rnd(c.seed) < 0.2f) {
+    wi = cosineHemisphere(u0, u1);
+    flags = kDiffuseRefl;
+    pdf = cosinePdf(wi.z);
+  } else {
+    wi = sampleGGX(u0, u1, wo, ax, ay);
+    f3 wh = makeNormal(wi + wo);
+    pdf = pdfGGX(wh, wo, ax, ay);
+    flags = kGlossyRefl;
+  }
+  f3 d = toWorld(h.X, h.Y, h.ffN, wi);
+  f3 w = pbrEval(d, h.V, h.ffN, h.X, h.Y, kd, ax, ay, metal, eta) * fabsf(wi.z);
+  if (pdf <= 0.0f || length(w) == 0.0f) {
+    c.stop = true;
+    return;
+  }
+  nextRay(c, h.pos, h.ffN, d, w / (pdf + kEps), pdf, flags);
+}
+
+// brdf_rough_plastic.rchit:82-136 (eval, pdf) with the signatures' parameter order.
+__device__ f3 rpEval(f3 L, f3 V, f3 N, f3 X, f3 Y, f3 kd, f3 ks, float eta, float fdrInt, float invEta2, float ax,
+                     float ay, uint32_t flags) {
+  f3 weight = mk1(0.f);
+  float NdotL = dot(L, N), NdotV = dot(V, N);
+  if (((flags & kLightArea) == 0) || NdotL < 0.0f || NdotV < 0.0f) return weight;
+  {
+    f3 H = makeNormal(L + V);
+    float Fs = fresnelDielectricExt(dot(H, V), eta);
+    float Gs = g1GGX(NdotV, dot(V, X), dot(V, Y), ax, ay);
+    Gs *= g1GGX(NdotL, dot(L, X), dot(L, Y), ax, ay);
+    float Ds = dAnisoGGX(dot(H, N), dot(H, X), dot(H, Y), ax, ay);
+    weight = weight + ks * Fs * Gs * Ds * NdotL;
+  }
+  {
+    float Fo = fresnelDielectricExt(NdotV, eta);
+    float Fi = fresnelDielectricExt(NdotL, eta);
+    f3 diff = kd / (mk1(1.f) - kd * fdrInt);
+    weight = weight + (1.0f - Fi) * (1.0f - Fo) * diff * invEta2 * kInvPi * NdotL;
+  }
+  return weight;
+}
+__device__ float rpPdf(f3 L, f3 V, f3 N, f3 X, f3 Y, float ax, float ay, float eta, float ssw, uint32_t flags) {
+  float pdf = 0.0f;
+  float NdotL = dot(L, N), NdotV = dot(V, N);
+  if (NdotL < 0.0f || NdotV < 0.0f || ((flags & kLightArea) == 0)) return pdf;
+  float Fo = fresnelDielectricExt(NdotV, eta);
+  float sub = ssw * (1.0f - Fo), spec = Fo;
+  float ps = spec / (spec + sub);
+  f3 wi = toLocal(X, Y, N, L), wo = toLocal(X, Y, N, V);
+  f3 wh = makeNormal(wi + wo);
+  pdf += ps * pdfGGX(wh, wo, ax, ay);
+  pdf += (1.0f - ps) * cosinePdf(wi.z);
+  return pdf;
+}
+
+__device__ void shadeRoughPlastic(PathCtx& c, Hit& h, const GpuMaterial& m) {
+  const GpuPushConstantRaytrace& pc = c.sp->pc;
+  const SceneView& sc = *c.sc;
+  f3 kd = ld3(m.diffuse);
+  float alx = m.anisoAlpha[0], aly = m.anisoAlpha[1];
+  if (m.diffuseTextureId >= 0) kd = xyz(tex2D(sc.textures[m.diffuseTextureId], h.uvx, h.uvy));
+  if (m.roughnessTextureId >= 0) {
+    float4 t = tex2D(sc.textures[m.roughnessTextureId], h.uvx, h.uvy);
+    alx = t.x;
+    aly = t.y;
+  }
+  if (m.normalTextureId >= 0) normalMap(sc, h, m.normalTextureId, pc.useFaceNormal);
+  float eta = m.ior, fdrInt = m.radiance[0], invEta2 = 1.0f / (eta * eta);
+  float ax = fmaxf(kEps, alx), ay = fmaxf(kEps, aly);
+  float ssw = luminance(kd);
+  f3 ks = mk1(1.f);
+  {
+    bool visible;
+    LSample l;
+    f3 radiance = sampleLights(c, h.pos, h.ffN, visible, l);
+    f3 Ld = mk1(0.f);
+    if (visible) {
+      // The shader passes (..., ax, ay, invEta2, EArea) into eval's
+      // (..., invEta2, ax, ay, flags) and (eta, ax, ay) into pdf's
+      // (ax, ay, eta) (brdf_rough_plastic.rchit:234-240).
+      f3 bw = rpEval(l.d, h.V, h.ffN, h.X, h.Y, kd, ks, eta, fdrInt, ax, ay, invEta2, kLightArea);
+      float bsdfPdf = rpPdf(l.d, h.V, h.ffN, h.X, h.Y, eta, ax, ay, ssw, l.flags);
+      float mis = powerHeuristic(l.pdf, bsdfPdf);
+      Ld = mis * bw * radiance * c.T / (l.pdf + kEps);
+    }
+    c.Ld = Ld;
+    c.shadow = visible;
+  }
+  float u0 = rnd(c.seed), u1 = rnd(c.seed);
+  // sampleBsdf, brdf_rough_plastic.rchit:138-189
+  f3 N = h.ffN, V = h.V, X = h.X, Y = h.Y;
+  float NdotV = dot(V, N);
+  float pdf = 0.0f;
+  uint32_t flags = kBsdfNull;
+  f3 d = mk1(0.f), w = mk1(0.f);
+  if (NdotV > 0.0f) {
+    float Fo = fresnelDielectricExt(NdotV, eta);
+    float sub = ssw * (1.0f - Fo), spec = Fo;
+    float ps = spec / (spec + sub);
+    if (u0 < ps) {
+      u0 = u0 / ps;
+      f3 wo = toLocal(X, Y, N, V);
+      f3 wi = sampleGGX(u0, u1, wo, ax, ay);
+      f3 Lw = toWorld(X, Y, N, wi);
+      d = Lw;
+      f3 H = makeNormal(V + Lw);
+      f3 wh = makeNormal(wi + wo);
+      float NdotL = dot(N, Lw);
+      float Fs = fresnelDielectricExt(dot(H, V), eta);
+      float Gs = g1GGX(NdotV, dot(V, X), dot(V, Y), ax, ay);
+      Gs *= g1GGX(NdotL, dot(Lw, X), dot(Lw, Y), ax, ay);
+      float Ds = dAnisoGGX(dot(H, N), dot(H, X), dot(H, Y), ax, ay);
+      flags = kGlossyRefl;
+      pdf = pdfGGX(wh, wo, ax, ay) * ps;
+      w = ks * Fs * Gs * Ds * NdotL;
+    } else {
+      u0 = (u0 - ps) / (1.0f + kEps - ps);
+      f3 wi = cosineHemisphere(u0, u1);
+      float Fi = fresnelDielectricExt(wi.z, eta);
+      f3 diff = kd / (mk1(1.f) - kd * fdrInt);
+      pdf = (1.0f - ps) * cosinePdf(wi.z);
+      d = toWorld(X, Y, N, wi);
+      flags = kDiffuseRefl;
+      w = (1.0f - Fi) * (1.0f - Fo) * invEta2 * diff * kInvPi * fabsf(wi.z);
+    }
+  }
+  if (pdf <= 0.0f || length(w) == 0.0f) {
+    c.stop = true;
+    return;
+  }
+  nextRay(c, h.pos, h.ffN, d, w / (pdf + kEps), pdf, flags);
+}
+
+// bsdf_dielectric.rchit:343-408.  The NEE term is always zero (eval tests
+// EDelta but is called with EArea), so only its RNG draws and the shadow
+// ray are kept; the shadow ray is counted but not traced (it cannot change
+// the radiance).
+__device__ void shadeDielectric(PathCtx& c, Hit& h, const GpuMaterial& m) {
+  const GpuPushConstantRaytrace& pc = c.sp->pc;
+  if (m.normalTextureId >= 0) normalMap(*c.sc, h, m.normalTextureId, pc.useFaceNormal);
+  float ior = m.ior;
+  float eta = dot(h.V, h.N) > 0.0f ? (1.0f / ior) : ior;
+  {
+    bool visible;
+    LSample l;
+    sampleLights(c, h.pos, h.ffN, visible, l);
+    c.Ld = mk1(0.f);
+    c.shadow = false;
+    c.fromDielectric = visible;
+  }
+  float u = rnd(c.seed);
+  f3 V = h.V, N = h.ffN;
+  float F = dielectricFresnel(fabsf(dot(V, N)), eta);
+  f3 d, w;
+  float pdf;
+  uint32_t flags;
+  if (u < F) {
+    d = makeNormal(reflect(-V, N));
+    pdf = F;
+    flags = kSpecRefl;
+    w = F * mk1(1.f);
+  } else {
+    d = makeNormal(refract(-V, N, eta));
+    pdf = 1.0f - F;
+    flags = kSpecTrans;
+    w = (1.0f - F) * mk1(1.f) * eta * eta;
+  }
+  if (pdf <= 0.0f || length(w) == 0.0f) {
+    c.stop = true;
+    return;
+  }
+  float sg = dot(d, h.N);
+  float sgn = sg > 0.0f ? 1.0f : (sg < 0.0f ? -1.0f : 0.0f);
+  c.bpdf = pdf;
+  c.bflags = flags;
+  c.ro = offsetRay(h.pos, sgn * h.N);
+  c.rd = d;
+  c.T = c.T * (w / (pdf + kEps));
+}
+
+__global__ __launch_bounds__(256) void kShade(SceneView sc, BatchDesc b, Wavefront w, int cur) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t count = w.ctr[cur];
+  if (i >= count) return;
+  int pid = w.queue[cur][i];
+  uint4 info = w.info[pid];
+  uint32_t shot = info.w >> 16;
+  const ShotParams& sp = b.shotTab[shot];
+  const GpuPushConstantRaytrace& pc = sp.pc;
+  float4 thr = w.thr[pid], rad = w.rad[pid];
+  PathCtx c;
+  c.sc = &sc;
+  c.sp = &sp;
+  c.seed = info.x;
+  c.depth = info.y & 0xffffu;
+  c.bflags = info.y >> 16;
+  c.ro = xyz(w.ro[pid]);
+  c.rd = xyz(w.rd[pid]);
+  c.T = xyz(thr);
+  c.bpdf = thr.w;
+  c.L = xyz(rad);
+  c.stop = false;
+  c.shadow = false;
+  c.fromDielectric = false;
+  c.Ld = mk1(0.f);
+  int4 hi = w.hitI[pid];
+  uint32_t pix = info.z;
+  bool frame0 = (b.firstFrame + int32_t(info.w & 0xffffu)) == 0;
+  if (hi.x < 0) {
+    // raytrace.default.rmiss:24-55
+    c.stop = true;
+    f3 env = sc.hasEnv && pc.hasEnvMap == 1 ? evalEnv(sc, sp.cam.envTransform, pc.envMapIntensity, c.rd)
+                                            : ld3(pc.bgColor);
+    float mis = 1.0f;
+    if (c.depth != 1 && isNonSpecular(c.bflags)) {
+      float envPdf = (sc.hasEnv && pc.hasEnvMap == 1)
+                         ? pdfEnv(sc, sp.cam.envTransform, pc.envMapResolution[0], pc.envMapResolution[1], c.rd)
+                         : kInv4Pi;
+      mis = powerHeuristic(c.bpdf, envPdf);
+    }
+    c.L = c.L + c.T * env * mis;
+  } else {
+    float4 hf = w.hitF[pid];
+    Hit h;
+    hitState(sc, hi.x, hi.y, hf.y, hf.z, c.rd, pc.useFaceNormal, h);
+    bool aov = frame0 && c.depth == 1;
+    if (aov && sp.depthChannel >= 0) writeAov(c, sp.depthChannel, mk1(hf.x), pix);
+    if (h.lightId >= 0) {
+      hitLight(c, h.lightId, h.pos);
+    } else {
+      const GpuMaterial& m = sc.mats[h.mat];
+      switch (m.type) {
+        case MaterialTypeBrdfLambertian: shadeLambertian(c, h, m, pix, aov); break;
+        case MaterialTypeBrdfPbrMetalnessRoughness: shadePbr(c, h, m, pix, aov); break;
+        case MaterialTypeBrdfRoughPlastic: shadeRoughPlastic(c, h, m); break;
+        case MaterialTypeBsdfDielectric: shadeDielectric(c, h, m); break;
+        case MaterialTypeBrdfEmissive: {
+          // brdf_emissive.rchit:12-26
+          c.stop = true;
+          f3 e = ld3(m.radiance);
+          if (m.radianceTextureId >= 0) e = ld3(m.radianceFactor) * xyz(tex2D(sc.textures[m.radianceTextureId], h.uvx, h.uvy));
+          if (pc.ignoreEmissive == 0) c.L = c.L + e * c.T;
+          break;
+        }
+        default: c.stop = true; break;
+      }
+    }
+  }
+  // Counters for Mrays/s (extension rays of this bounce, NEE rays).
+  waveCount(&w.ctr[5], true);
+  waveCount(&w.ctr[7], c.fromDielectric);
+  // NEE shadow ray: maxDist = dist - 2*EPS (rgen:117-122)
+  int si = waveAppend(&w.ctr[2], c.shadow);
+  if (si >= 0) {
+    w.shO[si] = f4(c.shO, c.shDist - 2.0f * kEps);
+    w.shD[si] = f4(c.shD, __int_as_float(pid));
+    w.ld[pid] = f4(c.Ld, 0.f);
+  }
+  uint32_t nd = c.depth + 1;
+  bool cont = !c.stop && int32_t(nd) <= pc.maxPathDepth;
+  int ni = waveAppend(&w.ctr[1 - cur], cont);
+  if (ni >= 0) w.queue[1 - cur][ni] = pid;
+  w.info[pid] = make_uint4(c.seed, (c.stop ? c.depth : nd) | (c.bflags << 16), info.z, info.w);
+  w.thr[pid] = f4(c.T, c.bpdf);
+  w.rad[pid] = f4(c.L, rad.w);
+  w.ro[pid] = f4(c.ro, 0.f);
+  w.rd[pid] = f4(c.rd, 0.f);
+}
+
+__global__ void kResetCounters(Wavefront w, int cur) {
+  if (threadIdx.x == 0) {
+    w.ctr[1 - cur] = 0;
+    w.ctr[2] = 0;
+    w.ctr[3] = 0;
+    w.ctr[4] = 0;
+  }
+}
+
+__global__ void kStatShadow(Wavefront w) {
+  if (threadIdx.x == 0) w.ctr[6] += w.ctr[2];
+}
+
+// Ordered fold of the batch's frames into the output images
+// (raytrace.projective.rgen:146-178).
+__global__ __launch_bounds__(256) void kAccumulate(BatchDesc b, Wavefront w) {
+  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= b.P * b.shots) return;
+  uint32_t pix = t % b.P, shot = t / b.P;
+  const ShotParams& sp = b.shotTab[shot];
+  size_t plane = size_t(sp.width) * sp.height;
+  float4* img0 = sp.images;
+  float4* img8 = sp.images + 8 * plane;
+  float4 mean = img0[pix];
+  float W = img8[pix].x;
+  for (uint32_t f = 0; f < b.frames; ++f) {
+    uint32_t p = (shot * b.frames + f) * b.P + pix;
+    float4 r = w.rad[p];
+    float fw = r.w;
+    f3 L = f3{clampf(r.x, 0.f, 10.f), clampf(r.y, 0.f, 10.f), clampf(r.z, 0.f, 10.f)};
+    f3 rws = fw * L;
+    if (b.firstFrame + int32_t(f) == 0) {
+      f3 m = rws / fw;
+      mean = f4(m, 1.f);
+      W = fw;
+    } else {
+      f3 oldRws = xyz(mean) * W;
+      float nW = W + fw;
+      f3 m = (oldRws + rws) / nW;
+      mean = f4(m, 1.f);
+      W = nW;
+    }
+  }
+  img0[pix] = mean;
+  img8[pix] = make_float4(W, W, W, W);
+}
+
+__global__ __launch_bounds__(256) void kFirstHitState(SceneView sc, const ShotParams* sp, Wavefront w, ash_hit* out,
+                                                      uint32_t P) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= P) return;
+  int4 hi = w.hitI[p];
+  ash_hit o;
+  if (hi.x < 0) {
+    o.t = kInf;
+    o.u = o.v = 0.f;
+    o.inst_id = -1;
+    o.prim_id = -1;
+    for (int k = 0; k < 3; ++k) o.normal[k] = 0.f, o.pos[k] = 0.f;
+  } else {
+    float4 hf = w.hitF[p];
+    Hit h;
+    hitState(sc, hi.x, hi.y, hf.y, hf.z, xyz(w.rd[p]), 0u, h);
+    o.t = hf.x;
+    o.u = hf.y;
+    o.v = hf.z;
+    o.inst_id = hi.x;
+    o.prim_id = hi.y;
+    o.normal[0] = h.N.x, o.normal[1] = h.N.y, o.normal[2] = h.N.z;
+    o.pos[0] = h.pos.x, o.pos[1] = h.pos.y, o.pos[2] = h.pos.z;
+  }
+  out[p] = o;
+}
+
+}  // namespace
+
+void launchRaygen(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
+  kRaygen<<<(b.n + 255) / 256, 256, 0, s>>>(b, w);
+}
+void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t, uint32_t grid, hipStream_t s) {
+  kExtend<<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+}
+void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
+                 hipStream_t s) {
+  kShade<<<(maxRays + 255) / 256, 256, 0, s>>>(sc, b, w, cur);
+  kStatShadow<<<1, 64, 0, s>>>(w);
+}
+void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t, uint32_t grid, hipStream_t s) {
+  kShadow<<<grid, kTraceBlock, 0, s>>>(sc, w);
+}
+void launchAccumulate(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
+  kAccumulate<<<(b.P * b.shots + 255) / 256, 256, 0, s>>>(b, w);
+}
+void launchResetCounters(const Wavefront& w, int cur, hipStream_t s) { kResetCounters<<<1, 64, 0, s>>>(w, cur); }
+void launchFirstHit(const SceneView& sc, const ShotParams* shot, const Wavefront& w, ash_hit* out, uint32_t P,
+                    hipStream_t s) {
+  kFirstHitState<<<(P + 255) / 256, 256, 0, s>>>(sc, shot, w, out, P);
+}
+
+}  // namespace ashd

@@ -1,0 +1,328 @@
+/*
+ * asuna_hip.h -- C ABI of the MI355X render library (libasuna.so).
+ *
+ * This is the seam that replaces Asuna's Vulkan ray-tracing pipeline:
+ *   PipelineRaytrace::init  (src/pipeline/pipeline_raytrace.cpp:7-23)
+ *   PipelineRaytrace::run   (src/pipeline/pipeline_raytrace.cpp:36-74)
+ *   PipelineGraphics::run / updateCameraBuffer (src/pipeline/pipeline_graphics.cpp:29-98)
+ *   Scene::submit device uploads (src/scene/scene.cpp:26-85)
+ * together with the POD structs those read (the headers under src/shared).  Every struct
+ * below is byte-identical to the GLSL/C++ shared header it names; sizes and
+ * offsets are checked by static assertions in the library and by the CPU
+ * test-suite through ash_pod_layout().
+ *
+ * Conventions (differences from the reference are deliberate):
+ *  - every function returns ASH_OK (0) or a negative ASH_E* code and never
+ *    exits the process (the reference calls exit(1), e.g. scene.cpp:320-348);
+ *    the message of the last failure is ash_last_error(ctx).
+ *  - the context owns all device memory; callers own host inputs, which are
+ *    copied during the call.
+ *  - there is no process-global camera (CameraManip, camera.h:27): the camera
+ *    and the push-constant state are passed to every render call.
+ */
+#ifndef ASUNA_HIP_H
+#define ASUNA_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ PODs */
+
+/* GpuVertex, src/shared/vertex.h:6-11 (44 B, scalar layout). */
+typedef struct GpuVertex {
+  float pos[3];
+  float uv[2];
+  float normal[3];
+  float tangent[3];
+} GpuVertex;
+
+/* GpuMaterial, src/shared/material.h:24-49 (132 B). */
+typedef struct GpuMaterial {
+  float diffuse[3];
+  float rhoSpec[3];
+  float anisoAlpha[2];
+  float ior;
+  float roughness;
+  float subsurface;
+  float specular;
+  float specularTint;
+  float anisotropic;
+  float sheen;
+  float sheenTint;
+  float clearcoat;
+  float clearcoatGloss;
+  float radiance[3];
+  float metalness;
+  float radianceFactor[3];
+  int32_t diffuseTextureId;
+  int32_t roughnessTextureId;
+  int32_t metalnessTextureId;
+  int32_t radianceTextureId;
+  int32_t normalTextureId;
+  int32_t tangentTextureId;
+  int32_t opacityTextureId;
+  uint32_t type;
+} GpuMaterial;
+
+/* MaterialType, src/shared/material.h:7-21. */
+enum {
+  MaterialTypeBrdfLambertian = 0,
+  MaterialTypeBrdfKang18 = 1,
+  MaterialTypeBrdfEmissive = 2,
+  MaterialTypeBrdfPbrMetalnessRoughness = 3,
+  MaterialTypeBrdfPlastic = 4,
+  MaterialTypeBrdfRoughPlastic = 5,
+  MaterialTypeBrdfConductor = 6,
+  MaterialTypeBrdfRoughConductor = 7,
+  MaterialTypeBrdfMirror = 8,
+  MaterialTypeBrdfDisney = 9,
+  MaterialTypeBsdfDielectric = 10,
+  MaterialTypeBrdfPhong = 11,
+  MaterialTypeNum = 12
+};
+
+/* GpuLight, src/shared/light.h:16-26 (76 B). */
+typedef struct GpuLight {
+  int32_t type;
+  float position[3];
+  float direction[3];
+  float radiance[3];
+  float u[3];
+  float v[3];
+  float radius;
+  float area;
+  uint32_t doubleSide;
+} GpuLight;
+
+/* LightType, src/shared/light.h:7-13. */
+enum {
+  LightTypeDirectional = 0,
+  LightTypeRect = 1,
+  LightTypeTriangle = 2,
+  LightTypePoint = 3,
+  LightTypeUndefined = 4
+};
+
+/* GpuCamera, src/shared/camera.h:15-24 (224 B).  Matrices are column-major
+ * (nvmath mat4f / GLSL mat4 storage: element (row r, col c) at [c*4 + r]). */
+typedef struct GpuCamera {
+  float rasterToCamera[16];
+  float cameraToWorld[16];
+  float envTransform[16];
+  float fxfycxcy[4];
+  uint32_t type;
+  float aperture;
+  float focalDistance;
+  float padding;
+} GpuCamera;
+
+/* CameraType, src/shared/camera.h:7-11. */
+enum { CameraTypePerspective = 0, CameraTypeOpencv = 1, CameraTypeUndefined = 2 };
+
+/* GpuPushConstantRaytrace, src/shared/pushconstant.h:10-34 (84 B). */
+typedef struct GpuPushConstantRaytrace {
+  int32_t spp;
+  int32_t curFrame;
+  int32_t maxPathDepth;
+  int32_t numLights;
+  float bgColor[3];
+  uint32_t useFaceNormal;
+  uint32_t ignoreEmissive;
+  uint32_t hasEnvMap;
+  float envMapResolution[2];
+  float envMapIntensity;
+  uint32_t nMultiChannel;
+  int32_t diffuseOutChannel;
+  int32_t specularOutChannel;
+  int32_t roughnessOutChannel;
+  int32_t normalOutChannel;
+  int32_t positionOutChannel;
+  int32_t tangentOutChannel;
+  int32_t uvOutChannel;
+} GpuPushConstantRaytrace;
+
+/* Number of RGBA32F output images, src/shared/binding.h:70.
+ * image 0 = running weighted mean of radiance, images 1..7 = multi-channel
+ * AOVs (channel c -> image c+1), image 8 = sum of filter weights
+ * (raytrace.projective.rgen:155-178).  Channel 7 aliases image 8 exactly as
+ * in the reference. */
+#define ASH_NUM_OUTPUT_IMAGES 9
+
+/* Extension channel (not in the reference): first-hit depth (ray t) written
+ * like an AOV when the push constant's extension slot asks for it; see
+ * ash_render_opts.depthOutChannel. */
+
+/* --------------------------------------------------------- scene upload */
+
+/* One mesh as Mesh/MeshAlloc hold it (src/core/mesh.cpp:51-101): de-indexed
+ * GpuVertex array plus a uint32 index triple per triangle. */
+typedef struct ash_mesh_desc {
+  uint32_t num_vertices;
+  uint32_t num_indices; /* multiple of 3 */
+  const GpuVertex* vertices;
+  const uint32_t* indices;
+} ash_mesh_desc;
+
+/* One TLAS instance (src/core/instance.h:10-31, pipeline_raytrace.cpp:120-147).
+ * Instances with light_id >= 0 are emitters and always shade with the
+ * Lambertian light-hit path (pipeline_raytrace.cpp:137-140). */
+typedef struct ash_instance_desc {
+  float transform[16]; /* object->world, column-major; row 3 must be 0,0,0,1 */
+  uint32_t mesh_id;
+  uint32_t material_id;
+  int32_t light_id;
+  uint32_t pad;
+} ash_instance_desc;
+
+/* RGBA32F image, row-major, row 0 first (readImage, src/core/texture.cpp:306-338). */
+typedef struct ash_image_desc {
+  uint32_t width;
+  uint32_t height;
+  const float* rgba;
+} ash_image_desc;
+
+/* Everything Scene::submit uploads (scene.cpp:41-72). Light 0 and material 0
+ * are the dummies Scene::reset adds (scene.cpp:93-111). */
+typedef struct ash_scene_desc {
+  uint32_t num_meshes;
+  const ash_mesh_desc* meshes;
+  uint32_t num_instances;
+  const ash_instance_desc* instances;
+  uint32_t num_materials;
+  const GpuMaterial* materials;
+  uint32_t num_lights;
+  const GpuLight* lights;
+  uint32_t num_textures;
+  const ash_image_desc* textures; /* texture id -> image */
+  ash_image_desc envmap;          /* rgba==NULL: no env map (1x1 dummy) */
+} ash_scene_desc;
+
+/* --------------------------------------------------------------- results */
+
+/* Frame-0 primary hit of one pixel (first-hit parity buffer). */
+typedef struct ash_hit {
+  float t;          /* ray parameter, INFINITY(1e10) on miss */
+  float u, v;       /* barycentrics of vertex 1 and 2 (GLSL _bary) */
+  int32_t inst_id;  /* gl_InstanceID, -1 on miss */
+  int32_t prim_id;  /* gl_PrimitiveID within the instance's mesh, -1 on miss */
+  float normal[3];  /* shading normal N of getHitState (rchit_layouts.glsl:67-95) */
+  float pos[3];     /* world hit position of getHitState */
+} ash_hit;
+
+typedef struct ash_stats {
+  uint64_t paths;          /* camera samples traced (pixels x frames x shots) */
+  uint64_t extension_rays; /* closest-hit rays traced */
+  uint64_t shadow_rays;    /* any-hit rays traced */
+  uint64_t dielectric_shadow_rays; /* of which from dielectric hits (always zero contribution) */
+  double render_ms;        /* device time of the render call (HIP events) */
+  double extend_ms;        /* sum of closest-hit kernel time, if timing enabled */
+  double shadow_ms;        /* sum of any-hit kernel time, if timing enabled */
+  double shade_ms;         /* sum of shading kernel time, if timing enabled */
+} ash_stats;
+
+typedef struct ash_build_opts {
+  uint32_t leaf_max_tris; /* max triangles per leaf after collapse (default 4) */
+  uint32_t quality;       /* 0 = LBVH, 1 = LBVH + SAH leaf collapse (default 1) */
+} ash_build_opts;
+
+typedef struct ash_build_info {
+  uint64_t num_triangles;
+  uint64_t num_nodes;
+  uint32_t max_depth;
+  uint32_t pad;
+  double build_ms;
+  double sah_cost;
+  uint64_t device_bytes; /* nodes + triangles + shading data */
+} ash_build_info;
+
+/* Render options beyond the push constants. */
+typedef struct ash_render_opts {
+  uint32_t queue_paths;    /* wavefront size in paths (0 = auto) */
+  uint32_t block_size;     /* threads per block of the traversal kernels (0 = auto) */
+  uint32_t timing;         /* 1 = time each stage with HIP events */
+  int32_t depthOutChannel; /* extension: AOV channel receiving first-hit t, -1 = none */
+  uint32_t max_bounce_iters; /* safety cap on wavefront iterations (0 = auto) */
+  uint32_t pad;
+} ash_render_opts;
+
+/* ------------------------------------------------------------ error codes */
+enum {
+  ASH_OK = 0,
+  ASH_E_INVALID = -1,    /* bad argument */
+  ASH_E_HIP = -2,        /* HIP runtime failure */
+  ASH_E_STATE = -3,      /* call out of order (e.g. render before upload) */
+  ASH_E_NOMEM = -4,      /* device allocation failed */
+  ASH_E_IO = -5,         /* file read / write failure */
+  ASH_E_PARSE = -6,      /* malformed scene file */
+  ASH_E_UNSUPPORTED = -7 /* feature outside the supported set */
+};
+
+typedef struct ash_ctx ash_ctx;
+
+/* Context on one device (--gpu_id, main.cpp:20 -> context.cpp:304). */
+int ash_create(int device_ordinal, ash_ctx** out);
+void ash_destroy(ash_ctx* ctx);
+const char* ash_last_error(const ash_ctx* ctx);
+
+/* Upload a scene (Scene::submit, scene.cpp:26-85).  Replaces any previous one.
+ * Env-map importance tables are built here (EnvMap::EnvMap, texture.cpp:144-226). */
+int ash_upload_scene(ash_ctx* ctx, const ash_scene_desc* scene);
+
+/* Build the acceleration structure on the device (replaces
+ * createBottomLevelAS/createTopLevelAS, pipeline_raytrace.cpp:107-147). */
+int ash_build_accel(ash_ctx* ctx, const ash_build_opts* opts, ash_build_info* info);
+
+/* Allocate/clear the 9 output images for a film of w x h
+ * (PipelineGraphics::createOffscreenResources, pipeline_graphics.cpp:105-196). */
+int ash_set_film(ash_ctx* ctx, uint32_t width, uint32_t height);
+
+/* Render frames [first_frame, first_frame + n_frames) of one shot
+ * (n_frames calls of PipelineRaytrace::run with curFrame = first_frame + i,
+ * tracer.cpp:218-226), accumulating into the output images exactly as the
+ * reference's running weighted mean.  pc->curFrame and pc->spp are ignored
+ * (the offline loop forces spp = 1, tracer.cpp:211). */
+int ash_render(ash_ctx* ctx, const GpuCamera* cam, const GpuPushConstantRaytrace* pc,
+               int32_t first_frame, int32_t n_frames, const ash_render_opts* opts,
+               ash_stats* stats);
+
+/* Batched multi-shot render: n_shots shots of one scene, each with its own
+ * camera and push constants, all frames [0, n_frames) in one wavefront.
+ * Output images of shot s are kept on the device; read them with
+ * ash_read_shot_output. */
+int ash_render_shots(ash_ctx* ctx, uint32_t n_shots, const GpuCamera* cams,
+                     const GpuPushConstantRaytrace* pcs, int32_t n_frames,
+                     const ash_render_opts* opts, ash_stats* stats);
+int ash_read_shot_output(ash_ctx* ctx, uint32_t shot, int32_t image, float* host_rgba);
+
+/* Copy output image `image` (0..8) to host RGBA32F (w*h*4 floats). */
+int ash_read_output(ash_ctx* ctx, int32_t image, float* host_rgba);
+
+/* Device pointer of output image `image` (RGBA32F, w*h*4 floats), for callers
+ * that keep the result in HBM (denoiser, serving). */
+int ash_output_device_ptr(ash_ctx* ctx, int32_t image, void** dev_ptr);
+
+/* Frame-0 primary hits for every pixel: raygen with jitter 0.5 (rgen:41) +
+ * one closest-hit trace + hit-state reconstruction.  out has w*h entries. */
+int ash_first_hit(ash_ctx* ctx, const GpuCamera* cam, ash_hit* out);
+
+/* Environment importance tables as the device uses them: marginal (H x 2
+ * floats: row/H, pdf1D) and conditional (W*H x 2 floats: col/W, pdf2D). */
+int ash_env_tables(ash_ctx* ctx, float* marginal_out, float* conditional_out);
+
+/* Size / offset table of every POD, for layout checks:
+ * writes pairs {sizeof, offsetof(last field)} for GpuVertex, GpuMaterial,
+ * GpuLight, GpuCamera, GpuPushConstantRaytrace (10 uint32). */
+void ash_pod_layout(uint32_t* out10);
+
+/* Library build string (arch, flags). */
+const char* ash_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASUNA_HIP_H */
