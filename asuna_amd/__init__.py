@@ -101,7 +101,9 @@ assert HIT_DTYPE.itemsize == C.sizeof(Hit)
 class Stats(C.Structure):
     _fields_ = [("paths", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("dielectric_shadow_rays", C.c_uint64), ("render_ms", C.c_double), ("extend_ms", C.c_double),
-                ("shadow_ms", C.c_double), ("shade_ms", C.c_double)]
+                ("shadow_ms", C.c_double), ("shade_ms", C.c_double), ("extend_launches", C.c_uint64),
+                ("shadow_launches", C.c_uint64), ("ext_node_visits", C.c_uint64), ("ext_tri_tests", C.c_uint64),
+                ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -122,7 +124,7 @@ class BuildInfo(C.Structure):
 
 class RenderOpts(C.Structure):
     _fields_ = [("queue_paths", C.c_uint32), ("block_size", C.c_uint32), ("timing", C.c_uint32),
-                ("depthOutChannel", C.c_int32), ("max_bounce_iters", C.c_uint32), ("pad", C.c_uint32)]
+                ("depthOutChannel", C.c_int32), ("max_bounce_iters", C.c_uint32), ("count_traversal", C.c_uint32)]
 
 
 class SceneState(C.Structure):
@@ -341,8 +343,9 @@ class Renderer:
         self.width, self.height = w, h
 
     def render(self, cam, pc, first_frame=0, n_frames=1, queue_paths=0, timing=False, depth_channel=-1,
-               max_bounce_iters=0) -> dict:
-        opts = RenderOpts(queue_paths, 0, 1 if timing else 0, depth_channel, max_bounce_iters, 0)
+               max_bounce_iters=0, count_traversal=False) -> dict:
+        opts = RenderOpts(queue_paths, 0, 1 if timing else 0, depth_channel, max_bounce_iters,
+                          1 if count_traversal else 0)
         st = Stats()
         self._check(lib().ash_render(self._c, C.byref(cam), C.byref(pc), first_frame, n_frames, C.byref(opts),
                                      C.byref(st)), "render")

@@ -79,6 +79,7 @@ struct ash_ctx {
   DBuf<int4> wHitI;
   DBuf<int> wQ0, wQ1, wSpill;
   DBuf<uint32_t> wCtr;
+  DBuf<unsigned long long> wCnt64;
   uint32_t traceGrid = 0;
 
   ~ash_ctx() {
@@ -89,7 +90,7 @@ struct ash_ctx {
       b->release();
     inst.release(), idx.release(), instTriOffset.release(), mats.release(), lights.release(), texTab.release();
     marg.release(), cond.release(), shotTab.release(), wInfo.release(), wHitI.release();
-    wQ0.release(), wQ1.release(), wSpill.release(), wCtr.release();
+    wQ0.release(), wQ1.release(), wSpill.release(), wCtr.release(), wCnt64.release();
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -178,6 +179,7 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   MALLOC(c, c->wQ0, cap);
   MALLOC(c, c->wQ1, cap);
   MALLOC(c, c->wCtr, 16);
+  MALLOC(c, c->wCnt64, 4);
   c->traceGrid = uint32_t(std::max(1, c->numCU) * 8);
   uint32_t spillThreads = c->traceGrid * kTraceBlock;
   MALLOC(c, c->wSpill, size_t(spillThreads) * kSpill);
@@ -198,7 +200,9 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   w.ctr = c->wCtr.p;
   w.spill = c->wSpill.p;
   w.spillThreads = spillThreads;
+  w.cnt64 = c->wCnt64.p;
   HIPC(c, hipMemsetAsync(c->wCtr.p, 0, 16 * sizeof(uint32_t), c->stream));
+  HIPC(c, hipMemsetAsync(c->wCnt64.p, 0, 4 * sizeof(unsigned long long), c->stream));
   return ASH_OK;
 }
 
@@ -220,6 +224,7 @@ static int runBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, const ash_rend
   SceneView sc = viewOf(c);
   hipStream_t s = c->stream;
   bool timing = o && o->timing;
+  bool count = o && o->count_traversal;
   hipEvent_t ev[2] = {nullptr, nullptr};
   if (timing) {
     HIPC(c, hipEventCreate(&ev[0]));
@@ -245,11 +250,13 @@ static int runBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, const ash_rend
   int rc = ASH_OK;
   for (uint32_t it = 0; it < iters && rc == ASH_OK; ++it) {
     launchResetCounters(c->wf, cur, s);
-    rc = timed(&st->extend_ms, [&] { launchExtend(sc, c->wf, cur, b.n, c->traceGrid, s); });
+    rc = timed(&st->extend_ms, [&] { launchExtend(sc, c->wf, cur, b.n, c->traceGrid, count, s); });
+    st->extend_launches++;
     if (rc) break;
     rc = timed(&st->shade_ms, [&] { launchShade(sc, b, c->wf, cur, b.n, s); });
     if (rc) break;
-    rc = timed(&st->shadow_ms, [&] { launchShadow(sc, c->wf, b.n, c->traceGrid, s); });
+    rc = timed(&st->shadow_ms, [&] { launchShadow(sc, c->wf, b.n, c->traceGrid, count, s); });
+    st->shadow_launches++;
     cur = 1 - cur;
   }
   if (rc == ASH_OK) launchAccumulate(b, c->wf, s);
@@ -293,6 +300,7 @@ static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots
   ash_stats* S = st ? st : &local;
   std::memset(S, 0, sizeof *S);
   HIPC(c, hipMemsetAsync(c->wf.ctr + 5, 0, 4 * sizeof(uint32_t), c->stream));
+  HIPC(c, hipMemsetAsync(c->wf.cnt64, 0, 4 * sizeof(unsigned long long), c->stream));
   hipEvent_t e0, e1;
   HIPC(c, hipEventCreate(&e0));
   HIPC(c, hipEventCreate(&e1));
@@ -329,6 +337,14 @@ static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots
   S->extension_rays = ext;
   S->shadow_rays = sh;
   S->dielectric_shadow_rays = dsh;
+  if (o && o->count_traversal) {
+    unsigned long long cnt[4];
+    HIPC(c, hipMemcpy(cnt, c->wf.cnt64, sizeof cnt, hipMemcpyDeviceToHost));
+    S->ext_node_visits = cnt[0];
+    S->ext_tri_tests = cnt[1];
+    S->shadow_node_visits = cnt[2];
+    S->shadow_tri_tests = cnt[3];
+  }
   return rc;
 }
 
@@ -642,7 +658,7 @@ int ash_first_hit(ash_ctx* c, const GpuCamera* cam, ash_hit* out) {
   SceneView sc = viewOf(c);
   launchRaygen(b, c->wf, c->stream);
   launchResetCounters(c->wf, 0, c->stream);
-  launchExtend(sc, c->wf, 0, P, c->traceGrid, c->stream);
+  launchExtend(sc, c->wf, 0, P, c->traceGrid, false, c->stream);
   launchFirstHit(sc, c->shotTab.p, c->wf, dout.p, P, c->stream);
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(out, dout.p, P * sizeof(ash_hit), hipMemcpyDeviceToHost, c->stream));

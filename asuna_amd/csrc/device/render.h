@@ -53,6 +53,7 @@ struct Wavefront {
                   // dielectric shadow rays, paths
   int* spill;     // traversal stack spill, kSpill ints per persistent thread
   uint32_t spillThreads;
+  unsigned long long* cnt64;  // [0] ext nodes, [1] ext tris, [2] shadow nodes, [3] shadow tris
 };
 
 struct BatchDesc {
@@ -72,10 +73,11 @@ constexpr int kTraceBlock = 256;
 // Host-side launchers (render.hip).
 void launchRaygen(const BatchDesc& b, const Wavefront& w, hipStream_t s);
 void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t maxRays, uint32_t grid,
-                  hipStream_t s);
+                  bool count, hipStream_t s);
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
                  hipStream_t s);
-void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t maxRays, uint32_t grid, hipStream_t s);
+void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t maxRays, uint32_t grid, bool count,
+                  hipStream_t s);
 void launchAccumulate(const BatchDesc& b, const Wavefront& w, hipStream_t s);
 void launchResetCounters(const Wavefront& w, int cur, hipStream_t s);
 void launchFirstHit(const SceneView& sc, const ShotParams* shot, const Wavefront& w, ash_hit* out, uint32_t P,

@@ -131,8 +131,9 @@ struct Stack {
 
 // Closest hit in [tmin, tmax]; ties at equal t go to the smaller global id.
 // Returns the leaf position of the hit triangle or -1.
+template <bool COUNT>
 __device__ int traceClosest(const SceneView& sc, const TRay& r, float tmin, float tmax, Stack& st, float& bt,
-                            float& bu, float& bv) {
+                            float& bu, float& bv, uint32_t& nNodes, uint32_t& nTris) {
   int best = -1, bestGid = 0x7fffffff;
   bt = tmax;
   st.sp = 0;
@@ -142,6 +143,7 @@ __device__ int traceClosest(const SceneView& sc, const TRay& r, float tmin, floa
       if (node != kEmptyRef) {
         const float4* np = sc.nodes + size_t(node) * 4;
         float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+        if (COUNT) ++nNodes;
         float t0, t1;
         float lim = best >= 0 ? bt : tmax;
         bool h0 = slab(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, r, tmin, lim, t0);
@@ -172,6 +174,7 @@ __device__ int traceClosest(const SceneView& sc, const TRay& r, float tmin, floa
       for (uint32_t k = first; k < first + cnt; ++k) {
         const float4* tp = sc.tris + size_t(k) * 3;
         float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+        if (COUNT) ++nTris;
         float t, u, v;
         if (hitTri(r, p0, p1, p2, tmin, best >= 0 ? bt : tmax, t, u, v)) {
           int gid = __float_as_int(p2.w);
@@ -191,7 +194,9 @@ __device__ int traceClosest(const SceneView& sc, const TRay& r, float tmin, floa
   return best;
 }
 
-__device__ bool traceAny(const SceneView& sc, const TRay& r, float tmin, float tmax, Stack& st) {
+template <bool COUNT>
+__device__ bool traceAny(const SceneView& sc, const TRay& r, float tmin, float tmax, Stack& st, uint32_t& nNodes,
+                         uint32_t& nTris) {
   st.sp = 0;
   int node = 0;
   for (;;) {
@@ -199,6 +204,7 @@ __device__ bool traceAny(const SceneView& sc, const TRay& r, float tmin, float t
       if (node != kEmptyRef) {
         const float4* np = sc.nodes + size_t(node) * 4;
         float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+        if (COUNT) ++nNodes;
         float t0, t1;
         bool h0 = slab(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, r, tmin, tmax, t0);
         bool h1 = slab(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, r, tmin, tmax, t1);
@@ -228,6 +234,7 @@ __device__ bool traceAny(const SceneView& sc, const TRay& r, float tmin, float t
       for (uint32_t k = first; k < first + cnt; ++k) {
         const float4* tp = sc.tris + size_t(k) * 3;
         float t, u, v;
+        if (COUNT) ++nTris;
         if (hitTri(r, tp[0], tp[1], tp[2], tmin, tmax, t, u, v)) return true;
       }
     }
@@ -302,6 +309,7 @@ __global__ __launch_bounds__(256) void kRaygen(BatchDesc b, Wavefront w) {
 }
 
 // ---------------------------------------------------------------- extend
+template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w, int cur) {
   __shared__ int lds[kStackLds * kTraceBlock];
   Stack st;
@@ -312,6 +320,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w
   const uint32_t count = w.ctr[cur];
   const int* q = w.queue[cur];
   int lane = __lane_id();
+  uint32_t nNodes = 0, nTris = 0;
   for (;;) {
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(&w.ctr[3], 64u);
@@ -323,7 +332,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w
       TRay r;
       setupRay(r, xyz(w.ro[pid]), xyz(w.rd[pid]));
       float t = 0.f, u = 0.f, v = 0.f;
-      int pos = traceClosest(sc, r, kMinimum, kInf, st, t, u, v);
+      int pos = traceClosest<COUNT>(sc, r, kMinimum, kInf, st, t, u, v, nNodes, nTris);
       if (pos >= 0) {
         const float4* tp = sc.tris + size_t(pos) * 3;
         w.hitF[pid] = make_float4(t, u, v, 0.f);
@@ -333,9 +342,14 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w
       }
     }
   }
+  if (COUNT) {
+    atomicAdd(&w.cnt64[0], (unsigned long long)nNodes);
+    atomicAdd(&w.cnt64[1], (unsigned long long)nTris);
+  }
 }
 
 // ---------------------------------------------------------------- shadow
+template <bool COUNT>
 __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w) {
   __shared__ int lds[kStackLds * kTraceBlock];
   Stack st;
@@ -345,6 +359,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w
   st.stride = w.spillThreads;
   const uint32_t count = w.ctr[2];
   int lane = __lane_id();
+  uint32_t nNodes = 0, nTris = 0;
   for (;;) {
     uint32_t base = 0;
     if (lane == 0) base = atomicAdd(&w.ctr[4], 64u);
@@ -356,11 +371,15 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w
       TRay r;
       setupRay(r, xyz(so), xyz(sd));
       int pid = __float_as_int(sd.w);
-      if (!traceAny(sc, r, 0.0f, so.w, st)) {
+      if (!traceAny<COUNT>(sc, r, 0.0f, so.w, st, nNodes, nTris)) {
         float4 L = w.rad[pid], d = w.ld[pid];
         w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
       }
     }
+  }
+  if (COUNT) {
+    atomicAdd(&w.cnt64[2], (unsigned long long)nNodes);
+    atomicAdd(&w.cnt64[3], (unsigned long long)nTris);
   }
 }
 
@@ -1050,16 +1069,19 @@ __global__ __launch_bounds__(256) void kFirstHitState(SceneView sc, const ShotPa
 void launchRaygen(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
   kRaygen<<<(b.n + 255) / 256, 256, 0, s>>>(b, w);
 }
-void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t, uint32_t grid, hipStream_t s) {
-  kExtend<<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t, uint32_t grid, bool count,
+                  hipStream_t s) {
+  if (count) kExtend<true><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+  else kExtend<false><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
 }
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
                  hipStream_t s) {
   kShade<<<(maxRays + 255) / 256, 256, 0, s>>>(sc, b, w, cur);
   kStatShadow<<<1, 64, 0, s>>>(w);
 }
-void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t, uint32_t grid, hipStream_t s) {
-  kShadow<<<grid, kTraceBlock, 0, s>>>(sc, w);
+void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t, uint32_t grid, bool count, hipStream_t s) {
+  if (count) kShadow<true><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  else kShadow<false><<<grid, kTraceBlock, 0, s>>>(sc, w);
 }
 void launchAccumulate(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
   kAccumulate<<<(b.P * b.shots + 255) / 256, 256, 0, s>>>(b, w);

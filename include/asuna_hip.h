@@ -222,6 +222,13 @@ typedef struct ash_stats {
   double extend_ms;        /* sum of closest-hit kernel time, if timing enabled */
   double shadow_ms;        /* sum of any-hit kernel time, if timing enabled */
   double shade_ms;         /* sum of shading kernel time, if timing enabled */
+  uint64_t extend_launches; /* closest-hit kernel launches */
+  uint64_t shadow_launches; /* any-hit kernel launches */
+  /* traversal work, filled when ash_render_opts.count_traversal = 1 */
+  uint64_t ext_node_visits; /* BVH nodes fetched by closest-hit rays */
+  uint64_t ext_tri_tests;   /* triangles fetched and tested by closest-hit rays */
+  uint64_t shadow_node_visits;
+  uint64_t shadow_tri_tests;
 } ash_stats;
 
 typedef struct ash_build_opts {
@@ -246,7 +253,7 @@ typedef struct ash_render_opts {
   uint32_t timing;         /* 1 = time each stage with HIP events */
   int32_t depthOutChannel; /* extension: AOV channel receiving first-hit t, -1 = none */
   uint32_t max_bounce_iters; /* safety cap on wavefront iterations (0 = auto) */
-  uint32_t pad;
+  uint32_t count_traversal;  /* 1 = count node visits / triangle tests (slower) */
 } ash_render_opts;
 
 /* ------------------------------------------------------------ error codes */
