@@ -189,6 +189,7 @@ def lib():
                                      P(P(C.c_float))]),
         "ash_free_image": (None, [P(C.c_float)]),
         "ash_tonemap": (None, [C.c_uint32, C.c_uint32, P(C.c_float)]),
+        "ash_env_tables_host": (None, [C.c_uint32, C.c_uint32, P(C.c_float), P(C.c_float), P(C.c_float)]),
         "ash_generate_scene": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_char_p, C.c_char_p,
                                          C.c_size_t]),
     }
@@ -289,6 +290,16 @@ def read_image(path: str, gamma: float = 1.0) -> np.ndarray:
     a = np.ctypeslib.as_array(p, (h.value, w.value, 4)).copy()
     lib().ash_free_image(p)
     return a
+
+
+def env_tables_host(rgba: np.ndarray):
+    """Env importance tables as the library builds them at upload: (marginal (H,2), conditional (H,W,2))."""
+    rgba = np.ascontiguousarray(rgba, dtype=np.float32)
+    h, w = rgba.shape[:2]
+    m = np.zeros((h, 2), np.float32)
+    c = np.zeros((h, w, 2), np.float32)
+    lib().ash_env_tables_host(w, h, fptr(rgba), fptr(m), fptr(c))
+    return m, c
 
 
 def pod_layout():

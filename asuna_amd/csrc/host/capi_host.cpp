@@ -142,6 +142,13 @@ void ash_free_image(float* p) { delete[] p; }
 
 void ash_tonemap(uint32_t type, uint32_t n, float* rgba) { ash::toneMap(type, n, rgba); }
 
+void ash_env_tables_host(uint32_t w, uint32_t h, const float* rgba, float* marg, float* cond) {
+  std::vector<float> m, c;
+  ash::buildEnvTables(w, h, rgba, m, c);
+  std::memcpy(marg, m.data(), m.size() * sizeof(float));
+  std::memcpy(cond, c.data(), c.size() * sizeof(float));
+}
+
 int ash_run_offline(const char* scenePath, const char* outName, const ash_offline_opts* o, ash_stats* total,
                     char* err, size_t errLen) {
   ash_host_scene* s = nullptr;

@@ -80,6 +80,12 @@ int ash_write_image(const char* path, uint32_t width, uint32_t height, const flo
 int ash_read_image(const char* path, float gamma, uint32_t* width, uint32_t* height, float** rgba);
 void ash_free_image(float* rgba);
 
+/* EnvMap::EnvMap importance tables (texture.cpp:144-226) as ash_upload_scene
+ * builds them: marginal H x 2 (column 0: row/H, pdf1D), conditional W*H x 2
+ * (col/W, pdf2D). */
+void ash_env_tables_host(uint32_t width, uint32_t height, const float* rgba, float* marginal,
+                         float* conditional);
+
 /* post.idle.frag tone mapping (types none..pbrt; custom without dither) of an
  * RGBA32F image in place. */
 void ash_tonemap(uint32_t tm_type, uint32_t n_pixels, float* rgba);

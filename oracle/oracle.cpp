@@ -1553,7 +1553,8 @@ void orc_camera_perspective(uint32_t w, uint32_t h, float fov, const float eye[3
   for (int r = 0; r < 3; ++r)
     for (int c = 0; c < 3; ++c) c2w[c * 4 + r] = w2c[r * 4 + c];
   for (int r = 0; r < 3; ++r) {
-    double t = -(double(w2c[0 * 4 + r]) * w2c[12] + double(w2c[1 * 4 + r]) * w2c[13] + double(w2c[2 * 4 + r]) * w2c[14]);
+    // -(R^T t)_r = -sum_k R(k, r) t_k with R(k, r) stored at [r*4 + k]
+    double t = -(double(w2c[r * 4 + 0]) * w2c[12] + double(w2c[r * 4 + 1]) * w2c[13] + double(w2c[r * 4 + 2]) * w2c[14]);
     c2w[12 + r] = float(t);
   }
   // cameraToRasterTransform (camera.cpp:28-62) in double, inverted in double.
@@ -1894,14 +1895,17 @@ float orc_compute_diffuse_fresnel(float ior, int sampleCount) {
   return float(diffuseFresnel);
 }
 
-static void localFrame(V3<float>& N, V3<float>& X, V3<float>& Y) {
+// Probe frame: shading normal +z, tangents from basis(+z), and the
+// face-forward normal the closest-hit shaders pass to eval/pdf/sample.
+static void localFrame(const V3<float>& V, V3<float>& N, V3<float>& X, V3<float>& Y) {
   N = V3<float>(0, 0, 1);
   basis(N, X, Y);
+  if (!(dot(N, V) > 0)) N = -N;
 }
 
 void orc_bsdf_eval(const GpuMaterial* m, const float Lp[3], const float Vp[3], uint32_t flags, float out[3]) {
   V3<float> N, X, Y, L = load3<float>(Lp), V = load3<float>(Vp), r(0);
-  localFrame(N, X, Y);
+  localFrame(V, N, X, Y);
   V3<float> kd = load3<float>(m->diffuse);
   switch (m->type) {
     case MaterialTypeBrdfLambertian: r = lamEval(L, V, N, kd, flags); break;
@@ -1917,7 +1921,7 @@ void orc_bsdf_eval(const GpuMaterial* m, const float Lp[3], const float Vp[3], u
       break;
     }
     case MaterialTypeBsdfDielectric: {
-      float eta = dot(V, N) > 0 ? 1.f / m->ior : m->ior;
+      float eta = V.z > 0 ? 1.f / m->ior : m->ior;
       r = dielEval(L, V, N, eta, flags);
       break;
     }
@@ -1928,7 +1932,7 @@ void orc_bsdf_eval(const GpuMaterial* m, const float Lp[3], const float Vp[3], u
 
 float orc_bsdf_pdf(const GpuMaterial* m, const float Lp[3], const float Vp[3], uint32_t flags) {
   V3<float> N, X, Y, L = load3<float>(Lp), V = load3<float>(Vp);
-  localFrame(N, X, Y);
+  localFrame(V, N, X, Y);
   switch (m->type) {
     case MaterialTypeBrdfLambertian: return lamPdf(L, N, flags);
     case MaterialTypeBrdfPbrMetalnessRoughness: {
@@ -1940,7 +1944,7 @@ float orc_bsdf_pdf(const GpuMaterial* m, const float Lp[3], const float Vp[3], u
       return rpPdf(L, V, N, X, Y, ax, ay, m->ior, luminance(load3<float>(m->diffuse)), flags);
     }
     case MaterialTypeBsdfDielectric: {
-      float eta = dot(V, N) > 0 ? 1.f / m->ior : m->ior;
+      float eta = V.z > 0 ? 1.f / m->ior : m->ior;
       return dielPdf(L, V, N, eta, flags);
     }
     default: return 0;
@@ -1950,7 +1954,7 @@ float orc_bsdf_pdf(const GpuMaterial* m, const float Lp[3], const float Vp[3], u
 void orc_bsdf_sample(const GpuMaterial* m, const float Vp[3], uint32_t* seed, float w[3], float dir[3], float* pdf,
                      uint32_t* flags) {
   V3<float> N, X, Y, V = load3<float>(Vp), d(0), r(0);
-  localFrame(N, X, Y);
+  localFrame(V, N, X, Y);
   V3<float> kd = load3<float>(m->diffuse);
   float p = 0;
   uint32_t f = 0;
@@ -1978,10 +1982,9 @@ void orc_bsdf_sample(const GpuMaterial* m, const float Vp[3], uint32_t* seed, fl
       break;
     }
     case MaterialTypeBsdfDielectric: {
-      float eta = dot(V, N) > 0 ? 1.f / m->ior : m->ior;
-      V3<float> ffN = dot(N, V) > 0 ? N : -N;
+      float eta = V.z > 0 ? 1.f / m->ior : m->ior;
       float u = rnd<float>(*seed);
-      r = dielSample(u, V, ffN, eta, d, p, f);
+      r = dielSample(u, V, N, eta, d, p, f);
       break;
     }
     default: break;

@@ -1,0 +1,223 @@
+"""CPU tests of libasuna.so's host side: exported C ABI, POD layouts, the
+scene loader (src/loader + src/scene semantics), OBJ/EXR/HDR/PNG/NPY IO and
+the procedural scene generator.  No GPU is touched."""
+import ctypes as C
+import filecmp
+import json
+import math
+import os
+import re
+
+import numpy as np
+import pytest
+
+import asuna_amd as A
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol():
+    L = A.lib()
+    names = set()
+    for hdr in ("asuna_hip.h", "asuna_host.h"):
+        text = open(os.path.join(ROOT, "include", hdr)).read()
+        names |= set(re.findall(r"^\s*(?:[\w\*]+\s+)+\**(ash_\w+)\s*\(", text, re.M))
+    assert len(names) >= 28
+    for n in sorted(names):
+        assert hasattr(L, n), n
+
+
+def test_pod_layouts_match_the_reference_structs():
+    # sizeof / offset of the last field: GpuVertex 44 (vertex.h), GpuMaterial 132
+    # (material.h), GpuLight 76 (light.h), GpuCamera 224 (camera.h), push
+    # constants 84 (pushconstant.h)
+    assert A.pod_layout() == [44, 32, 132, 128, 76, 72, 224, 220, 84, 80]
+    assert C.sizeof(A.GpuMaterial) == 132 and C.sizeof(A.GpuLight) == 76 and C.sizeof(A.GpuCamera) == 224
+    assert C.sizeof(A.GpuPushConstantRaytrace) == 84 and C.sizeof(A.GpuVertex) == 44
+
+
+def test_version_string():
+    assert b"gfx950" in A.lib().ash_version()
+
+
+def test_loader_box_scene(box_scene):
+    s = A.HostScene(box_scene)
+    d = s.desc
+    assert s.film == (512, 512) and s.state.num_shots == 1
+    # Scene::reset adds dummy material 0 and dummy light 0 (scene.cpp:93-111)
+    assert d.num_materials == 8 and d.num_lights == 2
+    l0, l1 = d.lights[0], d.lights[1]
+    assert l0.type == 0 and l0.doubleSide == 1 and list(l0.radiance) == [0, 0, 0]
+    assert l1.type == 1 and list(l1.radiance) == [17, 12, 4] and l1.doubleSide == 0
+    assert abs(l1.area - 0.0625) < 1e-7
+    np.testing.assert_allclose(np.cross(list(l1.u), list(l1.v)), [0, -0.0625, 0], atol=1e-7)
+    # rect light -> 4-vertex mesh + emitter instance, created before the
+    # scene's own meshes and instances (scene.cpp:220-244)
+    assert d.num_meshes == 7 and d.num_instances == 8
+    assert d.instances[0].light_id == 1 and d.meshes[d.instances[0].mesh_id].num_vertices == 4
+    assert [d.instances[i].light_id for i in range(1, 8)] == [-1] * 7
+    assert [d.instances[i].material_id for i in range(1, 8)] == list(range(1, 8))
+    cam, pc = s.shot(0)
+    assert pc.spp == 64 and pc.maxPathDepth == 8 and pc.numLights == 1 and pc.nMultiChannel == 3
+    assert (pc.diffuseOutChannel, pc.normalOutChannel, pc.positionOutChannel) == (0, 1, 2)
+    assert pc.uvOutChannel == -1 and pc.hasEnvMap == 0
+    assert s.state.output_hdr == 1 and s.state.tone_mapping == 4
+
+
+def test_loader_toworld_stack_left_multiplies(box_scene):
+    js = json.load(open(box_scene))
+    inst = [i for i in js["instances"] if i["mesh"] == "box"][0]
+    ops = inst["toworld"]
+    S = np.diag(ops[0]["value"] + [1.0])
+    a = math.radians(ops[1]["value"])
+    Ry = np.array([[math.cos(a), 0, math.sin(a), 0], [0, 1, 0, 0], [-math.sin(a), 0, math.cos(a), 0], [0, 0, 0, 1]])
+    T = np.eye(4)
+    T[:3, 3] = ops[2]["value"]
+    want = T @ Ry @ S
+    s = A.HostScene(box_scene)
+    idx = [i for i in range(s.desc.num_instances) if s.desc.instances[i].material_id == 6][0]
+    got = np.array(s.desc.instances[idx].transform).reshape(4, 4).T  # column-major storage
+    np.testing.assert_allclose(got, want, atol=1e-6)
+
+
+def test_loader_mesh_lights_and_env(spheres_small):
+    s = A.HostScene(spheres_small)
+    d = s.desc
+    tri = [d.lights[i] for i in range(1, d.num_lights)]
+    assert len(tri) == 4 and all(l.type == 2 for l in tri)  # "mesh" -> triangle lights
+    for l in tri:
+        assert abs(l.area - 0.5 * np.linalg.norm(np.cross(list(l.u), list(l.v)))) < 1e-6
+        assert np.cross(list(l.u), list(l.v))[1] < 0  # emits downwards
+        assert l.doubleSide == 0
+    assert d.envmap.width == 256 and d.envmap.height == 128
+    cam, pc = s.shot(0)
+    assert pc.hasEnvMap == 1 and list(pc.envMapResolution) == [256.0, 128.0] and pc.numLights == 4
+    assert s.state.depth_out_channel == 2
+
+
+def _write_scene(tmp_path, body, obj="v 0 0 0\nv 1 0 0\nv 0 1 0\nf 1 2 3\n"):
+    (tmp_path / "m.obj").write_text(obj)
+    p = tmp_path / "s.json"
+    p.write_text(json.dumps(body))
+    return str(p)
+
+
+def _base():
+    return {"state": {"path_tracing": {"spp": 2}}, "camera": {"type": "perspective", "film": {"resolution": [8, 6]}},
+            "materials": [{"name": "m", "type": "brdf_lambertian"}], "meshes": [{"name": "a", "path": "m.obj"}],
+            "instances": [{"mesh": "a", "material": "m"}],
+            "shots": [{"type": "lookat", "eye": [0, 0, 2], "lookat": [0, 0, 0], "up": [0, 1, 0]}]}
+
+
+def test_loader_triangle_light_is_a_half_area_rect(tmp_path):
+    b = _base()
+    b["lights"] = [{"type": "triangle", "radiance": [1, 1, 1], "position": [0, 1, 0], "v1": [1, 1, 0],
+                    "v2": [0, 1, 1]}]
+    s = A.HostScene(_write_scene(tmp_path, b))
+    l = s.desc.lights[1]
+    assert l.type == 1 and abs(l.area - 0.5) < 1e-7  # loader.cpp:284-296
+
+
+def test_loader_shot_state_override_and_material_defaults(tmp_path):
+    b = _base()
+    b["shots"][0]["state"] = {"path_tracing": {"spp": 7, "max_path_depth": 5, "background_color": [1, 2, 3]}}
+    b["materials"].append({"name": "g", "type": "bsdf_dielectric"})
+    b["materials"].append({"name": "rp", "type": "brdf_rough_plastic", "ior": 1.5})
+    s = A.HostScene(_write_scene(tmp_path, b))
+    cam, pc = s.shot(0)
+    assert pc.spp == 7 and pc.maxPathDepth == 5 and list(pc.bgColor) == [1, 2, 3]
+    m = s.desc.materials
+    assert m[2].type == 10 and abs(m[2].ior - 1.5) < 1e-7  # default ior (material.h:9-35)
+    assert m[1].type == 0 and abs(m[1].roughness - 0.5) < 1e-7 and list(m[1].radianceFactor) == [1, 1, 1]
+    assert abs(m[3].radiance[0] - 0.5963) < 2e-3  # fdrInt, internal diffuse Fresnel of ior 1.5
+
+
+@pytest.mark.parametrize("mutate,code", [
+    (lambda b: b.pop("instances"), "malformed"),
+    (lambda b: b["instances"][0].pop("material"), "malformed"),
+    (lambda b: b["instances"][0].update(material="nope"), "malformed"),
+    (lambda b: b["meshes"][0].update(path="missing.obj"), "i/o"),
+    (lambda b: b["materials"][0].update(type="brdf_unknown"), "malformed"),
+    (lambda b: b["camera"].update(type="fisheye"), "malformed"),
+])
+def test_loader_errors_are_codes_not_exits(tmp_path, mutate, code):
+    b = _base()
+    mutate(b)
+    with pytest.raises(A.AsunaError, match=code):
+        A.HostScene(_write_scene(tmp_path, b))
+
+
+def test_loader_rejects_broken_json(tmp_path):
+    p = tmp_path / "bad.json"
+    p.write_text('{"state": {,}')
+    with pytest.raises(A.AsunaError, match="malformed"):
+        A.HostScene(str(p))
+
+
+def test_obj_semantics_deindex_uvflip_fan(tmp_path):
+    """loadMesh (mesh.cpp:112-145): one vertex per face corner, index =
+    running count, uv.y flipped, missing normals zero, polygons fanned."""
+    obj = ("v 0 0 0\nv 1 0 0\nv 1 1 0\nv 0 1 0\nvt 0.25 0.1\nvt 0.75 0.9\nvn 0 0 1\n"
+           "f 1/1/1 2/2/1 3/1/1 4/2/1\nf -4 -3 -2\n")
+    b = _base()
+    s = A.HostScene(_write_scene(tmp_path, b, obj))
+    (v, idx), = s.meshes()
+    assert len(idx) == 9 and list(idx) == list(range(9))  # quad -> 2 tris + 1 tri
+    np.testing.assert_allclose(v[:3, 0:3], [[0, 0, 0], [1, 0, 0], [1, 1, 0]])
+    np.testing.assert_allclose(v[3:6, 0:3], [[0, 0, 0], [1, 1, 0], [0, 1, 0]])
+    np.testing.assert_allclose(v[0, 3:5], [0.25, 0.9], atol=1e-7)  # (u, 1 - v)
+    np.testing.assert_allclose(v[1, 3:5], [0.75, 0.1], atol=1e-7)
+    np.testing.assert_allclose(v[0, 5:8], [0, 0, 1])
+    np.testing.assert_allclose(v[6:9, 5:8], 0)  # no normal index -> zero normal
+
+
+def test_image_io_round_trips(tmp_path):
+    rng = np.random.default_rng(0)
+    img = rng.random((13, 17, 4)).astype(np.float32) * 4
+    img[..., 3] = 1
+    # EXR (half): values that are exact halves survive bit-exactly
+    half = img.astype(np.float16).astype(np.float32)
+    A.write_image(str(tmp_path / "a.exr"), half)
+    back = A.read_image(str(tmp_path / "a.exr"))
+    np.testing.assert_array_equal(back[..., :3], half[..., :3])
+    # HDR (RGBE): 8-bit shared-exponent mantissas
+    A.write_image(str(tmp_path / "a.hdr"), img)
+    back = A.read_image(str(tmp_path / "a.hdr"))
+    # shared exponent: every channel is quantised to 1/256 of the pixel's largest one
+    tol = img[..., :3].max(-1, keepdims=True) / 128
+    assert (np.abs(back[..., :3] - img[..., :3]) <= tol).all()
+    assert (back[..., :3] <= img[..., :3] + 1e-7).all()  # mantissas truncate
+    # PNG: v -> int(clamp(255 v + 0.5)), read back as v/255
+    ldr = rng.random((5, 7, 4)).astype(np.float32)
+    A.write_image(str(tmp_path / "a.png"), ldr)
+    back = A.read_image(str(tmp_path / "a.png"))
+    np.testing.assert_allclose(back, np.floor(np.clip(ldr * 255 + 0.5, 0, 255)) / 255, atol=1e-7)
+    # NPY
+    A.write_image(str(tmp_path / "a.npy"), img)
+    np.testing.assert_array_equal(np.load(str(tmp_path / "a.npy")), img)
+
+
+def test_tonemap_filmic_closed_form():
+    x = np.array([[0.0, 0.5, 2.0, 1.0]], np.float32)
+    y = x.copy()
+    A.lib().ash_tonemap(4, 1, A.fptr(y))
+    v = np.maximum(x[0, :3] - 0.004, 0)
+    np.testing.assert_allclose(y[0, :3], (v * (6.2 * v + 0.5)) / (v * (6.2 * v + 1.7) + 0.06), rtol=1e-6)
+
+
+def test_scene_generator_is_deterministic(tmp_path):
+    a = A.generate_scene(1, str(tmp_path / "a"), scale=1)
+    b = A.generate_scene(1, str(tmp_path / "b"), scale=1)
+    for f in os.listdir(os.path.dirname(a)):
+        assert filecmp.cmp(os.path.join(os.path.dirname(a), f), os.path.join(os.path.dirname(b), f), shallow=False), f
+
+
+def test_multishot_scene_has_64_shots_with_own_fov(multishot_scene):
+    s = A.HostScene(multishot_scene)
+    assert s.state.num_shots == 64
+    cams = [s.shot(i)[0] for i in range(64)]
+    fx = {round(c.rasterToCamera[0], 6) for c in cams}
+    assert len(fx) > 32  # per-shot fov extension reaches the camera matrices
+    eyes = np.array([[c.cameraToWorld[12], c.cameraToWorld[13], c.cameraToWorld[14]] for c in cams])
+    r = np.linalg.norm(eyes, axis=1)
+    assert r.min() >= 5 - 1e-4 and r.max() <= 9 + 1e-4
