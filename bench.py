@@ -52,6 +52,8 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--queue-paths", type=int, default=0)
     ap.add_argument("--scene-dir", default="")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="run a 256 MiB device-to-device copy first (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -60,7 +62,16 @@ def main():
 
     import asuna_amd as A
 
-    cfg_id = {"c1": 0, "c2": 1, "c3": 2}[args.config]
+    if args.copy_probe:
+        import torch  # plumbing only: a plain copy kernel of known size
+        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"[bench] copy probe: {src.numel() * 4} bytes read + written")
+        del src, dst
+
+    cfg_id ={"c1": 0, "c2": 1, "c3": 2}[args.config]
     sdir = args.scene_dir or os.path.join(tempfile.gettempdir(), f"asuna_bench_{args.config}_s{args.scale}")
     t0 = time.time()
     scene_path = os.path.join(sdir, {0: "box.json", 1: "spheres.json", 2: "multishot.json"}[cfg_id])
