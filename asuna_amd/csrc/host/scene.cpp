@@ -486,7 +486,7 @@ class Loader {
       l.area = len3(cross3(u, v));
       l.type = LightTypeRect;
       if (lj.contains("double_side")) l.doubleSide = lj["double_side"].b() ? 1 : 0;
-      if (t == "triangle") l.area *= 0.5f;  // stays a rect light (Appendix quirk)
+      if (t == "triangle") l.area *= 0.5f;  // still a rect light, with half the area
     } else if (t == "point") {
       checkKeys(lj, {"position"});
       set3(l.position, vec3(lj["position"]));

@@ -57,7 +57,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not built)", file=sys.stderr)
+              "(sharding across GPUs is designed but not built)", file=sys.stderr)
         sys.exit(2)
 
     import asuna_amd as A
@@ -71,7 +71,7 @@ def main():
         log(f"[bench] copy probe: {src.numel() * 4} bytes read + written")
         del src, dst
 
-    cfg_id ={"c1": 0, "c2": 1, "c3": 2}[args.config]
+    cfg_id = {"c1": 0, "c2": 1, "c3": 2}[args.config]
     sdir = args.scene_dir or os.path.join(tempfile.gettempdir(), f"asuna_bench_{args.config}_s{args.scale}")
     t0 = time.time()
     scene_path = os.path.join(sdir, {0: "box.json", 1: "spheres.json", 2: "multishot.json"}[cfg_id])
