@@ -103,28 +103,74 @@ class Stats(C.Structure):
                 ("dielectric_shadow_rays", C.c_uint64), ("render_ms", C.c_double), ("extend_ms", C.c_double),
                 ("shadow_ms", C.c_double), ("shade_ms", C.c_double), ("extend_launches", C.c_uint64),
                 ("shadow_launches", C.c_uint64), ("ext_node_visits", C.c_uint64), ("ext_tri_tests", C.c_uint64),
-                ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64)]
+                ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64), ("batches", C.c_uint64),
+                ("graph_launches", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class BuildOpts(C.Structure):
-    _fields_ = [("leaf_max_tris", C.c_uint32), ("quality", C.c_uint32)]
+    _fields_ = [("leaf_max_tris", C.c_uint32), ("quality", C.c_uint32), ("branching", C.c_uint32),
+                ("pad", C.c_uint32)]
 
 
 class BuildInfo(C.Structure):
     _fields_ = [("num_triangles", C.c_uint64), ("num_nodes", C.c_uint64), ("max_depth", C.c_uint32),
                 ("pad", C.c_uint32), ("build_ms", C.c_double), ("sah_cost", C.c_double),
-                ("device_bytes", C.c_uint64)]
+                ("device_bytes", C.c_uint64), ("num_wide_nodes", C.c_uint64), ("wide_depth", C.c_uint32),
+                ("branching", C.c_uint32)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
 
 
 class RenderOpts(C.Structure):
-    _fields_ = [("queue_paths", C.c_uint32), ("block_size", C.c_uint32), ("timing", C.c_uint32),
-                ("depthOutChannel", C.c_int32), ("max_bounce_iters", C.c_uint32), ("count_traversal", C.c_uint32)]
+    _fields_ = [("queue_paths", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("timing", C.c_uint32),
+                ("depthOutChannel", C.c_int32), ("max_bounce_iters", C.c_uint32), ("count_traversal", C.c_uint32),
+                ("no_graphs", C.c_uint32), ("pad", C.c_uint32)]
+
+    @staticmethod
+    def make(queue_paths=0, blocks_per_cu=0, timing=False, depth_channel=-1, max_bounce_iters=0,
+             count_traversal=False, graphs=True):
+        return RenderOpts(queue_paths, blocks_per_cu, int(bool(timing)), depth_channel, max_bounce_iters,
+                          int(bool(count_traversal)), 0 if graphs else 1, 0)
+
+
+class TuneResult(C.Structure):
+    _fields_ = [("queue_paths", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("candidates", C.c_uint32),
+                ("pad", C.c_uint32), ("best_ms", C.c_double), ("worst_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("queue_paths", C.c_uint32), ("blocks_per_cu", C.c_uint32),
+                ("max_jobs_per_batch", C.c_uint32), ("batches_ahead", C.c_uint32)]
+
+
+class JobDesc(C.Structure):
+    _fields_ = [("scene", C.c_uint32), ("width", C.c_uint32), ("height", C.c_uint32), ("spp", C.c_int32),
+                ("camera", GpuCamera), ("pc", GpuPushConstantRaytrace), ("depth_channel", C.c_int32),
+                ("flags", C.c_uint32)]
+
+
+class JobStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("pad", C.c_uint32), ("submit_ms", C.c_double), ("start_ms", C.c_double),
+                ("done_ms", C.c_double), ("paths", C.c_uint64)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [("jobs_submitted", C.c_uint64), ("jobs_done", C.c_uint64), ("batches", C.c_uint64),
+                ("paths", C.c_uint64), ("arena_bytes", C.c_uint64), ("arena_used_bytes", C.c_uint64),
+                ("busy_ms", C.c_double), ("uptime_ms", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class SceneState(C.Structure):
@@ -168,7 +214,18 @@ def lib():
                                  P(RenderOpts), P(Stats)]),
         "ash_render_shots": (C.c_int, [vp, C.c_uint32, P(GpuCamera), P(GpuPushConstantRaytrace), C.c_int32,
                                        P(RenderOpts), P(Stats)]),
+        "ash_autotune": (C.c_int, [vp, C.c_uint32, P(GpuCamera), P(GpuPushConstantRaytrace), C.c_int32,
+                                   P(RenderOpts), P(TuneResult)]),
         "ash_read_shot_output": (C.c_int, [vp, C.c_uint32, C.c_int32, P(C.c_float)]),
+        "ash_server_create": (C.c_int, [C.c_int, P(ServerOpts), P(vp)]),
+        "ash_server_destroy": (None, [vp]),
+        "ash_server_last_error": (C.c_char_p, [vp]),
+        "ash_server_add_scene": (C.c_int, [vp, P(SceneDesc), P(BuildOpts), P(C.c_uint32)]),
+        "ash_server_submit": (C.c_int, [vp, P(JobDesc), P(C.c_uint64)]),
+        "ash_server_wait": (C.c_int, [vp, C.c_uint64, C.c_int32, P(JobStatus)]),
+        "ash_server_read": (C.c_int, [vp, C.c_uint64, C.c_int32, P(C.c_float)]),
+        "ash_server_release": (C.c_int, [vp, C.c_uint64]),
+        "ash_server_stats": (C.c_int, [vp, P(ServerCounters)]),
         "ash_read_output": (C.c_int, [vp, C.c_int32, P(C.c_float)]),
         "ash_output_device_ptr": (C.c_int, [vp, C.c_int32, P(vp)]),
         "ash_first_hit": (C.c_int, [vp, P(GpuCamera), vp]),
@@ -343,9 +400,10 @@ class Renderer:
     def upload_desc(self, desc: SceneDesc):
         self._check(lib().ash_upload_scene(self._c, C.byref(desc)), "upload_scene")
 
-    def build(self, leaf_max_tris=4, quality=1) -> dict:
+    def build(self, leaf_max_tris=4, quality=1, branching=4) -> dict:
         info = BuildInfo()
-        self._check(lib().ash_build_accel(self._c, C.byref(BuildOpts(leaf_max_tris, quality)), C.byref(info)),
+        opts = BuildOpts(leaf_max_tris, quality, branching, 0)
+        self._check(lib().ash_build_accel(self._c, C.byref(opts), C.byref(info)),
                     "build_accel")
         return info.as_dict()
 
@@ -354,23 +412,34 @@ class Renderer:
         self.width, self.height = w, h
 
     def render(self, cam, pc, first_frame=0, n_frames=1, queue_paths=0, timing=False, depth_channel=-1,
-               max_bounce_iters=0, count_traversal=False) -> dict:
-        opts = RenderOpts(queue_paths, 0, 1 if timing else 0, depth_channel, max_bounce_iters,
-                          1 if count_traversal else 0)
+               max_bounce_iters=0, count_traversal=False, blocks_per_cu=0, graphs=True) -> dict:
+        opts = RenderOpts.make(queue_paths, blocks_per_cu, timing, depth_channel, max_bounce_iters, count_traversal,
+                               graphs)
         st = Stats()
         self._check(lib().ash_render(self._c, C.byref(cam), C.byref(pc), first_frame, n_frames, C.byref(opts),
                                      C.byref(st)), "render")
         return st.as_dict()
 
-    def render_shots(self, cams, pcs, n_frames, queue_paths=0, timing=False) -> dict:
+    def render_shots(self, cams, pcs, n_frames, queue_paths=0, timing=False, blocks_per_cu=0, graphs=True,
+                     count_traversal=False) -> dict:
         n = len(cams)
         ca = (GpuCamera * n)(*cams)
         pa = (GpuPushConstantRaytrace * n)(*pcs)
-        opts = RenderOpts(queue_paths, 0, 1 if timing else 0, -1, 0, 0)
+        opts = RenderOpts.make(queue_paths, blocks_per_cu, timing, -1, 0, count_traversal, graphs)
         st = Stats()
         self._check(lib().ash_render_shots(self._c, n, ca, pa, n_frames, C.byref(opts), C.byref(st)),
                     "render_shots")
         return st.as_dict()
+
+    def autotune(self, cams, pcs, probe_frames=1, **opts) -> tuple:
+        """Returns (RenderOpts with the fastest queue_paths / blocks_per_cu, report dict)."""
+        n = len(cams)
+        ca = (GpuCamera * n)(*cams)
+        pa = (GpuPushConstantRaytrace * n)(*pcs)
+        o = RenderOpts.make(**opts)
+        res = TuneResult()
+        self._check(lib().ash_autotune(self._c, n, ca, pa, probe_frames, C.byref(o), C.byref(res)), "autotune")
+        return o, res.as_dict()
 
     def read_shot(self, shot, image) -> np.ndarray:
         out = np.zeros((self.height, self.width, 4), np.float32)
@@ -396,10 +465,78 @@ class Renderer:
 
 def run_offline(scene: str, out: str, gpu_id: int = 0, output_scanline: bool = False, spp_override: int = 0,
                 quiet: bool = True) -> dict:
-    o = OfflineOpts(gpu_id, int(output_scanline), spp_override, int(quiet), RenderOpts(0, 0, 0, -1, 0, 0))
+    o = OfflineOpts(gpu_id, int(output_scanline), spp_override, int(quiet), RenderOpts.make())
     st = Stats()
     err = C.create_string_buffer(4096)
     rc = lib().ash_run_offline(scene.encode(), out.encode(), C.byref(o), C.byref(st), err, len(err))
     if rc != ASH_OK:
         raise AsunaError(f"run_offline failed ({ERRORS.get(rc, rc)}): {err.value.decode()}")
     return st.as_dict()
+
+
+# ------------------------------------------------------------ render server
+class Server:
+    """Resident render server (include/asuna_server.h): scenes stay in HBM,
+    jobs are admitted by an HBM budget and co-batched per scene."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0, queue_paths=0, blocks_per_cu=0, max_jobs_per_batch=0,
+                 batches_ahead=0):
+        self._s = C.c_void_p()
+        o = ServerOpts(hbm_budget_bytes, queue_paths, blocks_per_cu, max_jobs_per_batch, batches_ahead)
+        rc = lib().ash_server_create(device, C.byref(o), C.byref(self._s))
+        if rc != ASH_OK:
+            self._s = None
+            raise AsunaError(f"ash_server_create failed: {ERRORS.get(rc, rc)}")
+        self._sizes = {}
+        self._scenes = []
+
+    def _check(self, rc, what):
+        if rc != ASH_OK:
+            msg = lib().ash_server_last_error(self._s).decode()
+            raise AsunaError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}")
+
+    def add_scene(self, scene: "HostScene", leaf_max_tris=4, quality=1, branching=4) -> int:
+        sid = C.c_uint32()
+        self._check(lib().ash_server_add_scene(self._s, C.byref(scene.desc),
+                                               C.byref(BuildOpts(leaf_max_tris, quality, branching, 0)),
+                                               C.byref(sid)), "add_scene")
+        self._scenes.append(scene)  # keeps the host arrays alive
+        return sid.value
+
+    def submit(self, scene_id, cam, pc, width, height, spp, depth_channel=-1) -> int:
+        d = JobDesc(scene_id, width, height, spp, cam, pc, depth_channel, 0)
+        jid = C.c_uint64()
+        self._check(lib().ash_server_submit(self._s, C.byref(d), C.byref(jid)), "submit")
+        self._sizes[jid.value] = (height, width)
+        return jid.value
+
+    def wait(self, job_id, timeout_ms=-1) -> dict:
+        st = JobStatus()
+        self._check(lib().ash_server_wait(self._s, job_id, timeout_ms, C.byref(st)), "wait")
+        return st.as_dict()
+
+    def read(self, job_id, image=0) -> np.ndarray:
+        h, w = self._sizes[job_id]
+        out = np.zeros((h, w, 4), np.float32)
+        self._check(lib().ash_server_read(self._s, job_id, image, fptr(out)), "read")
+        return out
+
+    def release(self, job_id):
+        self._check(lib().ash_server_release(self._s, job_id), "release")
+        self._sizes.pop(job_id, None)
+
+    def stats(self) -> dict:
+        c = ServerCounters()
+        self._check(lib().ash_server_stats(self._s, C.byref(c)), "stats")
+        return c.as_dict()
+
+    def close(self):
+        if self._s:
+            lib().ash_server_destroy(self._s)
+            self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

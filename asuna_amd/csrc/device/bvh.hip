@@ -328,6 +328,94 @@ __global__ void kValidateParents(const unsigned* refcnt, uint32_t numNodes, unsi
   if (refcnt[i] != (i == 0 ? 0u : 1u)) atomicAdd(bad, 1u);
 }
 
+// ------------------------------------------------------ 4-wide collapse
+// One frontier level per launch: each 4-wide node takes a binary node and
+// repeatedly opens its largest-area internal child until it holds four
+// children (or only leaves remain).  Internal children get the next node ids
+// (siblings are allocated together, so a node's children are adjacent).
+__device__ __forceinline__ float slotArea(const float* lo, const float* hi) {
+  float dx = hi[0] - lo[0], dy = hi[1] - lo[1], dz = hi[2] - lo[2];
+  return dx * dy + dy * dz + dz * dx;
+}
+
+__global__ void kCollapse4(const float4* n2, const int2* front, uint32_t nFront, float4* n4, unsigned* n4Count,
+                           int2* next, unsigned* nextCount) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nFront) return;
+  int src = front[i].x, dst = front[i].y;
+  int ref[4];
+  float lo[4][3], hi[4][3];
+  auto loadChildren = [&](int node, int s0, int s1) {
+    float4 a = n2[size_t(node) * 4 + 0], b = n2[size_t(node) * 4 + 1], c = n2[size_t(node) * 4 + 2],
+           d = n2[size_t(node) * 4 + 3];
+    lo[s0][0] = a.x, hi[s0][0] = a.y, lo[s0][1] = a.z, hi[s0][1] = a.w, lo[s0][2] = c.x, hi[s0][2] = c.y;
+    lo[s1][0] = b.x, hi[s1][0] = b.y, lo[s1][1] = b.z, hi[s1][1] = b.w, lo[s1][2] = c.z, hi[s1][2] = c.w;
+    ref[s0] = __float_as_int(d.x);
+    ref[s1] = __float_as_int(d.y);
+  };
+  loadChildren(src, 0, 1);
+  int cnt = 2;
+  while (cnt < 4) {
+    int pick = -1;
+    float best = -1.f;
+    for (int k = 0; k < cnt; ++k) {
+      if (ref[k] < 0 || ref[k] == kEmptyRef) continue;
+      float a = slotArea(lo[k], hi[k]);
+      if (a > best) best = a, pick = k;
+    }
+    if (pick < 0) break;
+    loadChildren(ref[pick], pick, cnt);
+    ++cnt;
+  }
+  int internal = 0;
+  for (int k = 0; k < cnt; ++k) internal += (ref[k] >= 0 && ref[k] != kEmptyRef) ? 1 : 0;
+  unsigned base = internal ? atomicAdd(n4Count, unsigned(internal)) : 0u;
+  unsigned fbase = internal ? atomicAdd(nextCount, unsigned(internal)) : 0u;
+  int out[4];
+  const float far = 3.0e38f;
+  for (int k = 0; k < 4; ++k) {
+    if (k >= cnt) {
+      out[k] = kEmptyRef;
+      lo[k][0] = lo[k][1] = lo[k][2] = hi[k][0] = hi[k][1] = hi[k][2] = far;
+    } else if (ref[k] >= 0 && ref[k] != kEmptyRef) {
+      out[k] = int(base);
+      next[fbase] = make_int2(ref[k], int(base));
+      ++base;
+      ++fbase;
+    } else {
+      out[k] = ref[k];
+    }
+  }
+  float4* o = n4 + size_t(dst) * 8;
+  o[0] = make_float4(lo[0][0], lo[1][0], lo[2][0], lo[3][0]);
+  o[1] = make_float4(hi[0][0], hi[1][0], hi[2][0], hi[3][0]);
+  o[2] = make_float4(lo[0][1], lo[1][1], lo[2][1], lo[3][1]);
+  o[3] = make_float4(hi[0][1], hi[1][1], hi[2][1], hi[3][1]);
+  o[4] = make_float4(lo[0][2], lo[1][2], lo[2][2], lo[3][2]);
+  o[5] = make_float4(hi[0][2], hi[1][2], hi[2][2], hi[3][2]);
+  o[6] = make_float4(__int_as_float(out[0]), __int_as_float(out[1]), __int_as_float(out[2]), __int_as_float(out[3]));
+  o[7] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ void kValidate4(const float4* nodes, uint32_t numNodes, uint32_t numTris, unsigned* refcnt, unsigned* bad) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= numNodes) return;
+  float4 r4 = nodes[size_t(i) * 8 + 6];
+  int32_t refs[4] = {__float_as_int(r4.x), __float_as_int(r4.y), __float_as_int(r4.z), __float_as_int(r4.w)};
+  for (int q = 0; q < 4; ++q) {
+    int32_t r = refs[q];
+    if (r == kEmptyRef) continue;
+    if (r >= 0) {
+      if (uint32_t(r) >= numNodes || r == 0) atomicAdd(bad, 1u);
+      else atomicAdd(&refcnt[r], 1u);
+    } else {
+      uint32_t first, cnt;
+      decodeLeaf(r, first, cnt);
+      if (uint64_t(first) + cnt > numTris) atomicAdd(bad, 1u);
+    }
+  }
+}
+
 template <class T> struct DevBuf {
   T* p = nullptr;
   ~DevBuf() {
@@ -460,6 +548,51 @@ int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_
     HIPB(hipStreamSynchronize(s));
     if (nbad) {
       std::snprintf(err, errLen, "bvh: structural validation failed (%u violations)", nbad);
+      return ASH_E_INVALID;
+    }
+  }
+  if (a.branching == 4) {
+    DevBuf<int2> f0, f1;
+    DevBuf<unsigned> cnts;  // [0] 4-wide nodes allocated, [1] next frontier size
+    HIPB(hipMalloc(&r.nodes4, size_t(numNodes) * 8 * sizeof(float4)));
+    HIPB(hipMalloc(&f0.p, size_t(numNodes) * sizeof(int2)));
+    HIPB(hipMalloc(&f1.p, size_t(numNodes) * sizeof(int2)));
+    HIPB(hipMalloc(&cnts.p, 2 * sizeof(unsigned)));
+    int2 root = make_int2(0, 0);
+    unsigned init[2] = {1u, 0u};
+    HIPB(hipMemcpyAsync(f0.p, &root, sizeof root, hipMemcpyHostToDevice, s));
+    HIPB(hipMemcpyAsync(cnts.p, init, sizeof init, hipMemcpyHostToDevice, s));
+    uint32_t nFront = 1, levels = 0;
+    while (nFront) {
+      kCollapse4<<<(nFront + kBlock - 1) / kBlock, kBlock, 0, s>>>(r.nodes, f0.p, nFront, r.nodes4, cnts.p, f1.p,
+                                                                    cnts.p + 1);
+      unsigned c[2];
+      HIPB(hipMemcpyAsync(c, cnts.p, sizeof c, hipMemcpyDeviceToHost, s));
+      HIPB(hipMemsetAsync(cnts.p + 1, 0, sizeof(unsigned), s));
+      HIPB(hipStreamSynchronize(s));
+      std::swap(f0.p, f1.p);
+      nFront = c[1];
+      r.numNodes4 = c[0];
+      ++levels;
+      if (levels > 200 || r.numNodes4 > numNodes) {
+        std::snprintf(err, errLen, "bvh: 4-wide collapse did not terminate");
+        return ASH_E_INVALID;
+      }
+    }
+    r.depth4 = levels;
+    DevBuf<unsigned> refcnt, bad;
+    HIPB(hipMalloc(&refcnt.p, size_t(r.numNodes4) * sizeof(unsigned)));
+    HIPB(hipMalloc(&bad.p, sizeof(unsigned)));
+    HIPB(hipMemsetAsync(refcnt.p, 0, size_t(r.numNodes4) * sizeof(unsigned), s));
+    HIPB(hipMemsetAsync(bad.p, 0, sizeof(unsigned), s));
+    unsigned gv = (r.numNodes4 + kBlock - 1) / kBlock;
+    kValidate4<<<gv, kBlock, 0, s>>>(r.nodes4, r.numNodes4, n, refcnt.p, bad.p);
+    kValidateParents<<<gv, kBlock, 0, s>>>(refcnt.p, r.numNodes4, bad.p);
+    unsigned nbad = 0;
+    HIPB(hipMemcpyAsync(&nbad, bad.p, sizeof nbad, hipMemcpyDeviceToHost, s));
+    HIPB(hipStreamSynchronize(s));
+    if (nbad) {
+      std::snprintf(err, errLen, "bvh: 4-wide structural validation failed (%u violations)", nbad);
       return ASH_E_INVALID;
     }
   }

@@ -9,7 +9,7 @@
 #include <vector>
 
 #include "../host/scene.h"
-#include "render.h"
+#include "ctx_internal.h"
 
 using namespace ashd;
 
@@ -42,6 +42,31 @@ template <class T> struct DBuf {
 
 }  // namespace
 
+// Identity of a captured bounce loop: every kernel argument it recorded
+// (the shading kernel reads only the shot table / single-shot copy of the
+// batch descriptor).
+struct GraphKey {
+  SceneView sc;
+  Wavefront w;
+  const ShotParams* tab;
+  ShotParams one;
+  uint32_t grid, iters, n, single;
+};
+
+struct GraphEntry {
+  GraphKey key;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint64_t lastUse = 0;
+};
+
+static void releaseGraph(GraphEntry& g) {
+  if (g.exec) (void)hipGraphExecDestroy(g.exec);
+  if (g.graph) (void)hipGraphDestroy(g.graph);
+  g.exec = nullptr;
+  g.graph = nullptr;
+}
+
 struct ash_ctx {
   int device = 0;
   hipStream_t stream = nullptr;
@@ -63,7 +88,9 @@ struct ash_ctx {
   uint32_t numTris = 0, numInst = 0;
   uint32_t maxDepth = 0;
   float4* nodes = nullptr;
+  float4* nodes4 = nullptr;  // 4-wide tree when built with branching 4
   float4* tris = nullptr;
+  uint32_t stackNeed = 0;    // traversal stack entries the current tree can need
   uint64_t nodeCount = 0;
   // film (single shot)
   uint32_t width = 0, height = 0;
@@ -78,19 +105,26 @@ struct ash_ctx {
   DBuf<uint4> wInfo;
   DBuf<int4> wHitI;
   DBuf<int> wQ0, wQ1, wSpill;
+  DBuf<float> wSpillT;
   DBuf<uint32_t> wCtr;
   DBuf<unsigned long long> wCnt64;
-  uint32_t traceGrid = 0;
+  DBuf<WorkUnit> unitBuf;
+  GraphEntry graphs[4];
+  uint64_t graphClock = 0;
+  bool graphsBroken = false;
 
   ~ash_ctx() {
     if (nodes) (void)hipFree(nodes);
+    if (nodes4) (void)hipFree(nodes4);
     if (tris) (void)hipFree(tris);
     for (auto& t : texData) t.release();
     for (auto* b : {&verts, &env, &images, &shotImages, &wRo, &wRd, &wThr, &wRad, &wLd, &wHitF, &wShO, &wShD})
       b->release();
     inst.release(), idx.release(), instTriOffset.release(), mats.release(), lights.release(), texTab.release();
     marg.release(), cond.release(), shotTab.release(), wInfo.release(), wHitI.release();
-    wQ0.release(), wQ1.release(), wSpill.release(), wCtr.release(), wCnt64.release();
+    wQ0.release(), wQ1.release(), wSpill.release(), wSpillT.release(), wCtr.release(), wCnt64.release();
+    unitBuf.release();
+    for (auto& g : graphs) releaseGraph(g);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -146,7 +180,9 @@ static void invert3x4(const float* m, float* out) {
 
 static SceneView viewOf(const ash_ctx* c) {
   SceneView v;
+  std::memset(&v, 0, sizeof v);
   v.nodes = c->nodes;
+  v.nodes4 = c->nodes4;
   v.tris = c->tris;
   v.inst = c->inst.p;
   v.verts = c->verts.p;
@@ -179,10 +215,7 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   MALLOC(c, c->wQ0, cap);
   MALLOC(c, c->wQ1, cap);
   MALLOC(c, c->wCtr, 16);
-  MALLOC(c, c->wCnt64, 4);
-  c->traceGrid = uint32_t(std::max(1, c->numCU) * 8);
-  uint32_t spillThreads = c->traceGrid * kTraceBlock;
-  MALLOC(c, c->wSpill, size_t(spillThreads) * kSpill);
+  MALLOC(c, c->wCnt64, 8);
   Wavefront& w = c->wf;
   w.capacity = cap;
   w.ro = c->wRo.p;
@@ -198,11 +231,25 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   w.shO = c->wShO.p;
   w.shD = c->wShD.p;
   w.ctr = c->wCtr.p;
-  w.spill = c->wSpill.p;
-  w.spillThreads = spillThreads;
   w.cnt64 = c->wCnt64.p;
   HIPC(c, hipMemsetAsync(c->wCtr.p, 0, 16 * sizeof(uint32_t), c->stream));
-  HIPC(c, hipMemsetAsync(c->wCnt64.p, 0, 4 * sizeof(unsigned long long), c->stream));
+  HIPC(c, hipMemsetAsync(c->wCnt64.p, 0, 8 * sizeof(unsigned long long), c->stream));
+  return ASH_OK;
+}
+
+// Global spill space for the traversal stacks of all persistent threads,
+// sized to the deepest stack the current tree can produce.
+static int ensureSpill(ash_ctx* c, uint32_t grid) {
+  uint32_t cap = std::max<uint32_t>(uint32_t(kSpill), c->stackNeed > uint32_t(kStackLds) ? c->stackNeed - kStackLds : 0);
+  uint32_t threads = grid * kTraceBlock;
+  Wavefront& w = c->wf;
+  if (w.spill && w.spillCap >= cap && w.spillThreads == threads) return ASH_OK;
+  MALLOC(c, c->wSpill, size_t(threads) * cap);
+  MALLOC(c, c->wSpillT, size_t(threads) * cap);
+  w.spill = c->wSpill.p;
+  w.spillT = c->wSpillT.p;
+  w.spillThreads = threads;
+  w.spillCap = cap;
   return ASH_OK;
 }
 
@@ -219,68 +266,174 @@ static int checkPc(ash_ctx* c, const GpuPushConstantRaytrace* pc) {
   return ASH_OK;
 }
 
-// One batch of shots x frames through the bounce loop.
-static int runBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, const ash_render_opts* o, ash_stats* st) {
+// The shading grid covers the wavefront capacity (threads past the live
+// count exit), so one captured loop serves every batch size.
+static void launchBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid) {
+  int cur = 0;
+  for (uint32_t it = 0; it < iters; ++it) {
+    launchResetCounters(c->wf, cur, c->stream);
+    launchExtend(sc, c->wf, cur, b.n, grid, false, c->stream);
+    launchShade(sc, b, c->wf, cur, c->wf.capacity, c->stream);
+    launchShadow(sc, c->wf, b.n, grid, false, c->stream);
+    cur = 1 - cur;
+  }
+}
+
+// Replays the bounce loop of a batch from a captured HIP graph (captured on
+// first use of a key; a few keys are cached).  The loop holds kernel nodes
+// only.  Returns false when capture is unavailable (eager launch instead).
+static bool graphBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid) {
+  if (c->graphsBroken) return false;
+  GraphKey k;
+  std::memset(&k, 0, sizeof k);
+  std::memcpy(&k.sc, &sc, sizeof sc);
+  std::memcpy(&k.w, &c->wf, sizeof c->wf);
+  k.tab = b.shotTab;
+  if (b.shots == 1) std::memcpy(&k.one, &b.one, sizeof b.one);
+  k.grid = grid;
+  k.iters = iters;
+  k.n = c->wf.capacity;
+  k.single = b.shots == 1;
+  ++c->graphClock;
+  for (auto& g : c->graphs)
+    if (g.exec && std::memcmp(&g.key, &k, sizeof k) == 0) {
+      g.lastUse = c->graphClock;
+      return hipGraphLaunch(g.exec, c->stream) == hipSuccess;
+    }
+  GraphEntry* slot = &c->graphs[0];
+  for (auto& g : c->graphs)
+    if (!g.exec || g.lastUse < slot->lastUse) slot = &g;
+  releaseGraph(*slot);
+  if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    c->graphsBroken = true;
+    return false;
+  }
+  launchBounces(c, sc, b, iters, grid);
+  hipGraph_t graph = nullptr;
+  if (hipStreamEndCapture(c->stream, &graph) != hipSuccess || !graph) {
+    c->graphsBroken = true;
+    (void)hipGetLastError();
+    return false;
+  }
+  hipGraphExec_t exec = nullptr;
+  if (hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    (void)hipGraphDestroy(graph);
+    c->graphsBroken = true;
+    return false;
+  }
+  slot->key = k;
+  slot->graph = graph;
+  slot->exec = exec;
+  slot->lastUse = c->graphClock;
+  return hipGraphLaunch(exec, c->stream) == hipSuccess;
+}
+
+// One batch: ray generation, the bounce loop, the ordered accumulation.
+static int runBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, uint32_t grid, const ash_render_opts* o,
+                    ash_stats* st) {
   SceneView sc = viewOf(c);
   hipStream_t s = c->stream;
   bool timing = o && o->timing;
   bool count = o && o->count_traversal;
-  hipEvent_t ev[2] = {nullptr, nullptr};
-  if (timing) {
+  bool eager = timing || count || (o && o->no_graphs);
+  uint32_t iters = o && o->max_bounce_iters ? o->max_bounce_iters : uint32_t(std::max(maxDepth, 1));
+  launchRaygen(b, c->wf, s);
+  st->batches++;
+  st->extend_launches += iters;
+  st->shadow_launches += iters;
+  if (!eager && graphBounces(c, sc, b, iters, grid)) {
+    st->graph_launches++;
+  } else if (!timing) {
+    int cur = 0;
+    for (uint32_t it = 0; it < iters; ++it) {
+      launchResetCounters(c->wf, cur, s);
+      launchExtend(sc, c->wf, cur, b.n, grid, count, s);
+      launchShade(sc, b, c->wf, cur, b.n, s);
+      launchShadow(sc, c->wf, b.n, grid, count, s);
+      cur = 1 - cur;
+    }
+  } else {
+    hipEvent_t ev[2];
     HIPC(c, hipEventCreate(&ev[0]));
     HIPC(c, hipEventCreate(&ev[1]));
-  }
-  auto timed = [&](double* acc, auto&& fn) -> int {
-    if (!timing) {
+    auto timed = [&](double* acc, auto&& fn) -> int {
+      HIPC(c, hipEventRecord(ev[0], s));
       fn();
+      HIPC(c, hipEventRecord(ev[1], s));
+      HIPC(c, hipEventSynchronize(ev[1]));
+      float ms = 0;
+      HIPC(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
+      *acc += ms;
       return ASH_OK;
+    };
+    int cur = 0, rc = ASH_OK;
+    for (uint32_t it = 0; it < iters && rc == ASH_OK; ++it) {
+      launchResetCounters(c->wf, cur, s);
+      rc = timed(&st->extend_ms, [&] { launchExtend(sc, c->wf, cur, b.n, grid, count, s); });
+      if (rc == ASH_OK) rc = timed(&st->shade_ms, [&] { launchShade(sc, b, c->wf, cur, b.n, s); });
+      if (rc == ASH_OK) rc = timed(&st->shadow_ms, [&] { launchShadow(sc, c->wf, b.n, grid, count, s); });
+      cur = 1 - cur;
     }
-    HIPC(c, hipEventRecord(ev[0], s));
-    fn();
-    HIPC(c, hipEventRecord(ev[1], s));
-    HIPC(c, hipEventSynchronize(ev[1]));
-    float ms = 0;
-    HIPC(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
-    *acc += ms;
-    return ASH_OK;
-  };
-  launchRaygen(b, c->wf, s);
-  uint32_t iters = o && o->max_bounce_iters ? o->max_bounce_iters : uint32_t(std::max(maxDepth, 1));
-  int cur = 0;
-  int rc = ASH_OK;
-  for (uint32_t it = 0; it < iters && rc == ASH_OK; ++it) {
-    launchResetCounters(c->wf, cur, s);
-    rc = timed(&st->extend_ms, [&] { launchExtend(sc, c->wf, cur, b.n, c->traceGrid, count, s); });
-    st->extend_launches++;
-    if (rc) break;
-    rc = timed(&st->shade_ms, [&] { launchShade(sc, b, c->wf, cur, b.n, s); });
-    if (rc) break;
-    rc = timed(&st->shadow_ms, [&] { launchShadow(sc, c->wf, b.n, c->traceGrid, count, s); });
-    st->shadow_launches++;
-    cur = 1 - cur;
-  }
-  if (rc == ASH_OK) launchAccumulate(b, c->wf, s);
-  if (timing) {
     (void)hipEventDestroy(ev[0]);
     (void)hipEventDestroy(ev[1]);
+    if (rc) return rc;
   }
+  launchAccumulate(b, c->wf, s);
   HIPC(c, hipGetLastError());
-  return rc;
+  return ASH_OK;
 }
 
-static uint32_t framesPerBatch(uint32_t P, uint32_t shots, int32_t nFrames, const ash_render_opts* o) {
-  uint64_t target = o && o->queue_paths ? o->queue_paths : (1u << 22);
-  uint64_t perFrame = uint64_t(P) * shots;
-  uint64_t f = std::max<uint64_t>(1, target / std::max<uint64_t>(perFrame, 1));
-  return uint32_t(std::min<uint64_t>(f, uint64_t(nFrames)));
+// Splits shots x frames into wavefront batches of at most Q paths.  The
+// units of a batch cover disjoint (shot, pixel) sets and frames are the
+// outer loop, so every pixel folds its frames in increasing order.
+static void planBatches(uint32_t nShots, uint32_t P, int32_t first, int32_t nFrames, uint64_t Q,
+                        std::vector<std::vector<WorkUnit>>& out) {
+  out.clear();
+  auto unit = [](uint32_t s, uint32_t pix0, uint32_t np, int32_t f0, uint32_t nf) {
+    WorkUnit u;
+    std::memset(&u, 0, sizeof u);
+    u.shot = s, u.pix0 = pix0, u.np = np, u.f0 = f0, u.nf = nf;
+    return u;
+  };
+  Q = std::max<uint64_t>(Q, 64);
+  const int32_t end = first + nFrames;
+  uint64_t perFrame = uint64_t(P) * nShots;
+  if (perFrame <= Q) {
+    int32_t F = int32_t(std::min<uint64_t>(uint64_t(nFrames), Q / perFrame));
+    for (int32_t f0 = first; f0 < end; f0 += F) {
+      uint32_t nf = uint32_t(std::min(F, end - f0));
+      out.emplace_back();
+      for (uint32_t s = 0; s < nShots; ++s) out.back().push_back(unit(s, 0, P, f0, nf));
+    }
+  } else if (P <= Q) {
+    uint32_t g = uint32_t(Q / P);
+    for (int32_t f = first; f < end; ++f)
+      for (uint32_t s0 = 0; s0 < nShots; s0 += g) {
+        out.emplace_back();
+        for (uint32_t s = s0; s < std::min(nShots, s0 + g); ++s) out.back().push_back(unit(s, 0, P, f, 1));
+      }
+  } else {
+    uint32_t run = uint32_t(Q & ~uint64_t(63));
+    for (int32_t f = first; f < end; ++f)
+      for (uint32_t s = 0; s < nShots; ++s)
+        for (uint32_t p0 = 0; p0 < P; p0 += run) out.push_back({unit(s, p0, std::min(run, P - p0), f, 1)});
+  }
+  for (auto& bt : out) {
+    uint32_t base = 0, acc = 0;
+    for (auto& u : bt) {
+      u.base = base;
+      u.accBase = acc;
+      base += u.np * u.nf;
+      acc += u.np;
+    }
+  }
 }
 
 static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots, int32_t firstFrame, int32_t nFrames,
                         const ash_render_opts* o, ash_stats* st) {
   if (!c->hasScene || !c->hasAccel) return fail(c, ASH_E_STATE, "render called before upload_scene/build_accel");
-  if (c->maxDepth > uint32_t(kStackLds + kSpill))
-    return fail(c, ASH_E_UNSUPPORTED, "BVH depth %u exceeds the traversal stack (%d)", c->maxDepth, kStackLds + kSpill);
   if (nFrames <= 0 || firstFrame < 0) return fail(c, ASH_E_INVALID, "bad frame range");
+  if (nShots == 0 || nShots > 0xffff) return fail(c, ASH_E_INVALID, "1..65535 shots per render");
   uint32_t P = hostShots[0].width * hostShots[0].height;
   int maxDepth = 0;
   for (uint32_t s = 0; s < nShots; ++s) {
@@ -289,42 +442,56 @@ static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots
     maxDepth = std::max(maxDepth, hostShots[s].pc.maxPathDepth);
     if (hostShots[s].width * hostShots[s].height != P) return fail(c, ASH_E_INVALID, "shots must share one film size");
   }
-  uint32_t F = framesPerBatch(P, nShots, nFrames, o);
-  uint64_t cap64 = uint64_t(P) * nShots * F;
-  if (cap64 > 0x7fffffffull) return fail(c, ASH_E_INVALID, "batch too large");
-  int rc = ensureWavefront(c, uint32_t(cap64));
+  uint64_t Q = o && o->queue_paths ? o->queue_paths : (1u << 22);
+  if (Q > (1u << 30)) return fail(c, ASH_E_INVALID, "queue_paths above 2^30");
+  uint32_t bpc = o && o->blocks_per_cu ? o->blocks_per_cu : 8;
+  if (bpc > 64) return fail(c, ASH_E_INVALID, "blocks_per_cu above 64");
+  std::vector<std::vector<WorkUnit>> plan;
+  planBatches(nShots, P, firstFrame, nFrames, Q, plan);
+  uint32_t cap = 0;
+  size_t nUnits = 0;
+  for (auto& bt : plan) {
+    cap = std::max(cap, bt.back().base + bt.back().np * bt.back().nf);
+    nUnits += bt.size();
+  }
+  int rc = ensureWavefront(c, cap);
   if (rc) return rc;
-  MALLOC(c, c->shotTab, nShots);
-  HIPC(c, hipMemcpyAsync(c->shotTab.p, hostShots, nShots * sizeof(ShotParams), hipMemcpyHostToDevice, c->stream));
+  uint32_t grid = uint32_t(std::max(1, c->numCU)) * bpc;
+  rc = ensureSpill(c, grid);
+  if (rc) return rc;
+  std::vector<WorkUnit> flat;
+  flat.reserve(nUnits);
+  for (auto& bt : plan) flat.insert(flat.end(), bt.begin(), bt.end());
+  if (c->unitBuf.n < flat.size()) MALLOC(c, c->unitBuf, flat.size());
+  if (c->shotTab.n < nShots) MALLOC(c, c->shotTab, nShots);
+  HIPC(c, hipStreamSynchronize(c->stream));  // previous users of the tables are done
+  HIPC(c, hipMemcpy(c->unitBuf.p, flat.data(), flat.size() * sizeof(WorkUnit), hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(c->shotTab.p, hostShots, nShots * sizeof(ShotParams), hipMemcpyHostToDevice));
   ash_stats local{};
   ash_stats* S = st ? st : &local;
   std::memset(S, 0, sizeof *S);
-  HIPC(c, hipMemsetAsync(c->wf.ctr + 5, 0, 4 * sizeof(uint32_t), c->stream));
-  HIPC(c, hipMemsetAsync(c->wf.cnt64, 0, 4 * sizeof(unsigned long long), c->stream));
+  HIPC(c, hipMemsetAsync(c->wf.cnt64, 0, 8 * sizeof(unsigned long long), c->stream));
   hipEvent_t e0, e1;
   HIPC(c, hipEventCreate(&e0));
   HIPC(c, hipEventCreate(&e1));
   HIPC(c, hipEventRecord(e0, c->stream));
-  uint64_t ext = 0, sh = 0, dsh = 0;
-  for (int32_t f0 = firstFrame; f0 < firstFrame + nFrames; f0 += int32_t(F)) {
+  size_t off = 0;
+  for (auto& bt : plan) {
     BatchDesc b;
-    b.P = P;
-    b.width = hostShots[0].width;
-    b.frames = uint32_t(std::min<int32_t>(int32_t(F), firstFrame + nFrames - f0));
-    b.firstFrame = f0;
-    b.shots = nShots;
-    b.n = P * nShots * b.frames;
+    std::memset(&b, 0, sizeof b);
+    b.n = bt.back().base + bt.back().np * bt.back().nf;
+    b.nAcc = bt.back().accBase + bt.back().np;
+    b.nUnits = uint32_t(bt.size());
+    b.units = c->unitBuf.p + off;
     b.shotTab = c->shotTab.p;
-    rc = runBatch(c, b, maxDepth, o, S);
+    // kernel-argument shot constants only for single-shot renders: with
+    // several shots every batch reads the table, so one captured bounce
+    // loop serves all batches of the same size
+    b.shots = nShots;
+    b.one = hostShots[0];
+    off += bt.size();
+    rc = runBatch(c, b, maxDepth, grid, o, S);
     if (rc) break;
-    // ray counters are 32-bit on the device; fold them per batch
-    uint32_t cnt[4];
-    HIPC(c, hipMemcpyAsync(cnt, c->wf.ctr + 5, sizeof cnt, hipMemcpyDeviceToHost, c->stream));
-    HIPC(c, hipMemsetAsync(c->wf.ctr + 5, 0, 4 * sizeof(uint32_t), c->stream));
-    HIPC(c, hipStreamSynchronize(c->stream));
-    ext += cnt[0];
-    sh += cnt[1];
-    dsh += cnt[2];
   }
   HIPC(c, hipEventRecord(e1, c->stream));
   HIPC(c, hipEventSynchronize(e1));
@@ -332,20 +499,21 @@ static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots
   HIPC(c, hipEventElapsedTime(&ms, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  unsigned long long cnt[8];
+  HIPC(c, hipMemcpy(cnt, c->wf.cnt64, sizeof cnt, hipMemcpyDeviceToHost));
   S->render_ms = ms;
   S->paths = uint64_t(P) * nShots * uint64_t(nFrames);
-  S->extension_rays = ext;
-  S->shadow_rays = sh;
-  S->dielectric_shadow_rays = dsh;
+  S->extension_rays = cnt[4];
+  S->shadow_rays = cnt[5];
+  S->dielectric_shadow_rays = cnt[6];
   if (o && o->count_traversal) {
-    unsigned long long cnt[4];
-    HIPC(c, hipMemcpy(cnt, c->wf.cnt64, sizeof cnt, hipMemcpyDeviceToHost));
     S->ext_node_visits = cnt[0];
     S->ext_tri_tests = cnt[1];
     S->shadow_node_visits = cnt[2];
     S->shadow_tri_tests = cnt[3];
   }
-  return rc;
+  return ASH_OK;
 }
 
 extern "C" {
@@ -513,7 +681,9 @@ int ash_build_accel(ash_ctx* c, const ash_build_opts* o, ash_build_info* info) {
   if (!c->hasScene) return fail(c, ASH_E_STATE, "build_accel before upload_scene");
   HIPC(c, hipSetDevice(c->device));
   if (c->nodes) (void)hipFree(c->nodes), c->nodes = nullptr;
+  if (c->nodes4) (void)hipFree(c->nodes4), c->nodes4 = nullptr;
   if (c->tris) (void)hipFree(c->tris), c->tris = nullptr;
+  c->hasAccel = false;
   BuildArgs a;
   a.inst = c->inst.p;
   a.numInst = c->numInst;
@@ -523,18 +693,24 @@ int ash_build_accel(ash_ctx* c, const ash_build_opts* o, ash_build_info* info) {
   a.numTris = c->numTris;
   a.leafMax = o && o->leaf_max_tris ? o->leaf_max_tris : 4;
   a.quality = o ? o->quality : 1;
+  a.branching = o && o->branching ? o->branching : 4;
+  if (a.branching != 2 && a.branching != 4) return fail(c, ASH_E_INVALID, "branching must be 2 or 4");
   BuildResult r;
   char err[512] = {0};
   int rc = buildBvh(a, c->stream, r, err, sizeof err);
   if (rc) {
     if (r.nodes) (void)hipFree(r.nodes);
+    if (r.nodes4) (void)hipFree(r.nodes4);
     if (r.tris) (void)hipFree(r.tris);
     return fail(c, rc, "%s", err);
   }
   c->nodes = r.nodes;
+  c->nodes4 = r.nodes4;
   c->tris = r.tris;
   c->nodeCount = r.numNodes;
   c->maxDepth = r.maxDepth;
+  // A binary walk stacks at most one entry per level, a 4-wide walk three.
+  c->stackNeed = r.nodes4 ? 3 * r.depth4 : r.maxDepth;
   c->hasAccel = true;
   if (info) {
     info->num_triangles = c->numTris;
@@ -542,7 +718,11 @@ int ash_build_accel(ash_ctx* c, const ash_build_opts* o, ash_build_info* info) {
     info->max_depth = r.maxDepth;
     info->build_ms = r.ms;
     info->sah_cost = r.sahCost;
-    info->device_bytes = uint64_t(r.numNodes) * 64 + uint64_t(c->numTris) * 48 + c->verts.n * 16 + c->idx.n * 4;
+    info->device_bytes = uint64_t(r.numNodes) * 64 + uint64_t(r.numNodes4) * 128 + uint64_t(c->numTris) * 48 +
+                         c->verts.n * 16 + c->idx.n * 4;
+    info->num_wide_nodes = r.numNodes4;
+    info->wide_depth = r.depth4;
+    info->branching = a.branching;
   }
   return ASH_OK;
 }
@@ -605,6 +785,43 @@ int ash_render_shots(ash_ctx* c, uint32_t nShots, const GpuCamera* cams, const G
   return renderCommon(c, nShots, sps.data(), 0, nFrames, o, st);
 }
 
+int ash_autotune(ash_ctx* c, uint32_t nShots, const GpuCamera* cams, const GpuPushConstantRaytrace* pcs,
+                 int32_t probeFrames, ash_render_opts* io, ash_tune_result* res) {
+  if (!c || !cams || !pcs || !io || nShots == 0 || probeFrames <= 0) return ASH_E_INVALID;
+  ash_render_opts o = *io;
+  o.timing = 0;
+  o.count_traversal = 0;
+  double best = 1e300, worst = 0;
+  uint32_t bq = 0, bb = 0, n = 0;
+  for (uint32_t lq = 18; lq <= 22; ++lq)
+    for (uint32_t bpc : {4u, 8u, 16u}) {
+      o.queue_paths = 1u << lq;
+      o.blocks_per_cu = bpc;
+      ash_stats st;
+      double ms = 0;
+      for (int rep = 0; rep < 2; ++rep) {
+        int rc = nShots == 1 ? ash_render(c, cams, pcs, 0, probeFrames, &o, &st)
+                             : ash_render_shots(c, nShots, cams, pcs, probeFrames, &o, &st);
+        if (rc) return rc;
+        ms = st.render_ms;
+      }
+      ++n;
+      worst = std::max(worst, ms);
+      if (ms < best) best = ms, bq = o.queue_paths, bb = bpc;
+    }
+  io->queue_paths = bq;
+  io->blocks_per_cu = bb;
+  if (res) {
+    res->queue_paths = bq;
+    res->blocks_per_cu = bb;
+    res->candidates = n;
+    res->pad = 0;
+    res->best_ms = best;
+    res->worst_ms = worst;
+  }
+  return ASH_OK;
+}
+
 int ash_read_shot_output(ash_ctx* c, uint32_t shot, int32_t image, float* out) {
   if (!c || !out || image < 0 || image >= ASH_NUM_OUTPUT_IMAGES || shot >= c->shotCount) return ASH_E_INVALID;
   HIPC(c, hipSetDevice(c->device));
@@ -639,26 +856,37 @@ int ash_first_hit(ash_ctx* c, const GpuCamera* cam, ash_hit* out) {
   uint32_t P = c->width * c->height;
   int rc = ensureWavefront(c, P);
   if (rc) return rc;
+  uint32_t grid = uint32_t(std::max(1, c->numCU)) * 8;
+  rc = ensureSpill(c, grid);
+  if (rc) return rc;
   GpuPushConstantRaytrace pc;
   std::memset(&pc, 0, sizeof pc);
   pc.maxPathDepth = 1;
   ShotParams sp = makeShot(c, cam, &pc, nullptr, c->images.p, c->width, c->height);
-  MALLOC(c, c->shotTab, 1);
-  HIPC(c, hipMemcpyAsync(c->shotTab.p, &sp, sizeof sp, hipMemcpyHostToDevice, c->stream));
+  WorkUnit u;
+  std::memset(&u, 0, sizeof u);
+  u.np = P;
+  u.nf = 1;
+  if (c->shotTab.n < 1) MALLOC(c, c->shotTab, 1);
+  if (c->unitBuf.n < 1) MALLOC(c, c->unitBuf, 1);
+  HIPC(c, hipStreamSynchronize(c->stream));
+  HIPC(c, hipMemcpy(c->shotTab.p, &sp, sizeof sp, hipMemcpyHostToDevice));
+  HIPC(c, hipMemcpy(c->unitBuf.p, &u, sizeof u, hipMemcpyHostToDevice));
   DBuf<ash_hit> dout;
   MALLOC(c, dout, P);
   BatchDesc b;
-  b.P = P;
-  b.width = c->width;
-  b.frames = 1;
-  b.firstFrame = 0;
-  b.shots = 1;
+  std::memset(&b, 0, sizeof b);
   b.n = P;
+  b.nAcc = P;
+  b.nUnits = 1;
+  b.shots = 1;
+  b.units = c->unitBuf.p;
   b.shotTab = c->shotTab.p;
+  b.one = sp;
   SceneView sc = viewOf(c);
   launchRaygen(b, c->wf, c->stream);
   launchResetCounters(c->wf, 0, c->stream);
-  launchExtend(sc, c->wf, 0, P, c->traceGrid, false, c->stream);
+  launchExtend(sc, c->wf, 0, P, grid, false, c->stream);
   launchFirstHit(sc, c->shotTab.p, c->wf, dout.p, P, c->stream);
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(out, dout.p, P * sizeof(ash_hit), hipMemcpyDeviceToHost, c->stream));
@@ -684,6 +912,32 @@ void ash_pod_layout(uint32_t* o) {
   o[8] = sizeof(GpuPushConstantRaytrace), o[9] = offsetof(GpuPushConstantRaytrace, uvOutChannel);
 }
 
-const char* ash_version(void) { return "asuna-mi355x 0.1 (gfx950, wavefront BVH2, LBVH+SAH collapse)"; }
+const char* ash_version(void) { return "asuna-mi355x 0.2 (gfx950, wavefront path tracer, LBVH+SAH collapse, 4-wide BVH)"; }
 
 }  // extern "C"
+
+namespace ashd {
+
+hipStream_t ctxStream(ash_ctx* c) { return c->stream; }
+int ctxNumCU(ash_ctx* c) { return std::max(1, c->numCU); }
+int ctxPrepare(ash_ctx* c, uint32_t capacity, uint32_t grid) {
+  if (!c->hasScene || !c->hasAccel) return fail(c, ASH_E_STATE, "scene not uploaded/built");
+  int rc = ensureWavefront(c, capacity);
+  return rc ? rc : ensureSpill(c, grid);
+}
+int ctxCheckPc(ash_ctx* c, const GpuPushConstantRaytrace* pc) { return checkPc(c, pc); }
+int ctxRunBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, uint32_t grid, const ash_render_opts* o,
+                ash_stats* st) {
+  return runBatch(c, b, maxDepth, grid, o, st);
+}
+ShotParams ctxMakeShot(const GpuCamera* cam, const GpuPushConstantRaytrace* pc, int32_t depthChannel,
+                       float4* images, uint32_t w, uint32_t h) {
+  ash_render_opts o;
+  std::memset(&o, 0, sizeof o);
+  o.depthOutChannel = depthChannel;
+  return makeShot(nullptr, cam, pc, &o, images, w, h);
+}
+const char* ctxError(ash_ctx* c) { return c->err.c_str(); }
+
+}  // namespace ashd
+

@@ -1,0 +1,22 @@
+// Library-internal access to a render context (used by the render server).
+#pragma once
+
+#include "render.h"
+
+struct ash_ctx;
+
+namespace ashd {
+
+hipStream_t ctxStream(ash_ctx* c);
+int ctxNumCU(ash_ctx* c);
+// Wavefront buffers for `capacity` paths and traversal stacks for `grid` blocks.
+int ctxPrepare(ash_ctx* c, uint32_t capacity, uint32_t grid);
+int ctxCheckPc(ash_ctx* c, const GpuPushConstantRaytrace* pc);
+// Ray generation, bounce loop and accumulation of one batch on the context's stream.
+int ctxRunBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, uint32_t grid, const ash_render_opts* o,
+                ash_stats* st);
+ShotParams ctxMakeShot(const GpuCamera* cam, const GpuPushConstantRaytrace* pc, int32_t depthChannel,
+                       float4* images, uint32_t w, uint32_t h);
+const char* ctxError(ash_ctx* c);
+
+}  // namespace ashd

@@ -3,6 +3,9 @@
 //   vertices   float4[2*numVert]           (pos.xyz, uv.x), (normal.xyz, uv.y): 32 B/vertex
 //   indices    uint32[numIdx]              per-mesh vertex indices, 3 per triangle
 //   nodes      float4[4*numNodes]          BVH2, child boxes in the parent (64 B/node)
+//   nodes4     float4[8*numNodes4]         4-wide BVH collapsed from it, one 128 B L2 line per
+//                                           node: lo.x[4] hi.x[4] lo.y[4] hi.y[4] lo.z[4] hi.z[4]
+//                                           child refs[4], unused[4]
 //   tris       float4[3*numTris]           world-space v0,v1,v2 in leaf order (48 B/tri);
 //                                           .w of v0/v1/v2 = instance, primitive, global id
 //   materials  GpuMaterial[], lights GpuLight[], textures (RGBA32F), env + tables
@@ -50,13 +53,17 @@ struct BuildArgs {
   uint32_t numTris;
   uint32_t leafMax;   // 1..8
   uint32_t quality;   // 0 = LBVH (one triangle per leaf), 1 = SAH leaf collapse
+  uint32_t branching; // 2 or 4
 };
 
 struct BuildResult {
-  float4* nodes = nullptr;  // device, 4 float4 per node
-  float4* tris = nullptr;   // device, 3 float4 per triangle
+  float4* nodes = nullptr;   // device, 4 float4 per binary node
+  float4* nodes4 = nullptr;  // device, 8 float4 per 4-wide node (branching 4)
+  float4* tris = nullptr;    // device, 3 float4 per triangle
   uint32_t numNodes = 0;
   uint32_t maxDepth = 0;
+  uint32_t numNodes4 = 0;
+  uint32_t depth4 = 0;
   double sahCost = 0;
   double ms = 0;
 };

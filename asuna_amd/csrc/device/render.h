@@ -18,7 +18,8 @@ struct ShotParams {
 };
 
 struct SceneView {
-  const float4* nodes;
+  const float4* nodes;   // BVH2 (always built)
+  const float4* nodes4;  // 4-wide BVH, or null to traverse the BVH2
   const float4* tris;
   const DevInstance* inst;
   const float4* verts;
@@ -33,8 +34,7 @@ struct SceneView {
   const float2* cond;  // W*H entries
 };
 
-// Wavefront state for up to `capacity` paths.  Path p of a batch covers
-// shot = p / (frames*P), frame slot = (p / P) % frames, pixel = p % P.
+// Wavefront state for up to `capacity` paths (path numbering: WorkUnit).
 struct Wavefront {
   uint32_t capacity;
   float4* ro;    // ray origin of the current segment
@@ -42,32 +42,50 @@ struct Wavefront {
   float4* thr;   // throughput.xyz, w = pdf of the last BSDF sample
   float4* rad;   // radiance.xyz, w = Gaussian filter weight
   float4* ld;    // pending NEE radiance of the current bounce
-  uint4* info;   // x seed, y depth | bsdfFlags << 16, z pixel, w frame | shot << 16
+  uint4* info;   // x seed, y depth | bsdfFlags << 16, z pixel, w shot | (frame == 0) << 16
   float4* hitF;  // t, u, v
   int4* hitI;    // instance (-1 miss), primitive, global triangle id, leaf position
   int* queue[2];
   float4* shO;  // shadow rays: origin.xyz, tmax
   float4* shD;  // direction.xyz, path id
   uint32_t* ctr;  // [0],[1] queue counts, [2] shadow count, [3] extend fetch,
-                  // [4] shadow fetch, [5..8] stats: ext rays, shadow rays,
-                  // dielectric shadow rays, paths
-  int* spill;     // traversal stack spill, kSpill ints per persistent thread
+                  // [4] shadow fetch
+  int* spill;       // traversal stack spill: spillCap entries per persistent thread
+  float* spillT;    // entry distances of the spilled entries (4-wide closest hit)
   uint32_t spillThreads;
-  unsigned long long* cnt64;  // [0] ext nodes, [1] ext tris, [2] shadow nodes, [3] shadow tris
+  uint32_t spillCap;
+  unsigned long long* cnt64;  // [0] ext nodes, [1] ext tris, [2] shadow nodes, [3] shadow tris,
+                              // [4] extension rays, [5] shadow rays, [6] dielectric shadow rays
+};
+
+// A work unit: frames [f0, f0 + nf) of the linear pixel run [pix0, pix0 + np)
+// of one shot.  Its paths are base + fi * np + k (frame-major), and its
+// accumulate threads accBase + k.  The units of one batch cover disjoint
+// (shot, pixel) sets, so each pixel's frames are folded by one thread in
+// frame order.
+struct WorkUnit {
+  uint32_t shot;  // slot in the shot table
+  uint32_t pix0, np;
+  int32_t f0;
+  uint32_t nf;
+  uint32_t base;
+  uint32_t accBase;
+  uint32_t pad;
 };
 
 struct BatchDesc {
-  uint32_t P;          // pixels per shot
-  uint32_t width;
-  uint32_t frames;     // frames per shot in this batch
-  int32_t firstFrame;  // frame index of slot 0
-  uint32_t shots;      // shots in this batch
-  uint32_t n;          // = P * frames * shots
+  uint32_t n;       // paths (= sum of np * nf)
+  uint32_t nAcc;    // accumulate threads (= sum of np)
+  uint32_t nUnits;  // >= 1
+  uint32_t shots;   // 1: every path belongs to the shot copied into `one` (kernel-argument
+                    // constants); any other value: shading reads the shot table
+  const WorkUnit* units;      // device array of nUnits
   const ShotParams* shotTab;  // device array indexed by shot slot
+  ShotParams one;             // kernel-argument copy of the only shot (scalar loads)
 };
 
 constexpr int kStackLds = 16;    // per-thread traversal stack entries in LDS
-constexpr int kSpill = 48;       // further entries spilled to global memory
+constexpr int kSpill = 48;       // minimum further entries spilled to global memory
 constexpr int kTraceBlock = 256;
 
 // Host-side launchers (render.hip).

@@ -4,8 +4,9 @@
 //
 // One render batch = shots x frames x pixels paths.  Per bounce:
 //   extend  : closest hit of every live path (persistent waves fetching 64
-//             rays at a time; BVH2 traversal with a per-thread stack of 16
-//             entries in LDS and 48 spilled to global memory)
+//             rays at a time; 4-wide BVH traversal, nearest child first, with
+//             a per-thread stack of 16 entries in LDS spilling to global
+//             memory; the binary tree remains selectable)
 //   shade   : miss / light hit / material closest-hit logic; next rays and
 //             shadow rays are appended to compacted queues with one atomic per
 //             wave (ballot + popcount)
@@ -30,12 +31,12 @@ __device__ __forceinline__ int waveAppend(uint32_t* counter, bool pred) {
   uint64_t below = lane ? (mask & ((~0ull) >> (64 - lane))) : 0ull;
   return pred ? int(base + uint32_t(__popcll(below))) : -1;
 }
-__device__ __forceinline__ void waveCount(uint32_t* counter, bool pred) {
+__device__ __forceinline__ void waveCount64(unsigned long long* counter, bool pred) {
   uint64_t mask = __ballot(pred);
   if (mask == 0) return;
   int lane = __lane_id();
   int leader = __ffsll((unsigned long long)mask) - 1;
-  if (lane == leader) atomicAdd(counter, uint32_t(__popcll(mask)));
+  if (lane == leader) atomicAdd(counter, (unsigned long long)__popcll(mask));
 }
 
 // --------------------------------------------------------------- traversal
@@ -111,23 +112,96 @@ __device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, flo
   return tn <= tf;
 }
 
+// Per-thread traversal stack: kStackLds entries in LDS (stride = block
+// size), then `cap` entries in global memory (stride = all persistent
+// threads).  Entry distances are kept only by the 4-wide closest-hit walk.
 struct Stack {
   int* lds;
+  float* ldsT;
   int* spill;
+  float* spillT;
   uint32_t stride;
+  int cap;  // kStackLds + spilled entries
   int sp;
   __device__ __forceinline__ void push(int v) {
     if (sp < kStackLds) lds[sp * kTraceBlock] = v;
-    else if (sp < kStackLds + kSpill) spill[size_t(sp - kStackLds) * stride] = v;
+    else if (sp < cap) spill[size_t(sp - kStackLds) * stride] = v;
     ++sp;
   }
   __device__ __forceinline__ int pop() {
     --sp;
     if (sp < kStackLds) return lds[sp * kTraceBlock];
-    if (sp < kStackLds + kSpill) return spill[size_t(sp - kStackLds) * stride];
-    return kEmptyRef;  // overflowed entry (never happens when depth <= 64)
+    if (sp < cap) return spill[size_t(sp - kStackLds) * stride];
+    return kEmptyRef;  // overflowed entry (the host sizes `cap` to the tree depth)
+  }
+  __device__ __forceinline__ void push(int v, float t) {
+    if (sp < kStackLds) {
+      lds[sp * kTraceBlock] = v;
+      ldsT[sp * kTraceBlock] = t;
+    } else if (sp < cap) {
+      spill[size_t(sp - kStackLds) * stride] = v;
+      spillT[size_t(sp - kStackLds) * stride] = t;
+    }
+    ++sp;
+  }
+  __device__ __forceinline__ int pop(float& t) {
+    --sp;
+    if (sp < kStackLds) {
+      t = ldsT[sp * kTraceBlock];
+      return lds[sp * kTraceBlock];
+    }
+    if (sp < cap) {
+      t = spillT[size_t(sp - kStackLds) * stride];
+      return spill[size_t(sp - kStackLds) * stride];
+    }
+    t = 0.f;
+    return kEmptyRef;
   }
 };
+
+// ------------------------------------------------ 4-wide node traversal
+// Tests the four child boxes of a 128-byte node and returns the hit
+// children sorted by entry distance (sorting network; misses sort last).
+struct Kids {
+  float t[4];
+  int c[4];
+  int n;
+};
+__device__ __forceinline__ void cswap(Kids& k, int a, int b) {
+  if (k.t[b] < k.t[a]) {
+    float tt = k.t[a];
+    k.t[a] = k.t[b];
+    k.t[b] = tt;
+    int cc = k.c[a];
+    k.c[a] = k.c[b];
+    k.c[b] = cc;
+  }
+}
+__device__ __forceinline__ void testNode4(const float4* np, const TRay& r, float tmin, float lim, Kids& k) {
+  float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5], rf = np[6];
+  const float kMiss = 3.0e38f;
+  float t;
+  bool h;
+  h = slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r, tmin, lim, t);
+  k.t[0] = h ? t : kMiss;
+  k.n = h;
+  h = slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r, tmin, lim, t);
+  k.t[1] = h ? t : kMiss;
+  k.n += h;
+  h = slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r, tmin, lim, t);
+  k.t[2] = h ? t : kMiss;
+  k.n += h;
+  h = slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r, tmin, lim, t);
+  k.t[3] = h ? t : kMiss;
+  k.n += h;
+  k.c[0] = __float_as_int(rf.x), k.c[1] = __float_as_int(rf.y), k.c[2] = __float_as_int(rf.z),
+  k.c[3] = __float_as_int(rf.w);
+  cswap(k, 0, 1);
+  cswap(k, 2, 3);
+  cswap(k, 0, 2);
+  cswap(k, 1, 3);
+  cswap(k, 1, 2);
+}
 
 // Closest hit in [tmin, tmax]; ties at equal t go to the smaller global id.
 // Returns the leaf position of the hit triangle or -1.
@@ -244,6 +318,102 @@ __device__ bool traceAny(const SceneView& sc, const TRay& r, float tmin, float t
   return false;
 }
 
+// 4-wide closest hit.  Children are visited nearest first; stacked entries
+// keep their entry distance and are dropped on pop when it lies beyond the
+// current closest hit.  The drop test is the slab test's own far clamp
+// (tn <= lim * (1 + 2 gamma_3)), so the triangles tested include every one
+// the binary walk tests and the result is identical (same tie rule).
+template <bool COUNT>
+__device__ int traceClosest4(const SceneView& sc, const TRay& r, float tmin, float tmax, Stack& st, float& bt,
+                             float& bu, float& bv, uint32_t& nNodes, uint32_t& nTris) {
+  int best = -1, bestGid = 0x7fffffff;
+  bt = tmax;
+  st.sp = 0;
+  int node = 0;
+  for (;;) {
+    if (node >= 0) {
+      if (node != kEmptyRef) {
+        if (COUNT) ++nNodes;
+        Kids k;
+        testNode4(sc.nodes4 + size_t(node) * 8, r, tmin, best >= 0 ? bt : tmax, k);
+        if (k.n > 0) {
+          if (k.n > 3) st.push(k.c[3], k.t[3]);
+          if (k.n > 2) st.push(k.c[2], k.t[2]);
+          if (k.n > 1) st.push(k.c[1], k.t[1]);
+          node = k.c[0];
+          continue;
+        }
+      }
+    } else {
+      uint32_t first, cnt;
+      decodeLeaf(node, first, cnt);
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        const float4* tp = sc.tris + size_t(k) * 3;
+        float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+        if (COUNT) ++nTris;
+        float t, u, v;
+        if (hitTri(r, p0, p1, p2, tmin, best >= 0 ? bt : tmax, t, u, v)) {
+          int gid = __float_as_int(p2.w);
+          if (best < 0 || t < bt || (t == bt && gid < bestGid)) {
+            bt = t;
+            bu = u;
+            bv = v;
+            best = int(k);
+            bestGid = gid;
+          }
+        }
+      }
+    }
+    bool more = false;
+    while (st.sp > 0) {
+      float tn;
+      int c = st.pop(tn);
+      if (best < 0 || tn <= bt * 1.0000004f) {
+        node = c;
+        more = true;
+        break;
+      }
+    }
+    if (!more) break;
+  }
+  return best;
+}
+
+template <bool COUNT>
+__device__ bool traceAny4(const SceneView& sc, const TRay& r, float tmin, float tmax, Stack& st, uint32_t& nNodes,
+                          uint32_t& nTris) {
+  st.sp = 0;
+  int node = 0;
+  for (;;) {
+    if (node >= 0) {
+      if (node != kEmptyRef) {
+        if (COUNT) ++nNodes;
+        Kids k;
+        testNode4(sc.nodes4 + size_t(node) * 8, r, tmin, tmax, k);
+        if (k.n > 0) {
+          if (k.n > 3) st.push(k.c[3]);
+          if (k.n > 2) st.push(k.c[2]);
+          if (k.n > 1) st.push(k.c[1]);
+          node = k.c[0];
+          continue;
+        }
+      }
+    } else {
+      uint32_t first, cnt;
+      decodeLeaf(node, first, cnt);
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        const float4* tp = sc.tris + size_t(k) * 3;
+        float t, u, v;
+        if (COUNT) ++nTris;
+        if (hitTri(r, tp[0], tp[1], tp[2], tmin, tmax, t, u, v)) return true;
+      }
+    }
+    if (st.sp == 0) break;
+    node = st.pop();
+  }
+  return false;
+}
+
 // -------------------------------------------------------------- raygen
 // raytrace.projective.rgen:26-90 for one (shot, frame, pixel).
 __device__ __forceinline__ void cameraRay(const GpuCamera& cam, float px, float py, uint32_t& seed, f3& ro,
@@ -272,12 +442,26 @@ __device__ __forceinline__ void cameraRay(const GpuCamera& cam, float px, float 
   }
 }
 
+// Last unit whose first path (accumulate thread) is <= key.
+template <bool ACC>
+__device__ __forceinline__ uint32_t findUnit(const WorkUnit* u, uint32_t n, uint32_t key) {
+  uint32_t lo = 0, hi = n;
+  while (hi - lo > 1) {
+    uint32_t mid = (lo + hi) >> 1;
+    if ((ACC ? u[mid].accBase : u[mid].base) <= key) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 __global__ __launch_bounds__(256) void kRaygen(BatchDesc b, Wavefront w) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= b.n) return;
-  uint32_t pix = p % b.P, rest = p / b.P;
-  uint32_t fslot = rest % b.frames, shot = rest / b.frames;
-  int32_t frame = b.firstFrame + int32_t(fslot);
+  const WorkUnit un = b.units[findUnit<false>(b.units, b.nUnits, p)];
+  uint32_t rel = p - un.base;
+  uint32_t fslot = rel / un.np, pix = un.pix0 + rel % un.np;
+  int32_t frame = un.f0 + int32_t(fslot);
+  uint32_t shot = un.shot;
   const ShotParams& sp = b.shotTab[shot];
   uint32_t x = pix % sp.width, y = pix / sp.width;
   uint32_t seed = xxhash32Seed(x, y, uint32_t(frame));
@@ -299,7 +483,7 @@ __global__ __launch_bounds__(256) void kRaygen(BatchDesc b, Wavefront w) {
   w.rd[p] = f4(rd, 0.f);
   w.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
   w.rad[p] = make_float4(0.f, 0.f, 0.f, fw);
-  w.info[p] = make_uint4(seed, 1u, pix, uint32_t(fslot) | (shot << 16));
+  w.info[p] = make_uint4(seed, 1u, pix, shot | (frame == 0 ? 0x10000u : 0u));
   w.queue[0][p] = int(p);
   if (frame == 0) {
     size_t plane = size_t(sp.width) * sp.height;
@@ -309,14 +493,23 @@ __global__ __launch_bounds__(256) void kRaygen(BatchDesc b, Wavefront w) {
 }
 
 // ---------------------------------------------------------------- extend
-template <bool COUNT>
+template <bool WIDE>
+__device__ __forceinline__ void initStack(Stack& st, int* lds, float* ldsT, const Wavefront& w) {
+  uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
+  st.lds = lds + threadIdx.x;
+  st.ldsT = WIDE ? ldsT + threadIdx.x : nullptr;
+  st.spill = w.spill + gt;
+  st.spillT = WIDE ? w.spillT + gt : nullptr;
+  st.stride = w.spillThreads;
+  st.cap = kStackLds + int(w.spillCap);
+}
+
+template <bool COUNT, bool WIDE>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w, int cur) {
   __shared__ int lds[kStackLds * kTraceBlock];
+  __shared__ float ldsT[WIDE ? kStackLds * kTraceBlock : 1];
   Stack st;
-  st.lds = lds + threadIdx.x;
-  uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
-  st.spill = w.spill + gt;
-  st.stride = w.spillThreads;
+  initStack<WIDE>(st, lds, ldsT, w);
   const uint32_t count = w.ctr[cur];
   const int* q = w.queue[cur];
   int lane = __lane_id();
@@ -332,7 +525,8 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w
       TRay r;
       setupRay(r, xyz(w.ro[pid]), xyz(w.rd[pid]));
       float t = 0.f, u = 0.f, v = 0.f;
-      int pos = traceClosest<COUNT>(sc, r, kMinimum, kInf, st, t, u, v, nNodes, nTris);
+      int pos = WIDE ? traceClosest4<COUNT>(sc, r, kMinimum, kInf, st, t, u, v, nNodes, nTris)
+                     : traceClosest<COUNT>(sc, r, kMinimum, kInf, st, t, u, v, nNodes, nTris);
       if (pos >= 0) {
         const float4* tp = sc.tris + size_t(pos) * 3;
         w.hitF[pid] = make_float4(t, u, v, 0.f);
@@ -349,14 +543,11 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w
 }
 
 // ---------------------------------------------------------------- shadow
-template <bool COUNT>
+template <bool COUNT, bool WIDE>
 __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w) {
   __shared__ int lds[kStackLds * kTraceBlock];
   Stack st;
-  st.lds = lds + threadIdx.x;
-  uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
-  st.spill = w.spill + gt;
-  st.stride = w.spillThreads;
+  initStack<false>(st, lds, nullptr, w);
   const uint32_t count = w.ctr[2];
   int lane = __lane_id();
   uint32_t nNodes = 0, nTris = 0;
@@ -371,7 +562,9 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w
       TRay r;
       setupRay(r, xyz(so), xyz(sd));
       int pid = __float_as_int(sd.w);
-      if (!traceAny<COUNT>(sc, r, 0.0f, so.w, st, nNodes, nTris)) {
+      bool occluded = WIDE ? traceAny4<COUNT>(sc, r, 0.0f, so.w, st, nNodes, nTris)
+                           : traceAny<COUNT>(sc, r, 0.0f, so.w, st, nNodes, nTris);
+      if (!occluded) {
         float4 L = w.rad[pid], d = w.ld[pid];
         w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
       }
@@ -603,49 +796,6 @@ __device__ __forceinline__ void nextRay(PathCtx& c, f3 pos, f3 offN, f3 d, f3 we
   c.T = c.T * weightOverPdf;
 }
 
-// brdf_lambertian.rchit:70-151 (non-emitter part)
-__device__ void shadeLambertian(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t pix, bool aov) {
-  const GpuPushConstantRaytrace& pc = c.sp->pc;
-  f3 kd = ld3(m.diffuse);
-  if (m.diffuseTextureId >= 0) kd = xyz(tex2D(c.sc->textures[m.diffuseTextureId], h.uvx, h.uvy));
-  if (m.normalTextureId >= 0) normalMap(*c.sc, h, m.normalTextureId, pc.useFaceNormal);
-  if (aov) {
-    writeAov(c, pc.diffuseOutChannel, kd, pix);
-    writeAov(c, pc.normalOutChannel, h.N, pix);
-    writeAov(c, pc.specularOutChannel, mk1(0.f), pix);
-    writeAov(c, pc.tangentOutChannel, h.X, pix);
-    writeAov(c, pc.roughnessOutChannel, f3{1.f, 1.f, 0.f}, pix);
-    writeAov(c, pc.positionOutChannel, h.pos, pix);
-    writeAov(c, pc.uvOutChannel, f3{h.uvx, h.uvy, 1.f}, pix);
-  }
-  {
-    bool visible;
-    LSample l;
-    f3 radiance = sampleLights(c, h.pos, h.ffN, visible, l);
-    f3 Ld = mk1(0.f);
-    if (visible) {
-      f3 bw = mk1(0.f);
-      float NdotL = dot(h.ffN, l.d), NdotV = dot(h.ffN, h.V);
-      if (!(NdotL < 0.0f || NdotV < 0.0f)) bw = kd * kInvPi * NdotL;
-      float bsdfPdf = (l.flags & kLightArea) ? cosinePdf(dot(h.ffN, l.d)) : 0.0f;
-      float mis = powerHeuristic(l.pdf, bsdfPdf);
-      Ld = mis * bw * radiance * c.T / (l.pdf + kEps);
-    }
-    c.Ld = Ld;
-    c.shadow = visible;
-  }
-  float u0 = rnd(c.seed), u1 = rnd(c.seed);
-  f3 wi = cosineHemisphere(u0, u1);
-  float pdf = cosinePdf(wi.z);
-  f3 d = toWorld(h.X, h.Y, h.ffN, wi);
-  f3 w = kd * kInvPi * fabsf(wi.z);
-  if (pdf <= 0.0f || length(w) == 0.0f) {
-    c.stop = true;
-    return;
-  }
-  nextRay(c, h.pos, h.ffN, d, w / pdf, pdf, kDiffuseRefl);
-}
-
 // brdf_pbr_metalness_roughness.rchit:57-128
 __device__ f3 pbrEval(f3 L, f3 V, f3 N, f3 X, f3 Y, f3 albedo, float ax, float ay, float metal, float eta) {
   f3 weight = mk1(0.f);
@@ -668,77 +818,6 @@ __device__ f3 pbrEval(f3 L, f3 V, f3 N, f3 X, f3 Y, f3 albedo, float ax, float a
   return weight;
 }
 
-__device__ void shadePbr(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t pix, bool aov) {
-  const GpuPushConstantRaytrace& pc = c.sp->pc;
-  const SceneView& sc = *c.sc;
-  f3 kd = ld3(m.diffuse);
-  float metal = m.metalness, rough = m.roughness;
-  if (m.diffuseTextureId >= 0) kd = xyz(tex2D(sc.textures[m.diffuseTextureId], h.uvx, h.uvy));
-  if (m.metalnessTextureId >= 0) metal = tex2D(sc.textures[m.metalnessTextureId], h.uvx, h.uvy).x;
-  if (m.roughnessTextureId >= 0) rough = tex2D(sc.textures[m.roughnessTextureId], h.uvx, h.uvy).x;
-  if (m.normalTextureId >= 0) normalMap(sc, h, m.normalTextureId, pc.useFaceNormal);
-  float opacity = m.specular;
-  if (m.opacityTextureId >= 0) opacity = tex2D(sc.textures[m.opacityTextureId], h.uvx, h.uvy).x;
-  if (rnd(c.seed) < opacity) {
-    // pass through: same direction, same depth (pbr:156-159)
-    c.ro = offsetRay(h.pos, -h.ffN);
-    c.depth--;
-    return;
-  }
-  float ax = fmaxf(sqr(rough), 0.001f), ay = ax;
-  float eta = m.ior;
-  if (aov) {
-    writeAov(c, pc.diffuseOutChannel, kd, pix);
-    writeAov(c, pc.normalOutChannel, h.ffN, pix);
-  }
-  {
-    bool visible;
-    LSample l;
-    f3 radiance = sampleLights(c, h.pos, h.ffN, visible, l);
-    f3 Ld = mk1(0.f);
-    if (visible) {
-      f3 bw = pbrEval(l.d, h.V, h.ffN, h.X, h.Y, kd, ax, ay, metal, eta);
-      float bsdfPdf = 0.0f;
-      if (l.flags & kLightArea) {
-        float NdotL = dot(h.ffN, l.d), NdotV = dot(h.ffN, h.V);
-        if (!(NdotL <= 0.0f || NdotV <= 0.0f)) {
-          f3 H = makeNormal(l.d + h.V);
-          f3 wh = makeNormal(toLocal(h.X, h.Y, h.ffN, H));
-          f3 wo = makeNormal(toLocal(h.X, h.Y, h.ffN, h.V));
-          f3 wi = makeNormal(toLocal(h.X, h.Y, h.ffN, l.d));
-          bsdfPdf += 0.2f * cosinePdf(wi.z);
-          bsdfPdf += (1.0f - 0.2f) * pdfGGX(wh, wo, ax, ay);
-        }
-      }
-      float mis = powerHeuristic(l.pdf, bsdfPdf);
-      Ld = mis * bw * radiance * c.T / (l.pdf + kEps);
-    }
-    c.Ld = Ld;
-    c.shadow = visible;
-  }
-  float u0 = rnd(c.seed), u1 = rnd(c.seed);
-  f3 wo = makeNormal(toLocal(h.X, h.Y, h.ffN, h.V));
-  f3 wi;
-  float pdf = 0.0f;
-  uint32_t flags;
-  if (rnd(c.seed) < 0.2f) {
-    wi = cosineHemisphere(u0, u1);
-    flags = kDiffuseRefl;
-    pdf = cosinePdf(wi.z);
-  } else {
-    wi = sampleGGX(u0, u1, wo, ax, ay);
-    f3 wh = makeNormal(wi + wo);
-    pdf = pdfGGX(wh, wo, ax, ay);
-    flags = kGlossyRefl;
-  }
-  f3 d = toWorld(h.X, h.Y, h.ffN, wi);
-  f3 w = pbrEval(d, h.V, h.ffN, h.X, h.Y, kd, ax, ay, metal, eta) * fabsf(wi.z);
-  if (pdf <= 0.0f || length(w) == 0.0f) {
-    c.stop = true;
-    return;
-  }
-  nextRay(c, h.pos, h.ffN, d, w / (pdf + kEps), pdf, flags);
-}
 
 // brdf_rough_plastic.rchit:82-136 (eval, pdf) with the signatures' parameter order.
 __device__ f3 rpEval(f3 L, f3 V, f3 N, f3 X, f3 Y, f3 kd, f3 ks, float eta, float fdrInt, float invEta2, float ax,
@@ -776,75 +855,209 @@ __device__ float rpPdf(f3 L, f3 V, f3 N, f3 X, f3 Y, float ax, float ay, float e
   return pdf;
 }
 
-__device__ void shadeRoughPlastic(PathCtx& c, Hit& h, const GpuMaterial& m) {
+// Surface plugins (lambertian, pbr_metalness_roughness, rough_plastic,
+// dielectric).  Every closest-hit shader of the set runs the same three
+// steps -- material inputs (textures, normal map, AOVs, the pbr opacity
+// draw), one light sample through sampleLights, then its BSDF value toward
+// the light and its BSDF sample -- and the light sample sits at the same
+// point of the RNG stream in each.  The light sample is therefore taken once
+// for the whole wave instead of once per plugin present in it.
+//   brdf_lambertian.rchit:70-151, brdf_pbr_metalness_roughness.rchit:131-219,
+//   brdf_rough_plastic.rchit:191-269, bsdf_dielectric.rchit:74-138
+__device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t pix, bool aov) {
   const GpuPushConstantRaytrace& pc = c.sp->pc;
   const SceneView& sc = *c.sc;
+  const uint32_t type = m.type;
+  const bool lambert = type == MaterialTypeBrdfLambertian, pbr = type == MaterialTypeBrdfPbrMetalnessRoughness,
+             plastic = type == MaterialTypeBrdfRoughPlastic, dielectric = type == MaterialTypeBsdfDielectric;
+  // (1) material inputs
   f3 kd = ld3(m.diffuse);
-  float alx = m.anisoAlpha[0], aly = m.anisoAlpha[1];
-  if (m.diffuseTextureId >= 0) kd = xyz(tex2D(sc.textures[m.diffuseTextureId], h.uvx, h.uvy));
-  if (m.roughnessTextureId >= 0) {
-    float4 t = tex2D(sc.textures[m.roughnessTextureId], h.uvx, h.uvy);
-    alx = t.x;
-    aly = t.y;
+  float metal = 0.f, ax = 0.f, ay = 0.f, eta = m.ior, fdrInt = 0.f, invEta2 = 0.f, ssw = 0.f;
+  if (!dielectric && m.diffuseTextureId >= 0) kd = xyz(tex2D(sc.textures[m.diffuseTextureId], h.uvx, h.uvy));
+  if (pbr) {
+    metal = m.metalness;
+    float rough = m.roughness;
+    if (m.metalnessTextureId >= 0) metal = tex2D(sc.textures[m.metalnessTextureId], h.uvx, h.uvy).x;
+    if (m.roughnessTextureId >= 0) rough = tex2D(sc.textures[m.roughnessTextureId], h.uvx, h.uvy).x;
+    ax = ay = fmaxf(sqr(rough), 0.001f);
+  } else if (plastic) {
+    float alx = m.anisoAlpha[0], aly = m.anisoAlpha[1];
+    if (m.roughnessTextureId >= 0) {
+      float4 t = tex2D(sc.textures[m.roughnessTextureId], h.uvx, h.uvy);
+      alx = t.x;
+      aly = t.y;
+    }
+    fdrInt = m.radiance[0];
+    invEta2 = 1.0f / (eta * eta);
+    ax = fmaxf(kEps, alx);
+    ay = fmaxf(kEps, aly);
+    ssw = luminance(kd);
   }
   if (m.normalTextureId >= 0) normalMap(sc, h, m.normalTextureId, pc.useFaceNormal);
-  float eta = m.ior, fdrInt = m.radiance[0], invEta2 = 1.0f / (eta * eta);
-  float ax = fmaxf(kEps, alx), ay = fmaxf(kEps, aly);
-  float ssw = luminance(kd);
-  f3 ks = mk1(1.f);
-  {
-    bool visible;
-    LSample l;
-    f3 radiance = sampleLights(c, h.pos, h.ffN, visible, l);
-    f3 Ld = mk1(0.f);
-    if (visible) {
+  if (pbr) {
+    float opacity = m.specular;
+    if (m.opacityTextureId >= 0) opacity = tex2D(sc.textures[m.opacityTextureId], h.uvx, h.uvy).x;
+    if (rnd(c.seed) < opacity) {
+      // pass through: same direction, same depth (pbr:156-159)
+      c.ro = offsetRay(h.pos, -h.ffN);
+      c.depth--;
+      return;
+    }
+  }
+  if (dielectric) eta = dot(h.V, h.N) > 0.0f ? (1.0f / m.ior) : m.ior;
+  if (aov) {
+    if (lambert) {
+      writeAov(c, pc.diffuseOutChannel, kd, pix);
+      writeAov(c, pc.normalOutChannel, h.N, pix);
+      writeAov(c, pc.specularOutChannel, mk1(0.f), pix);
+      writeAov(c, pc.tangentOutChannel, h.X, pix);
+      writeAov(c, pc.roughnessOutChannel, f3{1.f, 1.f, 0.f}, pix);
+      writeAov(c, pc.positionOutChannel, h.pos, pix);
+      writeAov(c, pc.uvOutChannel, f3{h.uvx, h.uvy, 1.f}, pix);
+    } else if (pbr) {
+      writeAov(c, pc.diffuseOutChannel, kd, pix);
+      writeAov(c, pc.normalOutChannel, h.ffN, pix);
+    }
+  }
+  const f3 ks = mk1(1.f);
+  // (2) one light sample (sampleLights, rchit_layouts.glsl:118-167)
+  bool visible;
+  LSample l;
+  f3 radiance = sampleLights(c, h.pos, h.ffN, visible, l);
+  // (3) BSDF value and pdf toward the light.  The dielectric's NEE term is
+  // always zero (its eval tests EDelta but is called with EArea); its
+  // shadow ray is counted but not traced, since it cannot change the result.
+  f3 Ld = mk1(0.f);
+  if (visible && !dielectric) {
+    f3 bw = mk1(0.f);
+    float bsdfPdf = 0.0f;
+    if (lambert) {
+      float NdotL = dot(h.ffN, l.d), NdotV = dot(h.ffN, h.V);
+      if (!(NdotL < 0.0f || NdotV < 0.0f)) bw = kd * kInvPi * NdotL;
+      bsdfPdf = (l.flags & kLightArea) ? cosinePdf(dot(h.ffN, l.d)) : 0.0f;
+    } else if (pbr) {
+      bw = pbrEval(l.d, h.V, h.ffN, h.X, h.Y, kd, ax, ay, metal, eta);
+      if (l.flags & kLightArea) {
+        float NdotL = dot(h.ffN, l.d), NdotV = dot(h.ffN, h.V);
+        if (!(NdotL <= 0.0f || NdotV <= 0.0f)) {
+          f3 H = makeNormal(l.d + h.V);
+          f3 wh = makeNormal(toLocal(h.X, h.Y, h.ffN, H));
+          f3 wo = makeNormal(toLocal(h.X, h.Y, h.ffN, h.V));
+          f3 wi = makeNormal(toLocal(h.X, h.Y, h.ffN, l.d));
+          bsdfPdf += 0.2f * cosinePdf(wi.z);
+          bsdfPdf += (1.0f - 0.2f) * pdfGGX(wh, wo, ax, ay);
+        }
+      }
+    } else {
       // The shader passes (..., ax, ay, invEta2, EArea) into eval's
       // (..., invEta2, ax, ay, flags) and (eta, ax, ay) into pdf's
       // (ax, ay, eta) (brdf_rough_plastic.rchit:234-240).
-      f3 bw = rpEval(l.d, h.V, h.ffN, h.X, h.Y, kd, ks, eta, fdrInt, ax, ay, invEta2, kLightArea);
-      float bsdfPdf = rpPdf(l.d, h.V, h.ffN, h.X, h.Y, eta, ax, ay, ssw, l.flags);
-      float mis = powerHeuristic(l.pdf, bsdfPdf);
-      Ld = mis * bw * radiance * c.T / (l.pdf + kEps);
+      bw = rpEval(l.d, h.V, h.ffN, h.X, h.Y, kd, ks, eta, fdrInt, ax, ay, invEta2, kLightArea);
+      bsdfPdf = rpPdf(l.d, h.V, h.ffN, h.X, h.Y, eta, ax, ay, ssw, l.flags);
     }
-    c.Ld = Ld;
-    c.shadow = visible;
+    float mis = powerHeuristic(l.pdf, bsdfPdf);
+    Ld = mis * bw * radiance * c.T / (l.pdf + kEps);
+  }
+  c.Ld = Ld;
+  c.shadow = visible && !dielectric;
+  c.fromDielectric = visible && dielectric;
+  // (4) BSDF sample
+  if (dielectric) {
+    float u = rnd(c.seed);
+    f3 V = h.V, N = h.ffN;
+    float F = dielectricFresnel(fabsf(dot(V, N)), eta);
+    f3 d, w;
+    float pdf;
+    uint32_t flags;
+    if (u < F) {
+      d = makeNormal(reflect(-V, N));
+      pdf = F;
+      flags = kSpecRefl;
+      w = F * mk1(1.f);
+    } else {
+      d = makeNormal(refract(-V, N, eta));
+      pdf = 1.0f - F;
+      flags = kSpecTrans;
+      w = (1.0f - F) * mk1(1.f) * eta * eta;
+    }
+    if (pdf <= 0.0f || length(w) == 0.0f) {
+      c.stop = true;
+      return;
+    }
+    float sg = dot(d, h.N);
+    float sgn = sg > 0.0f ? 1.0f : (sg < 0.0f ? -1.0f : 0.0f);
+    c.bpdf = pdf;
+    c.bflags = flags;
+    c.ro = offsetRay(h.pos, sgn * h.N);
+    c.rd = d;
+    c.T = c.T * (w / (pdf + kEps));
+    return;
   }
   float u0 = rnd(c.seed), u1 = rnd(c.seed);
-  // sampleBsdf, brdf_rough_plastic.rchit:138-189
-  f3 N = h.ffN, V = h.V, X = h.X, Y = h.Y;
-  float NdotV = dot(V, N);
+  if (lambert) {
+    f3 wi = cosineHemisphere(u0, u1);
+    float pdf = cosinePdf(wi.z);
+    f3 d = toWorld(h.X, h.Y, h.ffN, wi);
+    f3 w = kd * kInvPi * fabsf(wi.z);
+    if (pdf <= 0.0f || length(w) == 0.0f) {
+      c.stop = true;
+      return;
+    }
+    nextRay(c, h.pos, h.ffN, d, w / pdf, pdf, kDiffuseRefl);
+    return;
+  }
+  f3 d = mk1(0.f), w = mk1(0.f);
   float pdf = 0.0f;
   uint32_t flags = kBsdfNull;
-  f3 d = mk1(0.f), w = mk1(0.f);
-  if (NdotV > 0.0f) {
-    float Fo = fresnelDielectricExt(NdotV, eta);
-    float sub = ssw * (1.0f - Fo), spec = Fo;
-    float ps = spec / (spec + sub);
-    if (u0 < ps) {
-      u0 = u0 / ps;
-      f3 wo = toLocal(X, Y, N, V);
-      f3 wi = sampleGGX(u0, u1, wo, ax, ay);
-      f3 Lw = toWorld(X, Y, N, wi);
-      d = Lw;
-      f3 H = makeNormal(V + Lw);
-      f3 wh = makeNormal(wi + wo);
-      float NdotL = dot(N, Lw);
-      float Fs = fresnelDielectricExt(dot(H, V), eta);
-      float Gs = g1GGX(NdotV, dot(V, X), dot(V, Y), ax, ay);
-      Gs *= g1GGX(NdotL, dot(Lw, X), dot(Lw, Y), ax, ay);
-      float Ds = dAnisoGGX(dot(H, N), dot(H, X), dot(H, Y), ax, ay);
-      flags = kGlossyRefl;
-      pdf = pdfGGX(wh, wo, ax, ay) * ps;
-      w = ks * Fs * Gs * Ds * NdotL;
-    } else {
-      u0 = (u0 - ps) / (1.0f + kEps - ps);
-      f3 wi = cosineHemisphere(u0, u1);
-      float Fi = fresnelDielectricExt(wi.z, eta);
-      f3 diff = kd / (mk1(1.f) - kd * fdrInt);
-      pdf = (1.0f - ps) * cosinePdf(wi.z);
-      d = toWorld(X, Y, N, wi);
+  if (pbr) {
+    f3 wo = makeNormal(toLocal(h.X, h.Y, h.ffN, h.V));
+    f3 wi;
+    if (rnd(c.seed) < 0.2f) {
+      wi = cosineHemisphere(u0, u1);
       flags = kDiffuseRefl;
-      w = (1.0f - Fi) * (1.0f - Fo) * invEta2 * diff * kInvPi * fabsf(wi.z);
+      pdf = cosinePdf(wi.z);
+    } else {
+      wi = sampleGGX(u0, u1, wo, ax, ay);
+      f3 wh = makeNormal(wi + wo);
+      pdf = pdfGGX(wh, wo, ax, ay);
+      flags = kGlossyRefl;
+    }
+    d = toWorld(h.X, h.Y, h.ffN, wi);
+    w = pbrEval(d, h.V, h.ffN, h.X, h.Y, kd, ax, ay, metal, eta) * fabsf(wi.z);
+  } else {
+    // sampleBsdf, brdf_rough_plastic.rchit:138-189
+    f3 N = h.ffN, V = h.V, X = h.X, Y = h.Y;
+    float NdotV = dot(V, N);
+    if (NdotV > 0.0f) {
+      float Fo = fresnelDielectricExt(NdotV, eta);
+      float sub = ssw * (1.0f - Fo), spec = Fo;
+      float ps = spec / (spec + sub);
+      if (u0 < ps) {
+        u0 = u0 / ps;
+        f3 wo = toLocal(X, Y, N, V);
+        f3 wi = sampleGGX(u0, u1, wo, ax, ay);
+        f3 Lw = toWorld(X, Y, N, wi);
+        d = Lw;
+        f3 H = makeNormal(V + Lw);
+        f3 wh = makeNormal(wi + wo);
+        float NdotL = dot(N, Lw);
+        float Fs = fresnelDielectricExt(dot(H, V), eta);
+        float Gs = g1GGX(NdotV, dot(V, X), dot(V, Y), ax, ay);
+        Gs *= g1GGX(NdotL, dot(Lw, X), dot(Lw, Y), ax, ay);
+        float Ds = dAnisoGGX(dot(H, N), dot(H, X), dot(H, Y), ax, ay);
+        flags = kGlossyRefl;
+        pdf = pdfGGX(wh, wo, ax, ay) * ps;
+        w = ks * Fs * Gs * Ds * NdotL;
+      } else {
+        u0 = (u0 - ps) / (1.0f + kEps - ps);
+        f3 wi = cosineHemisphere(u0, u1);
+        float Fi = fresnelDielectricExt(wi.z, eta);
+        f3 diff = kd / (mk1(1.f) - kd * fdrInt);
+        pdf = (1.0f - ps) * cosinePdf(wi.z);
+        d = toWorld(X, Y, N, wi);
+        flags = kDiffuseRefl;
+        w = (1.0f - Fi) * (1.0f - Fo) * invEta2 * diff * kInvPi * fabsf(wi.z);
+      }
     }
   }
   if (pdf <= 0.0f || length(w) == 0.0f) {
@@ -854,61 +1067,19 @@ __device__ void shadeRoughPlastic(PathCtx& c, Hit& h, const GpuMaterial& m) {
   nextRay(c, h.pos, h.ffN, d, w / (pdf + kEps), pdf, flags);
 }
 
-// bsdf_dielectric.rchit:343-408.  The NEE term is always zero (eval tests
-// EDelta but is called with EArea), so only its RNG draws and the shadow
-// ray are kept; the shadow ray is counted but not traced (it cannot change
-// the radiance).
-__device__ void shadeDielectric(PathCtx& c, Hit& h, const GpuMaterial& m) {
-  const GpuPushConstantRaytrace& pc = c.sp->pc;
-  if (m.normalTextureId >= 0) normalMap(*c.sc, h, m.normalTextureId, pc.useFaceNormal);
-  float ior = m.ior;
-  float eta = dot(h.V, h.N) > 0.0f ? (1.0f / ior) : ior;
-  {
-    bool visible;
-    LSample l;
-    sampleLights(c, h.pos, h.ffN, visible, l);
-    c.Ld = mk1(0.f);
-    c.shadow = false;
-    c.fromDielectric = visible;
-  }
-  float u = rnd(c.seed);
-  f3 V = h.V, N = h.ffN;
-  float F = dielectricFresnel(fabsf(dot(V, N)), eta);
-  f3 d, w;
-  float pdf;
-  uint32_t flags;
-  if (u < F) {
-    d = makeNormal(reflect(-V, N));
-    pdf = F;
-    flags = kSpecRefl;
-    w = F * mk1(1.f);
-  } else {
-    d = makeNormal(refract(-V, N, eta));
-    pdf = 1.0f - F;
-    flags = kSpecTrans;
-    w = (1.0f - F) * mk1(1.f) * eta * eta;
-  }
-  if (pdf <= 0.0f || length(w) == 0.0f) {
-    c.stop = true;
-    return;
-  }
-  float sg = dot(d, h.N);
-  float sgn = sg > 0.0f ? 1.0f : (sg < 0.0f ? -1.0f : 0.0f);
-  c.bpdf = pdf;
-  c.bflags = flags;
-  c.ro = offsetRay(h.pos, sgn * h.N);
-  c.rd = d;
-  c.T = c.T * (w / (pdf + kEps));
-}
-
+// One bounce of shading for the queued paths: miss, light hit, emissive or
+// surface plugin; appends the NEE shadow ray and the continuing path.
+// ONE: the batch holds a single shot whose constants come from the kernel
+// arguments (scalar loads) instead of the per-shot table.
+template <bool ONE>
 __global__ __launch_bounds__(256) void kShade(SceneView sc, BatchDesc b, Wavefront w, int cur) {
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t count = w.ctr[cur];
   if (i >= count) return;
   int pid = w.queue[cur][i];
   uint4 info = w.info[pid];
-  uint32_t shot = info.w >> 16;
-  const ShotParams& sp = b.shotTab[shot];
+  uint32_t shot = info.w & 0xffffu;
+  const ShotParams& sp = ONE ? b.one : b.shotTab[shot];
   const GpuPushConstantRaytrace& pc = sp.pc;
   float4 thr = w.thr[pid], rad = w.rad[pid];
   PathCtx c;
@@ -928,7 +1099,7 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, BatchDesc b, Wavefro
   c.Ld = mk1(0.f);
   int4 hi = w.hitI[pid];
   uint32_t pix = info.z;
-  bool frame0 = (b.firstFrame + int32_t(info.w & 0xffffu)) == 0;
+  bool frame0 = (info.w >> 16) & 1u;
   if (hi.x < 0) {
     // raytrace.default.rmiss:24-55
     c.stop = true;
@@ -952,26 +1123,24 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, BatchDesc b, Wavefro
       hitLight(c, h.lightId, h.pos);
     } else {
       const GpuMaterial& m = sc.mats[h.mat];
-      switch (m.type) {
-        case MaterialTypeBrdfLambertian: shadeLambertian(c, h, m, pix, aov); break;
-        case MaterialTypeBrdfPbrMetalnessRoughness: shadePbr(c, h, m, pix, aov); break;
-        case MaterialTypeBrdfRoughPlastic: shadeRoughPlastic(c, h, m); break;
-        case MaterialTypeBsdfDielectric: shadeDielectric(c, h, m); break;
-        case MaterialTypeBrdfEmissive: {
-          // brdf_emissive.rchit:12-26
-          c.stop = true;
-          f3 e = ld3(m.radiance);
-          if (m.radianceTextureId >= 0) e = ld3(m.radianceFactor) * xyz(tex2D(sc.textures[m.radianceTextureId], h.uvx, h.uvy));
-          if (pc.ignoreEmissive == 0) c.L = c.L + e * c.T;
-          break;
-        }
-        default: c.stop = true; break;
+      uint32_t type = m.type;
+      if (type == MaterialTypeBrdfEmissive) {
+        // brdf_emissive.rchit:12-26
+        c.stop = true;
+        f3 e = ld3(m.radiance);
+        if (m.radianceTextureId >= 0) e = ld3(m.radianceFactor) * xyz(tex2D(sc.textures[m.radianceTextureId], h.uvx, h.uvy));
+        if (pc.ignoreEmissive == 0) c.L = c.L + e * c.T;
+      } else if (type == MaterialTypeBrdfLambertian || type == MaterialTypeBrdfPbrMetalnessRoughness ||
+                 type == MaterialTypeBrdfRoughPlastic || type == MaterialTypeBsdfDielectric) {
+        shadeSurface(c, h, m, pix, aov);
+      } else {
+        c.stop = true;
       }
     }
   }
   // Counters for Mrays/s (extension rays of this bounce, NEE rays).
-  waveCount(&w.ctr[5], true);
-  waveCount(&w.ctr[7], c.fromDielectric);
+  waveCount64(&w.cnt64[4], true);
+  waveCount64(&w.cnt64[6], c.fromDielectric);
   // NEE shadow ray: maxDist = dist - 2*EPS (rgen:117-122)
   int si = waveAppend(&w.ctr[2], c.shadow);
   if (si >= 0) {
@@ -1000,28 +1169,30 @@ __global__ void kResetCounters(Wavefront w, int cur) {
 }
 
 __global__ void kStatShadow(Wavefront w) {
-  if (threadIdx.x == 0) w.ctr[6] += w.ctr[2];
+  if (threadIdx.x == 0) w.cnt64[5] += w.ctr[2];
 }
 
 // Ordered fold of the batch's frames into the output images
 // (raytrace.projective.rgen:146-178).
 __global__ __launch_bounds__(256) void kAccumulate(BatchDesc b, Wavefront w) {
   uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.P * b.shots) return;
-  uint32_t pix = t % b.P, shot = t / b.P;
-  const ShotParams& sp = b.shotTab[shot];
+  if (t >= b.nAcc) return;
+  const WorkUnit un = b.units[findUnit<true>(b.units, b.nUnits, t)];
+  uint32_t k = t - un.accBase;
+  uint32_t pix = un.pix0 + k;
+  const ShotParams& sp = b.shotTab[un.shot];
   size_t plane = size_t(sp.width) * sp.height;
   float4* img0 = sp.images;
   float4* img8 = sp.images + 8 * plane;
   float4 mean = img0[pix];
   float W = img8[pix].x;
-  for (uint32_t f = 0; f < b.frames; ++f) {
-    uint32_t p = (shot * b.frames + f) * b.P + pix;
+  for (uint32_t f = 0; f < un.nf; ++f) {
+    uint32_t p = un.base + f * un.np + k;
     float4 r = w.rad[p];
     float fw = r.w;
     f3 L = f3{clampf(r.x, 0.f, 10.f), clampf(r.y, 0.f, 10.f), clampf(r.z, 0.f, 10.f)};
     f3 rws = fw * L;
-    if (b.firstFrame + int32_t(f) == 0) {
+    if (un.f0 + int32_t(f) == 0) {
       f3 m = rws / fw;
       mean = f4(m, 1.f);
       W = fw;
@@ -1071,20 +1242,33 @@ void launchRaygen(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
 }
 void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t, uint32_t grid, bool count,
                   hipStream_t s) {
-  if (count) kExtend<true><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
-  else kExtend<false><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+  bool wide = sc.nodes4 != nullptr;
+  if (count) {
+    if (wide) kExtend<true, true><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtend<true, false><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+  } else {
+    if (wide) kExtend<false, true><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtend<false, false><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+  }
 }
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
                  hipStream_t s) {
-  kShade<<<(maxRays + 255) / 256, 256, 0, s>>>(sc, b, w, cur);
+  if (b.shots == 1) kShade<true><<<(maxRays + 255) / 256, 256, 0, s>>>(sc, b, w, cur);
+  else kShade<false><<<(maxRays + 255) / 256, 256, 0, s>>>(sc, b, w, cur);
   kStatShadow<<<1, 64, 0, s>>>(w);
 }
 void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t, uint32_t grid, bool count, hipStream_t s) {
-  if (count) kShadow<true><<<grid, kTraceBlock, 0, s>>>(sc, w);
-  else kShadow<false><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  bool wide = sc.nodes4 != nullptr;
+  if (count) {
+    if (wide) kShadow<true, true><<<grid, kTraceBlock, 0, s>>>(sc, w);
+    else kShadow<true, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  } else {
+    if (wide) kShadow<false, true><<<grid, kTraceBlock, 0, s>>>(sc, w);
+    else kShadow<false, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  }
 }
 void launchAccumulate(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
-  kAccumulate<<<(b.P * b.shots + 255) / 256, 256, 0, s>>>(b, w);
+  kAccumulate<<<(b.nAcc + 255) / 256, 256, 0, s>>>(b, w);
 }
 void launchResetCounters(const Wavefront& w, int cur, hipStream_t s) { kResetCounters<<<1, 64, 0, s>>>(w, cur); }
 void launchFirstHit(const SceneView& sc, const ShotParams* shot, const Wavefront& w, ash_hit* out, uint32_t P,

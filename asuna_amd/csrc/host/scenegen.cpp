@@ -292,11 +292,11 @@ void skyColor(V3 d, V3 sun, float out[3]) {
   out[0] = float(r), out[1] = float(g), out[2] = float(b);
 }
 
-int genSpheres(uint64_t master, int scale, const std::string& dir, std::string& scenePath) {
-  int level = scale > 0 ? 3 : 6;
-  int grid = scale > 0 ? 2 : 8;
-  int envW = scale > 0 ? 256 : 2048, envH = envW / 2;
-  uint32_t W = scale > 0 ? 320 : 1920, H = scale > 0 ? 180 : 1080;
+// grid x grid displaced icospheres of `level` subdivisions on a ground quad,
+// envW x envW/2 sky, 4 triangle lights, film W x H.
+int genSpheres(uint64_t master, int grid, int level, int envW, uint32_t W, uint32_t H, const std::string& dir,
+               std::string& scenePath) {
+  int envH = envW / 2;
   Mesh base = icosphere(level);
   const double radius = 0.4, spacing = 1.0;
   std::string mats, meshes, insts;
@@ -401,9 +401,26 @@ extern "C" int ash_generate_scene(int32_t config, uint64_t seed, int32_t scale, 
   uint64_t master = seed ? seed : 0xA5A52026ull;
   std::string path;
   int rc;
-  if (config == 0 || config == 2) rc = genBox(master, config, dir, path);
-  else if (config == 1) rc = genSpheres(master, scale, dir, path);
-  else return ASH_E_INVALID;
+  if (config == 0 || config == 2) {
+    rc = genBox(master, config, dir, path);
+  } else if (config == 1) {
+    rc = scale > 0 ? genSpheres(master, 2, 3, 256, 320, 180, dir, path) : genSpheres(master, 8, 6, 2048, 1920, 1080, dir, path);
+  } else if (config == 4) {
+    // resident serving scene of about `scale` triangles: the (grid, level)
+    // whose grid^2 * 20 * 4^level is closest to it, grid in 2..8, level 2..6
+    if (scale <= 0) return ASH_E_INVALID;
+    int bg = 2, bl = 2;
+    double best = 1e300;
+    for (int g = 2; g <= 8; ++g)
+      for (int l = 2; l <= 6; ++l) {
+        double t = double(g) * g * 20.0 * std::pow(4.0, l);
+        double e = std::fabs(std::log(t / double(scale)));
+        if (e < best) best = e, bg = g, bl = l;
+      }
+    rc = genSpheres(master, bg, bl, 1024, 512, 512, dir, path);
+  } else {
+    return ASH_E_INVALID;
+  }
   if (rc == ASH_OK && pathOut && pathLen) std::snprintf(pathOut, pathLen, "%s", path.c_str());
   return rc;
 }

@@ -15,7 +15,8 @@ images in HBM.  Scene upload and BVH build happen before the timed region.
 Prints one JSON line:
   value        Mpaths/s over the timed steps (camera samples per second)
   roofline     HBM roofline of the closest-hit traversal kernel: algorithmic
-               bytes per launch (node visits x 64 B + triangle tests x 48 B +
+               bytes per launch (node visits x node bytes (112 B for the 4-wide
+               tree, 64 B binary) + triangle tests x 48 B +
                68 B ray/hit I/O per ray, counted in a calibration render) over
                its average launch time (HIP events on the render stream)
   cpu_baseline the CPU oracle (the same estimator, FP32, own SAH BVH) on the
@@ -28,12 +29,17 @@ import sys
 import tempfile
 import time
 
+import numpy as np
+
 HERE = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, HERE)
 
 METRIC = "Mpaths/s & Mrays/s at 1920x1080; p50/p99 job latency at equal relMSE"
 HBM_PEAK_GBS = 8000.0
-NODE_BYTES, TRI_BYTES, RAY_IO_BYTES = 64, 48, 68
+# bytes a node visit reads: binary node 64 B; 4-wide node 7 x 16 B (6 box
+# planes + child refs) of its 128 B line
+NODE_BYTES = {2: 64, 4: 112}
+TRI_BYTES, RAY_IO_BYTES = 48, 68
 
 
 def log(*a):
@@ -43,15 +49,20 @@ def log(*a):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--config", choices=["c2", "c1", "c3"], default="c2")
+    ap.add_argument("--config", choices=["c2", "c1", "c3", "c5"], default="c2")
+    ap.add_argument("--jobs", type=int, default=200, help="c5: jobs per arrival rate")
+    ap.add_argument("--rates", default="10,50,200", help="c5: Poisson arrival rates (jobs/s)")
     ap.add_argument("--scale", type=int, default=0, help="shrink the procedural scene (tests)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=2)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--queue-paths", type=int, default=0)
     ap.add_argument("--scene-dir", default="")
+    ap.add_argument("--branching", type=int, choices=[2, 4], default=4)
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 256 MiB device-to-device copy first (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -61,6 +72,9 @@ def main():
         sys.exit(2)
 
     import asuna_amd as A
+
+    if args.config == "c5":
+        return run_serving(A, args)
 
     if args.copy_probe:
         import torch  # plumbing only: a plain copy kernel of known size
@@ -92,23 +106,32 @@ def main():
     t0 = time.time()
     r.upload(scene)
     t_up = time.time() - t0
-    binfo = r.build()
+    binfo = r.build(branching=args.branching)
     log(f"[bench] upload {t_up:.2f}s, BVH {binfo}")
     r.set_film(w, h)
     dch = scene.state.depth_out_channel
 
+    tune = None
+    qp, bpc = args.queue_paths, 0
+    if not args.no_autotune and not args.queue_paths:
+        opts, tune = r.autotune(list(cams), list(pcs), probe_frames=1)
+        qp, bpc = opts.queue_paths, opts.blocks_per_cu
+        log(f"[bench] autotune: {tune}")
+    graphs = not args.no_graphs
+
     def step():
         if cfg_id == 2:
-            return r.render_shots(list(cams), list(pcs), spp, queue_paths=args.queue_paths)
+            return r.render_shots(list(cams), list(pcs), spp, queue_paths=qp, blocks_per_cu=bpc, graphs=graphs)
         r.set_film(w, h)
-        return r.render(cams[0], pcs[0], 0, spp, queue_paths=args.queue_paths, depth_channel=dch)
+        return r.render(cams[0], pcs[0], 0, spp, queue_paths=qp, blocks_per_cu=bpc, depth_channel=dch,
+                        graphs=graphs)
 
     # calibration: traversal work per ray (not timed)
     cal = r.render(cams[0], pcs[0], 0, 1, queue_paths=args.queue_paths, count_traversal=True, depth_channel=dch)
     ext_rays = max(1, cal["extension_rays"])
     nodes_per_ray = cal["ext_node_visits"] / ext_rays
     tris_per_ray = cal["ext_tri_tests"] / ext_rays
-    bytes_per_ray = nodes_per_ray * NODE_BYTES + tris_per_ray * TRI_BYTES + RAY_IO_BYTES
+    bytes_per_ray = nodes_per_ray * NODE_BYTES[binfo["branching"]] + tris_per_ray * TRI_BYTES + RAY_IO_BYTES
     log(f"[bench] calibration: {nodes_per_ray:.2f} nodes, {tris_per_ray:.2f} tris per closest-hit ray "
         f"-> {bytes_per_ray:.0f} B/ray")
 
@@ -119,9 +142,11 @@ def main():
     hip = ctypes.CDLL("libamdhip64.so")
     hip.hipDeviceSynchronize()
     t0 = time.perf_counter()
-    stats = []
+    stats, lat = [], []
     for i in range(args.steps):
-        stats.append(step())
+        ts = time.perf_counter()
+        stats.append(step())  # returns after the render's last kernel has finished
+        lat.append((time.perf_counter() - ts) * 1e3)
     hip.hipDeviceSynchronize()
     elapsed = time.perf_counter() - t0
     paths = sum(s["paths"] for s in stats)
@@ -131,18 +156,31 @@ def main():
     mrays = rays / elapsed / 1e6
     log(f"[bench] {args.steps} steps in {elapsed:.3f}s: {mpaths:.1f} Mpaths/s, {mrays:.1f} Mrays/s")
 
+    # relMSE of the last timed render against the FP64 oracle at 4096 spp on
+    # 16 fixed 32x32 crops: mean of (x - ref)^2 / (ref^2 + 1e-2) over pixels and RGB
+    relmse = None
+    gold = os.path.join(HERE, "tests", "golden", "c2_reference_crops.npz")
+    if cfg_id == 1 and args.scale == 0 and os.path.exists(gold):
+        g = np.load(gold)
+        img = r.read(0)
+        c = int(g["crop"])
+        x = np.stack([img[y0:y0 + c, x0:x0 + c, :3] for x0, y0 in g["origins"]])
+        ref = g["ref"]
+        relmse = float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
+        log(f"[bench] relMSE vs FP64 oracle at {int(g['spp'])} spp (16 crops of {c}x{c}): {relmse:.5f}")
+
     # per-kernel timing pass (HIP events around every launch; not the timed steps)
     if cfg_id == 2:
-        tp = r.render_shots(list(cams), list(pcs), spp, queue_paths=args.queue_paths, timing=True)
+        tp = r.render_shots(list(cams), list(pcs), spp, queue_paths=qp, blocks_per_cu=bpc, timing=True)
     else:
         r.set_film(w, h)
-        tp = r.render(cams[0], pcs[0], 0, spp, queue_paths=args.queue_paths, timing=True, depth_channel=dch)
+        tp = r.render(cams[0], pcs[0], 0, spp, queue_paths=qp, blocks_per_cu=bpc, timing=True, depth_channel=dch)
     ext_ms_per_launch = tp["extend_ms"] / max(1, tp["extend_launches"])
     rays_per_launch = tp["extension_rays"] / max(1, tp["extend_launches"])
     bytes_per_launch = rays_per_launch * bytes_per_ray
     achieved = bytes_per_launch / (ext_ms_per_launch * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(HERE, "profiles", f"traffic_extend_{args.config}.json")
+    tfile = os.path.join(HERE, "profiles", f"traffic_extend_{args.config}_bvh{binfo['branching']}.json")
     if os.path.exists(tfile) and args.scale == 0:
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
@@ -181,9 +219,126 @@ def main():
                       "max_path_depth": pcs[0].maxPathDepth, "triangles": scene.state.num_triangles,
                       "shots": nshots, "bvh": binfo},
            "mrays_per_s": round(mrays, 1),
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2), "p99": round(float(np.percentile(lat, 99)), 2),
+                          "job": "one full render of the workload, submit to result in HBM"},
+           "relmse": relmse,
+           "relmse_ref": "FP64 CPU oracle, 4096 spp, 16 fixed 32x32 crops" if relmse is not None else None,
+           "autotune": tune, "graphs": graphs,
+           "batches_per_step": stats[-1]["batches"],
            "rays_per_path": round(rays / max(1, paths), 3),
            "roofline": roofline, "cpu_baseline": cpu,
            "gpu_over_cpu": round(mpaths / cpu["value"], 1) if cpu else None}
+    print(json.dumps(out), flush=True)
+
+
+def run_serving(A, args):
+    """Render serving: 8 resident procedural scenes (about 10k, 100k, 1M and 5M
+    triangles, two seeds each), Poisson arrivals of jobs with resolution from
+    {256^2, 512^2, 1024^2, 1920x1080} (weights .4/.3/.2/.1) and spp from
+    {1, 4, 16, 64}; scene uniform.  The server co-batches the jobs of a
+    scene and overlaps scenes; the baseline renders the same jobs one at a
+    time in arrival order (latency = FIFO queueing with the measured render
+    times)."""
+    import threading
+    rng = np.random.default_rng(0xA5A52026)
+    scenes = []
+    t0 = time.time()
+    for tris in (10_000, 100_000, 1_000_000, 5_000_000):
+        for seed in (1, 2):
+            d = os.path.join(tempfile.gettempdir(), f"asuna_serve_{tris}_{seed}")
+            p = os.path.join(d, "spheres.json")
+            if not os.path.exists(p):
+                p = A.generate_scene(4, d, scale=tris, seed=seed)
+            scenes.append(A.HostScene(p))
+    log(f"[serve] 8 scenes loaded in {time.time() - t0:.1f}s: {[s.state.num_triangles for s in scenes]}")
+    res = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
+    rates = [float(x) for x in args.rates.split(",")]
+
+    def make_jobs(n):
+        jobs = []
+        for _ in range(n):
+            sid = int(rng.integers(0, len(scenes)))
+            w, h = res[int(rng.choice(4, p=[0.4, 0.3, 0.2, 0.1]))]
+            spp = int(rng.choice([1, 4, 16, 64]))
+            eye = rng.normal(size=3)
+            eye = eye / np.linalg.norm(eye) * rng.uniform(3.0, 6.0)
+            eye[1] = abs(eye[1]) + 0.5
+            cam = A.camera_lookat(w, h, float(rng.uniform(35, 70)), tuple(eye), (0.0, 0.3, 0.0))
+            _, pc = scenes[sid].shot(0)
+            jobs.append((sid, cam, pc, w, h, spp, bool(rng.integers(0, 2))))
+        return jobs
+
+    srv = A.Server(0, hbm_budget_bytes=16 << 30, queue_paths=1 << 21)
+    for s in scenes:
+        srv.add_scene(s)
+    rs = []
+    for s in scenes:
+        r = A.Renderer(0)
+        r.upload(s)
+        r.build()
+        rs.append(r)
+    # warm both paths (graph capture, first-touch allocations)
+    for sid in range(len(scenes)):
+        cam, pc = scenes[sid].shot(0)
+        cam = A.camera_lookat(256, 256, 45.0, (0.0, 1.0, 4.0), (0.0, 0.3, 0.0))
+        j = srv.submit(sid, cam, pc, 256, 256, 1)
+        srv.wait(j)
+        srv.release(j)
+        rs[sid].set_film(256, 256)
+        rs[sid].render(cam, pc, 0, 1)
+    out = {"metric": "render serving: jobs/s and p50/p99 job latency", "unit": "jobs/s", "n_gpus": 1,
+           "config": {"workload": "8 resident scenes (1.2e4, 8.2e4, 1.0e6, 5.2e6 triangles x 2 seeds); Poisson "
+                                  "arrivals; resolution {256^2,512^2,1024^2,1920x1080} w {.4,.3,.2,.1}; "
+                                  "spp {1,4,16,64}; denoise flag drawn but not applied (no denoiser stage)",
+                      "jobs_per_rate": args.jobs, "queue_paths": 1 << 21},
+           "rates": []}
+    for lam in rates:
+        jobs = make_jobs(args.jobs)
+        arrivals = np.cumsum(rng.exponential(1.0 / lam, size=len(jobs)))
+        ids = [None] * len(jobs)
+        base = time.perf_counter()
+
+        def feeder():
+            for k, jb in enumerate(jobs):
+                dt = arrivals[k] - (time.perf_counter() - base)
+                if dt > 0:
+                    time.sleep(dt)
+                ids[k] = srv.submit(*jb[:6])
+
+        th = threading.Thread(target=feeder)
+        th.start()
+        th.join()
+        lat, paths = [], 0
+        for k in range(len(jobs)):
+            st = srv.wait(ids[k])
+            lat.append(st["done_ms"] - st["submit_ms"])
+            paths += st["paths"]
+            srv.release(ids[k])
+        span = (time.perf_counter() - base)
+        served = {"jobs_per_s": round(len(jobs) / span, 2), "p50_ms": round(float(np.percentile(lat, 50)), 1),
+                  "p99_ms": round(float(np.percentile(lat, 99)), 1), "mpaths_per_s": round(paths / span / 1e6, 1)}
+        # baseline: the same jobs rendered one at a time (FIFO)
+        service = []
+        for sid, cam, pc, w, h, spp, _ in jobs:
+            r = rs[sid]
+            ts = time.perf_counter()
+            r.set_film(w, h)
+            r.render(cam, pc, 0, spp)
+            service.append(time.perf_counter() - ts)
+        finish, blat = 0.0, []
+        for a, sv in zip(arrivals, service):
+            finish = max(finish, a) + sv
+            blat.append((finish - a) * 1e3)
+        single = {"jobs_per_s": round(len(jobs) / max(finish, arrivals[-1]), 2),
+                  "p50_ms": round(float(np.percentile(blat, 50)), 1), "p99_ms": round(float(np.percentile(blat, 99)), 1),
+                  "busy_s": round(sum(service), 2)}
+        log(f"[serve] rate {lam}/s: server {served}, one-at-a-time {single}")
+        out["rates"].append({"rate": lam, "server": served, "single_job": single})
+    srv.close()
+    for r in rs:
+        r.close()
+    out["value"] = out["rates"][-1]["server"]["jobs_per_s"]
+    out["higher_is_better"] = True
     print(json.dumps(out), flush=True)
 
 

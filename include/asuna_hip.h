@@ -229,11 +229,15 @@ typedef struct ash_stats {
   uint64_t ext_tri_tests;   /* triangles fetched and tested by closest-hit rays */
   uint64_t shadow_node_visits;
   uint64_t shadow_tri_tests;
+  uint64_t batches;        /* wavefront batches */
+  uint64_t graph_launches; /* bounce loops replayed from a captured HIP graph */
 } ash_stats;
 
 typedef struct ash_build_opts {
   uint32_t leaf_max_tris; /* max triangles per leaf after collapse (default 4) */
   uint32_t quality;       /* 0 = LBVH, 1 = LBVH + SAH leaf collapse (default 1) */
+  uint32_t branching;     /* 2 = binary BVH, 4 = 4-wide BVH traversed by the kernels (default 4) */
+  uint32_t pad;
 } ash_build_opts;
 
 typedef struct ash_build_info {
@@ -244,16 +248,21 @@ typedef struct ash_build_info {
   double build_ms;
   double sah_cost;
   uint64_t device_bytes; /* nodes + triangles + shading data */
+  uint64_t num_wide_nodes; /* 4-wide nodes (0 when branching = 2) */
+  uint32_t wide_depth;     /* levels of the 4-wide tree */
+  uint32_t branching;      /* tree the kernels traverse */
 } ash_build_info;
 
 /* Render options beyond the push constants. */
 typedef struct ash_render_opts {
-  uint32_t queue_paths;    /* wavefront size in paths (0 = auto) */
-  uint32_t block_size;     /* threads per block of the traversal kernels (0 = auto) */
+  uint32_t queue_paths;    /* paths per wavefront batch, the tile size of the work split (0 = 2^22) */
+  uint32_t blocks_per_cu;  /* persistent traversal blocks (256 threads) per CU (0 = 8) */
   uint32_t timing;         /* 1 = time each stage with HIP events */
   int32_t depthOutChannel; /* extension: AOV channel receiving first-hit t, -1 = none */
   uint32_t max_bounce_iters; /* safety cap on wavefront iterations (0 = auto) */
   uint32_t count_traversal;  /* 1 = count node visits / triangle tests (slower) */
+  uint32_t no_graphs;        /* 1 = launch the bounce loop eagerly instead of replaying a HIP graph */
+  uint32_t pad;
 } ash_render_opts;
 
 /* ------------------------------------------------------------ error codes */
@@ -304,6 +313,23 @@ int ash_render_shots(ash_ctx* ctx, uint32_t n_shots, const GpuCamera* cams,
                      const GpuPushConstantRaytrace* pcs, int32_t n_frames,
                      const ash_render_opts* opts, ash_stats* stats);
 int ash_read_shot_output(ash_ctx* ctx, uint32_t shot, int32_t image, float* host_rgba);
+
+/* Autotuner (no counterpart in the reference, whose dispatch shape is fixed
+ * at one thread per pixel): sweeps the wavefront batch size queue_paths over
+ * 2^18..2^22 and the persistent traversal grid blocks_per_cu over {4, 8, 16}
+ * on a probe render of `probe_frames` frames of the given shots (each
+ * candidate rendered twice, the second timed), then writes the fastest pair
+ * into `inout`.  The probe renders overwrite the shot outputs. */
+typedef struct ash_tune_result {
+  uint32_t queue_paths;
+  uint32_t blocks_per_cu;
+  uint32_t candidates;
+  uint32_t pad;
+  double best_ms;  /* probe render time of the chosen pair */
+  double worst_ms; /* slowest candidate */
+} ash_tune_result;
+int ash_autotune(ash_ctx* ctx, uint32_t n_shots, const GpuCamera* cams, const GpuPushConstantRaytrace* pcs,
+                 int32_t probe_frames, ash_render_opts* inout, ash_tune_result* result);
 
 /* Copy output image `image` (0..8) to host RGBA32F (w*h*4 floats). */
 int ash_read_output(ash_ctx* ctx, int32_t image, float* host_rgba);

@@ -1,0 +1,95 @@
+/* Resident render server on one GPU: libasuna.so.
+ *
+ * No counterpart in the reference, which renders one scene per process and
+ * one shot at a time (tracer.cpp:177-264).  Callers: render farms and
+ * interactive services that keep several scenes resident and submit many
+ * small jobs.
+ *
+ * Scenes stay in HBM (geometry, BVH, textures, env tables) once added.  Jobs
+ * (camera, push constants, resolution, frames) are admitted when their
+ * output images fit in the HBM budget, then co-batched: every wavefront
+ * batch of a scene takes work units (pixel runs x frame ranges) from all of
+ * that scene's admitted jobs, and the batches of different scenes run on
+ * their own streams at the same time.  A job's result does not depend on
+ * what it was batched with: the RNG is keyed by (pixel, frame) and every
+ * pixel folds its frames in order, so it equals ash_render of the same
+ * camera bit for bit.
+ */
+#ifndef ASUNA_SERVER_H
+#define ASUNA_SERVER_H
+
+#include "asuna_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ash_server ash_server;
+
+typedef struct ash_server_opts {
+  uint64_t hbm_budget_bytes; /* arena for admitted jobs' output images (0 = 8 GiB) */
+  uint32_t queue_paths;      /* paths per wavefront batch of one scene (0 = 2^21) */
+  uint32_t blocks_per_cu;    /* persistent traversal blocks per CU (0 = 8) */
+  uint32_t max_jobs_per_batch; /* shot-table entries per batch (0 = 256) */
+  uint32_t batches_ahead;    /* batches enqueued per scene before waiting (0 = 2, max 3) */
+} ash_server_opts;
+
+typedef struct ash_job_desc {
+  uint32_t scene;   /* id returned by ash_server_add_scene */
+  uint32_t width, height;
+  int32_t spp;      /* frames 0 .. spp-1 are accumulated */
+  GpuCamera camera;
+  GpuPushConstantRaytrace pc; /* as for ash_render */
+  int32_t depth_channel;      /* AOV channel for first-hit depth, -1 = none */
+  uint32_t flags;             /* reserved, 0 */
+} ash_job_desc;
+
+enum { ASH_JOB_QUEUED = 0, ASH_JOB_RUNNING = 1, ASH_JOB_DONE = 2 };
+
+typedef struct ash_job_status {
+  int32_t state;       /* ASH_JOB_* or a negative ASH_E_* code */
+  uint32_t pad;
+  double submit_ms;    /* server clock (ms since ash_server_create) */
+  double start_ms;     /* first batch enqueued */
+  double done_ms;      /* last batch observed complete */
+  uint64_t paths;
+} ash_job_status;
+
+typedef struct ash_server_counters {
+  uint64_t jobs_submitted, jobs_done;
+  uint64_t batches;
+  uint64_t paths;
+  uint64_t arena_bytes, arena_used_bytes;
+  double busy_ms;      /* wall time with at least one batch in flight */
+  double uptime_ms;
+} ash_server_counters;
+
+int ash_server_create(int device, const ash_server_opts* opts, ash_server** out);
+void ash_server_destroy(ash_server* srv);
+const char* ash_server_last_error(ash_server* srv);
+
+/* Upload a scene and build its BVH; it stays resident.  Not concurrent with
+ * running jobs of other scenes' uploads (call before submitting). */
+int ash_server_add_scene(ash_server* srv, const ash_scene_desc* desc, const ash_build_opts* opts,
+                         uint32_t* scene_id);
+
+/* Queue a job; returns at once with its id. */
+int ash_server_submit(ash_server* srv, const ash_job_desc* job, uint64_t* job_id);
+
+/* Wait up to timeout_ms (-1 = forever) for a job; fills status either way.
+ * Returns ASH_OK when done, ASH_E_STATE on timeout. */
+int ash_server_wait(ash_server* srv, uint64_t job_id, int32_t timeout_ms, ash_job_status* status);
+
+/* Copy output image `image` (0..8) of a finished job to host RGBA32F. */
+int ash_server_read(ash_server* srv, uint64_t job_id, int32_t image, float* host_rgba);
+
+/* Forget a finished job and return its images to the arena. */
+int ash_server_release(ash_server* srv, uint64_t job_id);
+
+int ash_server_stats(ash_server* srv, ash_server_counters* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* ASUNA_SERVER_H */

@@ -31,10 +31,10 @@ def renderer():
     r.close()
 
 
-def _load(r, path):
+def _load(r, path, branching=4):
     s = A.HostScene(path)
     r.upload(s)
-    info = r.build()
+    info = r.build(branching=branching)
     w, h = s.film
     r.set_film(w, h)
     return s, info
@@ -75,9 +75,10 @@ def test_first_hit_spheres_vs_oracle_bvh(renderer, spheres_small):
     assert dn <= 1e-5
 
 
-def test_first_hit_oblique_cameras(renderer, spheres_small):
-    """Random poses (rays at every angle to the BVH slabs)."""
-    s, _ = _load(renderer, spheres_small)
+@pytest.mark.parametrize("branching", [2, 4])
+def test_first_hit_oblique_cameras(renderer, spheres_small, branching):
+    """Random poses (rays at every angle to the BVH slabs), binary and 4-wide trees."""
+    s, _ = _load(renderer, spheres_small, branching)
     osc = O.OracleScene(s.desc)
     rng = np.random.default_rng(11)
     w, h = s.film
@@ -88,6 +89,35 @@ def test_first_hit_oblique_cameras(renderer, spheres_small):
         cam = A.camera_lookat(w, h, float(rng.uniform(30, 80)), tuple(eye), (0.0, 0.4, 0.0))
         prim, rel_t, dn, *_ = _first_hit_parity(renderer, s, osc, cam)
         assert prim >= 0.9999 and rel_t.max(initial=0) <= 1e-5 and dn <= 1e-5
+
+
+def test_wide_and_binary_trees_render_identically(renderer, spheres_small):
+    """The 4-wide walk (nearest-first, distance-culled stack) finds the same
+    closest hit as the binary walk, so whole renders are bit-identical; it
+    visits fewer nodes."""
+    s, i2 = _load(renderer, spheres_small, 2)
+    cam, pc = s.shot(0)
+    w, h = s.film
+    renderer.set_film(w, h)
+    st2 = renderer.render(cam, pc, 0, 4, count_traversal=True)
+    img2 = [renderer.read(i) for i in (0, 1, 2, 8)]
+    hit2 = renderer.first_hit(cam)
+    s, i4 = _load(renderer, spheres_small, 4)
+    renderer.set_film(w, h)
+    st4 = renderer.render(cam, pc, 0, 4, count_traversal=True)
+    img4 = [renderer.read(i) for i in (0, 1, 2, 8)]
+    hit4 = renderer.first_hit(cam)
+    log(f"binary {i2}\nwide {i4}\nnodes/ray {st2['ext_node_visits'] / st2['extension_rays']:.2f} -> "
+        f"{st4['ext_node_visits'] / st4['extension_rays']:.2f}")
+    assert i2["branching"] == 2 and i2["num_wide_nodes"] == 0
+    assert i4["branching"] == 4 and 0 < i4["num_wide_nodes"] < i4["num_nodes"]
+    assert i4["wide_depth"] < i4["max_depth"]
+    for a, b in zip(img2, img4):
+        np.testing.assert_array_equal(a, b)
+    for k in ("t", "prim_id", "inst_id", "normal"):
+        np.testing.assert_array_equal(hit2[k], hit4[k])
+    assert st2["extension_rays"] == st4["extension_rays"] and st2["shadow_rays"] == st4["shadow_rays"]
+    assert st4["ext_node_visits"] < st2["ext_node_visits"]
 
 
 def _radiance_parity(r, s, spp, crop=None, precision=0):
