@@ -128,21 +128,21 @@ class BuildInfo(C.Structure):
 class RenderOpts(C.Structure):
     _fields_ = [("queue_paths", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("timing", C.c_uint32),
                 ("depthOutChannel", C.c_int32), ("max_bounce_iters", C.c_uint32), ("count_traversal", C.c_uint32),
-                ("no_graphs", C.c_uint32), ("pad", C.c_uint32)]
+                ("no_graphs", C.c_uint32), ("lds_stack", C.c_uint32)]
 
     @staticmethod
     def make(queue_paths=0, blocks_per_cu=0, timing=False, depth_channel=-1, max_bounce_iters=0,
-             count_traversal=False, graphs=True):
+             count_traversal=False, graphs=True, lds_stack=0):
         return RenderOpts(queue_paths, blocks_per_cu, int(bool(timing)), depth_channel, max_bounce_iters,
-                          int(bool(count_traversal)), 0 if graphs else 1, 0)
+                          int(bool(count_traversal)), 0 if graphs else 1, lds_stack)
 
 
 class TuneResult(C.Structure):
     _fields_ = [("queue_paths", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("candidates", C.c_uint32),
-                ("pad", C.c_uint32), ("best_ms", C.c_double), ("worst_ms", C.c_double)]
+                ("lds_stack", C.c_uint32), ("best_ms", C.c_double), ("worst_ms", C.c_double)]
 
     def as_dict(self):
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class ServerOpts(C.Structure):
@@ -217,6 +217,18 @@ def lib():
         "ash_autotune": (C.c_int, [vp, C.c_uint32, P(GpuCamera), P(GpuPushConstantRaytrace), C.c_int32,
                                    P(RenderOpts), P(TuneResult)]),
         "ash_read_shot_output": (C.c_int, [vp, C.c_uint32, C.c_int32, P(C.c_float)]),
+        "ash_denoiser_create": (C.c_int, [C.c_int, C.c_uint32, P(vp)]),
+        "ash_denoiser_destroy": (None, [vp]),
+        "ash_denoiser_last_error": (C.c_char_p, [vp]),
+        "ash_denoiser_num_params": (C.c_int, [vp, P(C.c_uint64)]),
+        "ash_denoiser_get_params": (C.c_int, [vp, P(C.c_float)]),
+        "ash_denoiser_set_params": (C.c_int, [vp, P(C.c_float)]),
+        "ash_denoiser_forward": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_float), P(C.c_float)]),
+        "ash_denoiser_gradient": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_float), P(C.c_float),
+                                            P(C.c_float), P(C.c_float)]),
+        "ash_denoiser_train_step": (C.c_int, [vp, C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_float), P(C.c_float),
+                                              C.c_float, P(C.c_float)]),
+        "ash_denoise_film": (C.c_int, [vp, vp, C.c_int32, C.c_int32, C.c_int32, C.c_int32, P(C.c_double)]),
         "ash_server_create": (C.c_int, [C.c_int, P(ServerOpts), P(vp)]),
         "ash_server_destroy": (None, [vp]),
         "ash_server_last_error": (C.c_char_p, [vp]),
@@ -412,20 +424,20 @@ class Renderer:
         self.width, self.height = w, h
 
     def render(self, cam, pc, first_frame=0, n_frames=1, queue_paths=0, timing=False, depth_channel=-1,
-               max_bounce_iters=0, count_traversal=False, blocks_per_cu=0, graphs=True) -> dict:
+               max_bounce_iters=0, count_traversal=False, blocks_per_cu=0, graphs=True, lds_stack=0) -> dict:
         opts = RenderOpts.make(queue_paths, blocks_per_cu, timing, depth_channel, max_bounce_iters, count_traversal,
-                               graphs)
+                               graphs, lds_stack)
         st = Stats()
         self._check(lib().ash_render(self._c, C.byref(cam), C.byref(pc), first_frame, n_frames, C.byref(opts),
                                      C.byref(st)), "render")
         return st.as_dict()
 
     def render_shots(self, cams, pcs, n_frames, queue_paths=0, timing=False, blocks_per_cu=0, graphs=True,
-                     count_traversal=False) -> dict:
+                     count_traversal=False, lds_stack=0) -> dict:
         n = len(cams)
         ca = (GpuCamera * n)(*cams)
         pa = (GpuPushConstantRaytrace * n)(*pcs)
-        opts = RenderOpts.make(queue_paths, blocks_per_cu, timing, -1, 0, count_traversal, graphs)
+        opts = RenderOpts.make(queue_paths, blocks_per_cu, timing, -1, 0, count_traversal, graphs, lds_stack)
         st = Stats()
         self._check(lib().ash_render_shots(self._c, n, ca, pa, n_frames, C.byref(opts), C.byref(st)),
                     "render_shots")
@@ -534,6 +546,87 @@ class Server:
         if self._s:
             lib().ash_server_destroy(self._s)
             self._s = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------ denoiser
+class Denoiser:
+    """Learned denoiser (include/asuna_denoise.h): U-Net trained and run by the
+    library's BF16 MFMA kernels.  Batches are NHWC float32: input (N, H, W, 10)
+    = radiance RGB, albedo RGB, normal XYZ, depth; target (N, H, W, 3)."""
+
+    def __init__(self, device=0, seed=1):
+        self._d = C.c_void_p()
+        rc = lib().ash_denoiser_create(device, seed, C.byref(self._d))
+        if rc != ASH_OK:
+            self._d = None
+            raise AsunaError(f"ash_denoiser_create failed: {ERRORS.get(rc, rc)}")
+
+    def _check(self, rc, what):
+        if rc != ASH_OK:
+            msg = lib().ash_denoiser_last_error(self._d).decode()
+            raise AsunaError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}")
+
+    def num_params(self) -> int:
+        n = C.c_uint64()
+        self._check(lib().ash_denoiser_num_params(self._d, C.byref(n)), "num_params")
+        return n.value
+
+    def get_params(self) -> np.ndarray:
+        out = np.zeros(self.num_params(), np.float32)
+        self._check(lib().ash_denoiser_get_params(self._d, fptr(out)), "get_params")
+        return out
+
+    def set_params(self, theta):
+        t = np.ascontiguousarray(theta, np.float32)
+        assert t.size == self.num_params()
+        self._check(lib().ash_denoiser_set_params(self._d, fptr(t)), "set_params")
+
+    @staticmethod
+    def _batch(x, c):
+        x = np.ascontiguousarray(x, np.float32)
+        assert x.ndim == 4 and x.shape[-1] == c
+        return x
+
+    def forward(self, x) -> np.ndarray:
+        x = self._batch(x, 10)
+        n, h, w, _ = x.shape
+        out = np.zeros((n, h, w, 3), np.float32)
+        self._check(lib().ash_denoiser_forward(self._d, n, h, w, fptr(x), fptr(out)), "forward")
+        return out
+
+    def gradient(self, x, target):
+        x, t = self._batch(x, 10), self._batch(target, 3)
+        n, h, w, _ = x.shape
+        g = np.zeros(self.num_params(), np.float32)
+        loss = C.c_float()
+        self._check(lib().ash_denoiser_gradient(self._d, n, h, w, fptr(x), fptr(t), C.byref(loss), fptr(g)),
+                    "gradient")
+        return loss.value, g
+
+    def train_step(self, x, target, lr=1e-3) -> float:
+        x, t = self._batch(x, 10), self._batch(target, 3)
+        n, h, w, _ = x.shape
+        loss = C.c_float()
+        self._check(lib().ash_denoiser_train_step(self._d, n, h, w, fptr(x), fptr(t), lr, C.byref(loss)),
+                    "train_step")
+        return loss.value
+
+    def denoise_film(self, renderer, albedo_channel, normal_channel, depth_channel, out_image) -> float:
+        ms = C.c_double()
+        self._check(lib().ash_denoise_film(self._d, renderer._c, albedo_channel, normal_channel, depth_channel,
+                                           out_image, C.byref(ms)), "denoise_film")
+        return ms.value
+
+    def close(self):
+        if self._d:
+            lib().ash_denoiser_destroy(self._d)
+            self._d = None
 
     def __del__(self):
         try:

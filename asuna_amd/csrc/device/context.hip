@@ -50,7 +50,7 @@ struct GraphKey {
   Wavefront w;
   const ShotParams* tab;
   ShotParams one;
-  uint32_t grid, iters, n, single;
+  uint32_t grid, iters, n, single, lds;
 };
 
 struct GraphEntry {
@@ -240,7 +240,8 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
 // Global spill space for the traversal stacks of all persistent threads,
 // sized to the deepest stack the current tree can produce.
 static int ensureSpill(ash_ctx* c, uint32_t grid) {
-  uint32_t cap = std::max<uint32_t>(uint32_t(kSpill), c->stackNeed > uint32_t(kStackLds) ? c->stackNeed - kStackLds : 0);
+  uint32_t cap =
+      std::max<uint32_t>(uint32_t(kSpill), c->stackNeed > uint32_t(kStackLdsMin) ? c->stackNeed - kStackLdsMin : 0);
   uint32_t threads = grid * kTraceBlock;
   Wavefront& w = c->wf;
   if (w.spill && w.spillCap >= cap && w.spillThreads == threads) return ASH_OK;
@@ -268,11 +269,12 @@ static int checkPc(ash_ctx* c, const GpuPushConstantRaytrace* pc) {
 
 // The shading grid covers the wavefront capacity (threads past the live
 // count exit), so one captured loop serves every batch size.
-static void launchBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid) {
+static void launchBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid,
+                          uint32_t lds) {
   int cur = 0;
   for (uint32_t it = 0; it < iters; ++it) {
     launchResetCounters(c->wf, cur, c->stream);
-    launchExtend(sc, c->wf, cur, b.n, grid, false, c->stream);
+    launchExtend(sc, c->wf, cur, lds, grid, false, c->stream);
     launchShade(sc, b, c->wf, cur, c->wf.capacity, c->stream);
     launchShadow(sc, c->wf, b.n, grid, false, c->stream);
     cur = 1 - cur;
@@ -282,7 +284,8 @@ static void launchBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, u
 // Replays the bounce loop of a batch from a captured HIP graph (captured on
 // first use of a key; a few keys are cached).  The loop holds kernel nodes
 // only.  Returns false when capture is unavailable (eager launch instead).
-static bool graphBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid) {
+static bool graphBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid,
+                         uint32_t lds) {
   if (c->graphsBroken) return false;
   GraphKey k;
   std::memset(&k, 0, sizeof k);
@@ -294,6 +297,7 @@ static bool graphBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, ui
   k.iters = iters;
   k.n = c->wf.capacity;
   k.single = b.shots == 1;
+  k.lds = lds;
   ++c->graphClock;
   for (auto& g : c->graphs)
     if (g.exec && std::memcmp(&g.key, &k, sizeof k) == 0) {
@@ -308,7 +312,7 @@ static bool graphBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, ui
     c->graphsBroken = true;
     return false;
   }
-  launchBounces(c, sc, b, iters, grid);
+  launchBounces(c, sc, b, iters, grid, lds);
   hipGraph_t graph = nullptr;
   if (hipStreamEndCapture(c->stream, &graph) != hipSuccess || !graph) {
     c->graphsBroken = true;
@@ -337,17 +341,18 @@ static int runBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, uint32_t grid,
   bool count = o && o->count_traversal;
   bool eager = timing || count || (o && o->no_graphs);
   uint32_t iters = o && o->max_bounce_iters ? o->max_bounce_iters : uint32_t(std::max(maxDepth, 1));
+  uint32_t lds = o && o->lds_stack == uint32_t(kStackLdsMin) ? uint32_t(kStackLdsMin) : uint32_t(kStackLds);
   launchRaygen(b, c->wf, s);
   st->batches++;
   st->extend_launches += iters;
   st->shadow_launches += iters;
-  if (!eager && graphBounces(c, sc, b, iters, grid)) {
+  if (!eager && graphBounces(c, sc, b, iters, grid, lds)) {
     st->graph_launches++;
   } else if (!timing) {
     int cur = 0;
     for (uint32_t it = 0; it < iters; ++it) {
       launchResetCounters(c->wf, cur, s);
-      launchExtend(sc, c->wf, cur, b.n, grid, count, s);
+      launchExtend(sc, c->wf, cur, lds, grid, count, s);
       launchShade(sc, b, c->wf, cur, b.n, s);
       launchShadow(sc, c->wf, b.n, grid, count, s);
       cur = 1 - cur;
@@ -369,7 +374,7 @@ static int runBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, uint32_t grid,
     int cur = 0, rc = ASH_OK;
     for (uint32_t it = 0; it < iters && rc == ASH_OK; ++it) {
       launchResetCounters(c->wf, cur, s);
-      rc = timed(&st->extend_ms, [&] { launchExtend(sc, c->wf, cur, b.n, grid, count, s); });
+      rc = timed(&st->extend_ms, [&] { launchExtend(sc, c->wf, cur, lds, grid, count, s); });
       if (rc == ASH_OK) rc = timed(&st->shade_ms, [&] { launchShade(sc, b, c->wf, cur, b.n, s); });
       if (rc == ASH_OK) rc = timed(&st->shadow_ms, [&] { launchShadow(sc, c->wf, b.n, grid, count, s); });
       cur = 1 - cur;
@@ -446,6 +451,8 @@ static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots
   if (Q > (1u << 30)) return fail(c, ASH_E_INVALID, "queue_paths above 2^30");
   uint32_t bpc = o && o->blocks_per_cu ? o->blocks_per_cu : 8;
   if (bpc > 64) return fail(c, ASH_E_INVALID, "blocks_per_cu above 64");
+  if (o && o->lds_stack && o->lds_stack != uint32_t(kStackLdsMin) && o->lds_stack != uint32_t(kStackLds))
+    return fail(c, ASH_E_INVALID, "lds_stack must be 0, %d or %d", kStackLdsMin, kStackLds);
   std::vector<std::vector<WorkUnit>> plan;
   planBatches(nShots, P, firstFrame, nFrames, Q, plan);
   uint32_t cap = 0;
@@ -793,29 +800,33 @@ int ash_autotune(ash_ctx* c, uint32_t nShots, const GpuCamera* cams, const GpuPu
   o.count_traversal = 0;
   double best = 1e300, worst = 0;
   uint32_t bq = 0, bb = 0, n = 0;
+  uint32_t bl = 0;
   for (uint32_t lq = 18; lq <= 22; ++lq)
-    for (uint32_t bpc : {4u, 8u, 16u}) {
-      o.queue_paths = 1u << lq;
-      o.blocks_per_cu = bpc;
-      ash_stats st;
-      double ms = 0;
-      for (int rep = 0; rep < 2; ++rep) {
-        int rc = nShots == 1 ? ash_render(c, cams, pcs, 0, probeFrames, &o, &st)
-                             : ash_render_shots(c, nShots, cams, pcs, probeFrames, &o, &st);
-        if (rc) return rc;
-        ms = st.render_ms;
+    for (uint32_t bpc : {4u, 8u, 16u})
+      for (uint32_t lds : {uint32_t(kStackLdsMin), uint32_t(kStackLds)}) {
+        o.queue_paths = 1u << lq;
+        o.blocks_per_cu = bpc;
+        o.lds_stack = lds;
+        ash_stats st;
+        double ms = 0;
+        for (int rep = 0; rep < 2; ++rep) {
+          int rc = nShots == 1 ? ash_render(c, cams, pcs, 0, probeFrames, &o, &st)
+                               : ash_render_shots(c, nShots, cams, pcs, probeFrames, &o, &st);
+          if (rc) return rc;
+          ms = st.render_ms;
+        }
+        ++n;
+        worst = std::max(worst, ms);
+        if (ms < best) best = ms, bq = o.queue_paths, bb = bpc, bl = lds;
       }
-      ++n;
-      worst = std::max(worst, ms);
-      if (ms < best) best = ms, bq = o.queue_paths, bb = bpc;
-    }
   io->queue_paths = bq;
   io->blocks_per_cu = bb;
+  io->lds_stack = bl;
   if (res) {
     res->queue_paths = bq;
     res->blocks_per_cu = bb;
     res->candidates = n;
-    res->pad = 0;
+    res->lds_stack = bl;
     res->best_ms = best;
     res->worst_ms = worst;
   }
@@ -886,7 +897,7 @@ int ash_first_hit(ash_ctx* c, const GpuCamera* cam, ash_hit* out) {
   SceneView sc = viewOf(c);
   launchRaygen(b, c->wf, c->stream);
   launchResetCounters(c->wf, 0, c->stream);
-  launchExtend(sc, c->wf, 0, P, grid, false, c->stream);
+  launchExtend(sc, c->wf, 0, kStackLds, grid, false, c->stream);
   launchFirstHit(sc, c->shotTab.p, c->wf, dout.p, P, c->stream);
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(out, dout.p, P * sizeof(ash_hit), hipMemcpyDeviceToHost, c->stream));
@@ -938,6 +949,10 @@ ShotParams ctxMakeShot(const GpuCamera* cam, const GpuPushConstantRaytrace* pc, 
   return makeShot(nullptr, cam, pc, &o, images, w, h);
 }
 const char* ctxError(ash_ctx* c) { return c->err.c_str(); }
+void ctxFilm(ash_ctx* c, uint32_t* w, uint32_t* h) {
+  *w = c->images.p ? c->width : 0;
+  *h = c->images.p ? c->height : 0;
+}
 
 }  // namespace ashd
 

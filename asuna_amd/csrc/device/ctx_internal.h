@@ -18,5 +18,7 @@ int ctxRunBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, uint32_t grid, con
 ShotParams ctxMakeShot(const GpuCamera* cam, const GpuPushConstantRaytrace* pc, int32_t depthChannel,
                        float4* images, uint32_t w, uint32_t h);
 const char* ctxError(ash_ctx* c);
+// Film size of the single-shot output images (0 x 0 before ash_set_film).
+void ctxFilm(ash_ctx* c, uint32_t* w, uint32_t* h);
 
 }  // namespace ashd

@@ -112,47 +112,49 @@ __device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, flo
   return tn <= tf;
 }
 
-// Per-thread traversal stack: kStackLds entries in LDS (stride = block
-// size), then `cap` entries in global memory (stride = all persistent
-// threads).  Entry distances are kept only by the 4-wide closest-hit walk.
+// Per-thread traversal stack: the first `nl` entries in LDS (stride = block
+// size), then the rest in global memory (stride = all persistent threads),
+// `cap` entries in all.  Entry distances are kept only by the 4-wide
+// closest-hit walk.
 struct Stack {
   int* lds;
   float* ldsT;
   int* spill;
   float* spillT;
   uint32_t stride;
-  int cap;  // kStackLds + spilled entries
+  int nl;
+  int cap;
   int sp;
   __device__ __forceinline__ void push(int v) {
-    if (sp < kStackLds) lds[sp * kTraceBlock] = v;
-    else if (sp < cap) spill[size_t(sp - kStackLds) * stride] = v;
+    if (sp < nl) lds[sp * kTraceBlock] = v;
+    else if (sp < cap) spill[size_t(sp - nl) * stride] = v;
     ++sp;
   }
   __device__ __forceinline__ int pop() {
     --sp;
-    if (sp < kStackLds) return lds[sp * kTraceBlock];
-    if (sp < cap) return spill[size_t(sp - kStackLds) * stride];
+    if (sp < nl) return lds[sp * kTraceBlock];
+    if (sp < cap) return spill[size_t(sp - nl) * stride];
     return kEmptyRef;  // overflowed entry (the host sizes `cap` to the tree depth)
   }
   __device__ __forceinline__ void push(int v, float t) {
-    if (sp < kStackLds) {
+    if (sp < nl) {
       lds[sp * kTraceBlock] = v;
       ldsT[sp * kTraceBlock] = t;
     } else if (sp < cap) {
-      spill[size_t(sp - kStackLds) * stride] = v;
-      spillT[size_t(sp - kStackLds) * stride] = t;
+      spill[size_t(sp - nl) * stride] = v;
+      spillT[size_t(sp - nl) * stride] = t;
     }
     ++sp;
   }
   __device__ __forceinline__ int pop(float& t) {
     --sp;
-    if (sp < kStackLds) {
+    if (sp < nl) {
       t = ldsT[sp * kTraceBlock];
       return lds[sp * kTraceBlock];
     }
     if (sp < cap) {
-      t = spillT[size_t(sp - kStackLds) * stride];
-      return spill[size_t(sp - kStackLds) * stride];
+      t = spillT[size_t(sp - nl) * stride];
+      return spill[size_t(sp - nl) * stride];
     }
     t = 0.f;
     return kEmptyRef;
@@ -493,7 +495,7 @@ __global__ __launch_bounds__(256) void kRaygen(BatchDesc b, Wavefront w) {
 }
 
 // ---------------------------------------------------------------- extend
-template <bool WIDE>
+template <bool WIDE, int NL = kStackLds>
 __device__ __forceinline__ void initStack(Stack& st, int* lds, float* ldsT, const Wavefront& w) {
   uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
   st.lds = lds + threadIdx.x;
@@ -501,7 +503,8 @@ __device__ __forceinline__ void initStack(Stack& st, int* lds, float* ldsT, cons
   st.spill = w.spill + gt;
   st.spillT = WIDE ? w.spillT + gt : nullptr;
   st.stride = w.spillThreads;
-  st.cap = kStackLds + int(w.spillCap);
+  st.nl = NL;
+  st.cap = NL + int(w.spillCap);
 }
 
 template <bool COUNT, bool WIDE>
@@ -567,6 +570,221 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w
       if (!occluded) {
         float4 L = w.rad[pid], d = w.ld[pid];
         w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
+      }
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&w.cnt64[2], (unsigned long long)nNodes);
+    atomicAdd(&w.cnt64[3], (unsigned long long)nTris);
+  }
+}
+
+// ------------------------------------------- 4-wide walk with ray refill
+// Persistent waves keep every lane busy: a lane whose ray has finished
+// takes a new one from the queue as soon as kRefill lanes of its wave are
+// idle (one atomic per refill), and each loop iteration advances every busy
+// lane by one step -- one triangle test or one node visit -- so lanes at
+// leaves and lanes at inner nodes stay in lockstep instead of the wave
+// waiting for its slowest ray.
+constexpr int kRefill = 16;
+
+struct Trav {
+  TRay r;
+  int node;  // next inner node to visit, kEmptyRef = none
+  uint32_t leafFirst, leafLeft;
+  int best, bestGid;
+  float bt, bu, bv, tmax;
+};
+
+__device__ __forceinline__ void takeRef(Trav& s, int ref) {
+  if (ref < 0) {
+    decodeLeaf(ref, s.leafFirst, s.leafLeft);
+    s.node = kEmptyRef;
+  } else {
+    s.node = ref;
+  }
+}
+
+__device__ __forceinline__ void startRay(Trav& s, Stack& st, f3 o, f3 d, float tmax) {
+  setupRay(s.r, o, d);
+  s.node = 0;
+  s.leafFirst = s.leafLeft = 0;
+  s.best = -1;
+  s.bestGid = 0x7fffffff;
+  s.bt = s.tmax = tmax;
+  s.bu = s.bv = 0.f;
+  st.sp = 0;
+}
+
+// One step of the closest-hit walk; true when the ray has finished.  Same
+// visit order, culling and tie rule as traceClosest4.
+template <bool COUNT>
+__device__ __forceinline__ bool stepClosest4(const SceneView& sc, Trav& s, Stack& st, uint32_t& nN, uint32_t& nT) {
+  if (s.leafLeft) {
+    const float4* tp = sc.tris + size_t(s.leafFirst) * 3;
+    float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+    if (COUNT) ++nT;
+    float t, u, v;
+    if (hitTri(s.r, p0, p1, p2, kMinimum, s.best >= 0 ? s.bt : s.tmax, t, u, v)) {
+      int gid = __float_as_int(p2.w);
+      if (s.best < 0 || t < s.bt || (t == s.bt && gid < s.bestGid)) {
+        s.bt = t;
+        s.bu = u;
+        s.bv = v;
+        s.best = int(s.leafFirst);
+        s.bestGid = gid;
+      }
+    }
+    ++s.leafFirst;
+    --s.leafLeft;
+  } else if (s.node != kEmptyRef) {
+    if (COUNT) ++nN;
+    Kids k;
+    testNode4(sc.nodes4 + size_t(s.node) * 8, s.r, kMinimum, s.best >= 0 ? s.bt : s.tmax, k);
+    s.node = kEmptyRef;
+    if (k.n > 0) {
+      if (k.n > 3) st.push(k.c[3], k.t[3]);
+      if (k.n > 2) st.push(k.c[2], k.t[2]);
+      if (k.n > 1) st.push(k.c[1], k.t[1]);
+      takeRef(s, k.c[0]);
+    }
+  }
+  if (s.leafLeft == 0 && s.node == kEmptyRef) {
+    while (st.sp > 0) {
+      float tn;
+      int c = st.pop(tn);
+      if (s.best < 0 || tn <= s.bt * 1.0000004f) {
+        takeRef(s, c);
+        if (s.leafLeft || s.node != kEmptyRef) break;
+      }
+    }
+    return s.leafLeft == 0 && s.node == kEmptyRef;
+  }
+  return false;
+}
+
+// One step of the any-hit walk: 0 running, 1 finished unoccluded, 2 occluded.
+template <bool COUNT>
+__device__ __forceinline__ int stepAny4(const SceneView& sc, Trav& s, Stack& st, uint32_t& nN, uint32_t& nT) {
+  if (s.leafLeft) {
+    const float4* tp = sc.tris + size_t(s.leafFirst) * 3;
+    if (COUNT) ++nT;
+    float t, u, v;
+    if (hitTri(s.r, tp[0], tp[1], tp[2], 0.0f, s.tmax, t, u, v)) return 2;
+    ++s.leafFirst;
+    --s.leafLeft;
+  } else if (s.node != kEmptyRef) {
+    if (COUNT) ++nN;
+    Kids k;
+    testNode4(sc.nodes4 + size_t(s.node) * 8, s.r, 0.0f, s.tmax, k);
+    s.node = kEmptyRef;
+    if (k.n > 0) {
+      if (k.n > 3) st.push(k.c[3]);
+      if (k.n > 2) st.push(k.c[2]);
+      if (k.n > 1) st.push(k.c[1]);
+      takeRef(s, k.c[0]);
+    }
+  }
+  if (s.leafLeft == 0 && s.node == kEmptyRef) {
+    while (st.sp > 0) {
+      takeRef(s, st.pop());
+      if (s.leafLeft || s.node != kEmptyRef) break;
+    }
+    if (s.leafLeft == 0 && s.node == kEmptyRef) return 1;
+  }
+  return 0;
+}
+
+// Hands queue entries to the idle lanes of the wave when at least kRefill
+// of them are idle; `drained` is set once the queue has been handed out.
+__device__ __forceinline__ uint32_t refill(uint32_t* fetchCtr, uint32_t count, bool active, bool& drained,
+                                           bool& got) {
+  got = false;
+  uint64_t idle = __ballot(!active);
+  int nIdle = __popcll(idle);
+  if (drained || nIdle < kRefill) return 0;
+  int lane = __lane_id();
+  uint32_t base = 0;
+  if (lane == 0) base = atomicAdd(fetchCtr, uint32_t(nIdle));
+  base = __shfl(base, 0);
+  if (base + uint32_t(nIdle) >= count) drained = true;
+  uint32_t i = base + uint32_t(__popcll(idle & ((1ull << lane) - 1ull)));
+  got = !active && i < count;
+  return i;
+}
+
+template <bool COUNT, int NL>
+__global__ __launch_bounds__(kTraceBlock) void kExtendWide(SceneView sc, Wavefront w, int cur) {
+  __shared__ int lds[NL * kTraceBlock];
+  __shared__ float ldsT[NL * kTraceBlock];
+  Stack st;
+  initStack<true, NL>(st, lds, ldsT, w);
+  const uint32_t count = w.ctr[cur];
+  const int* q = w.queue[cur];
+  uint32_t nNodes = 0, nTris = 0;
+  Trav s;
+  int pid = -1;
+  bool active = false, drained = false;
+  for (;;) {
+    bool got;
+    uint32_t i = refill(&w.ctr[3], count, active, drained, got);
+    if (got) {
+      pid = q[i];
+      startRay(s, st, xyz(w.ro[pid]), xyz(w.rd[pid]), kInf);
+      active = true;
+    }
+    if (__ballot(active) == 0) {
+      if (drained) break;
+      continue;
+    }
+    if (active && stepClosest4<COUNT>(sc, s, st, nNodes, nTris)) {
+      active = false;
+      if (s.best >= 0) {
+        const float4* tp = sc.tris + size_t(s.best) * 3;
+        w.hitF[pid] = make_float4(s.bt, s.bu, s.bv, 0.f);
+        w.hitI[pid] = make_int4(__float_as_int(tp[0].w), __float_as_int(tp[1].w), __float_as_int(tp[2].w), s.best);
+      } else {
+        w.hitI[pid] = make_int4(-1, -1, -1, -1);
+      }
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&w.cnt64[0], (unsigned long long)nNodes);
+    atomicAdd(&w.cnt64[1], (unsigned long long)nTris);
+  }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kShadowWide(SceneView sc, Wavefront w) {
+  __shared__ int lds[kStackLds * kTraceBlock];
+  Stack st;
+  initStack<false>(st, lds, nullptr, w);
+  const uint32_t count = w.ctr[2];
+  uint32_t nNodes = 0, nTris = 0;
+  Trav s;
+  int pid = -1;
+  bool active = false, drained = false;
+  for (;;) {
+    bool got;
+    uint32_t i = refill(&w.ctr[4], count, active, drained, got);
+    if (got) {
+      float4 so = w.shO[i], sd = w.shD[i];
+      pid = __float_as_int(sd.w);
+      startRay(s, st, xyz(so), xyz(sd), so.w);
+      active = true;
+    }
+    if (__ballot(active) == 0) {
+      if (drained) break;
+      continue;
+    }
+    if (active) {
+      int r = stepAny4<COUNT>(sc, s, st, nNodes, nTris);
+      if (r) {
+        active = false;
+        if (r == 1) {
+          float4 L = w.rad[pid], d = w.ld[pid];
+          w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
+        }
       }
     }
   }
@@ -1240,15 +1458,18 @@ __global__ __launch_bounds__(256) void kFirstHitState(SceneView sc, const ShotPa
 void launchRaygen(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
   kRaygen<<<(b.n + 255) / 256, 256, 0, s>>>(b, w);
 }
-void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t, uint32_t grid, bool count,
+void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t grid, bool count,
                   hipStream_t s) {
   bool wide = sc.nodes4 != nullptr;
+  bool shallow = ldsStack == 8;
   if (count) {
-    if (wide) kExtend<true, true><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
-    else kExtend<true, false><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    if (!wide) kExtend<true, false><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else if (shallow) kExtendWide<true, 8><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtendWide<true, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
   } else {
-    if (wide) kExtend<false, true><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
-    else kExtend<false, false><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    if (!wide) kExtend<false, false><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else if (shallow) kExtendWide<false, 8><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtendWide<false, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
   }
 }
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
@@ -1260,10 +1481,10 @@ void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, in
 void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t, uint32_t grid, bool count, hipStream_t s) {
   bool wide = sc.nodes4 != nullptr;
   if (count) {
-    if (wide) kShadow<true, true><<<grid, kTraceBlock, 0, s>>>(sc, w);
+    if (wide) kShadowWide<true><<<grid, kTraceBlock, 0, s>>>(sc, w);
     else kShadow<true, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
   } else {
-    if (wide) kShadow<false, true><<<grid, kTraceBlock, 0, s>>>(sc, w);
+    if (wide) kShadowWide<false><<<grid, kTraceBlock, 0, s>>>(sc, w);
     else kShadow<false, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
   }
 }

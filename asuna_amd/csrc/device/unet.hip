@@ -1,0 +1,955 @@
+// Learned denoiser (include/asuna_denoise.h): a 5-level U-Net trained and
+// run by hand-written gfx950 kernels.
+//
+// Every 3x3 convolution is an implicit GEMM on v_mfma_f32_16x16x32_bf16
+// (BF16 operands, FP32 accumulation):
+//   forward      Y[p][co]  = b[co] + sum_tap sum_ci X[p + d(tap)][ci] W[tap][co][ci]
+//                GEMM rows = pixels, cols = cout, K = 9 x cin
+//   backward-x   dX[p][ci] = sum_tap sum_co dY[p - d(tap)][co] W[tap][co][ci]
+//                GEMM rows = pixels, cols = cin, K = 9 x cout
+//   backward-w   dW[tap][co][ci] = sum_p dY[p][co] X[p + d(tap)][ci]
+//                GEMM rows = cout, cols = cin, K = pixels (split over waves,
+//                FP32 atomics into dW)
+// Activations are NHWC BF16 with the channel count padded to a multiple of
+// 32 (padding channels hold zeros and their weights stay zero), so an A or B
+// fragment row is one 16-byte load.  Decoder convolutions read their input
+// as two sources -- the half-resolution tensor below, nearest-upsampled on
+// the fly, then the skip tensor -- so neither the upsampled tensor nor the
+// concatenation is materialised.  Gradients are FP32.
+//
+// MFMA 16x16x32 operand layout (lane l of 64, q = l / 16):
+//   A (16 x 32)  row l % 16, k = 8q .. 8q+7
+//   B (32 x 16)  col l % 16, k = 8q .. 8q+7
+//   D (16 x 16)  col l % 16, rows 4q .. 4q+3
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../../include/asuna_denoise.h"
+#include "ctx_internal.h"
+
+namespace {
+
+typedef __bf16 bf16;
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+constexpr int kLevels = 5;
+constexpr uint32_t kWidths[kLevels] = {32, 48, 64, 96, 128};
+constexpr uint32_t kInCh = 10, kOutCh = 3;
+constexpr int kMaxTiles = 14;  // 224 / 16: widest GEMM column count in 16-wide tiles
+
+__host__ __device__ inline uint32_t pad32(uint32_t c) { return (c + 31) & ~31u; }
+
+struct Src {
+  const bf16* a;  // activations [N][h][w][C]
+  float* g;       // gradient, same shape (backward-x target)
+  uint32_t C;     // stored channels (multiple of 32)
+  uint32_t up;    // 1: half-resolution tensor read nearest-upsampled
+};
+
+struct ConvArgs {
+  Src s0, s1;  // s1.C == 0: one source
+  uint32_t N, H, W;
+  uint32_t Cin, Cout;  // stored, multiples of 32
+  const bf16* Wf;      // [9][Cout][Cin]
+  const bf16* Wb;      // [9][Cin][Cout]
+  const float* bias;   // [Cout]
+  bf16* Y;             // BF16 output, or null
+  float* Yf;           // FP32 output (last layer), or null
+  const float* dY;     // output gradient (ReLU already applied)
+  float* upTmp;        // full-resolution gradient of an upsampled s0
+  float* dW;           // [9][Cout][Cin]
+  float* dB;           // [Cout]
+  uint32_t relu;
+  uint32_t splits;     // backward-w pixel splits
+};
+
+__device__ inline void pixelOf(uint32_t p, uint32_t H, uint32_t W, uint32_t& n, int& y, int& x) {
+  x = int(p % W);
+  uint32_t t = p / W;
+  y = int(t % H);
+  n = t / H;
+}
+
+// 8 BF16 channels [c, c + 8) of the input at (n, y, x); zeros outside.
+__device__ inline bf16x8 loadIn(const ConvArgs& a, uint32_t n, int y, int x, uint32_t c) {
+  bf16x8 z;
+  for (int i = 0; i < 8; ++i) z[i] = bf16(0.0f);
+  if (y < 0 || y >= int(a.H) || x < 0 || x >= int(a.W)) return z;
+  if (c < a.s0.C) {
+    size_t idx = a.s0.up ? ((size_t(n) * (a.H / 2) + uint32_t(y) / 2) * (a.W / 2) + uint32_t(x) / 2) * a.s0.C + c
+                         : ((size_t(n) * a.H + uint32_t(y)) * a.W + uint32_t(x)) * a.s0.C + c;
+    return *reinterpret_cast<const bf16x8*>(a.s0.a + idx);
+  }
+  c -= a.s0.C;
+  size_t idx = ((size_t(n) * a.H + uint32_t(y)) * a.W + uint32_t(x)) * a.s1.C + c;
+  return *reinterpret_cast<const bf16x8*>(a.s1.a + idx);
+}
+
+// One BF16 input value (backward-w B fragment); zero outside.
+__device__ inline float loadIn1(const ConvArgs& a, uint32_t n, int y, int x, uint32_t c) {
+  if (y < 0 || y >= int(a.H) || x < 0 || x >= int(a.W)) return 0.f;
+  if (c < a.s0.C) {
+    size_t idx = a.s0.up ? ((size_t(n) * (a.H / 2) + uint32_t(y) / 2) * (a.W / 2) + uint32_t(x) / 2) * a.s0.C + c
+                         : ((size_t(n) * a.H + uint32_t(y)) * a.W + uint32_t(x)) * a.s0.C + c;
+    return float(a.s0.a[idx]);
+  }
+  c -= a.s0.C;
+  return float(a.s1.a[((size_t(n) * a.H + uint32_t(y)) * a.W + uint32_t(x)) * a.s1.C + c]);
+}
+
+// ------------------------------------------------------------- forward
+// Block: 4 waves x 16 output pixels; each wave computes all output channels.
+__global__ __launch_bounds__(256) void kConvFwd(ConvArgs a) {
+  const uint32_t P = a.N * a.H * a.W;
+  const uint32_t lane = threadIdx.x & 63, q = lane >> 4, r16 = lane & 15;
+  const uint32_t p0 = blockIdx.x * 64 + (threadIdx.x >> 6) * 16;
+  const uint32_t p = p0 + r16;
+  const bool pin = p < P;
+  uint32_t n = 0;
+  int y = 0, x = 0;
+  if (pin) pixelOf(p, a.H, a.W, n, y, x);
+  const uint32_t nt = a.Cout / 16;
+  f32x4 acc[kMaxTiles];
+  for (int t = 0; t < kMaxTiles; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 9; ++tap) {
+    const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+    const bf16* wt = a.Wf + size_t(tap) * a.Cout * a.Cin;
+    for (uint32_t c0 = 0; c0 < a.Cin; c0 += 32) {
+      bf16x8 A;
+      if (pin) {
+        A = loadIn(a, n, yy, xx, c0 + 8 * q);
+      } else {
+        for (int i = 0; i < 8; ++i) A[i] = bf16(0.0f);
+      }
+      const bf16* wr = wt + c0 + 8 * q;
+#pragma unroll
+      for (int t = 0; t < kMaxTiles; ++t) {
+        if (uint32_t(t) < nt) {
+          bf16x8 B = *reinterpret_cast<const bf16x8*>(wr + size_t(16 * t + r16) * a.Cin);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < kMaxTiles; ++t) {
+    if (uint32_t(t) >= nt) continue;
+    const uint32_t co = 16 * t + r16;
+    const float b = a.bias[co];
+    for (int r = 0; r < 4; ++r) {
+      const uint32_t pp = p0 + 4 * q + r;
+      if (pp >= P) continue;
+      float v = acc[t][r] + b;
+      if (a.relu) v = fmaxf(v, 0.f);
+      if (a.Y) a.Y[size_t(pp) * a.Cout + co] = bf16(v);
+      if (a.Yf) a.Yf[size_t(pp) * a.Cout + co] = v;
+    }
+  }
+}
+
+// ---------------------------------------------------------- backward-x
+// Same tiling with the roles of cin and cout exchanged; gradients are
+// added (+=) into s0.g (or upTmp when s0 is upsampled) and s1.g.
+__global__ __launch_bounds__(256) void kConvBwdX(ConvArgs a) {
+  const uint32_t P = a.N * a.H * a.W;
+  const uint32_t lane = threadIdx.x & 63, q = lane >> 4, r16 = lane & 15;
+  const uint32_t p0 = blockIdx.x * 64 + (threadIdx.x >> 6) * 16;
+  const uint32_t p = p0 + r16;
+  const bool pin = p < P;
+  uint32_t n = 0;
+  int y = 0, x = 0;
+  if (pin) pixelOf(p, a.H, a.W, n, y, x);
+  const uint32_t nt = a.Cin / 16;
+  f32x4 acc[kMaxTiles];
+  for (int t = 0; t < kMaxTiles; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 9; ++tap) {
+    const int yy = y - (tap / 3 - 1), xx = x - (tap % 3 - 1);
+    const bool in = pin && yy >= 0 && yy < int(a.H) && xx >= 0 && xx < int(a.W);
+    const size_t src = in ? ((size_t(n) * a.H + uint32_t(yy)) * a.W + uint32_t(xx)) * a.Cout : 0;
+    const bf16* wt = a.Wb + size_t(tap) * a.Cin * a.Cout;
+    for (uint32_t c0 = 0; c0 < a.Cout; c0 += 32) {
+      bf16x8 A;
+      if (in) {
+        const float4* g = reinterpret_cast<const float4*>(a.dY + src + c0 + 8 * q);
+        float4 g0 = g[0], g1 = g[1];
+        A[0] = bf16(g0.x), A[1] = bf16(g0.y), A[2] = bf16(g0.z), A[3] = bf16(g0.w);
+        A[4] = bf16(g1.x), A[5] = bf16(g1.y), A[6] = bf16(g1.z), A[7] = bf16(g1.w);
+      } else {
+        for (int i = 0; i < 8; ++i) A[i] = bf16(0.0f);
+      }
+      const bf16* wr = wt + c0 + 8 * q;
+#pragma unroll
+      for (int t = 0; t < kMaxTiles; ++t) {
+        if (uint32_t(t) < nt) {
+          bf16x8 B = *reinterpret_cast<const bf16x8*>(wr + size_t(16 * t + r16) * a.Cout);
+          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int t = 0; t < kMaxTiles; ++t) {
+    if (uint32_t(t) >= nt) continue;
+    const uint32_t ci = 16 * t + r16;
+    for (int r = 0; r < 4; ++r) {
+      const uint32_t pp = p0 + 4 * q + r;
+      if (pp >= P) continue;
+      float* dst;
+      if (ci < a.s0.C) dst = (a.s0.up ? a.upTmp : a.s0.g) + size_t(pp) * a.s0.C + ci;
+      else dst = a.s1.g + size_t(pp) * a.s1.C + (ci - a.s0.C);
+      *dst += acc[t][r];
+    }
+  }
+}
+
+// ---------------------------------------------------------- backward-w
+// One wave = one tap x (32 cout) x (32 cin) block over one pixel split;
+// 2 x 2 MFMA tiles per K step of 32 pixels.
+__global__ __launch_bounds__(256) void kConvBwdW(ConvArgs a) {
+  const uint32_t P = a.N * a.H * a.W;
+  const uint32_t lane = threadIdx.x & 63, q = lane >> 4, r16 = lane & 15;
+  const uint32_t task = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const uint32_t nco = a.Cout / 32, nci = a.Cin / 32;
+  const uint32_t nTasks = 9 * nco * nci * a.splits;
+  if (task >= nTasks) return;
+  uint32_t t = task;
+  const uint32_t split = t % a.splits;
+  t /= a.splits;
+  const uint32_t cib = t % nci;
+  t /= nci;
+  const uint32_t cob = t % nco;
+  const int tap = int(t / nco);
+  const int dy = tap / 3 - 1, dx = tap % 3 - 1;
+  const uint32_t pBeg = uint32_t(uint64_t(P) * split / a.splits), pEnd = uint32_t(uint64_t(P) * (split + 1) / a.splits);
+  f32x4 acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint32_t co0 = cob * 32 + r16, ci0 = cib * 32 + r16;
+  for (uint32_t pp = pBeg; pp < pEnd; pp += 32) {
+    bf16x8 A[2], B[2];
+    uint32_t p = pp + 8 * q;
+    uint32_t n = 0;
+    int y = 0, x = 0;
+    if (p < pEnd) pixelOf(p, a.H, a.W, n, y, x);
+    for (int k = 0; k < 8; ++k, ++p) {
+      float g0 = 0.f, g1 = 0.f, x0 = 0.f, x1 = 0.f;
+      if (p < pEnd) {
+        const float* gr = a.dY + size_t(p) * a.Cout;
+        g0 = gr[co0];
+        g1 = gr[co0 + 16];
+        x0 = loadIn1(a, n, y + dy, x + dx, ci0);
+        x1 = loadIn1(a, n, y + dy, x + dx, ci0 + 16);
+      }
+      A[0][k] = bf16(g0);
+      A[1][k] = bf16(g1);
+      B[0][k] = bf16(x0);
+      B[1][k] = bf16(x1);
+      if (++x == int(a.W)) {
+        x = 0;
+        if (++y == int(a.H)) y = 0, ++n;
+      }
+    }
+    for (int i = 0; i < 2; ++i)
+      for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[i], B[j], acc[i][j], 0, 0, 0);
+  }
+  // D: col = cin (l % 16), rows = cout 4q .. 4q+3
+  float* dw = a.dW + size_t(tap) * a.Cout * a.Cin;
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j)
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t co = cob * 32 + 16 * i + 4 * q + r, ci = cib * 32 + 16 * j + r16;
+        atomicAdd(dw + size_t(co) * a.Cin + ci, acc[i][j][r]);
+      }
+}
+
+__global__ void kBiasGrad(const float* dY, uint32_t P, uint32_t C, float* dB) {
+  const uint32_t c = threadIdx.x % C, lanesPerC = blockDim.x / C;
+  const uint32_t g = threadIdx.x / C;
+  if (g >= lanesPerC) return;
+  const uint32_t chunk = 256;
+  const uint32_t p0 = blockIdx.x * chunk;
+  float s = 0.f;
+  for (uint32_t p = p0 + g; p < min(P, p0 + chunk); p += lanesPerC) s += dY[size_t(p) * C + c];
+  atomicAdd(dB + c, s);
+}
+
+// --------------------------------------------------------- elementwise
+__global__ void kReluMask(float* dY, const bf16* Y, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n && !(float(Y[i]) > 0.f)) dY[i] = 0.f;
+}
+
+__global__ void kPool(const bf16* in, bf16* out, uint32_t N, uint32_t H, uint32_t W, uint32_t C) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint32_t h = H / 2, w = W / 2;
+  if (i >= size_t(N) * h * w * C) return;
+  uint32_t c = uint32_t(i % C);
+  size_t pix = i / C;
+  uint32_t x = uint32_t(pix % w), y = uint32_t((pix / w) % h), n = uint32_t(pix / (size_t(w) * h));
+  size_t b = ((size_t(n) * H + 2 * y) * W + 2 * x) * C + c;
+  float s = float(in[b]) + float(in[b + C]) + float(in[b + size_t(W) * C]) + float(in[b + size_t(W) * C + C]);
+  out[i] = bf16(0.25f * s);
+}
+
+// dIn[full] += dOut[half] / 4
+__global__ void kPoolBwd(const float* dOut, float* dIn, uint32_t N, uint32_t H, uint32_t W, uint32_t C) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(N) * H * W * C) return;
+  uint32_t c = uint32_t(i % C);
+  size_t pix = i / C;
+  uint32_t x = uint32_t(pix % W), y = uint32_t((pix / W) % H), n = uint32_t(pix / (size_t(W) * H));
+  dIn[i] += 0.25f * dOut[((size_t(n) * (H / 2) + y / 2) * (W / 2) + x / 2) * C + c];
+}
+
+// dHalf += sum of the 2 x 2 full-resolution gradients (nearest upsampling)
+__global__ void kUpBwd(const float* tmp, float* dHalf, uint32_t N, uint32_t H, uint32_t W, uint32_t C) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint32_t h = H / 2, w = W / 2;
+  if (i >= size_t(N) * h * w * C) return;
+  uint32_t c = uint32_t(i % C);
+  size_t pix = i / C;
+  uint32_t x = uint32_t(pix % w), y = uint32_t((pix / w) % h), n = uint32_t(pix / (size_t(w) * h));
+  size_t b = ((size_t(n) * H + 2 * y) * W + 2 * x) * C + c;
+  dHalf[i] += tmp[b] + tmp[b + C] + tmp[b + size_t(W) * C] + tmp[b + size_t(W) * C + C];
+}
+
+// Raw (P, 10) FP32 -> features (P, 32) BF16
+__global__ void kFeatures(const float* raw, bf16* x, uint32_t P) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(P) * 32) return;
+  uint32_t c = uint32_t(i % 32);
+  size_t p = i / 32;
+  float v = 0.f;
+  if (c < kInCh) {
+    v = raw[p * kInCh + c];
+    if (c < 3 || c == 9) v = log1pf(fmaxf(v, 0.f));
+  }
+  x[i] = bf16(v);
+}
+
+// Film planes -> features, padded to (Hp, Wp) with zeros.
+__global__ void kFeaturesFilm(const float4* rad, const float4* alb, const float4* nrm, const float4* dep, uint32_t W,
+                              uint32_t H, uint32_t Wp, uint32_t Hp, bf16* x) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(Wp) * Hp) return;
+  uint32_t px = uint32_t(i % Wp), py = uint32_t(i / Wp);
+  float f[kInCh] = {};
+  if (px < W && py < H) {
+    size_t s = size_t(py) * W + px;
+    float4 r = rad[s];
+    f[0] = log1pf(fmaxf(r.x, 0.f)), f[1] = log1pf(fmaxf(r.y, 0.f)), f[2] = log1pf(fmaxf(r.z, 0.f));
+    if (alb) {
+      float4 a = alb[s];
+      f[3] = a.x, f[4] = a.y, f[5] = a.z;
+    }
+    if (nrm) {
+      float4 n = nrm[s];
+      f[6] = n.x, f[7] = n.y, f[8] = n.z;
+    }
+    if (dep) f[9] = log1pf(fmaxf(dep[s].x, 0.f));
+  }
+  bf16* o = x + i * 32;
+  for (uint32_t c = 0; c < 32; ++c) o[c] = bf16(c < kInCh ? f[c] : 0.f);
+}
+
+__global__ void kFilmOut(const float* y, uint32_t C, uint32_t W, uint32_t H, uint32_t Wp, float4* out) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(W) * H) return;
+  uint32_t px = uint32_t(i % W), py = uint32_t(i / W);
+  const float* v = y + (size_t(py) * Wp + px) * C;
+  out[i] = make_float4(fmaxf(expm1f(v[0]), 0.f), fmaxf(expm1f(v[1]), 0.f), fmaxf(expm1f(v[2]), 0.f), 1.f);
+}
+
+__global__ void kOutRgb(const float* y, uint32_t C, uint32_t P, float* rgb) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(P) * 3) return;
+  rgb[i] = expm1f(y[(i / 3) * C + i % 3]);
+}
+
+// Mean L1 in log space; dY = sign / (3P).  One partial sum per block.
+__global__ __launch_bounds__(256) void kLoss(const float* y, uint32_t C, const float* target, uint32_t P, float* dY,
+                                             float* partial) {
+  __shared__ float red[256];
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  float l = 0.f;
+  if (i < size_t(P) * C) {
+    uint32_t c = uint32_t(i % C);
+    size_t p = i / C;
+    float g = 0.f;
+    if (c < kOutCh) {
+      float d = y[i] - log1pf(fmaxf(target[p * kOutCh + c], 0.f));
+      l = fabsf(d);
+      g = (d > 0.f ? 1.f : (d < 0.f ? -1.f : 0.f)) / float(3.0 * double(P));
+    }
+    dY[i] = g;
+  }
+  red[threadIdx.x] = l;
+  __syncthreads();
+  for (int s = 128; s > 0; s >>= 1) {
+    if (int(threadIdx.x) < s) red[threadIdx.x] += red[threadIdx.x + s];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+__global__ void kAdam(float* w, const float* g, float* m, float* v, size_t n, float lr, float b1, float b2, float eps,
+                      float c1, float c2) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float gi = g[i];
+  float mi = b1 * m[i] + (1.f - b1) * gi;
+  float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+  m[i] = mi;
+  v[i] = vi;
+  w[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// FP32 master W[9][Cout][Cin] -> BF16 Wf (same layout) and Wb [9][Cin][Cout]
+__global__ void kPackW(const float* w, uint32_t Cout, uint32_t Cin, bf16* wf, bf16* wb) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= size_t(9) * Cout * Cin) return;
+  uint32_t ci = uint32_t(i % Cin);
+  size_t t = i / Cin;
+  uint32_t co = uint32_t(t % Cout), tap = uint32_t(t / Cout);
+  bf16 v = bf16(w[i]);
+  wf[i] = v;
+  wb[(size_t(tap) * Cin + ci) * Cout + co] = v;
+}
+
+template <class T> struct Buf {
+  T* p = nullptr;
+  size_t n = 0;
+  hipError_t alloc(size_t count) {
+    release();
+    n = count;
+    return hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct Layer {
+  uint32_t ca, cb, co;     // real channels (source 0, skip source, out)
+  uint32_t CaS, CbS, CoS;  // stored channels
+  bool relu;
+  size_t wOff, bOff;  // into the FP32 parameter vector
+  size_t fOff;        // into the BF16 packed weights (Wf and Wb each)
+};
+
+struct Act {
+  uint32_t N, H, W, C;
+  Buf<bf16> a;
+  Buf<float> g;
+  size_t count() const { return size_t(N) * H * W * C; }
+};
+
+}  // namespace
+
+struct ash_denoiser {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  std::vector<Layer> L;
+  size_t nParams = 0;  // padded FP32 parameter count
+  size_t nPacked = 0;  // BF16 weights per layout
+  Buf<float> theta, grad, m, v;
+  Buf<bf16> wf, wb;
+  uint64_t step = 0;
+  // activations for the current batch shape
+  uint32_t N = 0, H = 0, W = 0;
+  Act x, out;
+  Act enc[kLevels][2], pooled[kLevels], dec[kLevels][2];
+  Buf<float> upTmp, outF, lossPart, raw, target, rgb;
+  bool wDirty = true;
+};
+
+static int dfail(ash_denoiser* d, int code, const char* fmt, ...) {
+  char buf[512];
+  va_list ap;
+  va_start(ap, fmt);
+  std::vsnprintf(buf, sizeof buf, fmt, ap);
+  va_end(ap);
+  d->err = buf;
+  return code;
+}
+
+#define DH(d, x)                                                                              \
+  do {                                                                                        \
+    hipError_t e_ = (x);                                                                      \
+    if (e_ != hipSuccess) return dfail(d, ASH_E_HIP, "%s: %s", #x, hipGetErrorString(e_));   \
+  } while (0)
+
+static unsigned blocks(size_t n, unsigned b = 256) { return unsigned((n + b - 1) / b); }
+
+static void buildLayers(ash_denoiser* d) {
+  auto add = [&](uint32_t ca, uint32_t cb, uint32_t co, bool relu) {
+    Layer l;
+    l.ca = ca, l.cb = cb, l.co = co;
+    l.CaS = pad32(ca), l.CbS = cb ? pad32(cb) : 0, l.CoS = pad32(co);
+    l.relu = relu;
+    d->L.push_back(l);
+  };
+  uint32_t c = kInCh;
+  for (int lv = 0; lv < kLevels; ++lv) {
+    add(c, 0, kWidths[lv], true);
+    add(kWidths[lv], 0, kWidths[lv], true);
+    c = kWidths[lv];
+  }
+  for (int lv = kLevels - 2; lv >= 0; --lv) {
+    add(c, kWidths[lv], kWidths[lv], true);
+    add(kWidths[lv], 0, kWidths[lv], true);
+    c = kWidths[lv];
+  }
+  add(c, 0, kOutCh, false);
+  size_t off = 0, foff = 0;
+  for (auto& l : d->L) {
+    uint32_t cin = l.CaS + l.CbS;
+    l.wOff = off;
+    off += size_t(9) * l.CoS * cin;
+    l.bOff = off;
+    off += l.CoS;
+    l.fOff = foff;
+    foff += size_t(9) * l.CoS * cin;
+  }
+  d->nParams = off;
+  d->nPacked = foff;
+}
+
+// Real-channel index of stored input channel `s` of a layer, or -1 (padding).
+static int realIn(const Layer& l, uint32_t s) {
+  if (s < l.CaS) return s < l.ca ? int(s) : -1;
+  s -= l.CaS;
+  return s < l.cb ? int(l.ca + s) : -1;
+}
+
+static size_t realCount(const ash_denoiser* d) {
+  size_t n = 0;
+  for (auto& l : d->L) n += size_t(9) * l.co * (l.ca + l.cb) + l.co;
+  return n;
+}
+
+// padded device layout <-> real-channel flat order
+static void toPadded(const ash_denoiser* d, const float* real, std::vector<float>& pad) {
+  pad.assign(d->nParams, 0.f);
+  size_t k = 0;
+  for (auto& l : d->L) {
+    uint32_t cin = l.CaS + l.CbS, rin = l.ca + l.cb;
+    for (uint32_t t = 0; t < 9; ++t)
+      for (uint32_t co = 0; co < l.co; ++co)
+        for (uint32_t s = 0; s < cin; ++s) {
+          int r = realIn(l, s);
+          if (r >= 0) pad[l.wOff + (size_t(t) * l.CoS + co) * cin + s] = real[k + (size_t(t) * l.co + co) * rin + r];
+        }
+    k += size_t(9) * l.co * rin;
+    for (uint32_t co = 0; co < l.co; ++co) pad[l.bOff + co] = real[k + co];
+    k += l.co;
+  }
+}
+
+static void fromPadded(const ash_denoiser* d, const std::vector<float>& pad, float* real) {
+  size_t k = 0;
+  for (auto& l : d->L) {
+    uint32_t cin = l.CaS + l.CbS, rin = l.ca + l.cb;
+    for (uint32_t t = 0; t < 9; ++t)
+      for (uint32_t co = 0; co < l.co; ++co)
+        for (uint32_t s = 0; s < cin; ++s) {
+          int r = realIn(l, s);
+          if (r >= 0) real[k + (size_t(t) * l.co + co) * rin + r] = pad[l.wOff + (size_t(t) * l.CoS + co) * cin + s];
+        }
+    k += size_t(9) * l.co * rin;
+    for (uint32_t co = 0; co < l.co; ++co) real[k + co] = pad[l.bOff + co];
+    k += l.co;
+  }
+}
+
+static int packWeights(ash_denoiser* d) {
+  if (!d->wDirty) return ASH_OK;
+  for (auto& l : d->L) {
+    uint32_t cin = l.CaS + l.CbS;
+    size_t n = size_t(9) * l.CoS * cin;
+    kPackW<<<blocks(n), 256, 0, d->stream>>>(d->theta.p + l.wOff, l.CoS, cin, d->wf.p + l.fOff, d->wb.p + l.fOff);
+  }
+  DH(d, hipGetLastError());
+  d->wDirty = false;
+  return ASH_OK;
+}
+
+static int allocAct(ash_denoiser* d, Act& t, uint32_t N, uint32_t H, uint32_t W, uint32_t C) {
+  t.N = N, t.H = H, t.W = W, t.C = C;
+  DH(d, t.a.alloc(t.count()));
+  DH(d, t.g.alloc(t.count()));
+  return ASH_OK;
+}
+
+static int ensureShape(ash_denoiser* d, uint32_t N, uint32_t H, uint32_t W) {
+  if (H % 16 || W % 16 || !N) return dfail(d, ASH_E_INVALID, "batch spatial size must be a multiple of 16");
+  if (d->N == N && d->H == H && d->W == W) return ASH_OK;
+  int rc;
+#define CK(x) \
+  if ((rc = (x))) return rc
+  CK(allocAct(d, d->x, N, H, W, 32));
+  for (int lv = 0; lv < kLevels; ++lv) {
+    uint32_t h = H >> lv, w = W >> lv, c = pad32(kWidths[lv]);
+    if (lv) CK(allocAct(d, d->pooled[lv], N, h, w, pad32(kWidths[lv - 1])));
+    CK(allocAct(d, d->enc[lv][0], N, h, w, c));
+    CK(allocAct(d, d->enc[lv][1], N, h, w, c));
+    if (lv < kLevels - 1) {
+      CK(allocAct(d, d->dec[lv][0], N, h, w, c));
+      CK(allocAct(d, d->dec[lv][1], N, h, w, c));
+    }
+  }
+  CK(allocAct(d, d->out, N, H, W, 32));
+#undef CK
+  size_t P = size_t(N) * H * W;
+  DH(d, d->upTmp.alloc(P * 128));
+  DH(d, d->outF.alloc(P * 32));
+  DH(d, d->lossPart.alloc(blocks(P * 32)));
+  d->N = N, d->H = H, d->W = W;
+  return ASH_OK;
+}
+
+static ConvArgs convArgs(ash_denoiser* d, const Layer& l, const Act& s0, bool up, const Act* s1, uint32_t N, uint32_t H,
+                         uint32_t W) {
+  ConvArgs a;
+  std::memset(&a, 0, sizeof a);
+  a.s0 = Src{s0.a.p, s0.g.p, s0.C, up ? 1u : 0u};
+  if (s1) a.s1 = Src{s1->a.p, s1->g.p, s1->C, 0};
+  a.N = N, a.H = H, a.W = W;
+  a.Cin = l.CaS + l.CbS;
+  a.Cout = l.CoS;
+  a.Wf = d->wf.p + l.fOff;
+  a.Wb = d->wb.p + l.fOff;
+  a.bias = d->theta.p + l.bOff;
+  a.relu = l.relu ? 1 : 0;
+  a.upTmp = d->upTmp.p;
+  a.dW = d->grad.p + l.wOff;
+  a.dB = d->grad.p + l.bOff;
+  return a;
+}
+
+// Layer schedule: the conv inputs of every layer in network order.
+struct Step {
+  int layer;
+  const Act* s0;
+  bool up;
+  const Act* s1;
+  Act* out;  // null: the FP32 output
+};
+
+static std::vector<Step> schedule(ash_denoiser* d) {
+  std::vector<Step> s;
+  int li = 0;
+  const Act* h = &d->x;
+  for (int lv = 0; lv < kLevels; ++lv) {
+    if (lv) h = &d->pooled[lv];
+    s.push_back({li++, h, false, nullptr, &d->enc[lv][0]});
+    s.push_back({li++, &d->enc[lv][0], false, nullptr, &d->enc[lv][1]});
+    h = &d->enc[lv][1];
+  }
+  for (int lv = kLevels - 2; lv >= 0; --lv) {
+    s.push_back({li++, h, true, &d->enc[lv][1], &d->dec[lv][0]});
+    s.push_back({li++, &d->dec[lv][0], false, nullptr, &d->dec[lv][1]});
+    h = &d->dec[lv][1];
+  }
+  s.push_back({li++, h, false, nullptr, nullptr});
+  return s;
+}
+
+static int forwardPass(ash_denoiser* d) {
+  int rc = packWeights(d);
+  if (rc) return rc;
+  const uint32_t N = d->N;
+  auto sched = schedule(d);
+  for (const Step& st : sched) {
+    const Layer& l = d->L[st.layer];
+    const Act& o = st.out ? *st.out : d->out;
+    uint32_t H = o.H, W = o.W;
+    if (st.s0 != &d->x && !st.up && st.s0->H != H) return dfail(d, ASH_E_INVALID, "shape mismatch");
+    // the pooled input of an encoder level is produced just before its first conv
+    for (int lv = 1; lv < kLevels; ++lv)
+      if (st.s0 == &d->pooled[lv]) {
+        const Act& src = d->enc[lv - 1][1];
+        kPool<<<blocks(d->pooled[lv].count()), 256, 0, d->stream>>>(src.a.p, d->pooled[lv].a.p, N, src.H, src.W, src.C);
+      }
+    ConvArgs a = convArgs(d, l, *st.s0, st.up, st.s1, N, H, W);
+    if (a.Cin != a.s0.C + a.s1.C || a.Cout != o.C || a.Cin > 16 * kMaxTiles || a.Cout > 16 * kMaxTiles)
+      return dfail(d, ASH_E_INVALID, "layer %d: channel layout mismatch", st.layer);
+    if (st.out) a.Y = st.out->a.p;
+    else a.Yf = d->outF.p;
+    size_t P = size_t(N) * H * W;
+    kConvFwd<<<blocks(P, 64), 256, 0, d->stream>>>(a);
+  }
+  DH(d, hipGetLastError());
+  return ASH_OK;
+}
+
+static uint32_t splitsFor(size_t P) { return uint32_t(std::max<size_t>(1, std::min<size_t>(256, P / 2048))); }
+
+// Gradient of the loss w.r.t. all parameters into d->grad (after forwardPass
+// and the loss kernel wrote the output gradient into d->out.g).
+static int backwardPass(ash_denoiser* d) {
+  const uint32_t N = d->N;
+  DH(d, hipMemsetAsync(d->grad.p, 0, d->nParams * sizeof(float), d->stream));
+  auto zero = [&](Act& t) { return hipMemsetAsync(t.g.p, 0, t.count() * sizeof(float), d->stream); };
+  for (int lv = 0; lv < kLevels; ++lv) {
+    DH(d, zero(d->enc[lv][0]));
+    DH(d, zero(d->enc[lv][1]));
+    if (lv) DH(d, zero(d->pooled[lv]));
+    if (lv < kLevels - 1) {
+      DH(d, zero(d->dec[lv][0]));
+      DH(d, zero(d->dec[lv][1]));
+    }
+  }
+  auto sched = schedule(d);
+  for (int k = int(sched.size()) - 1; k >= 0; --k) {
+    const Step& st = sched[k];
+    const Layer& l = d->L[st.layer];
+    const Act& o = st.out ? *st.out : d->out;
+    const uint32_t H = o.H, W = o.W;
+    const size_t P = size_t(N) * H * W;
+    if (st.out && l.relu) kReluMask<<<blocks(o.count()), 256, 0, d->stream>>>(o.g.p, o.a.p, o.count());
+    ConvArgs a = convArgs(d, l, *st.s0, st.up, st.s1, N, H, W);
+    if (a.Cin != a.s0.C + a.s1.C || a.Cout != o.C) return dfail(d, ASH_E_INVALID, "layer %d: channel layout mismatch", st.layer);
+    a.dY = o.g.p;
+    a.splits = splitsFor(P);
+    size_t tasks = size_t(9) * (a.Cout / 32) * (a.Cin / 32) * a.splits;
+    kConvBwdW<<<blocks(tasks, 4), 256, 0, d->stream>>>(a);
+    kBiasGrad<<<blocks(P, 256), 256, 0, d->stream>>>(a.dY, uint32_t(P), a.Cout, a.dB);
+    if (st.s0 == &d->x) continue;  // no gradient for the input features
+    if (st.up) DH(d, hipMemsetAsync(d->upTmp.p, 0, P * st.s0->C * sizeof(float), d->stream));
+    kConvBwdX<<<blocks(P, 64), 256, 0, d->stream>>>(a);
+    if (st.up)
+      kUpBwd<<<blocks(st.s0->count()), 256, 0, d->stream>>>(d->upTmp.p, st.s0->g.p, N, H, W, st.s0->C);
+    for (int lv = 1; lv < kLevels; ++lv)
+      if (st.s0 == &d->pooled[lv]) {
+        Act& e = d->enc[lv - 1][1];
+        kPoolBwd<<<blocks(e.count()), 256, 0, d->stream>>>(d->pooled[lv].g.p, e.g.p, N, e.H, e.W, e.C);
+      }
+  }
+  DH(d, hipGetLastError());
+  return ASH_OK;
+}
+
+static int uploadBatch(ash_denoiser* d, uint32_t n, uint32_t h, uint32_t w, const float* input, const float* target) {
+  int rc = ensureShape(d, n, h, w);
+  if (rc) return rc;
+  size_t P = size_t(n) * h * w;
+  if (d->raw.n < P * kInCh) DH(d, d->raw.alloc(P * kInCh));
+  DH(d, hipMemcpyAsync(d->raw.p, input, P * kInCh * sizeof(float), hipMemcpyHostToDevice, d->stream));
+  kFeatures<<<blocks(P * 32), 256, 0, d->stream>>>(d->raw.p, d->x.a.p, uint32_t(P));
+  if (target) {
+    if (d->target.n < P * kOutCh) DH(d, d->target.alloc(P * kOutCh));
+    DH(d, hipMemcpyAsync(d->target.p, target, P * kOutCh * sizeof(float), hipMemcpyHostToDevice, d->stream));
+  }
+  DH(d, hipGetLastError());
+  return ASH_OK;
+}
+
+static int lossAndBackward(ash_denoiser* d, float* loss) {
+  size_t P = size_t(d->N) * d->H * d->W;
+  unsigned nb = blocks(P * 32);
+  kLoss<<<nb, 256, 0, d->stream>>>(d->outF.p, 32, d->target.p, uint32_t(P), d->out.g.p, d->lossPart.p);
+  std::vector<float> part(nb);
+  int rc = backwardPass(d);
+  if (rc) return rc;
+  DH(d, hipMemcpyAsync(part.data(), d->lossPart.p, nb * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+  DH(d, hipStreamSynchronize(d->stream));
+  double s = 0;
+  for (float v : part) s += v;
+  if (loss) *loss = float(s / (3.0 * double(P)));
+  return ASH_OK;
+}
+
+extern "C" {
+
+int ash_denoiser_create(int device, uint32_t seed, ash_denoiser** out) {
+  if (!out) return ASH_E_INVALID;
+  *out = nullptr;
+  int nd = 0;
+  if (hipGetDeviceCount(&nd) != hipSuccess || device < 0 || device >= nd) return ASH_E_HIP;
+  ash_denoiser* d = new ash_denoiser();
+  d->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete d;
+    return ASH_E_HIP;
+  }
+  buildLayers(d);
+  if (d->theta.alloc(d->nParams) || d->grad.alloc(d->nParams) || d->m.alloc(d->nParams) || d->v.alloc(d->nParams) ||
+      d->wf.alloc(d->nPacked) || d->wb.alloc(d->nPacked)) {
+    ash_denoiser_destroy(d);
+    return ASH_E_NOMEM;
+  }
+  // He-normal initialisation over the real fan-in, zero bias
+  std::mt19937_64 rng(seed ? seed : 1);
+  std::normal_distribution<double> nd01(0.0, 1.0);
+  std::vector<float> real(realCount(d));
+  size_t k = 0;
+  for (auto& l : d->L) {
+    size_t fan = size_t(9) * (l.ca + l.cb);
+    double sd = std::sqrt((l.relu ? 2.0 : 1.0) / double(fan));
+    for (size_t i = 0; i < fan * l.co; ++i) real[k++] = float(nd01(rng) * sd);
+    for (uint32_t i = 0; i < l.co; ++i) real[k++] = 0.f;
+  }
+  int rc = ash_denoiser_set_params(d, real.data());
+  if (rc) {
+    ash_denoiser_destroy(d);
+    return rc;
+  }
+  *out = d;
+  return ASH_OK;
+}
+
+void ash_denoiser_destroy(ash_denoiser* d) {
+  if (!d) return;
+  (void)hipSetDevice(d->device);
+  if (d->stream) (void)hipStreamSynchronize(d->stream);
+  for (auto* b : {&d->theta, &d->grad, &d->m, &d->v, &d->upTmp, &d->outF, &d->lossPart, &d->raw, &d->target, &d->rgb})
+    b->release();
+  d->wf.release(), d->wb.release();
+  auto rel = [](Act& t) { t.a.release(), t.g.release(); };
+  rel(d->x), rel(d->out);
+  for (int lv = 0; lv < kLevels; ++lv) {
+    rel(d->enc[lv][0]), rel(d->enc[lv][1]), rel(d->pooled[lv]), rel(d->dec[lv][0]), rel(d->dec[lv][1]);
+  }
+  if (d->stream) (void)hipStreamDestroy(d->stream);
+  delete d;
+}
+
+const char* ash_denoiser_last_error(ash_denoiser* d) { return d ? d->err.c_str() : "null denoiser"; }
+
+int ash_denoiser_num_params(ash_denoiser* d, uint64_t* n) {
+  if (!d || !n) return ASH_E_INVALID;
+  *n = realCount(d);
+  return ASH_OK;
+}
+
+int ash_denoiser_get_params(ash_denoiser* d, float* host) {
+  if (!d || !host) return ASH_E_INVALID;
+  std::vector<float> pad(d->nParams);
+  DH(d, hipSetDevice(d->device));
+  DH(d, hipMemcpyAsync(pad.data(), d->theta.p, d->nParams * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+  DH(d, hipStreamSynchronize(d->stream));
+  fromPadded(d, pad, host);
+  return ASH_OK;
+}
+
+int ash_denoiser_set_params(ash_denoiser* d, const float* host) {
+  if (!d || !host) return ASH_E_INVALID;
+  std::vector<float> pad;
+  toPadded(d, host, pad);
+  DH(d, hipSetDevice(d->device));
+  DH(d, hipMemcpyAsync(d->theta.p, pad.data(), d->nParams * sizeof(float), hipMemcpyHostToDevice, d->stream));
+  DH(d, hipMemsetAsync(d->m.p, 0, d->nParams * sizeof(float), d->stream));
+  DH(d, hipMemsetAsync(d->v.p, 0, d->nParams * sizeof(float), d->stream));
+  DH(d, hipStreamSynchronize(d->stream));
+  d->step = 0;
+  d->wDirty = true;
+  return ASH_OK;
+}
+
+int ash_denoiser_forward(ash_denoiser* d, uint32_t n, uint32_t h, uint32_t w, const float* input, float* out) {
+  if (!d || !input || !out) return ASH_E_INVALID;
+  DH(d, hipSetDevice(d->device));
+  int rc = uploadBatch(d, n, h, w, input, nullptr);
+  if (!rc) rc = forwardPass(d);
+  if (rc) return rc;
+  size_t P = size_t(n) * h * w;
+  if (d->rgb.n < P * 3) DH(d, d->rgb.alloc(P * 3));
+  kOutRgb<<<blocks(P * 3), 256, 0, d->stream>>>(d->outF.p, 32, uint32_t(P), d->rgb.p);
+  DH(d, hipMemcpyAsync(out, d->rgb.p, P * 3 * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+  DH(d, hipStreamSynchronize(d->stream));
+  return ASH_OK;
+}
+
+int ash_denoiser_gradient(ash_denoiser* d, uint32_t n, uint32_t h, uint32_t w, const float* input, const float* target,
+                          float* loss, float* grad) {
+  if (!d || !input || !target || !grad) return ASH_E_INVALID;
+  DH(d, hipSetDevice(d->device));
+  int rc = uploadBatch(d, n, h, w, input, target);
+  if (!rc) rc = forwardPass(d);
+  if (!rc) rc = lossAndBackward(d, loss);
+  if (rc) return rc;
+  std::vector<float> pad(d->nParams);
+  DH(d, hipMemcpyAsync(pad.data(), d->grad.p, d->nParams * sizeof(float), hipMemcpyDeviceToHost, d->stream));
+  DH(d, hipStreamSynchronize(d->stream));
+  fromPadded(d, pad, grad);
+  return ASH_OK;
+}
+
+int ash_denoiser_train_step(ash_denoiser* d, uint32_t n, uint32_t h, uint32_t w, const float* input,
+                            const float* target, float lr, float* loss) {
+  if (!d || !input || !target || !(lr > 0.f)) return ASH_E_INVALID;
+  DH(d, hipSetDevice(d->device));
+  int rc = uploadBatch(d, n, h, w, input, target);
+  if (!rc) rc = forwardPass(d);
+  if (!rc) rc = lossAndBackward(d, loss);
+  if (rc) return rc;
+  ++d->step;
+  const float b1 = 0.9f, b2 = 0.999f;
+  float c1 = 1.f - std::pow(b1, float(d->step)), c2 = 1.f - std::pow(b2, float(d->step));
+  kAdam<<<blocks(d->nParams), 256, 0, d->stream>>>(d->theta.p, d->grad.p, d->m.p, d->v.p, d->nParams, lr, b1, b2,
+                                                  1e-8f, c1, c2);
+  DH(d, hipGetLastError());
+  d->wDirty = true;
+  DH(d, hipStreamSynchronize(d->stream));
+  return ASH_OK;
+}
+
+int ash_denoise_film(ash_denoiser* d, ash_ctx* ctx, int32_t albedoCh, int32_t normalCh, int32_t depthCh, int32_t outImage,
+                     double* ms) {
+  if (!d || !ctx || outImage < 0 || outImage >= ASH_NUM_OUTPUT_IMAGES) return ASH_E_INVALID;
+  uint32_t W = 0, H = 0;
+  ashd::ctxFilm(ctx, &W, &H);
+  if (!W || !H) return dfail(d, ASH_E_STATE, "context has no film");
+  auto plane = [&](int32_t ch) -> const float4* {
+    if (ch < 0 || ch >= ASH_NUM_OUTPUT_IMAGES - 1) return nullptr;
+    void* p = nullptr;
+    return ash_output_device_ptr(ctx, ch + 1, &p) == ASH_OK ? static_cast<const float4*>(p) : nullptr;
+  };
+  void* rad = nullptr;
+  void* outp = nullptr;
+  if (ash_output_device_ptr(ctx, 0, &rad) || ash_output_device_ptr(ctx, outImage, &outp))
+    return dfail(d, ASH_E_STATE, "film planes unavailable");
+  DH(d, hipSetDevice(d->device));
+  const uint32_t Hp = (H + 15) & ~15u, Wp = (W + 15) & ~15u;
+  int rc = ensureShape(d, 1, Hp, Wp);
+  if (rc) return rc;
+  hipStream_t s = ashd::ctxStream(ctx);
+  hipEvent_t e0, e1, ready;
+  DH(d, hipEventCreate(&e0));
+  DH(d, hipEventCreate(&e1));
+  DH(d, hipEventCreate(&ready));
+  // the denoiser works on its own stream after the context's pending renders
+  DH(d, hipEventRecord(ready, s));
+  DH(d, hipStreamWaitEvent(d->stream, ready, 0));
+  DH(d, hipEventRecord(e0, d->stream));
+  kFeaturesFilm<<<blocks(size_t(Wp) * Hp), 256, 0, d->stream>>>(static_cast<const float4*>(rad), plane(albedoCh),
+                                                               plane(normalCh), plane(depthCh), W, H, Wp, Hp, d->x.a.p);
+  rc = forwardPass(d);
+  if (rc) return rc;
+  kFilmOut<<<blocks(size_t(W) * H), 256, 0, d->stream>>>(d->outF.p, 32, W, H, Wp, static_cast<float4*>(outp));
+  DH(d, hipGetLastError());
+  DH(d, hipEventRecord(e1, d->stream));
+  DH(d, hipEventSynchronize(e1));
+  float t = 0;
+  DH(d, hipEventElapsedTime(&t, e0, e1));
+  if (ms) *ms = t;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipEventDestroy(ready);
+  return ASH_OK;
+}
+
+}  // extern "C"

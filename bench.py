@@ -112,22 +112,23 @@ def main():
     dch = scene.state.depth_out_channel
 
     tune = None
-    qp, bpc = args.queue_paths, 0
+    qp, bpc, lds = args.queue_paths, 0, 0
     if not args.no_autotune and not args.queue_paths:
         opts, tune = r.autotune(list(cams), list(pcs), probe_frames=1)
-        qp, bpc = opts.queue_paths, opts.blocks_per_cu
+        qp, bpc, lds = opts.queue_paths, opts.blocks_per_cu, opts.lds_stack
         log(f"[bench] autotune: {tune}")
     graphs = not args.no_graphs
 
     def step():
         if cfg_id == 2:
-            return r.render_shots(list(cams), list(pcs), spp, queue_paths=qp, blocks_per_cu=bpc, graphs=graphs)
+            return r.render_shots(list(cams), list(pcs), spp, queue_paths=qp, blocks_per_cu=bpc, graphs=graphs,
+                                  lds_stack=lds)
         r.set_film(w, h)
         return r.render(cams[0], pcs[0], 0, spp, queue_paths=qp, blocks_per_cu=bpc, depth_channel=dch,
-                        graphs=graphs)
+                        graphs=graphs, lds_stack=lds)
 
     # calibration: traversal work per ray (not timed)
-    cal = r.render(cams[0], pcs[0], 0, 1, queue_paths=args.queue_paths, count_traversal=True, depth_channel=dch)
+    cal = r.render(cams[0], pcs[0], 0, 1, count_traversal=True, depth_channel=dch)
     ext_rays = max(1, cal["extension_rays"])
     nodes_per_ray = cal["ext_node_visits"] / ext_rays
     tris_per_ray = cal["ext_tri_tests"] / ext_rays
@@ -171,10 +172,11 @@ def main():
 
     # per-kernel timing pass (HIP events around every launch; not the timed steps)
     if cfg_id == 2:
-        tp = r.render_shots(list(cams), list(pcs), spp, queue_paths=qp, blocks_per_cu=bpc, timing=True)
+        tp = r.render_shots(list(cams), list(pcs), spp, queue_paths=qp, blocks_per_cu=bpc, timing=True, lds_stack=lds)
     else:
         r.set_film(w, h)
-        tp = r.render(cams[0], pcs[0], 0, spp, queue_paths=qp, blocks_per_cu=bpc, timing=True, depth_channel=dch)
+        tp = r.render(cams[0], pcs[0], 0, spp, queue_paths=qp, blocks_per_cu=bpc, timing=True, depth_channel=dch,
+                      lds_stack=lds)
     ext_ms_per_launch = tp["extend_ms"] / max(1, tp["extend_launches"])
     rays_per_launch = tp["extension_rays"] / max(1, tp["extend_launches"])
     bytes_per_launch = rays_per_launch * bytes_per_ray

@@ -262,7 +262,7 @@ typedef struct ash_render_opts {
   uint32_t max_bounce_iters; /* safety cap on wavefront iterations (0 = auto) */
   uint32_t count_traversal;  /* 1 = count node visits / triangle tests (slower) */
   uint32_t no_graphs;        /* 1 = launch the bounce loop eagerly instead of replaying a HIP graph */
-  uint32_t pad;
+  uint32_t lds_stack;        /* closest-hit stack entries per thread held in LDS: 8 or 16 (0 = 16) */
 } ash_render_opts;
 
 /* ------------------------------------------------------------ error codes */
@@ -316,20 +316,21 @@ int ash_read_shot_output(ash_ctx* ctx, uint32_t shot, int32_t image, float* host
 
 /* Autotuner (no counterpart in the reference, whose dispatch shape is fixed
  * at one thread per pixel): sweeps the wavefront batch size queue_paths over
- * 2^18..2^22 and the persistent traversal grid blocks_per_cu over {4, 8, 16}
- * on a probe render of `probe_frames` frames of the given shots (each
+ * 2^18..2^22, the persistent traversal grid blocks_per_cu over {4, 8, 16} and
+ * the LDS share of the closest-hit stack over {8, 16} on a probe render of `probe_frames` frames of the given shots (each
  * candidate rendered twice, the second timed), then writes the fastest pair
  * into `inout`.  The probe renders overwrite the shot outputs. */
 typedef struct ash_tune_result {
   uint32_t queue_paths;
   uint32_t blocks_per_cu;
   uint32_t candidates;
-  uint32_t pad;
+  uint32_t lds_stack;
   double best_ms;  /* probe render time of the chosen pair */
   double worst_ms; /* slowest candidate */
 } ash_tune_result;
 int ash_autotune(ash_ctx* ctx, uint32_t n_shots, const GpuCamera* cams, const GpuPushConstantRaytrace* pcs,
                  int32_t probe_frames, ash_render_opts* inout, ash_tune_result* result);
+/* The three parameters are written into `inout` (queue_paths, blocks_per_cu, lds_stack). */
 
 /* Copy output image `image` (0..8) to host RGBA32F (w*h*4 floats). */
 int ash_read_output(ash_ctx* ctx, int32_t image, float* host_rgba);

@@ -1,0 +1,87 @@
+"""Batching, HIP-graph replay and the autotuner change how the work is
+launched, never the estimator: every form must reproduce the plain eager
+render bit for bit (the RNG is keyed by pixel and frame only, and every pixel
+folds its frames in order)."""
+import numpy as np
+import pytest
+
+import asuna_amd as A
+
+pytestmark = pytest.mark.gpu
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+@pytest.fixture(scope="module")
+def renderer():
+    r = A.Renderer(0)
+    yield r
+    r.close()
+
+
+def _load(r, path):
+    s = A.HostScene(path)
+    r.upload(s)
+    r.build()
+    r.set_film(*s.film)
+    return s
+
+
+def _render(r, s, frames, **kw):
+    cam, pc = s.shot(0)
+    r.set_film(*s.film)
+    st = r.render(cam, pc, 0, frames, **kw)
+    return [r.read(i) for i in (0, 1, 8)], st
+
+
+def test_graph_replays_equal_eager(renderer, spheres_small):
+    """Each replay of the captured bounce loop, with eager work and
+    synchronisations between replays, equals the eager render."""
+    s = _load(renderer, spheres_small)
+    ref, st0 = _render(renderer, s, 6, graphs=False, queue_paths=1 << 16)
+    assert st0["graph_launches"] == 0 and st0["batches"] > 1
+    for rep in range(3):
+        got, st = _render(renderer, s, 6, queue_paths=1 << 16)
+        log(f"replay {rep}: batches {st['batches']}, graph launches {st['graph_launches']}")
+        assert st["graph_launches"] == st["batches"]
+        for a, b in zip(ref, got):
+            np.testing.assert_array_equal(a, b)
+        _render(renderer, s, 1, graphs=False)  # eager work in between
+    assert st["extension_rays"] == st0["extension_rays"] and st["shadow_rays"] == st0["shadow_rays"]
+
+
+def test_batch_plans_are_equivalent(renderer, multishot_scene):
+    """64 shots x 2 frames split three ways: several frames per batch, groups
+    of whole shots per batch, and pixel runs of one shot per batch."""
+    s = _load(renderer, multishot_scene)
+    n = s.state.num_shots
+    cams, pcs = zip(*[s.shot(i) for i in range(n)])
+    w, h = s.film
+    outs = []
+    for q in (1 << 26, 1 << 22, 1 << 18, 1 << 16):
+        st = renderer.render_shots(list(cams), list(pcs), 2, queue_paths=q)
+        log(f"queue {q}: {st['batches']} batches, {st['render_ms']:.1f} ms")
+        outs.append([renderer.read_shot(i, img) for i in (0, 31, 63) for img in (0, 8)])
+        assert st["paths"] == n * w * h * 2
+    for o in outs[1:]:
+        for a, b in zip(outs[0], o):
+            np.testing.assert_array_equal(a, b)
+
+
+def test_autotune_picks_a_candidate_and_keeps_results(renderer, multishot_scene):
+    s = _load(renderer, multishot_scene)
+    cams, pcs = zip(*[s.shot(i) for i in range(8)])
+    opts, rep = renderer.autotune(list(cams), list(pcs), probe_frames=1)
+    log(f"autotune: {rep}")
+    assert rep["candidates"] == 30
+    assert opts.queue_paths in [1 << k for k in range(18, 23)] and opts.blocks_per_cu in (4, 8, 16)
+    assert opts.lds_stack in (8, 16) and rep["lds_stack"] == opts.lds_stack
+    assert 0 < rep["best_ms"] <= rep["worst_ms"]
+    renderer.render_shots(list(cams), list(pcs), 2, queue_paths=opts.queue_paths, blocks_per_cu=opts.blocks_per_cu,
+                          lds_stack=opts.lds_stack)
+    tuned = [renderer.read_shot(i, 0) for i in range(8)]
+    renderer.render_shots(list(cams), list(pcs), 2, graphs=False)
+    for i in range(8):
+        np.testing.assert_array_equal(tuned[i], renderer.read_shot(i, 0))
