@@ -257,7 +257,7 @@ def lib():
         "ash_read_image": (C.c_int, [C.c_char_p, C.c_float, P(C.c_uint32), P(C.c_uint32),
                                      P(P(C.c_float))]),
         "ash_free_image": (None, [P(C.c_float)]),
-        "ash_tonemap": (None, [C.c_uint32, C.c_uint32, P(C.c_float)]),
+        "ash_tonemap": (None, [C.c_uint32, C.c_uint32, C.c_uint32, P(C.c_float)]),
         "ash_env_tables_host": (None, [C.c_uint32, C.c_uint32, P(C.c_float), P(C.c_float), P(C.c_float)]),
         "ash_generate_scene": (C.c_int, [C.c_int32, C.c_uint64, C.c_int32, C.c_char_p, C.c_char_p,
                                          C.c_size_t]),

@@ -140,7 +140,7 @@ int ash_read_image(const char* path, float gamma, uint32_t* w, uint32_t* h, floa
 
 void ash_free_image(float* p) { delete[] p; }
 
-void ash_tonemap(uint32_t type, uint32_t n, float* rgba) { ash::toneMap(type, n, rgba); }
+void ash_tonemap(uint32_t type, uint32_t w, uint32_t h, float* rgba) { ash::toneMap(type, w, h, rgba); }
 
 void ash_env_tables_host(uint32_t w, uint32_t h, const float* rgba, float* marg, float* cond) {
   std::vector<float> m, c;
@@ -210,7 +210,7 @@ int ash_run_offline(const char* scenePath, const char* outName, const ash_offlin
       if (!ash::writeImage(name, W, H, px.data(), e)) return bail(ASH_E_IO);
     } else {
       std::snprintf(name, sizeof name, "%s_shot_%04u.png", out.c_str(), shot);
-      ash::toneMap(state.tmType, size_t(W) * H, px.data());
+      ash::toneMap(state.tmType, W, H, px.data());
       if (!ash::writeImage(name, W, H, px.data(), e)) return bail(ASH_E_IO);
     }
     std::vector<int> valid;  // tracer.cpp:344-392 (--output_scanline)

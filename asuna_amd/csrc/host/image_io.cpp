@@ -723,11 +723,60 @@ bool writeNpy(const std::string& path, const std::vector<unsigned long>& shape, 
   return true;
 }
 
-// post.idle.frag:71-134 tone mapping operators (custom without the dither).
-void toneMap(uint32_t type, size_t n, float* rgba) {
+// Uncharted 2 curve, utils/tonemapping.glsl:43-54
+static float uncharted2(float x) {
+  const float A = 0.15f, B = 0.50f, C = 0.10f, D = 0.20f, E = 0.02f, F = 0.30f;
+  return ((x * (A * x + C * B) + D * E) / (x * (A * x + B) + D * F)) - E / F;
+}
+
+// pcg3d integer hash, utils/random.glsl:76-86
+static void pcg3d(uint32_t v[3]) {
+  for (int k = 0; k < 3; ++k) v[k] = v[k] * 1664525u + 1013904223u;
+  v[0] += v[1] * v[2];
+  v[1] += v[2] * v[0];
+  v[2] += v[0] * v[1];
+  for (int k = 0; k < 3; ++k) v[k] ^= v[k] >> 16;
+  v[0] += v[1] * v[2];
+  v[1] += v[2] * v[0];
+  v[2] += v[0] * v[1];
+}
+
+// The custom operator with the post state the loader leaves at its
+// defaults (state.h:47-58; loader.cpp:199-227 reads only the type):
+// exposure avgLum = 1, Uncharted 2 + gamma (TONEMAP_UNCHARTED), dither to
+// 1/255 with pcg3d noise of the pixel, contrast / brightness / saturation 1,
+// vignette 0, no auto exposure.  post.idle.frag:108-131.
+static void toneMapCustom(uint32_t x, uint32_t y, float* c) {
+  const float g = 2.2f, ig = 1.0f / 2.2f, quant = 1.0f / 255.0f;
+  const float white = 1.0f / uncharted2(11.2f);
+  uint32_t r[3] = {x, y, 0u};
+  pcg3d(r);
+  for (int k = 0; k < 3; ++k) {
+    float t = std::pow(uncharted2(c[k] * 1.0f * 2.0f) * white, ig);  // toneMap(hdr, avgLum)
+    float lin = std::pow(t, g);                                       // sRGBToLinear
+    uint32_t bits = 0x3f800000u | (r[k] >> 9);
+    float noise;
+    std::memcpy(&noise, &bits, 4);
+    noise -= 1.0f;
+    float c0 = std::floor(std::pow(lin, ig) / quant) * quant;  // dither()
+    float c1 = c0 + quant;
+    float discr = std::pow(c0, g) * (1.0f - noise) + std::pow(c1, g) * noise;
+    float v = discr < lin ? c1 : c0;
+    v = std::min(1.0f, std::max(0.0f, v));  // contrast 1 then clamp; brightness, saturation 1; vignette 0
+    c[k] = v;
+  }
+}
+
+// post.idle.frag:71-134 tone mapping operators; pixel (x, y) = index % w, / w.
+void toneMap(uint32_t type, uint32_t w, uint32_t h, float* rgba) {
   const float invGamma = 1.0f / 2.2f;
+  const size_t n = size_t(w) * h;
   for (size_t i = 0; i < n; ++i) {
     float* c = rgba + i * 4;
+    if (type == 6) {
+      toneMapCustom(uint32_t(i % w), uint32_t(i / w), c);
+      continue;
+    }
     for (int k = 0; k < 3; ++k) {
       float x = c[k];
       float y = x;

@@ -93,5 +93,5 @@ bool readImage(const std::string& path, float gamma, Image& out, std::string& er
 bool writeImage(const std::string& path, uint32_t w, uint32_t h, const float* rgba, std::string& err);
 bool writeNpy(const std::string& path, const std::vector<unsigned long>& shape, const float* data,
               std::string& err);
-void toneMap(uint32_t type, size_t n, float* rgba);
+void toneMap(uint32_t type, uint32_t width, uint32_t height, float* rgba);
 }  // namespace ash

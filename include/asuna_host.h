@@ -86,9 +86,10 @@ void ash_free_image(float* rgba);
 void ash_env_tables_host(uint32_t width, uint32_t height, const float* rgba, float* marginal,
                          float* conditional);
 
-/* post.idle.frag tone mapping (types none..pbrt; custom without dither) of an
- * RGBA32F image in place. */
-void ash_tonemap(uint32_t tm_type, uint32_t n_pixels, float* rgba);
+/* post.idle.frag tone mapping of a width x height RGBA32F image in place
+ * (types none, gamma, reinhard, aces, filmic, pbrt, custom; custom with the
+ * loader's default post state: no auto exposure). */
+void ash_tonemap(uint32_t tm_type, uint32_t width, uint32_t height, float* rgba);
 
 /* Procedural benchmark scenes written as JSON + OBJ (+ env map) into dir:
  *   config 0: closed box, 5 walls + 2 boxes + 0.25 x 0.25 rect light, 512^2,

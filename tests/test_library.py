@@ -200,9 +200,56 @@ def test_image_io_round_trips(tmp_path):
 def test_tonemap_filmic_closed_form():
     x = np.array([[0.0, 0.5, 2.0, 1.0]], np.float32)
     y = x.copy()
-    A.lib().ash_tonemap(4, 1, A.fptr(y))
+    A.lib().ash_tonemap(4, 1, 1, A.fptr(y))
     v = np.maximum(x[0, :3] - 0.004, 0)
     np.testing.assert_allclose(y[0, :3], (v * (6.2 * v + 0.5)) / (v * (6.2 * v + 1.7) + 0.06), rtol=1e-6)
+
+
+def _pcg3d(v):
+    v = [(x * 1664525 + 1013904223) & 0xFFFFFFFF for x in v]
+    m = 0xFFFFFFFF
+    v[0] = (v[0] + v[1] * v[2]) & m
+    v[1] = (v[1] + v[2] * v[0]) & m
+    v[2] = (v[2] + v[0] * v[1]) & m
+    v = [x ^ (x >> 16) for x in v]
+    v[0] = (v[0] + v[1] * v[2]) & m
+    v[1] = (v[1] + v[2] * v[0]) & m
+    v[2] = (v[2] + v[0] * v[1]) & m
+    return v
+
+
+def test_tonemap_custom_is_dithered_uncharted2():
+    """Custom operator: Uncharted 2 (Hable 2010, exposure bias 2, white 11.2)
+    then gamma 2.2, dithered to one of the two neighbouring 1/255 levels,
+    chosen by pcg3d noise of (x, y, 0) (Jarzynski & Olano 2020)."""
+    w, h = 7, 5
+    rng = np.random.default_rng(3)
+    img = np.concatenate([rng.uniform(0, 4, (h, w, 3)), np.ones((h, w, 1))], -1).astype(np.float32)
+    out = img.copy()
+    A.lib().ash_tonemap(6, w, h, A.fptr(out))
+
+    def u2(x):
+        a, b, c, d, e, f = 0.15, 0.50, 0.10, 0.20, 0.02, 0.30
+        return (x * (a * x + c * b) + d * e) / (x * (a * x + b) + d * f) - e / f
+
+    t = (u2(2.0 * img[..., :3].astype(np.float64)) / u2(11.2)) ** (1 / 2.2)
+    lo = np.floor(t * 255 + 1e-9) / 255
+    got = out[..., :3].astype(np.float64)
+    # every value is one of the two 1/255 levels around the tone-mapped value
+    assert np.all((np.abs(got - lo) < 2e-6) | (np.abs(got - (lo + 1 / 255)) < 2e-6) | (np.abs(got - t) < 2e-6))
+    # and the upper level is taken exactly when the noise falls below the
+    # linear-space position of the value between the two levels
+    for y in range(h):
+        for x in range(w):
+            r = _pcg3d([x, y, 0])
+            for k in range(3):
+                noise = np.frombuffer(np.uint32(0x3F800000 | (r[k] >> 9)).tobytes(), np.float32)[0] - 1.0
+                l0, l1 = lo[y, x, k], lo[y, x, k] + 1 / 255
+                lin = t[y, x, k] ** 2.2
+                frac = (lin - l0 ** 2.2) / (l1 ** 2.2 - l0 ** 2.2)
+                if abs(frac - noise) > 1e-4:  # away from the decision boundary
+                    assert abs(got[y, x, k] - (l1 if noise < frac else l0)) < 2e-6
+    np.testing.assert_array_equal(out[..., 3], 1.0)
 
 
 def test_scene_generator_is_deterministic(tmp_path):
