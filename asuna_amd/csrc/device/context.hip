@@ -104,7 +104,7 @@ struct ash_ctx {
   DBuf<float4> wRo, wRd, wThr, wRad, wLd, wHitF, wShO, wShD;
   DBuf<uint4> wInfo;
   DBuf<int4> wHitI;
-  DBuf<int> wQ0, wQ1, wSpill;
+  DBuf<int> wQ0, wQ1, wQ2, wSpill;
   DBuf<float> wSpillT;
   DBuf<uint32_t> wCtr;
   DBuf<unsigned long long> wCnt64;
@@ -122,7 +122,7 @@ struct ash_ctx {
       b->release();
     inst.release(), idx.release(), instTriOffset.release(), mats.release(), lights.release(), texTab.release();
     marg.release(), cond.release(), shotTab.release(), wInfo.release(), wHitI.release();
-    wQ0.release(), wQ1.release(), wSpill.release(), wSpillT.release(), wCtr.release(), wCnt64.release();
+    wQ0.release(), wQ1.release(), wQ2.release(), wSpill.release(), wSpillT.release(), wCtr.release(), wCnt64.release();
     unitBuf.release();
     for (auto& g : graphs) releaseGraph(g);
     if (stream) (void)hipStreamDestroy(stream);
@@ -214,6 +214,7 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   MALLOC(c, c->wHitI, cap);
   MALLOC(c, c->wQ0, cap);
   MALLOC(c, c->wQ1, cap);
+  MALLOC(c, c->wQ2, cap);
   MALLOC(c, c->wCtr, 16);
   MALLOC(c, c->wCnt64, 8);
   Wavefront& w = c->wf;
@@ -228,6 +229,7 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   w.hitI = c->wHitI.p;
   w.queue[0] = c->wQ0.p;
   w.queue[1] = c->wQ1.p;
+  w.queue[2] = c->wQ2.p;
   w.shO = c->wShO.p;
   w.shD = c->wShD.p;
   w.ctr = c->wCtr.p;
@@ -592,8 +594,7 @@ int ash_upload_scene(ash_ctx* c, const ash_scene_desc* d) {
     if (id.light_id < 0) {
       if (id.material_id >= d->num_materials) return fail(c, ASH_E_INVALID, "instance %u: material id out of range", i);
       uint32_t t = d->materials[id.material_id].type;
-      if (t != MaterialTypeBrdfLambertian && t != MaterialTypeBrdfPbrMetalnessRoughness &&
-          t != MaterialTypeBrdfRoughPlastic && t != MaterialTypeBsdfDielectric && t != MaterialTypeBrdfEmissive)
+      if (t >= MaterialTypeNum || t == MaterialTypeBrdfDisney)
         return fail(c, ASH_E_UNSUPPORTED, "instance %u: material type %u is outside the supported plugin set", i, t);
     }
     DevInstance& di = hinst[i];

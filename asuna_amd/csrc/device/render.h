@@ -45,11 +45,11 @@ struct Wavefront {
   uint4* info;   // x seed, y depth | bsdfFlags << 16, z pixel, w shot | (frame == 0) << 16
   float4* hitF;  // t, u, v
   int4* hitI;    // instance (-1 miss), primitive, global triangle id, leaf position
-  int* queue[2];
+  int* queue[3];   // [0], [1] extension queues (alternating), [2] second-group shading queue
   float4* shO;  // shadow rays: origin.xyz, tmax
   float4* shD;  // direction.xyz, path id
   uint32_t* ctr;  // [0],[1] queue counts, [2] shadow count, [3] extend fetch,
-                  // [4] shadow fetch
+                  // [4] shadow fetch, [8] second-group shading queue count
   int* spill;       // traversal stack spill: spillCap entries per persistent thread
   float* spillT;    // entry distances of the spilled entries (4-wide closest hit)
   uint32_t spillThreads;

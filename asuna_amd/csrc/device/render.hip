@@ -797,6 +797,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadowWide(SceneView sc, Wavefro
 // ----------------------------------------------------------------- shade
 struct Hit {
   int lightId;
+  int inst;
   float uvx, uvy;
   f3 pos, V, N, ffN, X, Y;
   uint32_t mat;
@@ -826,6 +827,7 @@ __device__ __forceinline__ void hitState(const SceneView& sc, int inst, int prim
   h.N = makeNormal(nw);
   h.V = makeNormal(-rayDir);
   h.mat = in.material;
+  h.inst = inst;
   if (useFaceNormal == 1) h.N = mk1(0.f);  // geoN is never assigned (rchit_layouts.glsl:62)
   basis(h.N, h.X, h.Y);
   h.ffN = dot(h.N, h.V) > 0.0f ? h.N : -h.N;
@@ -1073,24 +1075,110 @@ __device__ float rpPdf(f3 L, f3 V, f3 N, f3 X, f3 Y, float ax, float ay, float e
   return pdf;
 }
 
-// Surface plugins (lambertian, pbr_metalness_roughness, rough_plastic,
-// dielectric).  Every closest-hit shader of the set runs the same three
-// steps -- material inputs (textures, normal map, AOVs, the pbr opacity
-// draw), one light sample through sampleLights, then its BSDF value toward
-// the light and its BSDF sample -- and the light sample sits at the same
-// point of the RNG stream in each.  The light sample is therefore taken once
-// for the whole wave instead of once per plugin present in it.
+// Conductor Fresnel after Shirley (brdf_conductor.rchit:14-34).
+__device__ __forceinline__ float conductorR(float eta, float k, float cosThetaI) {
+  float cosThetaISq = cosThetaI * cosThetaI;
+  float sinThetaISq = fmaxf(1.0f - cosThetaISq, 0.0f);
+  float sinThetaIQu = sinThetaISq * sinThetaISq;
+  float innerTerm = eta * eta - k * k - sinThetaISq;
+  float aSqPlusBSq = sqrtf(fmaxf(innerTerm * innerTerm + 4.0f * eta * eta * k * k, 0.0f));
+  float a = sqrtf(fmaxf((aSqPlusBSq + innerTerm) * 0.5f, 0.0f));
+  float Rs = ((aSqPlusBSq + cosThetaISq) - (2.0f * a * cosThetaI)) / ((aSqPlusBSq + cosThetaISq) + (2.0f * a * cosThetaI));
+  float Rp = ((cosThetaISq * aSqPlusBSq + sinThetaIQu) - (2.0f * a * cosThetaI * sinThetaISq)) /
+             ((cosThetaISq * aSqPlusBSq + sinThetaIQu) + (2.0f * a * cosThetaI * sinThetaISq));
+  return 0.5f * (Rs + Rs * Rp);
+}
+__device__ __forceinline__ f3 conductorR3(f3 eta, f3 k, float c) {
+  return f3{conductorR(eta.x, k.x, c), conductorR(eta.y, k.y, c), conductorR(eta.z, k.z, c)};
+}
+// kang18's own luminance weights (brdf_kang18.rchit:60-62)
+__device__ __forceinline__ float lum18(f3 c) { return 0.212671f * c.x + 0.715160f * c.y + 0.072169f * c.z; }
+
+// rough conductor eval, brdf_rough_conductor.rchit:72-91
+__device__ f3 rcEval(f3 L, f3 V, f3 N, f3 X, f3 Y, f3 kd, f3 eta, f3 k, float ax, float ay) {
+  float NdotL = dot(L, N), NdotV = dot(V, N);
+  if (NdotL < 0.0f || NdotV < 0.0f) return mk1(0.f);
+  f3 H = makeNormal(L + V);
+  f3 Fs = conductorR3(eta, k, NdotL);
+  float Gs = g1GGX(NdotV, dot(V, X), dot(V, Y), ax, ay);
+  Gs *= g1GGX(NdotL, dot(L, X), dot(L, Y), ax, ay);
+  float Ds = dAnisoGGX(dot(H, N), dot(H, X), dot(H, Y), ax, ay);
+  return mk1(0.f) + kd * Fs * Gs * Ds * NdotL;
+}
+// phong, brdf_phong.rchit:12-52
+__device__ f3 phongEval(f3 L, f3 V, f3 N, f3 kd, f3 ks, float sh) {
+  float NdotL = dot(N, L), NdotV = dot(N, V);
+  if (NdotL < 0.0f || NdotV < 0.0f) return mk1(0.f);
+  f3 H = makeNormal(L + V);
+  f3 diffuseLobe = kd * kInvPi;
+  f3 specularLobe = ks * powf(fmaxf(dot(H, N), 0.0f), sh) * (sh + 2.0f) * kInv2Pi;
+  float db = luminance(kd), sb = luminance(ks);
+  float dw = db / (db + sb);
+  float sw = 1.0f - dw;
+  return (dw * diffuseLobe + sw * specularLobe) * NdotL;
+}
+__device__ __forceinline__ float phongLobePdf(f3 L, f3 V, f3 N, float sh) {
+  f3 H = makeNormal(L + V);
+  float NdotH = fmaxf(dot(H, N), 0.0f), VdotH = fmaxf(dot(H, V), 0.0f);
+  return (sh + 1.0f) * kInv2Pi * powf(NdotH, sh) * 0.25f / (VdotH + kEps);
+}
+// kang18, brdf_kang18.rchit:64-124
+__device__ f3 kangEval(f3 L, f3 V, f3 N, f3 X, f3 Y, f3 kd, f3 ks, float ax, float ay, float eta) {
+  float NdotL = dot(N, L), NdotV = dot(N, V);
+  if (NdotL <= 0.0f || NdotV <= 0.0f) return mk1(0.f);
+  f3 H = makeNormal(L + V);
+  float HdotN = dot(H, N), HdotX = dot(H, X), HdotY = dot(H, Y), HdotV = dot(H, V);
+  float F0 = sqr((eta - 1.0f) / (eta + 1.0f));
+  float Fs = F0 + (1.0f - F0) * powf(clampf(1.0f - HdotV, 0.0f, 1.0f), 5.0f);
+  float Ds = dAnisoGGX(HdotN, HdotX, HdotY, ax, ay);
+  float Gs = g1GGX(NdotV, dot(V, X), dot(V, Y), ax, ay);
+  Gs *= g1GGX(NdotL, dot(L, X), dot(L, Y), ax, ay);
+  f3 weight = mk1(0.f);
+  weight = weight + kd * kInvPi;
+  weight = weight + ks * Fs * Ds * Gs;
+  return weight * NdotL;
+}
+
+// Surface plugins.  Every closest-hit shader of the set runs the same three
+// steps -- material inputs (textures, normal map, AOVs, opacity draws), one
+// light sample through sampleLights, then its BSDF value toward the light
+// and its BSDF sample -- and the light sample sits at the same point of the
+// RNG stream in each.  The light sample is therefore taken once for the
+// whole wave instead of once per plugin present in it.
 //   brdf_lambertian.rchit:70-151, brdf_pbr_metalness_roughness.rchit:131-219,
-//   brdf_rough_plastic.rchit:191-269, bsdf_dielectric.rchit:74-138
+//   brdf_rough_plastic.rchit:191-269, bsdf_dielectric.rchit:74-138,
+//   brdf_mirror.rchit:39-101, brdf_conductor.rchit:89-155,
+//   brdf_rough_conductor.rchit:147-220, brdf_plastic.rchit:133-204,
+//   brdf_phong.rchit:112-193, brdf_kang18.rchit:141-249
+//
+// The plugins are compiled as two groups (GROUP2 = false: lambertian, pbr,
+// rough plastic, dielectric; true: mirror, conductor, rough conductor,
+// plastic, phong, kang18) so that each shading kernel keeps the register
+// budget of one group.
+__device__ __forceinline__ bool isGroup2(uint32_t t) {
+  return t == MaterialTypeBrdfMirror || t == MaterialTypeBrdfConductor || t == MaterialTypeBrdfRoughConductor ||
+         t == MaterialTypeBrdfPlastic || t == MaterialTypeBrdfPhong || t == MaterialTypeBrdfKang18;
+}
+
+template <bool GROUP2>
 __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t pix, bool aov) {
   const GpuPushConstantRaytrace& pc = c.sp->pc;
   const SceneView& sc = *c.sc;
   const uint32_t type = m.type;
-  const bool lambert = type == MaterialTypeBrdfLambertian, pbr = type == MaterialTypeBrdfPbrMetalnessRoughness,
-             plastic = type == MaterialTypeBrdfRoughPlastic, dielectric = type == MaterialTypeBsdfDielectric;
+  const bool lambert = !GROUP2 && type == MaterialTypeBrdfLambertian,
+             pbr = !GROUP2 && type == MaterialTypeBrdfPbrMetalnessRoughness,
+             plastic = !GROUP2 && type == MaterialTypeBrdfRoughPlastic,
+             dielectric = !GROUP2 && type == MaterialTypeBsdfDielectric,
+             mirror = GROUP2 && type == MaterialTypeBrdfMirror, conductor = GROUP2 && type == MaterialTypeBrdfConductor,
+             rcond = GROUP2 && type == MaterialTypeBrdfRoughConductor, smooth = GROUP2 && type == MaterialTypeBrdfPlastic,
+             phong = GROUP2 && type == MaterialTypeBrdfPhong, kang = GROUP2 && type == MaterialTypeBrdfKang18;
+  // NEE is identically zero for the delta lobes (their eval tests EDelta but
+  // is called with EArea); their shadow ray is counted, not traced.
+  const bool zeroNee = dielectric || mirror || conductor;
   // (1) material inputs
-  f3 kd = ld3(m.diffuse);
+  f3 kd = ld3(m.diffuse), ks = mk1(1.f);
   float metal = 0.f, ax = 0.f, ay = 0.f, eta = m.ior, fdrInt = 0.f, invEta2 = 0.f, ssw = 0.f;
+  f3 ceta = mk1(0.f), ck = mk1(0.f);
   if (!dielectric && m.diffuseTextureId >= 0) kd = xyz(tex2D(sc.textures[m.diffuseTextureId], h.uvx, h.uvy));
   if (pbr) {
     metal = m.metalness;
@@ -1098,20 +1186,66 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
     if (m.metalnessTextureId >= 0) metal = tex2D(sc.textures[m.metalnessTextureId], h.uvx, h.uvy).x;
     if (m.roughnessTextureId >= 0) rough = tex2D(sc.textures[m.roughnessTextureId], h.uvx, h.uvy).x;
     ax = ay = fmaxf(sqr(rough), 0.001f);
-  } else if (plastic) {
+  } else if (plastic || rcond || kang) {
     float alx = m.anisoAlpha[0], aly = m.anisoAlpha[1];
     if (m.roughnessTextureId >= 0) {
       float4 t = tex2D(sc.textures[m.roughnessTextureId], h.uvx, h.uvy);
       alx = t.x;
       aly = t.y;
     }
+    ax = fmaxf(plastic ? kEps : alx, plastic ? alx : kEps);
+    ay = fmaxf(plastic ? kEps : aly, plastic ? aly : kEps);
+    if (plastic) {
+      fdrInt = m.radiance[0];
+      invEta2 = 1.0f / (eta * eta);
+      ssw = luminance(kd);
+    }
+  } else if (smooth) {
     fdrInt = m.radiance[0];
+    float dAvg = luminance(kd), sAvg = luminance(mk1(1.f));
+    ssw = sAvg / (dAvg + sAvg);
     invEta2 = 1.0f / (eta * eta);
-    ax = fmaxf(kEps, alx);
-    ay = fmaxf(kEps, aly);
-    ssw = luminance(kd);
   }
-  if (m.normalTextureId >= 0) normalMap(sc, h, m.normalTextureId, pc.useFaceNormal);
+  if (conductor || rcond) {
+    ceta = ld3(m.radiance);
+    ck = ld3(m.radianceFactor);
+  }
+  if (phong) {
+    ks = ld3(m.rhoSpec);
+    metal = m.specular;  // shininess
+  }
+  if (kang) {
+    ks = ld3(m.rhoSpec);
+    if (m.metalnessTextureId >= 0) ks = xyz(tex2D(sc.textures[m.metalnessTextureId], h.uvx, h.uvy));
+    float opacity = m.metalness;
+    if (m.opacityTextureId >= 0) opacity = tex2D(sc.textures[m.opacityTextureId], h.uvx, h.uvy).x;
+    const float* q = sc.inst[h.inst].w2o;
+    auto toWorldN = [&](f3 n) {  // (n * gl_WorldToObjectEXT).xyz
+      return f3{n.x * q[0] + n.y * q[4] + n.z * q[8], n.x * q[1] + n.y * q[5] + n.z * q[9],
+                n.x * q[2] + n.y * q[6] + n.z * q[10]};
+    };
+    if (m.normalTextureId >= 0) {
+      f3 cn = xyz(tex2D(sc.textures[m.normalTextureId], h.uvx, h.uvy));
+      h.N = makeNormal(toWorldN(2.0f * cn - mk1(1.f)));
+      if (pc.useFaceNormal == 1) h.N = mk1(0.f);
+      basis(h.N, h.X, h.Y);
+      h.ffN = dot(h.N, h.V) > 0.0f ? h.N : -h.N;
+    }
+    if (m.tangentTextureId >= 0) {
+      f3 ct = xyz(tex2D(sc.textures[m.tangentTextureId], h.uvx, h.uvy));
+      h.X = makeNormal(toWorldN(2.0f * ct - mk1(1.f)));
+    }
+    h.Y = makeNormal(cross(h.N, h.X));
+    h.X = makeNormal(cross(h.Y, h.N));
+    h.ffN = dot(h.N, h.V) > 0.0f ? h.N : -h.N;
+    if (rnd(c.seed) < opacity) {
+      c.ro = offsetRay(h.pos, -h.ffN);
+      c.depth--;
+      return;
+    }
+  } else if (m.normalTextureId >= 0) {
+    normalMap(sc, h, m.normalTextureId, pc.useFaceNormal);
+  }
   if (pbr) {
     float opacity = m.specular;
     if (m.opacityTextureId >= 0) opacity = tex2D(sc.textures[m.opacityTextureId], h.uvx, h.uvy).x;
@@ -1135,50 +1269,93 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
     } else if (pbr) {
       writeAov(c, pc.diffuseOutChannel, kd, pix);
       writeAov(c, pc.normalOutChannel, h.ffN, pix);
+    } else if (phong || kang) {
+      writeAov(c, pc.diffuseOutChannel, kd, pix);
+      writeAov(c, pc.normalOutChannel, h.ffN, pix);
+      writeAov(c, pc.specularOutChannel, ks, pix);
+      writeAov(c, pc.tangentOutChannel, h.X, pix);
+      writeAov(c, pc.roughnessOutChannel, kang ? f3{ax, ay, 0.f} : f3{1.f, 1.f, 0.f}, pix);
+      writeAov(c, pc.positionOutChannel, h.pos, pix);
+      writeAov(c, pc.uvOutChannel, f3{h.uvx, h.uvy, 1.f}, pix);
     }
   }
-  const f3 ks = mk1(1.f);
   // (2) one light sample (sampleLights, rchit_layouts.glsl:118-167)
   bool visible;
   LSample l;
   f3 radiance = sampleLights(c, h.pos, h.ffN, visible, l);
-  // (3) BSDF value and pdf toward the light.  The dielectric's NEE term is
-  // always zero (its eval tests EDelta but is called with EArea); its
-  // shadow ray is counted but not traced, since it cannot change the result.
+  // (3) BSDF value and pdf toward the light
   f3 Ld = mk1(0.f);
-  if (visible && !dielectric) {
+  if (visible && !zeroNee) {
     f3 bw = mk1(0.f);
     float bsdfPdf = 0.0f;
+    const f3 N = h.ffN, V = h.V;
     if (lambert) {
-      float NdotL = dot(h.ffN, l.d), NdotV = dot(h.ffN, h.V);
+      float NdotL = dot(N, l.d), NdotV = dot(N, V);
       if (!(NdotL < 0.0f || NdotV < 0.0f)) bw = kd * kInvPi * NdotL;
-      bsdfPdf = (l.flags & kLightArea) ? cosinePdf(dot(h.ffN, l.d)) : 0.0f;
+      bsdfPdf = (l.flags & kLightArea) ? cosinePdf(dot(N, l.d)) : 0.0f;
     } else if (pbr) {
-      bw = pbrEval(l.d, h.V, h.ffN, h.X, h.Y, kd, ax, ay, metal, eta);
+      bw = pbrEval(l.d, V, N, h.X, h.Y, kd, ax, ay, metal, eta);
       if (l.flags & kLightArea) {
-        float NdotL = dot(h.ffN, l.d), NdotV = dot(h.ffN, h.V);
+        float NdotL = dot(N, l.d), NdotV = dot(N, V);
         if (!(NdotL <= 0.0f || NdotV <= 0.0f)) {
-          f3 H = makeNormal(l.d + h.V);
-          f3 wh = makeNormal(toLocal(h.X, h.Y, h.ffN, H));
-          f3 wo = makeNormal(toLocal(h.X, h.Y, h.ffN, h.V));
-          f3 wi = makeNormal(toLocal(h.X, h.Y, h.ffN, l.d));
+          f3 H = makeNormal(l.d + V);
+          f3 wh = makeNormal(toLocal(h.X, h.Y, N, H));
+          f3 wo = makeNormal(toLocal(h.X, h.Y, N, V));
+          f3 wi = makeNormal(toLocal(h.X, h.Y, N, l.d));
           bsdfPdf += 0.2f * cosinePdf(wi.z);
           bsdfPdf += (1.0f - 0.2f) * pdfGGX(wh, wo, ax, ay);
         }
       }
-    } else {
+    } else if (plastic) {
       // The shader passes (..., ax, ay, invEta2, EArea) into eval's
       // (..., invEta2, ax, ay, flags) and (eta, ax, ay) into pdf's
       // (ax, ay, eta) (brdf_rough_plastic.rchit:234-240).
-      bw = rpEval(l.d, h.V, h.ffN, h.X, h.Y, kd, ks, eta, fdrInt, ax, ay, invEta2, kLightArea);
-      bsdfPdf = rpPdf(l.d, h.V, h.ffN, h.X, h.Y, eta, ax, ay, ssw, l.flags);
+      bw = rpEval(l.d, V, N, h.X, h.Y, kd, ks, eta, fdrInt, ax, ay, invEta2, kLightArea);
+      bsdfPdf = rpPdf(l.d, V, N, h.X, h.Y, eta, ax, ay, ssw, l.flags);
+    } else if (rcond) {
+      bw = rcEval(l.d, V, N, h.X, h.Y, kd, ceta, ck, ax, ay);
+      float NdotL = dot(l.d, N), NdotV = dot(V, N);
+      if (!(NdotL < 0.0f || NdotV < 0.0f) && (l.flags & kLightArea)) {
+        f3 wi = toLocal(h.X, h.Y, N, l.d), wo = toLocal(h.X, h.Y, N, V);
+        bsdfPdf = pdfGGX(makeNormal(wi + wo), wo, ax, ay);
+      }
+    } else if (smooth) {
+      float NdotL = dot(l.d, N), NdotV = dot(V, N);
+      if (!(NdotL < 0.0f || NdotV < 0.0f)) {
+        float Fo = fresnelDielectricExt(NdotV, eta), Fi = fresnelDielectricExt(NdotL, eta);
+        f3 diff = kd / (mk1(1.f) - kd * fdrInt);
+        bw = (1.0f - Fi) * (1.0f - Fo) * diff * invEta2 * kInvPi * NdotL;
+        float ps = (Fo * ssw) / (Fo * ssw + (1.0f - Fo) * (1.0f - ssw));
+        if (l.flags & kLightDelta) bsdfPdf = fabsf(dot(reflect(-V, N), l.d) - 1.0f) < kEps ? ps : 0.0f;
+        else if (l.flags & kLightArea) bsdfPdf = NdotL * (1.0f - ps);  // no 1/pi, as written
+      }
+    } else if (phong) {
+      bw = phongEval(l.d, V, N, kd, ks, metal);
+      if (l.flags & kLightArea) {
+        float db = luminance(kd), sb = luminance(ks);
+        float dw = db / (db + sb);
+        float sw = 1.0f - dw;
+        bsdfPdf = dw * cosinePdf(dot(N, l.d)) + sw * phongLobePdf(l.d, V, N, metal);
+      }
+    } else if (kang) {  // kang18: eval ignores the flags
+      bw = kangEval(l.d, V, N, h.X, h.Y, kd, ks, ax, ay, eta);
+      float NdotL = dot(N, l.d), NdotV = dot(N, V);
+      if ((l.flags & kLightArea) && !(NdotL <= 0.0f || NdotV <= 0.0f)) {
+        f3 H = makeNormal(l.d + V);
+        f3 wh = makeNormal(toLocal(h.X, h.Y, N, H)), wo = makeNormal(toLocal(h.X, h.Y, N, V)),
+           wi = makeNormal(toLocal(h.X, h.Y, N, l.d));
+        float pd = lum18(kd), ps = lum18(ks);
+        float P = pd / (pd + ps + kEps);
+        bsdfPdf += P * cosinePdf(wi.z);
+        bsdfPdf += (1.0f - P) * pdfGGX(wh, wo, ax, ay);
+      }
     }
     float mis = powerHeuristic(l.pdf, bsdfPdf);
     Ld = mis * bw * radiance * c.T / (l.pdf + kEps);
   }
   c.Ld = Ld;
-  c.shadow = visible && !dielectric;
-  c.fromDielectric = visible && dielectric;
+  c.shadow = visible && !zeroNee;
+  c.fromDielectric = visible && zeroNee;
   // (4) BSDF sample
   if (dielectric) {
     float u = rnd(c.seed);
@@ -1211,6 +1388,14 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
     c.T = c.T * (w / (pdf + kEps));
     return;
   }
+  if (mirror) {  // sampleBsdf takes no random numbers
+    if (length(kd) == 0.0f) {
+      c.stop = true;
+      return;
+    }
+    nextRay(c, h.pos, h.ffN, reflect(-h.V, h.ffN), kd / 1.0f, 1.0f, kSpecRefl);
+    return;
+  }
   float u0 = rnd(c.seed), u1 = rnd(c.seed);
   if (lambert) {
     f3 wi = cosineHemisphere(u0, u1);
@@ -1227,8 +1412,10 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
   f3 d = mk1(0.f), w = mk1(0.f);
   float pdf = 0.0f;
   uint32_t flags = kBsdfNull;
+  bool epsInDivisor = true;
+  const f3 N = h.ffN, V = h.V, X = h.X, Y = h.Y;
   if (pbr) {
-    f3 wo = makeNormal(toLocal(h.X, h.Y, h.ffN, h.V));
+    f3 wo = makeNormal(toLocal(X, Y, N, V));
     f3 wi;
     if (rnd(c.seed) < 0.2f) {
       wi = cosineHemisphere(u0, u1);
@@ -1240,11 +1427,10 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
       pdf = pdfGGX(wh, wo, ax, ay);
       flags = kGlossyRefl;
     }
-    d = toWorld(h.X, h.Y, h.ffN, wi);
-    w = pbrEval(d, h.V, h.ffN, h.X, h.Y, kd, ax, ay, metal, eta) * fabsf(wi.z);
-  } else {
+    d = toWorld(X, Y, N, wi);
+    w = pbrEval(d, V, N, X, Y, kd, ax, ay, metal, eta) * fabsf(wi.z);
+  } else if (plastic) {
     // sampleBsdf, brdf_rough_plastic.rchit:138-189
-    f3 N = h.ffN, V = h.V, X = h.X, Y = h.Y;
     float NdotV = dot(V, N);
     if (NdotV > 0.0f) {
       float Fo = fresnelDielectricExt(NdotV, eta);
@@ -1277,24 +1463,106 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
         w = (1.0f - Fi) * (1.0f - Fo) * invEta2 * diff * kInvPi * fabsf(wi.z);
       }
     }
+  } else if (conductor) {  // the two draws are unused
+    if (dot(V, N) > 0.0f) {
+      d = reflect(-V, N);
+      pdf = 1.0f;
+      flags = kSpecRefl;
+      w = kd * conductorR3(ceta, ck, dot(N, d));
+    }
+  } else if (rcond) {
+    float NdotV = dot(V, N);
+    if (NdotV > 0.0f) {
+      f3 wo = toLocal(X, Y, N, V);
+      f3 wi = sampleGGX(u0, u1, wo, ax, ay);
+      f3 L = toWorld(X, Y, N, wi);
+      d = L;
+      f3 H = makeNormal(V + L);
+      f3 wh = makeNormal(wi + wo);
+      float NdotL = dot(N, L);
+      f3 Fs = conductorR3(ceta, ck, NdotL);
+      float Gs = g1GGX(NdotV, dot(V, X), dot(V, Y), ax, ay);
+      Gs *= g1GGX(NdotL, dot(L, X), dot(L, Y), ax, ay);
+      float Ds = dAnisoGGX(dot(H, N), dot(H, X), dot(H, Y), ax, ay);
+      flags = kGlossyRefl;
+      pdf = pdfGGX(wh, wo, ax, ay);
+      w = kd * Fs * Gs * Ds * NdotL;
+    }
+  } else if (smooth) {
+    float NdotV = dot(V, N);
+    if (NdotV > 0.0f) {
+      float Fo = fresnelDielectricExt(NdotV, eta);
+      float ps = (Fo * ssw) / (Fo * ssw + (1.0f - Fo) * (1.0f - ssw));
+      if (u0 < ps) {
+        d = makeNormal(reflect(-V, N));
+        flags = kSpecRefl;
+        pdf = ps;
+        w = mk1(1.f) * Fo;
+      } else {
+        u0 = (u0 - ps) / (1.0f + kEps - ps);
+        f3 wi = cosineHemisphere(u0, u1);
+        float Fi = fresnelDielectricExt(wi.z, eta);
+        f3 diff = kd / (mk1(1.f) - kd * fdrInt);
+        pdf = (1.0f - ps) * cosinePdf(wi.z);
+        d = toWorld(X, Y, N, wi);
+        flags = kDiffuseRefl;
+        w = (1.0f - Fi) * (1.0f - Fo) * invEta2 * diff * kInvPi * fabsf(wi.z);
+      }
+    }
+  } else if (phong) {
+    epsInDivisor = false;
+    float db = luminance(kd), sb = luminance(ks);
+    float dw = db / (db + sb);
+    float sw = 1.0f - dw;
+    if (u0 < dw) {
+      u0 /= dw;
+      f3 wi = cosineHemisphere(u0, u1);
+      pdf = cosinePdf(wi.z);
+      d = toWorld(X, Y, N, wi);
+      flags = kDiffuseRefl;
+    } else {
+      u0 = (u0 - dw) / sw;
+      float cosTheta = powf(u0, 1.0f / (metal + 1.0f));
+      float phi = kTwoPi * u1;
+      float sinTheta = sqrtf(fmaxf(0.0f, 1.0f - cosTheta * cosTheta));
+      f3 wh = f3{sinTheta * sinf(phi), sinTheta * cosf(phi), cosTheta};
+      f3 H = toWorld(X, Y, N, wh);
+      d = reflect(-V, H);
+      pdf = phongLobePdf(d, V, N, metal);
+      flags = kGlossyRefl;
+    }
+    w = phongEval(d, V, N, kd, ks, metal);
+  } else if (kang) {
+    f3 wo = makeNormal(toLocal(X, Y, N, V));
+    float pd = lum18(kd), ps = lum18(ks);
+    float P = pd / (pd + ps + kEps);
+    f3 wi;
+    if (rnd(c.seed) < P) {
+      wi = cosineHemisphere(u0, u1);
+      flags = kDiffuseRefl;
+      pdf = cosinePdf(wi.z);
+    } else {
+      wi = sampleGGX(u0, u1, wo, ax, ay);
+      pdf = pdfGGX(makeNormal(wi + wo), wo, ax, ay);
+      flags = kGlossyRefl;
+    }
+    d = toWorld(X, Y, N, wi);
+    w = kangEval(d, V, N, X, Y, kd, ks, ax, ay, eta);
   }
   if (pdf <= 0.0f || length(w) == 0.0f) {
     c.stop = true;
     return;
   }
-  nextRay(c, h.pos, h.ffN, d, w / (pdf + kEps), pdf, flags);
+  nextRay(c, h.pos, h.ffN, d, epsInDivisor ? w / (pdf + kEps) : w / pdf, pdf, flags);
 }
 
-// One bounce of shading for the queued paths: miss, light hit, emissive or
+// One bounce of shading for one queued path: miss, light hit, emissive or
 // surface plugin; appends the NEE shadow ray and the continuing path.
 // ONE: the batch holds a single shot whose constants come from the kernel
 // arguments (scalar loads) instead of the per-shot table.
-template <bool ONE>
-__global__ __launch_bounds__(256) void kShade(SceneView sc, BatchDesc b, Wavefront w, int cur) {
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-  const uint32_t count = w.ctr[cur];
-  if (i >= count) return;
-  int pid = w.queue[cur][i];
+template <bool ONE, bool GROUP2>
+__device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur,
+                                         int pid) {
   uint4 info = w.info[pid];
   uint32_t shot = info.w & 0xffffu;
   const ShotParams& sp = ONE ? b.one : b.shotTab[shot];
@@ -1348,9 +1616,8 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, BatchDesc b, Wavefro
         f3 e = ld3(m.radiance);
         if (m.radianceTextureId >= 0) e = ld3(m.radianceFactor) * xyz(tex2D(sc.textures[m.radianceTextureId], h.uvx, h.uvy));
         if (pc.ignoreEmissive == 0) c.L = c.L + e * c.T;
-      } else if (type == MaterialTypeBrdfLambertian || type == MaterialTypeBrdfPbrMetalnessRoughness ||
-                 type == MaterialTypeBrdfRoughPlastic || type == MaterialTypeBsdfDielectric) {
-        shadeSurface(c, h, m, pix, aov);
+      } else if (type != MaterialTypeBrdfDisney && type < MaterialTypeNum) {
+        shadeSurface<GROUP2>(c, h, m, pix, aov);
       } else {
         c.stop = true;
       }
@@ -1377,12 +1644,43 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, BatchDesc b, Wavefro
   w.rd[pid] = f4(c.rd, 0.f);
 }
 
+// Shades the queue; paths whose surface belongs to the second plugin group
+// are moved to the side queue (queue[2], count ctr[8]) for kShadeGroup2.
+template <bool ONE>
+__global__ __launch_bounds__(256) void kShade(SceneView sc, BatchDesc b, Wavefront w, int cur) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t count = w.ctr[cur];
+  const bool valid = i < count;
+  const int pid = valid ? w.queue[cur][i] : -1;
+  bool side = false;
+  if (valid) {
+    int inst = w.hitI[pid].x;
+    if (inst >= 0) {
+      const DevInstance& in = sc.inst[inst];
+      side = in.light < 0 && isGroup2(sc.mats[in.material].type);
+    }
+  }
+  int si = waveAppend(&w.ctr[8], side);
+  if (side) w.queue[2][si] = pid;
+  if (!valid || side) return;
+  shadeOne<ONE, false>(sc, b, w, cur, pid);
+}
+
+// Second plugin group, grid-stride over the side queue.
+template <bool ONE>
+__global__ __launch_bounds__(256) void kShadeGroup2(SceneView sc, BatchDesc b, Wavefront w, int cur) {
+  const uint32_t count = w.ctr[8];
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
+    shadeOne<ONE, true>(sc, b, w, cur, w.queue[2][i]);
+}
+
 __global__ void kResetCounters(Wavefront w, int cur) {
   if (threadIdx.x == 0) {
     w.ctr[1 - cur] = 0;
     w.ctr[2] = 0;
     w.ctr[3] = 0;
     w.ctr[4] = 0;
+    w.ctr[8] = 0;
   }
 }
 
@@ -1474,8 +1772,13 @@ void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t lds
 }
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
                  hipStream_t s) {
-  if (b.shots == 1) kShade<true><<<(maxRays + 255) / 256, 256, 0, s>>>(sc, b, w, cur);
-  else kShade<false><<<(maxRays + 255) / 256, 256, 0, s>>>(sc, b, w, cur);
+  if (b.shots == 1) {
+    kShade<true><<<(maxRays + 255) / 256, 256, 0, s>>>(sc, b, w, cur);
+    kShadeGroup2<true><<<512, 256, 0, s>>>(sc, b, w, cur);
+  } else {
+    kShade<false><<<(maxRays + 255) / 256, 256, 0, s>>>(sc, b, w, cur);
+    kShadeGroup2<false><<<512, 256, 0, s>>>(sc, b, w, cur);
+  }
   kStatShadow<<<1, 64, 0, s>>>(w);
 }
 void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t, uint32_t grid, bool count, hipStream_t s) {

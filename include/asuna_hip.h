@@ -217,7 +217,8 @@ typedef struct ash_stats {
   uint64_t paths;          /* camera samples traced (pixels x frames x shots) */
   uint64_t extension_rays; /* closest-hit rays traced */
   uint64_t shadow_rays;    /* any-hit rays traced */
-  uint64_t dielectric_shadow_rays; /* of which from dielectric hits (always zero contribution) */
+  uint64_t dielectric_shadow_rays; /* of which from delta lobes (dielectric, mirror, conductor):
+                                      always zero contribution, counted but not traced */
   double render_ms;        /* device time of the render call (HIP events) */
   double extend_ms;        /* sum of closest-hit kernel time, if timing enabled */
   double shadow_ms;        /* sum of any-hit kernel time, if timing enabled */

@@ -633,6 +633,7 @@ template <class R> struct HitState {
   R uvx, uvy;
   V3<R> pos, V, N, ffN, X, Y;
   GpuMaterial mat;
+  const float* w2o;  // gl_WorldToObjectEXT of the instance (row-major 3x4)
 };
 
 template <class R>
@@ -661,6 +662,7 @@ HitState<R> getHitState(const orc_scene& s, const HitR& h, V3<R> rayDir, uint32_
   V3<R> nw(n.x * R(w[0]) + n.y * R(w[4]) + n.z * R(w[8]), n.x * R(w[1]) + n.y * R(w[5]) + n.z * R(w[9]),
            n.x * R(w[2]) + n.y * R(w[6]) + n.z * R(w[10]));
   st.N = makeNormal(nw);
+  st.w2o = in.w2o;
   st.V = makeNormal(-rayDir);
   if (st.lightId < 0) st.mat = s.mats[in.material];
   // configureShadingFrame: geoN is never assigned in the reference
@@ -1336,6 +1338,441 @@ template <class R> void chsDielectric(const RCtx& c, Payload<R>& pl, HitState<R>
   pl.throughput = pl.throughput * (bw / (bpdf + K<R>::EPS));
 }
 
+// ---------------------------------------------- further plugins of the set
+// brdf_conductor.rchit:14-34 (Shirley's conductor Fresnel; identical in
+// brdf_rough_conductor.rchit:14-34)
+template <class R> R conductorReflectance(R eta, R k, R cosThetaI) {
+  R cosThetaISq = cosThetaI * cosThetaI;
+  R sinThetaISq = std::max(R(1) - cosThetaISq, R(0));
+  R sinThetaIQu = sinThetaISq * sinThetaISq;
+  R innerTerm = eta * eta - k * k - sinThetaISq;
+  R aSqPlusBSq = std::sqrt(std::max(innerTerm * innerTerm + R(4) * eta * eta * k * k, R(0)));
+  R a = std::sqrt(std::max((aSqPlusBSq + innerTerm) * R(0.5), R(0)));
+  R Rs = ((aSqPlusBSq + cosThetaISq) - (R(2) * a * cosThetaI)) / ((aSqPlusBSq + cosThetaISq) + (R(2) * a * cosThetaI));
+  R Rp = ((cosThetaISq * aSqPlusBSq + sinThetaIQu) - (R(2) * a * cosThetaI * sinThetaISq)) /
+         ((cosThetaISq * aSqPlusBSq + sinThetaIQu) + (R(2) * a * cosThetaI * sinThetaISq));
+  return R(0.5) * (Rs + Rs * Rp);
+}
+template <class R> V3<R> conductorReflectance3(V3<R> eta, V3<R> k, R c) {
+  return V3<R>(conductorReflectance(eta.x, k.x, c), conductorReflectance(eta.y, k.y, c),
+               conductorReflectance(eta.z, k.z, c));
+}
+
+// Next-event estimate with a BSDF value / pdf pair (the USE_MIS block every
+// closest-hit shader repeats).
+template <class R, class EvalF, class PdfF>
+void directLight(const RCtx& c, Payload<R>& pl, HitState<R>& st, EvalF eval, PdfF pdf) {
+  V3<R> Ld(0);
+  bool visible;
+  LRec<R> l;
+  V3<R> radiance = sampleLights(c, pl, st.pos, st.ffN, visible, l);
+  if (visible) {
+    V3<R> bw = eval(l);
+    R bsdfPdf = pdf(l);
+    R mis = powerHeuristic(l.pdf, bsdfPdf);
+    Ld = Ld + mis * bw * radiance * pl.throughput / (l.pdf + K<R>::EPS);
+  }
+  pl.dRad = Ld;
+  pl.dSkip = !visible;
+}
+
+template <class R> void diffuseAndNormalMaps(const RCtx& c, HitState<R>& st) {
+  R t4[4];
+  if (st.mat.diffuseTextureId >= 0) {
+    V3<R> d = textureEval3(*c.s, st.mat.diffuseTextureId, st.uvx, st.uvy, t4);
+    st.mat.diffuse[0] = float(d.x), st.mat.diffuse[1] = float(d.y), st.mat.diffuse[2] = float(d.z);
+  }
+}
+
+// Continue the path along a sampled direction (offset along ffN).
+template <class R>
+void nextRay(Payload<R>& pl, HitState<R>& st, V3<R> bd, V3<R> bw, R bpdf, uint32_t bflags, bool epsInDivisor) {
+  pl.bd = bd;
+  pl.bpdf = bpdf;
+  pl.bflags = bflags;
+  pl.ro = offsetPositionAlongNormal(st.pos, st.ffN);
+  pl.rd = bd;
+  pl.throughput = pl.throughput * (epsInDivisor ? bw / (bpdf + K<R>::EPS) : bw / bpdf);
+}
+
+// brdf_mirror.rchit:12-101.  Its NEE term is always zero (eval tests EDelta
+// and is called with EArea); the shadow ray is still issued when visible.
+template <class R> void chsMirror(const RCtx& c, Payload<R>& pl, HitState<R>& st) {
+  diffuseAndNormalMaps(c, st);
+  if (st.mat.normalTextureId >= 0) applyNormalMap(*c.s, st, c.pc->useFaceNormal);
+  V3<R> kd = load3<R>(st.mat.diffuse);
+  directLight(
+      c, pl, st, [&](const LRec<R>&) { return V3<R>(0); },  // eval(..., EArea)
+      [&](const LRec<R>& l) {
+        if ((l.flags & EDelta) == 0) return R(0);
+        return std::fabs(dot(reflect(-st.V, st.ffN), l.d) - R(1)) < K<R>::EPS ? R(1) : R(0);
+      });
+  pl.fromDielectric = true;
+  V3<R> bd = reflect(-st.V, st.ffN);
+  if (length(kd) == R(0)) {
+    pl.stop = true;
+    return;
+  }
+  nextRay(pl, st, bd, kd, R(1), ESpecularReflection, false);
+}
+
+// brdf_conductor.rchit:36-155.  NEE is always zero as for the mirror; the
+// sample consumes two draws it does not use.
+template <class R> void chsConductor(const RCtx& c, Payload<R>& pl, HitState<R>& st) {
+  diffuseAndNormalMaps(c, st);
+  if (st.mat.normalTextureId >= 0) applyNormalMap(*c.s, st, c.pc->useFaceNormal);
+  V3<R> kd = load3<R>(st.mat.diffuse), eta = load3<R>(st.mat.radiance), k = load3<R>(st.mat.radianceFactor);
+  directLight(
+      c, pl, st, [&](const LRec<R>&) { return V3<R>(0); },
+      [&](const LRec<R>& l) {
+        R NdotL = dot(l.d, st.ffN), NdotV = dot(st.V, st.ffN);
+        if (NdotL < R(0) || NdotV < R(0) || (l.flags & EDelta) == 0) return R(0);
+        return std::fabs(dot(reflect(-st.V, st.ffN), l.d) - R(1)) < K<R>::EPS ? R(1) : R(0);
+      });
+  pl.fromDielectric = true;
+  (void)rnd<R>(pl.seed);
+  (void)rnd<R>(pl.seed);
+  R NdotV = dot(st.V, st.ffN);
+  if (NdotV <= R(0)) {
+    pl.stop = true;
+    return;
+  }
+  V3<R> bd = reflect(-st.V, st.ffN);
+  V3<R> bw = kd * conductorReflectance3(eta, k, dot(st.ffN, bd));
+  if (length(bw) == R(0)) {
+    pl.stop = true;
+    return;
+  }
+  nextRay(pl, st, bd, bw, R(1), ESpecularReflection, true);
+}
+
+// brdf_rough_conductor.rchit:72-220
+template <class R>
+V3<R> roughConductorEval(V3<R> L, V3<R> V, V3<R> N, V3<R> X, V3<R> Y, V3<R> kd, V3<R> eta, V3<R> k, R ax, R ay,
+                         uint32_t flags) {
+  V3<R> weight(0);
+  R NdotL = dot(L, N), NdotV = dot(V, N);
+  if ((flags & EArea) == 0 || NdotL < R(0) || NdotV < R(0)) return weight;
+  V3<R> H = makeNormal(L + V);
+  V3<R> Fs = conductorReflectance3(eta, k, NdotL);
+  R Gs = g1SmithAnisoGGX(NdotV, dot(V, X), dot(V, Y), ax, ay);
+  Gs *= g1SmithAnisoGGX(NdotL, dot(L, X), dot(L, Y), ax, ay);
+  R Ds = dAnisoGGX(dot(H, N), dot(H, X), dot(H, Y), ax, ay);
+  return weight + kd * Fs * Gs * Ds * NdotL;
+}
+
+template <class R> void chsRoughConductor(const RCtx& c, Payload<R>& pl, HitState<R>& st) {
+  R t4[4];
+  diffuseAndNormalMaps(c, st);
+  if (st.mat.roughnessTextureId >= 0) {
+    textureEval3(*c.s, st.mat.roughnessTextureId, st.uvx, st.uvy, t4);
+    st.mat.anisoAlpha[0] = float(t4[0]), st.mat.anisoAlpha[1] = float(t4[1]);
+  }
+  if (st.mat.normalTextureId >= 0) applyNormalMap(*c.s, st, c.pc->useFaceNormal);
+  V3<R> kd = load3<R>(st.mat.diffuse), eta = load3<R>(st.mat.radiance), k = load3<R>(st.mat.radianceFactor);
+  R ax = std::max(R(st.mat.anisoAlpha[0]), K<R>::EPS), ay = std::max(R(st.mat.anisoAlpha[1]), K<R>::EPS);
+  directLight(
+      c, pl, st,
+      [&](const LRec<R>& l) { return roughConductorEval(l.d, st.V, st.ffN, st.X, st.Y, kd, eta, k, ax, ay, EArea); },
+      [&](const LRec<R>& l) {
+        R NdotL = dot(l.d, st.ffN), NdotV = dot(st.V, st.ffN);
+        if (NdotL < R(0) || NdotV < R(0) || (l.flags & EArea) == 0) return R(0);
+        V3<R> wi = toLocal(st.X, st.Y, st.ffN, l.d), wo = toLocal(st.X, st.Y, st.ffN, st.V);
+        return importanceAnisoGGXPdf(makeNormal(wi + wo), wo, ax, ay);
+      });
+  R u0 = rnd<R>(pl.seed), u1 = rnd<R>(pl.seed);
+  V3<R> N = st.ffN, V = st.V, X = st.X, Y = st.Y;
+  R NdotV = dot(V, N);
+  if (NdotV <= R(0)) {
+    pl.stop = true;
+    return;
+  }
+  V3<R> wo = toLocal(X, Y, N, V);
+  V3<R> wi = importanceSampleAnisoGGX(u0, u1, wo, ax, ay);
+  V3<R> L = toWorld(X, Y, N, wi);
+  V3<R> H = makeNormal(V + L);
+  V3<R> wh = makeNormal(wi + wo);
+  R NdotL = dot(N, L);
+  V3<R> Fs = conductorReflectance3(eta, k, NdotL);
+  R Gs = g1SmithAnisoGGX(NdotV, dot(V, X), dot(V, Y), ax, ay);
+  Gs *= g1SmithAnisoGGX(NdotL, dot(L, X), dot(L, Y), ax, ay);
+  R Ds = dAnisoGGX(dot(H, N), dot(H, X), dot(H, Y), ax, ay);
+  R bpdf = importanceAnisoGGXPdf(wh, wo, ax, ay);
+  V3<R> bw = kd * Fs * Gs * Ds * NdotL;
+  if (bpdf <= R(0) || length(bw) == R(0)) {
+    pl.stop = true;
+    return;
+  }
+  nextRay(pl, st, L, bw, bpdf, EGlossyReflection, true);
+}
+
+// brdf_plastic.rchit:45-93 eval / pdf (smooth coating over a diffuse base).
+// The diffuse-lobe pdf is NdotL * (1 - probSpecular), without 1/pi, as written.
+template <class R>
+V3<R> plasticEval(V3<R> L, V3<R> V, V3<R> N, V3<R> kd, R eta, R fdrInt, R invEta2, uint32_t flags) {
+  R NdotL = dot(L, N), NdotV = dot(V, N);
+  if (NdotL < R(0) || NdotV < R(0)) return V3<R>(0);
+  R Fo = fresnelDielectricExt(NdotV, eta);
+  if (flags & EDelta) return std::fabs(dot(reflect(-V, N), L) - R(1)) < K<R>::EPS ? Fo * V3<R>(1) : V3<R>(0);
+  if (flags & EArea) {
+    R Fi = fresnelDielectricExt(NdotL, eta);
+    V3<R> diff = kd / (V3<R>(1) - kd * fdrInt);
+    return (R(1) - Fi) * (R(1) - Fo) * diff * invEta2 * K<R>::INV_PI * NdotL;
+  }
+  return V3<R>(0);
+}
+template <class R> R plasticPdf(V3<R> L, V3<R> V, V3<R> N, R eta, R ssw, uint32_t flags) {
+  R NdotL = dot(L, N), NdotV = dot(V, N);
+  if (NdotL < R(0) || NdotV < R(0)) return R(0);
+  R Fo = fresnelDielectricExt(NdotV, eta);
+  R ps = (Fo * ssw) / (Fo * ssw + (R(1) - Fo) * (R(1) - ssw));
+  if (flags & EDelta) return std::fabs(dot(reflect(-V, N), L) - R(1)) < K<R>::EPS ? ps : R(0);
+  if (flags & EArea) return NdotL * (R(1) - ps);
+  return R(0);
+}
+
+// brdf_plastic.rchit:133-204
+template <class R> void chsPlastic(const RCtx& c, Payload<R>& pl, HitState<R>& st) {
+  diffuseAndNormalMaps(c, st);
+  if (st.mat.normalTextureId >= 0) applyNormalMap(*c.s, st, c.pc->useFaceNormal);
+  V3<R> kd = load3<R>(st.mat.diffuse);
+  R eta = R(st.mat.ior), fdrInt = R(st.mat.radiance[0]);
+  R dAvg = luminance(kd), sAvg = luminance(V3<R>(1));
+  R ssw = sAvg / (dAvg + sAvg);
+  R invEta2 = R(1) / (eta * eta);
+  V3<R> N = st.ffN, V = st.V;
+  auto probSpec = [&](R Fo) { return (Fo * ssw) / (Fo * ssw + (R(1) - Fo) * (R(1) - ssw)); };
+  directLight(
+      c, pl, st, [&](const LRec<R>& l) { return plasticEval(l.d, V, N, kd, eta, fdrInt, invEta2, EArea); },
+      [&](const LRec<R>& l) { return plasticPdf(l.d, V, N, eta, ssw, l.flags); });
+  R u0 = rnd<R>(pl.seed), u1 = rnd<R>(pl.seed);
+  R NdotV = dot(V, N);
+  if (NdotV <= R(0)) {
+    pl.stop = true;
+    return;
+  }
+  R Fo = fresnelDielectricExt(NdotV, eta);
+  R ps = probSpec(Fo);
+  V3<R> bd, bw;
+  R bpdf;
+  uint32_t bflags;
+  if (u0 < ps) {
+    bd = makeNormal(reflect(-V, N));
+    bflags = ESpecularReflection;
+    bpdf = ps;
+    bw = V3<R>(1) * Fo;
+  } else {
+    u0 = (u0 - ps) / (R(1) + K<R>::EPS - ps);
+    V3<R> wi = cosineSampleHemisphere(u0, u1);
+    R Fi = fresnelDielectricExt(wi.z, eta);
+    V3<R> diff = kd / (V3<R>(1) - kd * fdrInt);
+    bpdf = (R(1) - ps) * cosineHemispherePdf(wi.z);
+    bd = toWorld(st.X, st.Y, N, wi);
+    bflags = EDiffuseReflection;
+    bw = (R(1) - Fi) * (R(1) - Fo) * invEta2 * diff * K<R>::INV_PI * std::fabs(wi.z);
+  }
+  if (bpdf <= R(0) || length(bw) == R(0)) {
+    pl.stop = true;
+    return;
+  }
+  nextRay(pl, st, bd, bw, bpdf, bflags, true);
+}
+
+// brdf_phong.rchit:12-193 (shininess in mat.specular, specular albedo in rhoSpec)
+template <class R> V3<R> phongEval(V3<R> L, V3<R> V, V3<R> N, V3<R> kd, V3<R> ks, R sh, uint32_t flags) {
+  V3<R> weight(0);
+  if ((flags & EArea) == 0) return weight;
+  R NdotL = dot(N, L), NdotV = dot(N, V);
+  if (NdotL < R(0) || NdotV < R(0)) return weight;
+  V3<R> H = makeNormal(L + V);
+  V3<R> diffuseLobe = kd * K<R>::INV_PI;
+  V3<R> specularLobe = ks * std::pow(std::max(dot(H, N), R(0)), sh) * (sh + R(2)) * K<R>::INV_2PI;
+  R db = luminance(kd), sb = luminance(ks);
+  R dw = db / (db + sb);
+  R sw = R(1) - dw;
+  return (dw * diffuseLobe + sw * specularLobe) * NdotL;
+}
+template <class R> R phongLobePdf(V3<R> L, V3<R> V, V3<R> N, R sh) {
+  V3<R> H = makeNormal(L + V);
+  R NdotH = std::max(dot(H, N), R(0)), VdotH = std::max(dot(H, V), R(0));
+  return (sh + R(1)) * K<R>::INV_2PI * std::pow(NdotH, sh) * R(0.25) / (VdotH + K<R>::EPS);
+}
+template <class R> R phongPdf(V3<R> L, V3<R> V, V3<R> N, V3<R> kd, V3<R> ks, R sh, uint32_t flags) {
+  if ((flags & EArea) == 0) return R(0);
+  R db = luminance(kd), sb = luminance(ks);
+  R dw = db / (db + sb);
+  R sw = R(1) - dw;
+  return dw * cosineHemispherePdf(dot(N, L)) + sw * phongLobePdf(L, V, N, sh);
+}
+
+template <class R> void chsPhong(const RCtx& c, Payload<R>& pl, HitState<R>& st) {
+  const GpuPushConstantRaytrace& pc = *c.pc;
+  diffuseAndNormalMaps(c, st);
+  if (st.mat.normalTextureId >= 0) applyNormalMap(*c.s, st, pc.useFaceNormal);
+  V3<R> kd = load3<R>(st.mat.diffuse), ks = load3<R>(st.mat.rhoSpec);
+  R sh = R(st.mat.specular);
+  if (pl.depth == 1) {
+    if (pc.diffuseOutChannel >= 0) pl.ch[pc.diffuseOutChannel] = kd;
+    if (pc.normalOutChannel >= 0) pl.ch[pc.normalOutChannel] = st.ffN;
+    if (pc.specularOutChannel >= 0) pl.ch[pc.specularOutChannel] = ks;
+    if (pc.tangentOutChannel >= 0) pl.ch[pc.tangentOutChannel] = st.X;
+    if (pc.roughnessOutChannel >= 0) pl.ch[pc.roughnessOutChannel] = V3<R>(1, 1, 0);
+    if (pc.positionOutChannel >= 0) pl.ch[pc.positionOutChannel] = st.pos;
+    if (pc.uvOutChannel >= 0) pl.ch[pc.uvOutChannel] = V3<R>(st.uvx, st.uvy, 1);
+  }
+  directLight(
+      c, pl, st, [&](const LRec<R>& l) { return phongEval(l.d, st.V, st.ffN, kd, ks, sh, EArea); },
+      [&](const LRec<R>& l) { return phongPdf(l.d, st.V, st.ffN, kd, ks, sh, l.flags); });
+  R u0 = rnd<R>(pl.seed), u1 = rnd<R>(pl.seed);
+  R db = luminance(kd), sb = luminance(ks);
+  R dw = db / (db + sb);
+  R sw = R(1) - dw;
+  V3<R> bd;
+  R bpdf;
+  uint32_t bflags;
+  if (u0 < dw) {
+    u0 /= dw;
+    V3<R> wi = cosineSampleHemisphere(u0, u1);
+    bpdf = cosineHemispherePdf(wi.z);
+    bd = toWorld(st.X, st.Y, st.ffN, wi);
+    bflags = EDiffuseReflection;
+  } else {
+    u0 = (u0 - dw) / sw;
+    R cosTheta = std::pow(u0, R(1) / (sh + R(1)));
+    R phi = K<R>::TWO_PI * u1;
+    R sinTheta = safeSqrt(R(1) - cosTheta * cosTheta);
+    V3<R> wh(sinTheta * std::sin(phi), sinTheta * std::cos(phi), cosTheta);
+    V3<R> H = toWorld(st.X, st.Y, st.ffN, wh);
+    V3<R> L = reflect(-st.V, H);
+    bpdf = phongLobePdf(L, st.V, st.ffN, sh);
+    bd = L;
+    bflags = EGlossyReflection;
+  }
+  V3<R> bw = phongEval(bd, st.V, st.ffN, kd, ks, sh, EArea);
+  if (bpdf <= R(0) || length(bw) == R(0)) {
+    pl.stop = true;
+    return;
+  }
+  nextRay(pl, st, bd, bw, bpdf, bflags, false);
+}
+
+// brdf_kang18.rchit:57-249 (GGX + Lambert with its own luminance weights)
+template <class R> R lum18(V3<R> c) { return R(0.212671) * c.x + R(0.715160) * c.y + R(0.072169) * c.z; }
+template <class R>
+V3<R> kangEval(V3<R> L, V3<R> V, V3<R> N, V3<R> X, V3<R> Y, V3<R> kd, V3<R> ks, R ax, R ay, R eta) {
+  V3<R> weight(0);
+  R NdotL = dot(N, L), NdotV = dot(N, V);
+  if (NdotL <= R(0) || NdotV <= R(0)) return weight;
+  V3<R> H = makeNormal(L + V);
+  R HdotN = dot(H, N), HdotX = dot(H, X), HdotY = dot(H, Y), HdotV = dot(H, V);
+  R F0 = sqr((eta - R(1)) / (eta + R(1)));
+  R Fs = F0 + (R(1) - F0) * std::pow(std::min(std::max(R(1) - HdotV, R(0)), R(1)), R(5));
+  R Ds = dAnisoGGX(HdotN, HdotX, HdotY, ax, ay);
+  R Gs = g1SmithAnisoGGX(NdotV, dot(V, X), dot(V, Y), ax, ay);
+  Gs *= g1SmithAnisoGGX(NdotL, dot(L, X), dot(L, Y), ax, ay);
+  V3<R> diffuse = kd * K<R>::INV_PI;
+  V3<R> specular = ks * Fs * Ds * Gs;
+  weight = weight + diffuse;
+  weight = weight + specular;
+  return weight * NdotL;
+}
+template <class R>
+R kangPdf(V3<R> L, V3<R> V, V3<R> N, V3<R> X, V3<R> Y, R ax, R ay, V3<R> kd, V3<R> ks, uint32_t flags) {
+  if ((flags & EArea) == 0) return R(0);
+  R NdotL = dot(N, L), NdotV = dot(N, V);
+  if (NdotL <= R(0) || NdotV <= R(0)) return R(0);
+  V3<R> H = makeNormal(L + V);
+  V3<R> wh = makeNormal(toLocal(X, Y, N, H)), wo = makeNormal(toLocal(X, Y, N, V)), wi = makeNormal(toLocal(X, Y, N, L));
+  R pd = lum18(kd), ps = lum18(ks);
+  R P = pd / (pd + ps + K<R>::EPS);
+  R pdf = 0;
+  pdf += P * cosineHemispherePdf(wi.z);
+  pdf += (R(1) - P) * importanceAnisoGGXPdf(wh, wo, ax, ay);
+  return pdf;
+}
+
+template <class R> void chsKang18(const RCtx& c, Payload<R>& pl, HitState<R>& st) {
+  const orc_scene& s = *c.s;
+  const GpuPushConstantRaytrace& pc = *c.pc;
+  R t4[4];
+  diffuseAndNormalMaps(c, st);
+  if (st.mat.metalnessTextureId >= 0) {
+    V3<R> v = textureEval3(s, st.mat.metalnessTextureId, st.uvx, st.uvy, t4);
+    st.mat.rhoSpec[0] = float(v.x), st.mat.rhoSpec[1] = float(v.y), st.mat.rhoSpec[2] = float(v.z);
+  }
+  if (st.mat.roughnessTextureId >= 0) {
+    textureEval3(s, st.mat.roughnessTextureId, st.uvx, st.uvy, t4);
+    st.mat.anisoAlpha[0] = float(t4[0]), st.mat.anisoAlpha[1] = float(t4[1]);
+  }
+  R opacity = R(st.mat.metalness);
+  if (st.mat.opacityTextureId >= 0) {
+    textureEval3(s, st.mat.opacityTextureId, st.uvx, st.uvy, t4);
+    opacity = t4[0];
+  }
+  const float* w = st.w2o;
+  auto objToWorldNormal = [&](V3<R> n) {  // (n * gl_WorldToObjectEXT).xyz
+    return V3<R>(n.x * R(w[0]) + n.y * R(w[4]) + n.z * R(w[8]), n.x * R(w[1]) + n.y * R(w[5]) + n.z * R(w[9]),
+                 n.x * R(w[2]) + n.y * R(w[6]) + n.z * R(w[10]));
+  };
+  if (st.mat.normalTextureId >= 0) {
+    V3<R> cn = textureEval3(s, st.mat.normalTextureId, st.uvx, st.uvy, t4);
+    st.N = makeNormal(objToWorldNormal(R(2) * cn - V3<R>(1)));
+    if (pc.useFaceNormal == 1) st.N = V3<R>(0);
+    basis(st.N, st.X, st.Y);
+    st.ffN = dot(st.N, st.V) > R(0) ? st.N : -st.N;
+  }
+  if (st.mat.tangentTextureId >= 0) {
+    V3<R> ct = textureEval3(s, st.mat.tangentTextureId, st.uvx, st.uvy, t4);
+    st.X = makeNormal(objToWorldNormal(R(2) * ct - V3<R>(1)));
+  }
+  st.Y = makeNormal(cross(st.N, st.X));
+  st.X = makeNormal(cross(st.Y, st.N));
+  st.ffN = dot(st.N, st.V) > R(0) ? st.N : -st.N;
+  if (rnd<R>(pl.seed) < opacity) {
+    pl.ro = offsetPositionAlongNormal(st.pos, -st.ffN);
+    pl.depth--;
+    return;
+  }
+  R ax = std::max(R(st.mat.anisoAlpha[0]), K<R>::EPS), ay = std::max(R(st.mat.anisoAlpha[1]), K<R>::EPS);
+  R eta = R(st.mat.ior);
+  V3<R> kd = load3<R>(st.mat.diffuse), ks = load3<R>(st.mat.rhoSpec);
+  if (pl.depth == 1) {
+    if (pc.diffuseOutChannel >= 0) pl.ch[pc.diffuseOutChannel] = kd;
+    if (pc.normalOutChannel >= 0) pl.ch[pc.normalOutChannel] = st.ffN;
+    if (pc.specularOutChannel >= 0) pl.ch[pc.specularOutChannel] = ks;
+    if (pc.tangentOutChannel >= 0) pl.ch[pc.tangentOutChannel] = st.X;
+    if (pc.roughnessOutChannel >= 0) pl.ch[pc.roughnessOutChannel] = V3<R>(ax, ay, 0);
+    if (pc.positionOutChannel >= 0) pl.ch[pc.positionOutChannel] = st.pos;
+    if (pc.uvOutChannel >= 0) pl.ch[pc.uvOutChannel] = V3<R>(st.uvx, st.uvy, 1);
+  }
+  directLight(
+      c, pl, st, [&](const LRec<R>& l) { return kangEval(l.d, st.V, st.ffN, st.X, st.Y, kd, ks, ax, ay, eta); },
+      [&](const LRec<R>& l) { return kangPdf(l.d, st.V, st.ffN, st.X, st.Y, ax, ay, kd, ks, l.flags); });
+  R u0 = rnd<R>(pl.seed), u1 = rnd<R>(pl.seed);
+  V3<R> wo = makeNormal(toLocal(st.X, st.Y, st.ffN, st.V));
+  R pd = lum18(kd), ps = lum18(ks);
+  R P = pd / (pd + ps + K<R>::EPS);
+  V3<R> wi;
+  R bpdf;
+  uint32_t bflags;
+  if (rnd<R>(pl.seed) < P) {
+    wi = cosineSampleHemisphere(u0, u1);
+    bflags = EDiffuseReflection;
+    bpdf = cosineHemispherePdf(wi.z);
+  } else {
+    wi = importanceSampleAnisoGGX(u0, u1, wo, ax, ay);
+    bpdf = importanceAnisoGGXPdf(makeNormal(wi + wo), wo, ax, ay);
+    bflags = EGlossyReflection;
+  }
+  V3<R> bd = toWorld(st.X, st.Y, st.ffN, wi);
+  V3<R> bw = kangEval(bd, st.V, st.ffN, st.X, st.Y, kd, ks, ax, ay, eta);
+  if (bpdf <= R(0) || length(bw) == R(0)) {
+    pl.stop = true;
+    return;
+  }
+  nextRay(pl, st, bd, bw, bpdf, bflags, true);
+}
+
 template <class R> void chsEmissive(const RCtx& c, Payload<R>& pl, HitState<R>& st) {
   // brdf_emissive.rchit:12-26
   pl.stop = true;
@@ -1440,6 +1877,12 @@ void renderPixel(const RCtx& c, uint32_t x, uint32_t y, int32_t frame, R& wsum, 
           case MaterialTypeBrdfRoughPlastic: chsRoughPlastic(c, pl, st); break;
           case MaterialTypeBsdfDielectric: chsDielectric(c, pl, st); break;
           case MaterialTypeBrdfEmissive: chsEmissive(c, pl, st); break;
+          case MaterialTypeBrdfMirror: chsMirror(c, pl, st); break;
+          case MaterialTypeBrdfConductor: chsConductor(c, pl, st); break;
+          case MaterialTypeBrdfRoughConductor: chsRoughConductor(c, pl, st); break;
+          case MaterialTypeBrdfPlastic: chsPlastic(c, pl, st); break;
+          case MaterialTypeBrdfPhong: chsPhong(c, pl, st); break;
+          case MaterialTypeBrdfKang18: chsKang18(c, pl, st); break;
           default: pl.stop = true; break;
         }
       }
@@ -1925,6 +2368,31 @@ void orc_bsdf_eval(const GpuMaterial* m, const float Lp[3], const float Vp[3], u
       r = dielEval(L, V, N, eta, flags);
       break;
     }
+    case MaterialTypeBrdfMirror:
+      if ((flags & EDelta) && std::fabs(dot(reflect(-V, N), L) - 1.f) < K<float>::EPS) r = kd;
+      break;
+    case MaterialTypeBrdfConductor: {
+      float NdotL = dot(L, N), NdotV = dot(V, N);
+      if (!(NdotL < 0 || NdotV < 0 || (flags & EDelta) == 0) && std::fabs(dot(reflect(-V, N), L) - 1.f) < K<float>::EPS)
+        r = kd * conductorReflectance3(load3<float>(m->radiance), load3<float>(m->radianceFactor), NdotL);
+      break;
+    }
+    case MaterialTypeBrdfRoughConductor: {
+      float ax = std::max(m->anisoAlpha[0], K<float>::EPS), ay = std::max(m->anisoAlpha[1], K<float>::EPS);
+      r = roughConductorEval(L, V, N, X, Y, kd, load3<float>(m->radiance), load3<float>(m->radianceFactor), ax, ay, flags);
+      break;
+    }
+    case MaterialTypeBrdfPlastic: {
+      float eta = m->ior;
+      r = plasticEval(L, V, N, kd, eta, m->radiance[0], 1.f / (eta * eta), flags);
+      break;
+    }
+    case MaterialTypeBrdfPhong: r = phongEval(L, V, N, kd, load3<float>(m->rhoSpec), m->specular, flags); break;
+    case MaterialTypeBrdfKang18: {
+      float ax = std::max(m->anisoAlpha[0], K<float>::EPS), ay = std::max(m->anisoAlpha[1], K<float>::EPS);
+      r = kangEval(L, V, N, X, Y, kd, load3<float>(m->rhoSpec), ax, ay, m->ior);
+      break;
+    }
     default: break;
   }
   out[0] = r.x, out[1] = r.y, out[2] = r.z;
@@ -1947,9 +2415,33 @@ float orc_bsdf_pdf(const GpuMaterial* m, const float Lp[3], const float Vp[3], u
       float eta = V.z > 0 ? 1.f / m->ior : m->ior;
       return dielPdf(L, V, N, eta, flags);
     }
+    case MaterialTypeBrdfMirror:
+    case MaterialTypeBrdfConductor:
+      return (flags & EDelta) && dot(L, N) >= 0 && dot(V, N) >= 0 && std::fabs(dot(reflect(-V, N), L) - 1.f) < K<float>::EPS
+                 ? 1.f
+                 : 0.f;
+    case MaterialTypeBrdfRoughConductor: {
+      float ax = std::max(m->anisoAlpha[0], K<float>::EPS), ay = std::max(m->anisoAlpha[1], K<float>::EPS);
+      if (dot(L, N) < 0 || dot(V, N) < 0 || (flags & EArea) == 0) return 0;
+      V3<float> wi = toLocal(X, Y, N, L), wo = toLocal(X, Y, N, V);
+      return importanceAnisoGGXPdf(makeNormal(wi + wo), wo, ax, ay);
+    }
+    case MaterialTypeBrdfPlastic: {
+      V3<float> kd = load3<float>(m->diffuse);
+      float dAvg = luminance(kd), sAvg = luminance(V3<float>(1));
+      return plasticPdf(L, V, N, m->ior, sAvg / (dAvg + sAvg), flags);
+    }
+    case MaterialTypeBrdfPhong:
+      return phongPdf(L, V, N, load3<float>(m->diffuse), load3<float>(m->rhoSpec), m->specular, flags);
+    case MaterialTypeBrdfKang18: {
+      float ax = std::max(m->anisoAlpha[0], K<float>::EPS), ay = std::max(m->anisoAlpha[1], K<float>::EPS);
+      return kangPdf(L, V, N, X, Y, ax, ay, load3<float>(m->diffuse), load3<float>(m->rhoSpec), flags);
+    }
     default: return 0;
   }
 }
+
+float orc_conductor_reflectance(float eta, float k, float cosThetaI) { return conductorReflectance(eta, k, cosThetaI); }
 
 void orc_bsdf_sample(const GpuMaterial* m, const float Vp[3], uint32_t* seed, float w[3], float dir[3], float* pdf,
                      uint32_t* flags) {

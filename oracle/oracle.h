@@ -98,6 +98,8 @@ float orc_compute_diffuse_fresnel(float ior, int sample_count);
 void orc_bsdf_eval(const GpuMaterial* m, const float L[3], const float V[3], uint32_t flags,
                    float out[3]);
 float orc_bsdf_pdf(const GpuMaterial* m, const float L[3], const float V[3], uint32_t flags);
+/* Shirley's conductor Fresnel (brdf_conductor.rchit:14-34). */
+float orc_conductor_reflectance(float eta, float k, float cos_theta_i);
 /* Sample: draws from *seed in reference order; returns weight (f*cos, or
  * f*cos/pdf where the shader divides later is NOT applied), direction,
  * pdf, flags. */

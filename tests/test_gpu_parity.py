@@ -211,6 +211,44 @@ def test_rough_plastic_and_emissive_scene(renderer, tmp_path):
     np.testing.assert_allclose(mg, mo, rtol=5e-3)
 
 
+def test_more_plugins_scene_matches_oracle(renderer, tmp_path):
+    """Mirror, conductor (Au), rough conductor (Cu, anisotropic), smooth
+    plastic, phong and kang18 surfaces in the box, lit by its rect light plus
+    a point light (delta light paths through every plugin's pdf)."""
+    d = tmp_path / "more"
+    d.mkdir()
+    sp = A.generate_scene(0, str(d))
+    import json
+    js = json.load(open(sp))
+    mats = js["materials"]
+    mats[0] = {"name": mats[0]["name"], "type": "brdf_mirror", "diffuse_reflectance": [0.9, 0.85, 0.8]}
+    mats[1] = {"name": mats[1]["name"], "type": "brdf_conductor", "material": "Au"}
+    mats[2] = {"name": mats[2]["name"], "type": "brdf_rough_conductor", "material": "Cu", "alpha": [0.2, 0.35],
+               "diffuse_reflectance": [1, 1, 1]}
+    mats[3] = {"name": mats[3]["name"], "type": "brdf_plastic", "diffuse_reflectance": [0.2, 0.4, 0.7], "ior": 1.6}
+    mats[4] = {"name": mats[4]["name"], "type": "brdf_phong", "diffuse_reflectance": [0.5, 0.3, 0.2],
+               "specular_reflectance": [0.3, 0.3, 0.3], "shininess": 30}
+    mats[5] = {"name": mats[5]["name"], "type": "brdf_kang18", "diffuse_reflectance": [0.3, 0.5, 0.3],
+               "specular_reflectance": [0.5, 0.5, 0.5], "alpha": [0.2, 0.4]}
+    for m in mats:
+        if m["type"] in ("brdf_conductor",):
+            m["diffuse_reflectance"] = [1, 1, 1]
+    js["lights"].append({"type": "point", "radiance": [1.0, 1.0, 1.0], "position": [0.3, 0.6, 0.2]})
+    json.dump(js, open(sp, "w"))
+    s, _ = _load(renderer, sp)
+    spp = 16
+    st, ost, g, o, sq = _radiance_parity(renderer, s, spp)
+    frac_bad = _three_sigma(g[0], o[0], sq, spp)
+    mg, mo = g[0][..., :3].mean((0, 1)), o[0][..., :3].mean((0, 1))
+    log(f"more plugins: outside 3 sigma {frac_bad:.5f}, means {mg} vs {mo}; rays gpu {st['extension_rays']} "
+        f"oracle {ost['extension_rays']}, zero-NEE shadow {st['dielectric_shadow_rays']} vs "
+        f"{ost['dielectric_shadow_rays']}")
+    assert frac_bad <= 0.01
+    np.testing.assert_allclose(mg, mo, rtol=1e-2)
+    assert abs(st["extension_rays"] - ost["extension_rays"]) / ost["extension_rays"] < 1e-3
+    assert st["dielectric_shadow_rays"] > 0
+
+
 def test_env_tables_on_device_equal_oracle(renderer, spheres_small):
     s, _ = _load(renderer, spheres_small)
     env = np.ctypeslib.as_array(s.desc.envmap.rgba, (s.desc.envmap.height, s.desc.envmap.width, 4)).copy()

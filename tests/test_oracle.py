@@ -548,3 +548,102 @@ def test_accumulation_weighted_mean_recurrence(tmp_path):
     # frame 0 is unjittered: weight (1 - e^-8)^2 ; 15 jittered frames follow
     w0 = (1 - math.exp(-8)) ** 2
     assert img[8, 0, 0, 0] > w0 and img[8, 0, 0, 0] < 16 * w0
+
+
+# ------------------------------------------------------------------ further plugins of the set
+def test_conductor_fresnel_closed_forms():
+    """Shirley's conductor Fresnel: normal incidence ((n-1)^2 + k^2)/((n+1)^2 + k^2);
+    grazing incidence 1; with k = 0 it is the unpolarised dielectric Fresnel."""
+    L = O.lib()
+    for n, k in [(0.2004, 3.9129), (1.6575, 9.2239), (2.5, 0.5)]:
+        f0 = L.orc_conductor_reflectance(n, k, 1.0)
+        assert abs(f0 - ((n - 1) ** 2 + k ** 2) / ((n + 1) ** 2 + k ** 2)) < 2e-6
+        assert abs(L.orc_conductor_reflectance(n, k, 0.0) - 1.0) < 1e-6
+    for n in (1.33, 1.5, 2.4):
+        for ct in (0.95, 0.7, 0.3):
+            st = math.sqrt(1 - ct * ct)
+            ctt = math.sqrt(1 - (st / n) ** 2)
+            rs = (ct - n * ctt) / (ct + n * ctt)
+            rp = (n * ct - ctt) / (n * ct + ctt)
+            assert abs(L.orc_conductor_reflectance(n, 0.0, ct) - 0.5 * (rs * rs + rp * rp)) < 2e-5
+
+
+def test_mirror_and_conductor_are_delta_lobes():
+    """eval / pdf are nonzero only for EDelta and the mirror direction; the
+    value there is the albedo (mirror) or albedo x Fresnel (conductor)."""
+    L = O.lib()
+    V = (0.6, 0.0, 0.8)
+    Lr = (-0.6, 0.0, 0.8)
+    out = (C.c_float * 3)()
+    mi = _mat(8, diffuse=(0.9, 0.5, 0.2))
+    L.orc_bsdf_eval(C.byref(mi), O.f3(Lr), O.f3(V), 1, out)
+    np.testing.assert_allclose(list(out), [0.9, 0.5, 0.2], rtol=1e-6)
+    L.orc_bsdf_eval(C.byref(mi), O.f3(Lr), O.f3(V), 2, out)
+    assert list(out) == [0, 0, 0]
+    assert L.orc_bsdf_pdf(C.byref(mi), O.f3(Lr), O.f3(V), 1) == 1.0
+    assert L.orc_bsdf_pdf(C.byref(mi), O.f3((0.0, 0.6, 0.8)), O.f3(V), 1) == 0.0
+    co = _mat(6, diffuse=(1.0, 1.0, 1.0), radiance=(0.2004, 0.9240, 1.1022), radianceFactor=(3.9129, 2.4528, 2.1422))
+    L.orc_bsdf_eval(C.byref(co), O.f3(Lr), O.f3(V), 1, out)
+    for c, (n, k) in enumerate([(0.2004, 3.9129), (0.9240, 2.4528), (1.1022, 2.1422)]):
+        assert abs(out[c] - L.orc_conductor_reflectance(n, k, 0.8)) < 1e-6
+
+
+def test_plastic_pdf_keeps_the_missing_one_over_pi():
+    """brdf_plastic's diffuse-lobe pdf is NdotL (1 - p_spec) without 1/pi, so
+    it integrates to pi (1 - p_spec) over the hemisphere (p_spec from the
+    Fresnel at the view angle and the sampling weight 1 / (1 + lum(kd)))."""
+    L = O.lib()
+    kd = (0.5, 0.4, 0.3)
+    m = _mat(4, diffuse=kd, ior=1.5)
+    V = (0.0, 0.0, 1.0)
+    total = _hemisphere_integral(lambda l: L.orc_bsdf_pdf(C.byref(m), O.f3(l), O.f3(V), 2), 200, 8)
+    fo = ((1.5 - 1) / (1.5 + 1)) ** 2
+    lum = 0.2126 * kd[0] + 0.7152 * kd[1] + 0.0722 * kd[2]
+    ssw = 1.0 / (lum + 1.0)
+    ps = fo * ssw / (fo * ssw + (1 - fo) * (1 - ssw))
+    assert abs(total - math.pi * (1 - ps)) < 2e-3 * math.pi
+
+
+def test_phong_pdf_integrates_to_one():
+    """Mixture of the cosine pdf and the normalised Phong half-vector lobe
+    (n+1)/(2 pi) cos^n with the 1/(4 h.v) Jacobian: integrates to ~1 at
+    normal view (the +EPS guard biases it by < 1 %)."""
+    L = O.lib()
+    for sh in (10.0, 40.0):
+        m = _mat(11, diffuse=(0.5, 0.5, 0.5), rhoSpec=(0.4, 0.4, 0.4), specular=sh)
+        V = O.f3((0.0, 0.0, 1.0))
+        total = _hemisphere_integral(lambda l: L.orc_bsdf_pdf(C.byref(m), O.f3(l), V, 2), 600, 16)
+        assert 0.985 < total < 1.001, (sh, total)
+
+
+def test_phong_specular_lobe_is_normalised():
+    """With no diffuse albedo the value is (n+2)/(2 pi) cos^n(theta_h) cos(theta_l);
+    at the mirror direction of normal view that is (n+2)/(2 pi)."""
+    L = O.lib()
+    out = (C.c_float * 3)()
+    m = _mat(11, diffuse=(0.0, 0.0, 0.0), rhoSpec=(1.0, 1.0, 1.0), specular=20.0)
+    L.orc_bsdf_eval(C.byref(m), O.f3((0.0, 0.0, 1.0)), O.f3((0.0, 0.0, 1.0)), 2, out)
+    assert abs(out[0] - 22.0 / (2 * math.pi)) < 1e-4
+
+
+def test_kang18_pdf_integrates_to_one():
+    """Lambert + anisotropic GGX mixture with kang18's own luminance weights."""
+    L = O.lib()
+    m = _mat(1, diffuse=(0.3, 0.3, 0.3), rhoSpec=(0.6, 0.6, 0.6), anisoAlpha=(0.3, 0.5), ior=1.5)
+    V = O.f3((0.0, 0.0, 1.0))
+    total = _hemisphere_integral(lambda l: L.orc_bsdf_pdf(C.byref(m), O.f3(l), V, 2), 800, 64)
+    # Expected from the definitions: the cosine lobe integrates to 1; the GGX
+    # lobe keeps only half-vectors within 45 deg of n (others reflect V = n
+    # below the horizon), a fraction E_phi[1 / (1 + ax^2 cos^2 + ay^2 sin^2)]
+    # of the anisotropic GGX; the lobe weight is lum18(kd) / (lum18(kd) +
+    # lum18(ks) + EPS) with kang18's weights (0.212671, 0.715160, 0.072169).
+    phi = (np.arange(4096) + 0.5) / 4096 * 2 * np.pi
+    inside = np.mean(1.0 / (1.0 + 0.3 ** 2 * np.cos(phi) ** 2 + 0.5 ** 2 * np.sin(phi) ** 2))
+    P = 0.3 / (0.3 + 0.6 + 1e-3)
+    expected = P + (1 - P) * inside
+    assert abs(total - expected) < 0.01 * expected, (total, expected)
+    # delta lights: pdf 0 (the shader's pdf requires EArea) but eval is still the BRDF
+    assert L.orc_bsdf_pdf(C.byref(m), O.f3((0.0, 0.0, 1.0)), V, 1) == 0.0
+    out = (C.c_float * 3)()
+    L.orc_bsdf_eval(C.byref(m), O.f3((0.0, 0.0, 1.0)), V, 1, out)
+    assert out[0] > 0.3 / math.pi
