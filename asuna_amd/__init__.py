@@ -158,7 +158,7 @@ class JobDesc(C.Structure):
 
 class JobStatus(C.Structure):
     _fields_ = [("state", C.c_int32), ("pad", C.c_uint32), ("submit_ms", C.c_double), ("start_ms", C.c_double),
-                ("done_ms", C.c_double), ("paths", C.c_uint64)]
+                ("done_ms", C.c_double), ("paths", C.c_uint64), ("denoise_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
@@ -234,6 +234,8 @@ def lib():
         "ash_server_last_error": (C.c_char_p, [vp]),
         "ash_server_add_scene": (C.c_int, [vp, P(SceneDesc), P(BuildOpts), P(C.c_uint32)]),
         "ash_server_submit": (C.c_int, [vp, P(JobDesc), P(C.c_uint64)]),
+        "ash_server_set_denoiser": (C.c_int, [vp, vp]),
+        "ash_denoise_planes": (C.c_int, [vp, vp, vp, vp, vp, C.c_uint32, C.c_uint32, vp, P(C.c_double)]),
         "ash_server_wait": (C.c_int, [vp, C.c_uint64, C.c_int32, P(JobStatus)]),
         "ash_server_read": (C.c_int, [vp, C.c_uint64, C.c_int32, P(C.c_float)]),
         "ash_server_release": (C.c_int, [vp, C.c_uint64]),
@@ -515,8 +517,12 @@ class Server:
         self._scenes.append(scene)  # keeps the host arrays alive
         return sid.value
 
-    def submit(self, scene_id, cam, pc, width, height, spp, depth_channel=-1) -> int:
-        d = JobDesc(scene_id, width, height, spp, cam, pc, depth_channel, 0)
+    def set_denoiser(self, denoiser):
+        self._den = denoiser  # keeps it alive
+        self._check(lib().ash_server_set_denoiser(self._s, denoiser._d), "set_denoiser")
+
+    def submit(self, scene_id, cam, pc, width, height, spp, depth_channel=-1, denoise=False) -> int:
+        d = JobDesc(scene_id, width, height, spp, cam, pc, depth_channel, 1 if denoise else 0)
         jid = C.c_uint64()
         self._check(lib().ash_server_submit(self._s, C.byref(d), C.byref(jid)), "submit")
         self._sizes[jid.value] = (height, width)

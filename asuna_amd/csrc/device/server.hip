@@ -78,6 +78,8 @@ struct Job {
   int inflight = 0;
   int32_t state = ASH_JOB_QUEUED;
   double tSubmit = 0, tStart = -1, tDone = -1;
+  double denoiseMs = 0;
+  uint32_t planes = ASH_NUM_OUTPUT_IMAGES;  // + 1 denoised plane with ASH_JOB_DENOISE
 };
 
 struct Batch {
@@ -120,6 +122,7 @@ struct ash_server {
   ash_server_counters st{};
   double busySince = -1;
   int loopError = 0;
+  ash_denoiser* den = nullptr;
 
   double now() const { return std::chrono::duration<double, std::milli>(Clock::now() - t0).count(); }
 };
@@ -237,10 +240,22 @@ static void schedulerLoop(ash_server* s) {
         for (Job* j : bt.jobs) {
           if (--j->inflight == 0 && j->issued) {
             j->state = q == hipSuccess ? ASH_JOB_DONE : ASH_E_HIP;
-            j->tDone = t;
+            if (j->state == ASH_JOB_DONE && (j->d.flags & ASH_JOB_DENOISE)) {
+              // the batch event has completed: the job's planes are final
+              const size_t P = j->P;
+              auto plane = [&](int32_t ch) -> const void* {
+                return ch >= 0 && ch < ASH_NUM_OUTPUT_IMAGES - 1 ? j->sp.images + size_t(ch + 1) * P : nullptr;
+              };
+              int rc = ash_denoise_planes(s->den, j->sp.images, plane(j->d.pc.diffuseOutChannel),
+                                          plane(j->d.pc.normalOutChannel), plane(j->d.depth_channel), j->d.width,
+                                          j->d.height, j->sp.images + 9 * P, &j->denoiseMs);
+              if (rc) j->state = rc;
+            }
+            j->tDone = s->now();
             s->st.jobs_done++;
           }
         }
+        (void)t;
         sc->inflight.pop_front();
         progress = true;
         s->cvDone.notify_all();
@@ -394,6 +409,13 @@ int ash_server_add_scene(ash_server* s, const ash_scene_desc* desc, const ash_bu
   return ASH_OK;
 }
 
+int ash_server_set_denoiser(ash_server* s, ash_denoiser* den) {
+  if (!s) return ASH_E_INVALID;
+  std::lock_guard<std::mutex> lk(s->m);
+  s->den = den;
+  return ASH_OK;
+}
+
 int ash_server_submit(ash_server* s, const ash_job_desc* d, uint64_t* id) {
   if (!s || !d || !id) return ASH_E_INVALID;
   std::lock_guard<std::mutex> lk(s->m);
@@ -402,13 +424,16 @@ int ash_server_submit(ash_server* s, const ash_job_desc* d, uint64_t* id) {
   if (d->width == 0 || d->height == 0 || d->spp <= 0 || uint64_t(d->width) * d->height > (1u << 28))
     return sfail(s, ASH_E_INVALID, "bad job size");
   if (d->depth_channel >= int32_t(ASH_NUM_OUTPUT_IMAGES - 1)) return sfail(s, ASH_E_INVALID, "depth channel out of range");
+  if (d->flags & ~uint32_t(ASH_JOB_DENOISE)) return sfail(s, ASH_E_INVALID, "unknown job flags");
+  if ((d->flags & ASH_JOB_DENOISE) && !s->den) return sfail(s, ASH_E_STATE, "denoise flag without a denoiser");
   int rc = ctxCheckPc(s->scenes[d->scene]->ctx, &d->pc);
   if (rc) return sfail(s, rc, "bad push constants: %s", ctxError(s->scenes[d->scene]->ctx));
   Job* j = new Job();
   j->id = s->nextId++;
   j->d = *d;
   j->P = d->width * d->height;
-  j->bytes = uint64_t(j->P) * ASH_NUM_OUTPUT_IMAGES * sizeof(float4);
+  j->planes = ASH_NUM_OUTPUT_IMAGES + ((d->flags & ASH_JOB_DENOISE) ? 1 : 0);
+  j->bytes = uint64_t(j->P) * j->planes * sizeof(float4);
   if (Arena::round(j->bytes) > s->arena.size) {
     delete j;
     return sfail(s, ASH_E_NOMEM, "job images exceed the HBM budget");
@@ -431,6 +456,7 @@ static void fillStatus(const Job* j, ash_job_status* st) {
   st->start_ms = j->tStart;
   st->done_ms = j->tDone;
   st->paths = uint64_t(j->P) * uint64_t(j->d.spp);
+  st->denoise_ms = j->denoiseMs;
 }
 
 int ash_server_wait(ash_server* s, uint64_t id, int32_t timeout_ms, ash_job_status* st) {
@@ -449,7 +475,7 @@ int ash_server_wait(ash_server* s, uint64_t id, int32_t timeout_ms, ash_job_stat
 }
 
 int ash_server_read(ash_server* s, uint64_t id, int32_t image, float* out) {
-  if (!s || !out || image < 0 || image >= ASH_NUM_OUTPUT_IMAGES) return ASH_E_INVALID;
+  if (!s || !out || image < 0 || image > ASH_NUM_OUTPUT_IMAGES) return ASH_E_INVALID;
   const float4* src;
   size_t plane;
   {
@@ -458,6 +484,7 @@ int ash_server_read(ash_server* s, uint64_t id, int32_t image, float* out) {
     if (it == s->jobs.end()) return sfail(s, ASH_E_INVALID, "unknown job");
     Job* j = it->second;
     if (j->state != ASH_JOB_DONE) return sfail(s, ASH_E_STATE, "job not finished");
+    if (uint32_t(image) >= j->planes) return sfail(s, ASH_E_INVALID, "image %d not kept for this job", image);
     plane = j->P;
     src = j->sp.images + size_t(image) * plane;
   }

@@ -907,39 +907,23 @@ int ash_denoiser_train_step(ash_denoiser* d, uint32_t n, uint32_t h, uint32_t w,
   return ASH_OK;
 }
 
-int ash_denoise_film(ash_denoiser* d, ash_ctx* ctx, int32_t albedoCh, int32_t normalCh, int32_t depthCh, int32_t outImage,
-                     double* ms) {
-  if (!d || !ctx || outImage < 0 || outImage >= ASH_NUM_OUTPUT_IMAGES) return ASH_E_INVALID;
-  uint32_t W = 0, H = 0;
-  ashd::ctxFilm(ctx, &W, &H);
-  if (!W || !H) return dfail(d, ASH_E_STATE, "context has no film");
-  auto plane = [&](int32_t ch) -> const float4* {
-    if (ch < 0 || ch >= ASH_NUM_OUTPUT_IMAGES - 1) return nullptr;
-    void* p = nullptr;
-    return ash_output_device_ptr(ctx, ch + 1, &p) == ASH_OK ? static_cast<const float4*>(p) : nullptr;
-  };
-  void* rad = nullptr;
-  void* outp = nullptr;
-  if (ash_output_device_ptr(ctx, 0, &rad) || ash_output_device_ptr(ctx, outImage, &outp))
-    return dfail(d, ASH_E_STATE, "film planes unavailable");
+int ash_denoise_planes(ash_denoiser* d, const void* radiance, const void* albedo, const void* normal,
+                       const void* depth, uint32_t W, uint32_t H, void* out, double* ms) {
+  if (!d || !radiance || !out || !W || !H) return ASH_E_INVALID;
   DH(d, hipSetDevice(d->device));
   const uint32_t Hp = (H + 15) & ~15u, Wp = (W + 15) & ~15u;
   int rc = ensureShape(d, 1, Hp, Wp);
   if (rc) return rc;
-  hipStream_t s = ashd::ctxStream(ctx);
-  hipEvent_t e0, e1, ready;
+  hipEvent_t e0, e1;
   DH(d, hipEventCreate(&e0));
   DH(d, hipEventCreate(&e1));
-  DH(d, hipEventCreate(&ready));
-  // the denoiser works on its own stream after the context's pending renders
-  DH(d, hipEventRecord(ready, s));
-  DH(d, hipStreamWaitEvent(d->stream, ready, 0));
   DH(d, hipEventRecord(e0, d->stream));
-  kFeaturesFilm<<<blocks(size_t(Wp) * Hp), 256, 0, d->stream>>>(static_cast<const float4*>(rad), plane(albedoCh),
-                                                               plane(normalCh), plane(depthCh), W, H, Wp, Hp, d->x.a.p);
+  kFeaturesFilm<<<blocks(size_t(Wp) * Hp), 256, 0, d->stream>>>(
+      static_cast<const float4*>(radiance), static_cast<const float4*>(albedo), static_cast<const float4*>(normal),
+      static_cast<const float4*>(depth), W, H, Wp, Hp, d->x.a.p);
   rc = forwardPass(d);
   if (rc) return rc;
-  kFilmOut<<<blocks(size_t(W) * H), 256, 0, d->stream>>>(d->outF.p, 32, W, H, Wp, static_cast<float4*>(outp));
+  kFilmOut<<<blocks(size_t(W) * H), 256, 0, d->stream>>>(d->outF.p, 32, W, H, Wp, static_cast<float4*>(out));
   DH(d, hipGetLastError());
   DH(d, hipEventRecord(e1, d->stream));
   DH(d, hipEventSynchronize(e1));
@@ -948,8 +932,33 @@ int ash_denoise_film(ash_denoiser* d, ash_ctx* ctx, int32_t albedoCh, int32_t no
   if (ms) *ms = t;
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(ready);
   return ASH_OK;
+}
+
+int ash_denoise_film(ash_denoiser* d, ash_ctx* ctx, int32_t albedoCh, int32_t normalCh, int32_t depthCh, int32_t outImage,
+                     double* ms) {
+  if (!d || !ctx || outImage < 0 || outImage >= ASH_NUM_OUTPUT_IMAGES) return ASH_E_INVALID;
+  uint32_t W = 0, H = 0;
+  ashd::ctxFilm(ctx, &W, &H);
+  if (!W || !H) return dfail(d, ASH_E_STATE, "context has no film");
+  auto plane = [&](int32_t ch) -> const void* {
+    if (ch < 0 || ch >= ASH_NUM_OUTPUT_IMAGES - 1) return nullptr;
+    void* p = nullptr;
+    return ash_output_device_ptr(ctx, ch + 1, &p) == ASH_OK ? p : nullptr;
+  };
+  void* rad = nullptr;
+  void* outp = nullptr;
+  if (ash_output_device_ptr(ctx, 0, &rad) || ash_output_device_ptr(ctx, outImage, &outp))
+    return dfail(d, ASH_E_STATE, "film planes unavailable");
+  DH(d, hipSetDevice(d->device));
+  // the denoiser works on its own stream after the context's pending renders
+  hipEvent_t ready;
+  DH(d, hipEventCreate(&ready));
+  DH(d, hipEventRecord(ready, ashd::ctxStream(ctx)));
+  DH(d, hipStreamWaitEvent(d->stream, ready, 0));
+  int rc = ash_denoise_planes(d, rad, plane(albedoCh), plane(normalCh), plane(depthCh), W, H, outp, ms);
+  (void)hipEventDestroy(ready);
+  return rc;
 }
 
 }  // extern "C"

@@ -374,6 +374,8 @@ def run_serving(A, args):
         return jobs
 
     srv = A.Server(0, hbm_budget_bytes=16 << 30, queue_paths=1 << 21)
+    den = A.Denoiser(0, seed=1)  # weights do not change the cost; c4 trains one
+    srv.set_denoiser(den)
     for s in scenes:
         srv.add_scene(s)
     rs = []
@@ -386,15 +388,17 @@ def run_serving(A, args):
     for sid in range(len(scenes)):
         cam, pc = scenes[sid].shot(0)
         cam = A.camera_lookat(256, 256, 45.0, (0.0, 1.0, 4.0), (0.0, 0.3, 0.0))
-        j = srv.submit(sid, cam, pc, 256, 256, 1)
+        st = scenes[sid].state
+        j = srv.submit(sid, cam, pc, 256, 256, 1, depth_channel=st.depth_out_channel, denoise=True)
         srv.wait(j)
         srv.release(j)
         rs[sid].set_film(256, 256)
-        rs[sid].render(cam, pc, 0, 1)
+        rs[sid].render(cam, pc, 0, 1, depth_channel=st.depth_out_channel)
+        den.denoise_film(rs[sid], pc.diffuseOutChannel, pc.normalOutChannel, st.depth_out_channel, 7)
     out = {"metric": "render serving: jobs/s and p50/p99 job latency", "unit": "jobs/s", "n_gpus": 1,
            "config": {"workload": "8 resident scenes (1.2e4, 8.2e4, 1.0e6, 5.2e6 triangles x 2 seeds); Poisson "
                                   "arrivals; resolution {256^2,512^2,1024^2,1920x1080} w {.4,.3,.2,.1}; "
-                                  "spp {1,4,16,64}; denoise flag drawn but not applied (no denoiser stage)",
+                                  "spp {1,4,16,64}; denoise flag on 50 % of jobs (U-Net on the job's planes)",
                       "jobs_per_rate": args.jobs, "queue_paths": 1 << 21},
            "rates": []}
     for lam in rates:
@@ -408,7 +412,8 @@ def run_serving(A, args):
                 dt = arrivals[k] - (time.perf_counter() - base)
                 if dt > 0:
                     time.sleep(dt)
-                ids[k] = srv.submit(*jb[:6])
+                sid = jb[0]
+                ids[k] = srv.submit(*jb[:6], depth_channel=scenes[sid].state.depth_out_channel, denoise=jb[6])
 
         th = threading.Thread(target=feeder)
         th.start()
@@ -424,11 +429,14 @@ def run_serving(A, args):
                   "p99_ms": round(float(np.percentile(lat, 99)), 1), "mpaths_per_s": round(paths / span / 1e6, 1)}
         # baseline: the same jobs rendered one at a time (FIFO)
         service = []
-        for sid, cam, pc, w, h, spp, _ in jobs:
+        for sid, cam, pc, w, h, spp, dn in jobs:
             r = rs[sid]
+            dch = scenes[sid].state.depth_out_channel
             ts = time.perf_counter()
             r.set_film(w, h)
-            r.render(cam, pc, 0, spp)
+            r.render(cam, pc, 0, spp, depth_channel=dch)
+            if dn:
+                den.denoise_film(r, pc.diffuseOutChannel, pc.normalOutChannel, dch, 7)
             service.append(time.perf_counter() - ts)
         finish, blat = 0.0, []
         for a, sv in zip(arrivals, service):
@@ -440,6 +448,7 @@ def run_serving(A, args):
         log(f"[serve] rate {lam}/s: server {served}, one-at-a-time {single}")
         out["rates"].append({"rate": lam, "server": served, "single_job": single})
     srv.close()
+    den.close()
     for r in rs:
         r.close()
     out["value"] = out["rates"][-1]["server"]["jobs_per_s"]

@@ -48,9 +48,16 @@ int ash_denoiser_train_step(ash_denoiser* d, uint32_t n, uint32_t h, uint32_t w,
 /* Denoise a render context's film in HBM: radiance = image 0, albedo /
  * normal / depth = AOV channels (image index = channel + 1; depth uses .x),
  * result written to image `out_image` (RGBA32F, alpha 1).  Runs on the
- * context's stream; returns after completion.  ms = device time. */
+ * denoiser's stream after the context's pending work; returns after
+ * completion.  ms = device time. */
 int ash_denoise_film(ash_denoiser* d, ash_ctx* ctx, int32_t albedo_channel, int32_t normal_channel,
                      int32_t depth_channel, int32_t out_image, double* ms);
+
+/* Same on raw device planes (RGBA32F, width x height; albedo / normal /
+ * depth may be null = zero features).  The caller orders the planes'
+ * producers before this call (it runs on the denoiser's own stream). */
+int ash_denoise_planes(ash_denoiser* d, const void* radiance, const void* albedo, const void* normal,
+                       const void* depth, uint32_t width, uint32_t height, void* out, double* ms);
 
 #ifdef __cplusplus
 }

@@ -18,6 +18,7 @@
 #ifndef ASUNA_SERVER_H
 #define ASUNA_SERVER_H
 
+#include "asuna_denoise.h"
 #include "asuna_hip.h"
 
 #ifdef __cplusplus
@@ -41,18 +42,23 @@ typedef struct ash_job_desc {
   GpuCamera camera;
   GpuPushConstantRaytrace pc; /* as for ash_render */
   int32_t depth_channel;      /* AOV channel for first-hit depth, -1 = none */
-  uint32_t flags;             /* reserved, 0 */
+  uint32_t flags;             /* ASH_JOB_DENOISE or 0 */
 } ash_job_desc;
 
 enum { ASH_JOB_QUEUED = 0, ASH_JOB_RUNNING = 1, ASH_JOB_DONE = 2 };
+/* Job flag: after its last frame the job's radiance is denoised (guides:
+ * pc.diffuseOutChannel, pc.normalOutChannel, depth_channel) into image 9,
+ * which ash_server_read then accepts.  Needs ash_server_set_denoiser. */
+enum { ASH_JOB_DENOISE = 1 };
 
 typedef struct ash_job_status {
   int32_t state;       /* ASH_JOB_* or a negative ASH_E_* code */
   uint32_t pad;
   double submit_ms;    /* server clock (ms since ash_server_create) */
   double start_ms;     /* first batch enqueued */
-  double done_ms;      /* last batch observed complete */
+  double done_ms;      /* last batch observed complete (and denoised) */
   uint64_t paths;
+  double denoise_ms;   /* device time of the denoise step, 0 if none */
 } ash_job_status;
 
 typedef struct ash_server_counters {
@@ -73,6 +79,10 @@ const char* ash_server_last_error(ash_server* srv);
 int ash_server_add_scene(ash_server* srv, const ash_scene_desc* desc, const ash_build_opts* opts,
                          uint32_t* scene_id);
 
+/* Denoiser used for ASH_JOB_DENOISE jobs (owned by the caller; not used
+ * elsewhere while the server runs). */
+int ash_server_set_denoiser(ash_server* srv, ash_denoiser* d);
+
 /* Queue a job; returns at once with its id. */
 int ash_server_submit(ash_server* srv, const ash_job_desc* job, uint64_t* job_id);
 
@@ -80,7 +90,7 @@ int ash_server_submit(ash_server* srv, const ash_job_desc* job, uint64_t* job_id
  * Returns ASH_OK when done, ASH_E_STATE on timeout. */
 int ash_server_wait(ash_server* srv, uint64_t job_id, int32_t timeout_ms, ash_job_status* status);
 
-/* Copy output image `image` (0..8) of a finished job to host RGBA32F. */
+/* Copy output image `image` (0..8, or 9 = denoised) of a finished job to host RGBA32F. */
 int ash_server_read(ash_server* srv, uint64_t job_id, int32_t image, float* host_rgba);
 
 /* Forget a finished job and return its images to the arena. */

@@ -90,3 +90,33 @@ def test_server_admission_waits_for_budget(box_scene):
     with pytest.raises(A.AsunaError):
         srv.submit(sid, cam, pc, 1024, 1024, 1)  # larger than the whole budget
     srv.close()
+
+
+def test_server_denoise_flag_equals_film_denoise(spheres_small):
+    """A job with the denoise flag gets image 9 = the denoiser run on its own
+    radiance / albedo / normal / depth planes, identical to denoising the
+    same render in a plain context."""
+    s = A.HostScene(spheres_small)
+    st = s.state
+    cam, pc = s.shot(0)
+    w, h = s.film
+    den = A.Denoiser(0, seed=3)
+    srv = A.Server(0, hbm_budget_bytes=256 << 20, queue_paths=1 << 17)
+    srv.set_denoiser(den)
+    sid = srv.add_scene(s)
+    jid = srv.submit(sid, cam, pc, w, h, 4, depth_channel=st.depth_out_channel, denoise=True)
+    status = srv.wait(jid, 60000)
+    got = srv.read(jid, 9)
+    log(f"server denoise: {status['denoise_ms']:.2f} ms")
+    assert status["denoise_ms"] > 0
+    srv.release(jid)
+    srv.close()
+    r = A.Renderer(0)
+    r.upload(s)
+    r.build()
+    r.set_film(w, h)
+    r.render(cam, pc, 0, 4, depth_channel=st.depth_out_channel)
+    den.denoise_film(r, pc.diffuseOutChannel, pc.normalOutChannel, st.depth_out_channel, 7)
+    np.testing.assert_array_equal(got, r.read(7))
+    r.close()
+    den.close()
