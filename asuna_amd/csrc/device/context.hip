@@ -594,7 +594,7 @@ int ash_upload_scene(ash_ctx* c, const ash_scene_desc* d) {
     if (id.light_id < 0) {
       if (id.material_id >= d->num_materials) return fail(c, ASH_E_INVALID, "instance %u: material id out of range", i);
       uint32_t t = d->materials[id.material_id].type;
-      if (t >= MaterialTypeNum || t == MaterialTypeBrdfDisney)
+      if (t >= MaterialTypeNum)
         return fail(c, ASH_E_UNSUPPORTED, "instance %u: material type %u is outside the supported plugin set", i, t);
     }
     DevInstance& di = hinst[i];

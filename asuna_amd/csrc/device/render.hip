@@ -1139,6 +1139,187 @@ __device__ f3 kangEval(f3 L, f3 V, f3 N, f3 X, f3 Y, f3 kd, f3 ks, float ax, flo
   return weight * NdotL;
 }
 
+// Disney BRDF, brdf_disney.rchit:31-405: Burley diffuse with the
+// Hanrahan-Krueger subsurface blend and sheen, anisotropic GGX specular
+// sampled through visible normals, GTR1 clearcoat.  GLSL mix(x, y, a) is
+// x (1 - a) + y a.  The lobe parameters not held in registers (subsurface,
+// tints, sheen, clearcoat) are read from the material record.
+struct DsnMat {
+  f3 base;
+  float metallic, rough, ax, ay;  // rough = max(roughness^2, 0.001)
+  const GpuMaterial* m;
+};
+__device__ __forceinline__ float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+__device__ __forceinline__ f3 mix3(f3 x, f3 y, float a) { return x * (1.0f - a) + y * a; }
+__device__ __forceinline__ float dsnSchlick(float u) {
+  float x = clampf(1.0f - u, 0.0f, 1.0f);
+  float x2 = x * x;
+  return x2 * x2 * x;
+}
+__device__ __forceinline__ float dsnDielF(float ci, float eta) {
+  float st2 = eta * eta * (1.0f - ci * ci);
+  if (st2 > 1.0f) return 1.0f;
+  float ct = sqrtf(fmaxf(1.0f - st2, 0.0f));
+  float rs = (eta * ct - ci) / (eta * ct + ci);
+  float rp = (eta * ci - ct) / (eta * ci + ct);
+  return 0.5f * (rs * rs + rp * rp);
+}
+__device__ __forceinline__ float dsnFresnel(float metallic, float eta, float LdotH, float VdotH) {
+  return mixf(dsnDielF(fabsf(VdotH), eta), dsnSchlick(LdotH), metallic);
+}
+__device__ __forceinline__ float dsnSmithG(float NdotV, float alphaG) {
+  float a = alphaG * alphaG, b = NdotV * NdotV;
+  return (2.0f * NdotV) / (NdotV + sqrtf(a + b - a * b));
+}
+__device__ __forceinline__ float dsnSmithGAniso(float NdotV, float VdotX, float VdotY, float ax, float ay) {
+  float a = VdotX * ax, b = VdotY * ay, c = NdotV;
+  return (2.0f * NdotV) / (NdotV + sqrtf(a * a + b * b + c * c));
+}
+__device__ __forceinline__ float dsnGTR1(float NdotH, float a) {
+  if (a >= 1.0f) return kInvPi;
+  float a2 = a * a;
+  float t = 1.0f + (a2 - 1.0f) * NdotH * NdotH;
+  return (a2 - 1.0f) / (kPi * logf(a2) * t);
+}
+__device__ __forceinline__ float dsnGTR2Aniso(float NdotH, float HdotX, float HdotY, float ax, float ay) {
+  float a = HdotX / ax, b = HdotY / ay;
+  float c = a * a + b * b + NdotH * NdotH;
+  return 1.0f / (kPi * ax * ay * c * c);
+}
+__device__ __forceinline__ float dsnCcRough(const DsnMat& d) { return mixf(0.1f, 0.001f, d.m->clearcoatGloss); }
+__device__ __forceinline__ f3 dsnEvalDiffuse(const DsnMat& d, f3 csheen, f3 V, f3 L, f3 H, float& pdf) {
+  pdf = 0.0f;
+  if (L.z <= 0.0f) return mk1(0.f);
+  float FL = dsnSchlick(L.z), FV = dsnSchlick(V.z), FH = dsnSchlick(dot(L, H));
+  float LH = dot(L, H);
+  float Fd90 = 0.5f + 2.0f * LH * LH * d.rough;
+  float Fd = mixf(1.0f, Fd90, FL) * mixf(1.0f, Fd90, FV);
+  float Fss90 = LH * LH * d.rough;
+  float Fss = mixf(1.0f, Fss90, FL) * mixf(1.0f, Fss90, FV);
+  float ss = 1.25f * (Fss * (1.0f / (L.z + V.z) - 0.5f) + 0.5f);
+  f3 Fsheen = FH * d.m->sheen * csheen;
+  pdf = L.z * kInvPi;
+  return (1.0f - d.metallic) * (kInvPi * mixf(Fd, ss, d.m->subsurface) * d.base + Fsheen);
+}
+__device__ __forceinline__ f3 dsnEvalSpec(const DsnMat& d, float eta, f3 specCol, f3 V, f3 L, f3 H, float& pdf) {
+  pdf = 0.0f;
+  if (L.z <= 0.0f) return mk1(0.f);
+  float FM = dsnFresnel(d.metallic, eta, dot(L, H), dot(V, H));
+  f3 F = mix3(specCol, mk1(1.f), FM);
+  float D = dsnGTR2Aniso(H.z, H.x, H.y, d.ax, d.ay);
+  float G1 = dsnSmithGAniso(fabsf(V.z), V.x, V.y, d.ax, d.ay);
+  float G2 = G1 * dsnSmithGAniso(fabsf(L.z), L.x, L.y, d.ax, d.ay);
+  pdf = G1 * D / (4.0f * V.z);
+  return F * D * G2 / (4.0f * L.z * V.z);
+}
+__device__ __forceinline__ f3 dsnEvalClearcoat(const DsnMat& d, f3 V, f3 L, f3 H, float& pdf) {
+  pdf = 0.0f;
+  if (L.z <= 0.0f) return mk1(0.f);
+  float FH = dsnDielF(dot(V, H), 1.0f / 1.5f);
+  float F = mixf(0.04f, 1.0f, FH);
+  float D = dsnGTR1(H.z, dsnCcRough(d));
+  float G = dsnSmithG(L.z, 0.25f) * dsnSmithG(V.z, 0.25f);
+  float jac = 1.0f / (4.0f * dot(V, H));
+  pdf = D * H.z * jac;
+  return mk1(0.25f) * d.m->clearcoat * F * D * G / (4.0f * L.z * V.z);
+}
+__device__ __forceinline__ void dsnColors(const DsnMat& d, float eta, f3& specCol, f3& sheenCol) {
+  float lum = lum18(d.base);
+  f3 ctint = lum > 0.0f ? d.base / lum : mk1(1.f);
+  float F0 = (1.0f - eta) / (1.0f + eta);
+  specCol = mix3(F0 * F0 * mix3(mk1(1.f), ctint, d.m->specularTint), d.base, d.metallic);
+  sheenCol = mix3(mk1(1.f), ctint, d.m->sheenTint);
+}
+__device__ __forceinline__ void dsnLobes(const DsnMat& d, f3 specCol, float approxF, float& dw, float& sw, float& cw) {
+  dw = lum18(d.base) * (1.0f - d.metallic);
+  sw = lum18(mix3(specCol, mk1(1.f), approxF));
+  cw = 0.25f * d.m->clearcoat * (1.0f - d.metallic);
+  float total = dw + sw + cw;
+  dw /= total;
+  sw /= total;
+  cw /= total;
+}
+// eval (EArea), brdf_disney.rchit:242-296
+__device__ f3 dsnEval(f3 Lw, f3 Vw, f3 N, f3 X, f3 Y, const DsnMat& d, float eta, float& bsdfPdf) {
+  bsdfPdf = 0.0f;
+  f3 V = toLocal(X, Y, N, Vw), L = toLocal(X, Y, N, Lw);
+  if (L.z <= 0.0f || V.z <= 0.0f) return mk1(0.f);
+  f3 H = makeNormal(L + V);
+  if (H.z < 0.0f) H = -H;
+  f3 specCol, sheenCol;
+  dsnColors(d, eta, specCol, sheenCol);
+  float dw, sw, cw;
+  dsnLobes(d, specCol, dsnFresnel(d.metallic, eta, dot(L, H), dot(V, H)), dw, sw, cw);
+  f3 w = mk1(0.f);
+  float pdf;
+  if (dw > 0.0f) {
+    w = w + dsnEvalDiffuse(d, sheenCol, V, L, H, pdf);
+    bsdfPdf += pdf * dw;
+  }
+  if (sw > 0.0f) {
+    w = w + dsnEvalSpec(d, eta, specCol, V, L, H, pdf);
+    bsdfPdf += pdf * sw;
+  }
+  if (cw > 0.0f) {
+    w = w + dsnEvalClearcoat(d, V, L, H, pdf);
+    bsdfPdf += pdf * cw;
+  }
+  return w * L.z;
+}
+// sampleBsdf, brdf_disney.rchit:298-363.  SampleGTR1 draws phi and
+// cos theta from the same r1 (r2 unused), as written.  The weight is
+// f |dot(N, L)| with N in world space and L local, as written.
+__device__ f3 dsnSample(float r1, float r2, f3 Vw, f3 N, f3 X, f3 Y, const DsnMat& d, float eta, f3& dir,
+                        float& pdf, uint32_t& flags) {
+  pdf = 0.0f;
+  f3 V = toLocal(X, Y, N, Vw), L, f;
+  f3 specCol, sheenCol;
+  dsnColors(d, eta, specCol, sheenCol);
+  float dw, sw, cw;
+  dsnLobes(d, specCol, dsnFresnel(d.metallic, eta, V.z, V.z), dw, sw, cw);
+  float cdf0 = dw, cdf1 = cdf0 + cw;
+  if (r1 < cdf0) {
+    r1 /= cdf0;
+    L = cosineHemisphere(r1, r2);
+    f = dsnEvalDiffuse(d, sheenCol, V, L, makeNormal(L + V), pdf);
+    pdf *= dw;
+    flags = kDiffuseRefl;
+  } else if (r1 < cdf1) {
+    r1 = (r1 - cdf0) / (cdf1 - cdf0);
+    float a = fmaxf(0.001f, dsnCcRough(d));
+    float a2 = a * a;
+    float phi = r1 * kTwoPi;
+    float ct = sqrtf((1.0f - powf(a2, 1.0f - r1)) / (1.0f - a2));
+    float st = clampf(sqrtf(1.0f - ct * ct), 0.0f, 1.0f);
+    f3 H = f3{st * cosf(phi), st * sinf(phi), ct};
+    if (H.z < 0.0f) H = -H;
+    L = makeNormal(reflect(-V, H));
+    f = dsnEvalClearcoat(d, V, L, H, pdf);
+    pdf *= cw;
+    flags = kGlossyRefl;
+  } else {
+    r1 = (r1 - cdf1) / (1.0f - cdf1);
+    // visible-normal sampling (Heitz, JCGT 2018), brdf_disney.rchit:96-115
+    f3 Vh = makeNormal(f3{d.ax * V.x, d.ay * V.y, V.z});
+    float lensq = Vh.x * Vh.x + Vh.y * Vh.y;
+    f3 T1 = lensq > 0.0f ? f3{-Vh.y, Vh.x, 0.0f} * (1.0f / sqrtf(lensq)) : f3{1.0f, 0.0f, 0.0f};
+    f3 T2 = cross(Vh, T1);
+    float r = sqrtf(r1), phi = 2.0f * kPi * r2;
+    float t1 = r * cosf(phi), t2 = r * sinf(phi);
+    float sv = 0.5f * (1.0f + Vh.z);
+    t2 = (1.0f - sv) * sqrtf(1.0f - t1 * t1) + sv * t2;
+    f3 Nh = t1 * T1 + t2 * T2 + sqrtf(fmaxf(0.0f, 1.0f - t1 * t1 - t2 * t2)) * Vh;
+    f3 H = makeNormal(f3{d.ax * Nh.x, d.ay * Nh.y, fmaxf(0.0f, Nh.z)});
+    if (H.z < 0.0f) H = -H;
+    L = makeNormal(reflect(-V, H));
+    f = dsnEvalSpec(d, eta, specCol, V, L, H, pdf);
+    pdf *= sw;
+    flags = kGlossyRefl;
+  }
+  dir = toWorld(X, Y, N, L);
+  return f * fabsf(dot(N, L));
+}
+
 // Surface plugins.  Every closest-hit shader of the set runs the same three
 // steps -- material inputs (textures, normal map, AOVs, opacity draws), one
 // light sample through sampleLights, then its BSDF value toward the light
@@ -1149,15 +1330,17 @@ __device__ f3 kangEval(f3 L, f3 V, f3 N, f3 X, f3 Y, f3 kd, f3 ks, float ax, flo
 //   brdf_rough_plastic.rchit:191-269, bsdf_dielectric.rchit:74-138,
 //   brdf_mirror.rchit:39-101, brdf_conductor.rchit:89-155,
 //   brdf_rough_conductor.rchit:147-220, brdf_plastic.rchit:133-204,
-//   brdf_phong.rchit:112-193, brdf_kang18.rchit:141-249
+//   brdf_phong.rchit:112-193, brdf_kang18.rchit:141-249,
+//   brdf_disney.rchit:365-477
 //
 // The plugins are compiled as two groups (GROUP2 = false: lambertian, pbr,
 // rough plastic, dielectric; true: mirror, conductor, rough conductor,
-// plastic, phong, kang18) so that each shading kernel keeps the register
-// budget of one group.
+// plastic, phong, kang18, disney) so that each shading kernel keeps the
+// register budget of one group.
 __device__ __forceinline__ bool isGroup2(uint32_t t) {
   return t == MaterialTypeBrdfMirror || t == MaterialTypeBrdfConductor || t == MaterialTypeBrdfRoughConductor ||
-         t == MaterialTypeBrdfPlastic || t == MaterialTypeBrdfPhong || t == MaterialTypeBrdfKang18;
+         t == MaterialTypeBrdfPlastic || t == MaterialTypeBrdfPhong || t == MaterialTypeBrdfKang18 ||
+         t == MaterialTypeBrdfDisney;
 }
 
 template <bool GROUP2>
@@ -1171,7 +1354,8 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
              dielectric = !GROUP2 && type == MaterialTypeBsdfDielectric,
              mirror = GROUP2 && type == MaterialTypeBrdfMirror, conductor = GROUP2 && type == MaterialTypeBrdfConductor,
              rcond = GROUP2 && type == MaterialTypeBrdfRoughConductor, smooth = GROUP2 && type == MaterialTypeBrdfPlastic,
-             phong = GROUP2 && type == MaterialTypeBrdfPhong, kang = GROUP2 && type == MaterialTypeBrdfKang18;
+             phong = GROUP2 && type == MaterialTypeBrdfPhong, kang = GROUP2 && type == MaterialTypeBrdfKang18,
+             disney = GROUP2 && type == MaterialTypeBrdfDisney;
   // NEE is identically zero for the delta lobes (their eval tests EDelta but
   // is called with EArea); their shadow ray is counted, not traced.
   const bool zeroNee = dielectric || mirror || conductor;
@@ -1180,12 +1364,18 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
   float metal = 0.f, ax = 0.f, ay = 0.f, eta = m.ior, fdrInt = 0.f, invEta2 = 0.f, ssw = 0.f;
   f3 ceta = mk1(0.f), ck = mk1(0.f);
   if (!dielectric && m.diffuseTextureId >= 0) kd = xyz(tex2D(sc.textures[m.diffuseTextureId], h.uvx, h.uvy));
-  if (pbr) {
+  if (pbr || disney) {
     metal = m.metalness;
     float rough = m.roughness;
     if (m.metalnessTextureId >= 0) metal = tex2D(sc.textures[m.metalnessTextureId], h.uvx, h.uvy).x;
     if (m.roughnessTextureId >= 0) rough = tex2D(sc.textures[m.roughnessTextureId], h.uvx, h.uvy).x;
     ax = ay = fmaxf(sqr(rough), 0.001f);
+    if (disney) {  // brdf_disney.rchit:419-421
+      float aspect = sqrtf(1.0f - m.anisotropic * 0.9f);
+      fdrInt = ax;  // max(roughness^2, 0.001)
+      ax = fmaxf(0.001f, rough * rough / aspect);
+      ay = fmaxf(0.001f, rough * rough * aspect);
+    }
   } else if (plastic || rcond || kang) {
     float alx = m.anisoAlpha[0], aly = m.anisoAlpha[1];
     if (m.roughnessTextureId >= 0) {
@@ -1246,8 +1436,8 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
   } else if (m.normalTextureId >= 0) {
     normalMap(sc, h, m.normalTextureId, pc.useFaceNormal);
   }
-  if (pbr) {
-    float opacity = m.specular;
+  if (pbr || disney) {
+    float opacity = pbr ? m.specular : m.rhoSpec[0];
     if (m.opacityTextureId >= 0) opacity = tex2D(sc.textures[m.opacityTextureId], h.uvx, h.uvy).x;
     if (rnd(c.seed) < opacity) {
       // pass through: same direction, same depth (pbr:156-159)
@@ -1256,7 +1446,8 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
       return;
     }
   }
-  if (dielectric) eta = dot(h.V, h.N) > 0.0f ? (1.0f / m.ior) : m.ior;
+  if (dielectric || disney) eta = dot(h.V, h.N) > 0.0f ? (1.0f / m.ior) : m.ior;
+  const DsnMat dm{kd, metal, fdrInt, ax, ay, &m};
   if (aov) {
     if (lambert) {
       writeAov(c, pc.diffuseOutChannel, kd, pix);
@@ -1349,6 +1540,8 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
         bsdfPdf += P * cosinePdf(wi.z);
         bsdfPdf += (1.0f - P) * pdfGGX(wh, wo, ax, ay);
       }
+    } else if (disney) {  // eval is called with EArea whatever the light
+      bw = dsnEval(l.d, V, N, h.X, h.Y, dm, eta, bsdfPdf);
     }
     float mis = powerHeuristic(l.pdf, bsdfPdf);
     Ld = mis * bw * radiance * c.T / (l.pdf + kEps);
@@ -1548,6 +1741,9 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
     }
     d = toWorld(X, Y, N, wi);
     w = kangEval(d, V, N, X, Y, kd, ks, ax, ay, eta);
+  } else if (disney) {
+    epsInDivisor = false;
+    w = dsnSample(u0, u1, V, N, X, Y, dm, eta, d, pdf, flags);
   }
   if (pdf <= 0.0f || length(w) == 0.0f) {
     c.stop = true;
@@ -1616,7 +1812,7 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
         f3 e = ld3(m.radiance);
         if (m.radianceTextureId >= 0) e = ld3(m.radianceFactor) * xyz(tex2D(sc.textures[m.radianceTextureId], h.uvx, h.uvy));
         if (pc.ignoreEmissive == 0) c.L = c.L + e * c.T;
-      } else if (type != MaterialTypeBrdfDisney && type < MaterialTypeNum) {
+      } else if (type < MaterialTypeNum) {
         shadeSurface<GROUP2>(c, h, m, pix, aov);
       } else {
         c.stop = true;

@@ -461,7 +461,7 @@ class Loader {
         if (mj.contains("alpha")) vec2(mj["alpha"], m.anisoAlpha[0], m.anisoAlpha[1]);
         optTex("alpha_texture", m.roughnessTextureId);
       }
-    } else if (t == "brdf_disney") {  // material.cpp:358-388 (parsed; not rendered)
+    } else if (t == "brdf_disney") {  // material.cpp:358-388
       m.type = MaterialTypeBrdfDisney;
       optTex("normal_texture", m.normalTextureId);
       opt3("diffuse_reflectance", m.diffuse);

@@ -1773,6 +1773,264 @@ template <class R> void chsKang18(const RCtx& c, Payload<R>& pl, HitState<R>& st
   nextRay(pl, st, bd, bw, bpdf, bflags, true);
 }
 
+// brdf_disney.rchit:31-405 (the reflection lobes: diffuse with retro-
+// reflection / fake subsurface / sheen, anisotropic GGX through visible-
+// normal sampling, GTR1 clearcoat).  GLSL mix(x, y, a) = x (1 - a) + y a.
+template <class R> struct DisneyMat {
+  V3<R> baseColor;
+  R anisotropic, metallic, roughness, subsurface, specularTint, sheen, sheenTint, clearcoat, clearcoatRoughness, ior,
+      ax, ay;
+};
+template <class R> R mixf(R x, R y, R a) { return x * (R(1) - a) + y * a; }
+template <class R> R clamp01(R x) { return std::fmin(std::fmax(x, R(0)), R(1)); }
+template <class R> R dsnLum(V3<R> c) { return R(0.212671) * c.x + R(0.715160) * c.y + R(0.072169) * c.z; }
+template <class R> R dsnGTR1(R NDotH, R a) {
+  if (a >= R(1)) return K<R>::INV_PI;
+  R a2 = a * a;
+  R t = R(1) + (a2 - R(1)) * NDotH * NDotH;
+  return (a2 - R(1)) / (K<R>::PI * std::log(a2) * t);
+}
+template <class R> V3<R> dsnSampleGTR1(R rgh, R r1, R r2) {
+  R a = std::max(R(0.001), rgh);
+  R a2 = a * a;
+  R phi = r1 * K<R>::TWO_PI;
+  R cosTheta = std::sqrt((R(1) - std::pow(a2, R(1) - r1)) / (R(1) - a2));
+  R sinTheta = clamp01(std::sqrt(R(1) - (cosTheta * cosTheta)));
+  return V3<R>(sinTheta * std::cos(phi), sinTheta * std::sin(phi), cosTheta);
+}
+template <class R> V3<R> dsnSampleVNDF(V3<R> V, R ax, R ay, R r1, R r2) {
+  V3<R> Vh = makeNormal(V3<R>(ax * V.x, ay * V.y, V.z));
+  R lensq = Vh.x * Vh.x + Vh.y * Vh.y;
+  V3<R> T1 = lensq > R(0) ? V3<R>(-Vh.y, Vh.x, R(0)) * (R(1) / std::sqrt(lensq)) : V3<R>(1, 0, 0);
+  V3<R> T2 = cross(Vh, T1);
+  R r = std::sqrt(r1);
+  R phi = R(2) * K<R>::PI * r2;
+  R t1 = r * std::cos(phi), t2 = r * std::sin(phi);
+  R sv = R(0.5) * (R(1) + Vh.z);
+  t2 = (R(1) - sv) * std::sqrt(R(1) - t1 * t1) + sv * t2;
+  V3<R> Nh = t1 * T1 + t2 * T2 + std::sqrt(std::max(R(0), R(1) - t1 * t1 - t2 * t2)) * Vh;
+  return makeNormal(V3<R>(ax * Nh.x, ay * Nh.y, std::max(R(0), Nh.z)));
+}
+template <class R> R dsnGTR2Aniso(R NDotH, R HDotX, R HDotY, R ax, R ay) {
+  R a = HDotX / ax, b = HDotY / ay;
+  R c = a * a + b * b + NDotH * NDotH;
+  return R(1) / (K<R>::PI * ax * ay * c * c);
+}
+template <class R> R dsnSmithG(R NDotV, R alphaG) {
+  R a = alphaG * alphaG, b = NDotV * NDotV;
+  return (R(2) * NDotV) / (NDotV + std::sqrt(a + b - a * b));
+}
+template <class R> R dsnSmithGAniso(R NDotV, R VDotX, R VDotY, R ax, R ay) {
+  R a = VDotX * ax, b = VDotY * ay, c = NDotV;
+  return (R(2) * NDotV) / (NDotV + std::sqrt(a * a + b * b + c * c));
+}
+template <class R> R dsnSchlick(R u) {
+  R m = clamp01(R(1) - u);
+  R m2 = m * m;
+  return m2 * m2 * m;
+}
+template <class R> R dsnDielectricFresnel(R cosThetaI, R eta) {
+  R sinThetaTSq = eta * eta * (R(1) - cosThetaI * cosThetaI);
+  if (sinThetaTSq > R(1)) return R(1);
+  R cosThetaT = std::sqrt(std::max(R(1) - sinThetaTSq, R(0)));
+  R rs = (eta * cosThetaT - cosThetaI) / (eta * cosThetaT + cosThetaI);
+  R rp = (eta * cosThetaI - cosThetaT) / (eta * cosThetaI + cosThetaT);
+  return R(0.5) * (rs * rs + rp * rp);
+}
+template <class R> R dsnFresnel(R metallic, R eta, R LDotH, R VDotH) {
+  return mixf(dsnDielectricFresnel(std::fabs(VDotH), eta), dsnSchlick(LDotH), metallic);
+}
+template <class R> V3<R> dsnEvalDiffuse(const DisneyMat<R>& m, V3<R> Csheen, V3<R> V, V3<R> L, V3<R> H, R& pdf) {
+  pdf = 0;
+  if (L.z <= R(0)) return V3<R>(0);
+  R FL = dsnSchlick(L.z), FV = dsnSchlick(V.z), FH = dsnSchlick(dot(L, H));
+  R Fd90 = R(0.5) + R(2) * dot(L, H) * dot(L, H) * m.roughness;
+  R Fd = mixf(R(1), Fd90, FL) * mixf(R(1), Fd90, FV);
+  R Fss90 = dot(L, H) * dot(L, H) * m.roughness;
+  R Fss = mixf(R(1), Fss90, FL) * mixf(R(1), Fss90, FV);
+  R ss = R(1.25) * (Fss * (R(1) / (L.z + V.z) - R(0.5)) + R(0.5));
+  V3<R> Fsheen = FH * m.sheen * Csheen;
+  pdf = L.z * K<R>::INV_PI;
+  return (R(1) - m.metallic) * (K<R>::INV_PI * mixf(Fd, ss, m.subsurface) * m.baseColor + Fsheen);
+}
+template <class R>
+V3<R> dsnEvalSpec(const DisneyMat<R>& m, R eta, V3<R> specCol, V3<R> V, V3<R> L, V3<R> H, R& pdf) {
+  pdf = 0;
+  if (L.z <= R(0)) return V3<R>(0);
+  R FM = dsnFresnel(m.metallic, eta, dot(L, H), dot(V, H));
+  V3<R> F = mixv(specCol, V3<R>(1), FM);
+  R D = dsnGTR2Aniso(H.z, H.x, H.y, m.ax, m.ay);
+  R G1 = dsnSmithGAniso(std::fabs(V.z), V.x, V.y, m.ax, m.ay);
+  R G2 = G1 * dsnSmithGAniso(std::fabs(L.z), L.x, L.y, m.ax, m.ay);
+  pdf = G1 * D / (R(4) * V.z);
+  return F * D * G2 / (R(4) * L.z * V.z);
+}
+template <class R> V3<R> dsnEvalClearcoat(const DisneyMat<R>& m, V3<R> V, V3<R> L, V3<R> H, R& pdf) {
+  pdf = 0;
+  if (L.z <= R(0)) return V3<R>(0);
+  R FH = dsnDielectricFresnel(dot(V, H), R(1) / R(1.5));
+  R F = mixf(R(0.04), R(1), FH);
+  R D = dsnGTR1(H.z, m.clearcoatRoughness);
+  R G = dsnSmithG(L.z, R(0.25)) * dsnSmithG(V.z, R(0.25));
+  R jacobian = R(1) / (R(4) * dot(V, H));
+  pdf = D * H.z * jacobian;
+  return V3<R>(0.25) * m.clearcoat * F * D * G / (R(4) * L.z * V.z);
+}
+template <class R> void dsnSpecColor(const DisneyMat<R>& m, R eta, V3<R>& specCol, V3<R>& sheenCol) {
+  R lum = dsnLum(m.baseColor);
+  V3<R> ctint = lum > R(0) ? m.baseColor / lum : V3<R>(1);
+  R F0 = (R(1) - eta) / (R(1) + eta);
+  specCol = mixv(F0 * F0 * mixv(V3<R>(1), ctint, m.specularTint), m.baseColor, m.metallic);
+  sheenCol = mixv(V3<R>(1), ctint, m.sheenTint);
+}
+template <class R> void dsnLobes(const DisneyMat<R>& m, V3<R> specCol, R approxFresnel, R& dw, R& sw, R& cw) {
+  dw = dsnLum(m.baseColor) * (R(1) - m.metallic);
+  sw = dsnLum(mixv(specCol, V3<R>(1), approxFresnel));
+  cw = R(0.25) * m.clearcoat * (R(1) - m.metallic);
+  R total = dw + sw + cw;
+  dw /= total;
+  sw /= total;
+  cw /= total;
+}
+template <class R>
+V3<R> dsnEval(V3<R> L, V3<R> V, V3<R> N, V3<R> X, V3<R> Y, const DisneyMat<R>& m, R eta, uint32_t flags, R& bsdfPdf) {
+  V3<R> weight(0);
+  bsdfPdf = 0;
+  if ((flags & EArea) == 0) return weight;
+  V = toLocal(X, Y, N, V);
+  L = toLocal(X, Y, N, L);
+  if (L.z <= R(0) || V.z <= R(0)) return weight;
+  V3<R> H = makeNormal(L + V);
+  if (H.z < R(0)) H = -H;
+  V3<R> specCol, sheenCol;
+  dsnSpecColor(m, eta, specCol, sheenCol);
+  R dw, sw, cw;
+  dsnLobes(m, specCol, dsnFresnel(m.metallic, eta, dot(L, H), dot(V, H)), dw, sw, cw);
+  R pdf = 0;
+  if (dw > R(0) && L.z > R(0)) {
+    weight = weight + dsnEvalDiffuse(m, sheenCol, V, L, H, pdf);
+    bsdfPdf += pdf * dw;
+  }
+  if (sw > R(0) && L.z > R(0) && V.z > R(0)) {
+    weight = weight + dsnEvalSpec(m, eta, specCol, V, L, H, pdf);
+    bsdfPdf += pdf * sw;
+  }
+  if (cw > R(0) && L.z > R(0) && V.z > R(0)) {
+    weight = weight + dsnEvalClearcoat(m, V, L, H, pdf);
+    bsdfPdf += pdf * cw;
+  }
+  return weight * L.z;
+}
+// sampleBsdf, brdf_disney.rchit:298-363.  The returned weight is
+// f * |dot(N, L)| with N in world space and L still local, as written.
+template <class R>
+V3<R> dsnSample(R r1, R r2, V3<R> Vw, V3<R> N, V3<R> X, V3<R> Y, const DisneyMat<R>& m, R eta, V3<R>& d, R& pdf,
+                uint32_t& flags) {
+  pdf = 0;
+  V3<R> f(0);
+  V3<R> V = toLocal(X, Y, N, Vw), L;
+  V3<R> specCol, sheenCol;
+  dsnSpecColor(m, eta, specCol, sheenCol);
+  R dw, sw, cw;
+  dsnLobes(m, specCol, dsnFresnel(m.metallic, eta, V.z, V.z), dw, sw, cw);
+  R cdf0 = dw, cdf1 = cdf0 + cw;
+  if (r1 < cdf0) {
+    r1 /= cdf0;
+    L = cosineSampleHemisphere(r1, r2);
+    V3<R> H = makeNormal(L + V);
+    f = dsnEvalDiffuse(m, sheenCol, V, L, H, pdf);
+    pdf *= dw;
+    flags = EDiffuseReflection;
+  } else if (r1 < cdf1) {
+    r1 = (r1 - cdf0) / (cdf1 - cdf0);
+    V3<R> H = dsnSampleGTR1(m.clearcoatRoughness, r1, r2);
+    if (H.z < R(0)) H = -H;
+    L = makeNormal(reflect(-V, H));
+    f = dsnEvalClearcoat(m, V, L, H, pdf);
+    pdf *= cw;
+    flags = EGlossyReflection;
+  } else {
+    r1 = (r1 - cdf1) / (R(1) - cdf1);
+    V3<R> H = dsnSampleVNDF(V, m.ax, m.ay, r1, r2);
+    if (H.z < R(0)) H = -H;
+    L = makeNormal(reflect(-V, H));
+    f = dsnEvalSpec(m, eta, specCol, V, L, H, pdf);
+    pdf *= sw;
+    flags = EGlossyReflection;
+  }
+  d = toWorld(X, Y, N, L);
+  return f * std::fabs(dot(N, L));
+}
+template <class R> DisneyMat<R> disneyMat(const GpuMaterial& g, R rough) {
+  DisneyMat<R> m;
+  R aspect = std::sqrt(R(1) - R(g.anisotropic) * R(0.9));
+  m.ax = std::max(R(0.001), rough * rough / aspect);
+  m.ay = std::max(R(0.001), rough * rough * aspect);
+  m.baseColor = load3<R>(g.diffuse);
+  m.anisotropic = R(g.anisotropic);
+  m.metallic = R(g.metalness);
+  m.roughness = std::max(rough * rough, R(0.001));
+  m.subsurface = R(g.subsurface);
+  m.specularTint = R(g.specularTint);
+  m.sheen = R(g.sheen);
+  m.sheenTint = R(g.sheenTint);
+  m.clearcoat = R(g.clearcoat);
+  m.clearcoatRoughness = mixf(R(0.1), R(0.001), R(g.clearcoatGloss));
+  m.ior = R(g.ior);
+  return m;
+}
+
+template <class R> void chsDisney(const RCtx& c, Payload<R>& pl, HitState<R>& st) {
+  const orc_scene& s = *c.s;
+  R t4[4];
+  diffuseAndNormalMaps(c, st);
+  if (st.mat.metalnessTextureId >= 0) {
+    textureEval3(s, st.mat.metalnessTextureId, st.uvx, st.uvy, t4);
+    st.mat.metalness = float(t4[0]);
+  }
+  if (st.mat.roughnessTextureId >= 0) {
+    textureEval3(s, st.mat.roughnessTextureId, st.uvx, st.uvy, t4);
+    st.mat.roughness = float(t4[0]);
+  }
+  if (st.mat.normalTextureId >= 0) applyNormalMap(s, st, c.pc->useFaceNormal);
+  R opacity = R(st.mat.rhoSpec[0]);
+  if (st.mat.opacityTextureId >= 0) {
+    textureEval3(s, st.mat.opacityTextureId, st.uvx, st.uvy, t4);
+    opacity = t4[0];
+  }
+  if (rnd<R>(pl.seed) < opacity) {
+    pl.ro = offsetPositionAlongNormal(st.pos, -st.ffN);
+    pl.depth--;
+    return;
+  }
+  DisneyMat<R> m = disneyMat<R>(st.mat, R(st.mat.roughness));
+  R eta = dot(st.V, st.N) > R(0) ? (R(1) / m.ior) : m.ior;
+  {
+    V3<R> Ld(0);
+    bool visible;
+    LRec<R> l;
+    V3<R> radiance = sampleLights(c, pl, st.pos, st.ffN, visible, l);
+    if (visible) {
+      R bsdfPdf;
+      V3<R> bw = dsnEval(l.d, st.V, st.ffN, st.X, st.Y, m, eta, EArea, bsdfPdf);
+      R mis = powerHeuristic(l.pdf, bsdfPdf);
+      Ld = Ld + mis * bw * radiance * pl.throughput / (l.pdf + K<R>::EPS);
+    }
+    pl.dRad = Ld;
+    pl.dSkip = !visible;
+  }
+  R u0 = rnd<R>(pl.seed), u1 = rnd<R>(pl.seed);
+  V3<R> bd;
+  R bpdf;
+  uint32_t bflags = EBsdfNull;
+  V3<R> bw = dsnSample(u0, u1, st.V, st.ffN, st.X, st.Y, m, eta, bd, bpdf, bflags);
+  if (bpdf <= R(0) || length(bw) == R(0)) {
+    pl.stop = true;
+    return;
+  }
+  nextRay(pl, st, bd, bw, bpdf, bflags, false);
+}
+
 template <class R> void chsEmissive(const RCtx& c, Payload<R>& pl, HitState<R>& st) {
   // brdf_emissive.rchit:12-26
   pl.stop = true;
@@ -1883,6 +2141,7 @@ void renderPixel(const RCtx& c, uint32_t x, uint32_t y, int32_t frame, R& wsum, 
           case MaterialTypeBrdfPlastic: chsPlastic(c, pl, st); break;
           case MaterialTypeBrdfPhong: chsPhong(c, pl, st); break;
           case MaterialTypeBrdfKang18: chsKang18(c, pl, st); break;
+          case MaterialTypeBrdfDisney: chsDisney(c, pl, st); break;
           default: pl.stop = true; break;
         }
       }
@@ -2393,6 +2652,12 @@ void orc_bsdf_eval(const GpuMaterial* m, const float Lp[3], const float Vp[3], u
       r = kangEval(L, V, N, X, Y, kd, load3<float>(m->rhoSpec), ax, ay, m->ior);
       break;
     }
+    case MaterialTypeBrdfDisney: {
+      DisneyMat<float> dm = disneyMat<float>(*m, m->roughness);
+      float eta = V.z > 0 ? 1.f / dm.ior : dm.ior, p;
+      r = dsnEval(L, V, N, X, Y, dm, eta, flags, p);
+      break;
+    }
     default: break;
   }
   out[0] = r.x, out[1] = r.y, out[2] = r.z;
@@ -2437,6 +2702,12 @@ float orc_bsdf_pdf(const GpuMaterial* m, const float Lp[3], const float Vp[3], u
       float ax = std::max(m->anisoAlpha[0], K<float>::EPS), ay = std::max(m->anisoAlpha[1], K<float>::EPS);
       return kangPdf(L, V, N, X, Y, ax, ay, load3<float>(m->diffuse), load3<float>(m->rhoSpec), flags);
     }
+    case MaterialTypeBrdfDisney: {
+      DisneyMat<float> dm = disneyMat<float>(*m, m->roughness);
+      float eta = V.z > 0 ? 1.f / dm.ior : dm.ior, p;
+      dsnEval(L, V, N, X, Y, dm, eta, flags, p);
+      return p;
+    }
     default: return 0;
   }
 }
@@ -2477,6 +2748,13 @@ void orc_bsdf_sample(const GpuMaterial* m, const float Vp[3], uint32_t* seed, fl
       float eta = V.z > 0 ? 1.f / m->ior : m->ior;
       float u = rnd<float>(*seed);
       r = dielSample(u, V, N, eta, d, p, f);
+      break;
+    }
+    case MaterialTypeBrdfDisney: {
+      DisneyMat<float> dm = disneyMat<float>(*m, m->roughness);
+      float eta = V.z > 0 ? 1.f / dm.ior : dm.ior;
+      float u0 = rnd<float>(*seed), u1 = rnd<float>(*seed);
+      r = dsnSample(u0, u1, V, N, X, Y, dm, eta, d, p, f);
       break;
     }
     default: break;

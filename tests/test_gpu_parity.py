@@ -249,6 +249,44 @@ def test_more_plugins_scene_matches_oracle(renderer, tmp_path):
     assert st["dielectric_shadow_rays"] > 0
 
 
+def test_disney_scene_matches_oracle(renderer, tmp_path):
+    """Disney BRDF surfaces: diffuse with retro-reflection, metal, partial
+    opacity (pass-through draws) and -- through the material records, since
+    the JSON loader reads only base colour, metallic, roughness and opacity --
+    subsurface, sheen, tints, anisotropy and clearcoat; rect + point light."""
+    d = tmp_path / "disney"
+    d.mkdir()
+    sp = A.generate_scene(0, str(d))
+    import json
+    js = json.load(open(sp))
+    mats = js["materials"]
+    for k, (base, metal, rough, opac) in enumerate([((0.8, 0.7, 0.6), 0.0, 0.6, 0.0), ((0.9, 0.6, 0.3), 1.0, 0.3, 0.0),
+                                                    ((0.3, 0.5, 0.8), 0.2, 0.5, 0.0), ((0.7, 0.7, 0.2), 0.0, 0.2, 0.0),
+                                                    ((0.5, 0.3, 0.6), 0.5, 0.7, 0.3), ((0.2, 0.6, 0.4), 0.0, 0.4, 0.0)]):
+        mats[k] = {"name": mats[k]["name"], "type": "brdf_disney", "diffuse_reflectance": list(base),
+                   "metallic": metal, "roughness": rough, "opacity": opac}
+    js["lights"].append({"type": "point", "radiance": [1.0, 1.0, 1.0], "position": [0.3, 0.6, 0.2]})
+    json.dump(js, open(sp, "w"))
+    s = A.HostScene(sp)
+    m = s.desc.materials
+    m[2].subsurface, m[2].sheen, m[2].sheenTint, m[2].specularTint = 0.8, 1.0, 0.5, 0.7
+    m[3].clearcoat, m[3].clearcoatGloss, m[3].anisotropic = 1.0, 0.7, 0.8
+    m[5].clearcoat, m[5].clearcoatGloss, m[5].ior = 0.5, 0.2, 1.8
+    assert all(m[k].type == 9 for k in range(6))
+    renderer.upload(s)
+    renderer.build()
+    spp = 16
+    st, ost, g, o, sq = _radiance_parity(renderer, s, spp)
+    frac_bad = _three_sigma(g[0], o[0], sq, spp)
+    mg, mo = g[0][..., :3].mean((0, 1)), o[0][..., :3].mean((0, 1))
+    log(f"disney: outside 3 sigma {frac_bad:.5f}, means {mg} vs {mo}; rays gpu {st['extension_rays']} "
+        f"oracle {ost['extension_rays']}")
+    assert np.isfinite(g[0]).all()
+    assert frac_bad <= 0.01
+    np.testing.assert_allclose(mg, mo, rtol=1e-2)
+    assert abs(st["extension_rays"] - ost["extension_rays"]) / ost["extension_rays"] < 2e-3
+
+
 def test_env_tables_on_device_equal_oracle(renderer, spheres_small):
     s, _ = _load(renderer, spheres_small)
     env = np.ctypeslib.as_array(s.desc.envmap.rgba, (s.desc.envmap.height, s.desc.envmap.width, 4)).copy()

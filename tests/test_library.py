@@ -132,6 +132,24 @@ def test_loader_shot_state_override_and_material_defaults(tmp_path):
     assert abs(m[3].radiance[0] - 0.5963) < 2e-3  # fdrInt, internal diffuse Fresnel of ior 1.5
 
 
+def test_loader_disney_reads_four_inputs(tmp_path):
+    """brdf_disney reads base colour, metallic, roughness and opacity (held
+    in rhoSpec.x, default 0); its other lobe inputs keep the material
+    defaults (material.h:9-35, loader/material.cpp:359-389)."""
+    b = _base()
+    b["materials"].append({"name": "d", "type": "brdf_disney", "diffuse_reflectance": [0.1, 0.2, 0.3],
+                           "metallic": 0.4, "roughness": 0.3, "opacity": 0.25, "sheen": 9.0})
+    b["materials"].append({"name": "d2", "type": "brdf_disney"})
+    s = A.HostScene(_write_scene(tmp_path, b))
+    m = s.desc.materials
+    d, d2 = m[2], m[3]
+    assert d.type == 9 and d2.type == 9
+    np.testing.assert_allclose(list(d.diffuse), [0.1, 0.2, 0.3], rtol=1e-7)
+    assert abs(d.metalness - 0.4) < 1e-7 and abs(d.roughness - 0.3) < 1e-7 and abs(d.rhoSpec[0] - 0.25) < 1e-7
+    assert d.sheen == 0 and d.clearcoat == 0 and d.subsurface == 0 and d.anisotropic == 0 and abs(d.ior - 1.5) < 1e-7
+    assert d2.rhoSpec[0] == 0 and abs(d2.roughness - 0.5) < 1e-7 and d2.metalness == 0
+
+
 @pytest.mark.parametrize("mutate,code", [
     (lambda b: b.pop("instances"), "malformed"),
     (lambda b: b["instances"][0].pop("material"), "malformed"),

@@ -647,3 +647,194 @@ def test_kang18_pdf_integrates_to_one():
     out = (C.c_float * 3)()
     L.orc_bsdf_eval(C.byref(m), O.f3((0.0, 0.0, 1.0)), V, 1, out)
     assert out[0] > 0.3 / math.pi
+
+
+# ----------------------------------------------------------------- disney
+# Material inputs the Disney plugin reads (brdf_disney.rchit:408-436):
+# baseColor = diffuse, metallic = metalness, roughness -> alpha = roughness^2
+# stretched by aspect = sqrt(1 - 0.9 anisotropic) (ax = a / aspect along the
+# first tangent, ay = a * aspect), clearcoatGloss -> GTR1 alpha mix(0.1, 0.001, g).
+DISNEY = 9
+
+
+def _disney(**kw):
+    base = dict(diffuse=(0.0, 0.0, 0.0), metalness=0.0, roughness=0.5, ior=1.5, subsurface=0.0, sheen=0.0,
+                sheenTint=0.0, specularTint=0.0, clearcoat=0.0, clearcoatGloss=0.0, anisotropic=0.0)
+    base.update(kw)
+    return _mat(DISNEY, **base)
+
+
+def _aniso_inside(ax, ay, n=8192):
+    """Fraction of an anisotropic GGX distribution with theta_h < 45 deg.
+    In slope space stretched by (ax, ay) the distribution is radially
+    1/(pi (1 + r^2)^2); the disk tan(theta_h) < 1 becomes the ellipse
+    r^2 (ax^2 cos^2 psi + ay^2 sin^2 psi) < 1, whose mass is
+    E_psi[1 / (1 + ax^2 cos^2 psi + ay^2 sin^2 psi)] for uniform psi."""
+    psi = (np.arange(n) + 0.5) / n * 2 * np.pi
+    return np.mean(1.0 / (1.0 + ax ** 2 * np.cos(psi) ** 2 + ay ** 2 * np.sin(psi) ** 2))
+
+
+@pytest.mark.parametrize("rough,aniso", [(0.5, 0.0), (0.5, 0.9), (0.7, 0.5)])
+def test_disney_metal_lobe_pdf_is_the_visible_normal_pdf(rough, aniso):
+    """metallic = 1 leaves only the specular lobe.  At normal view G1(V) = 1
+    and the visible-normal pdf reflected about H is D(H) / 4, so the pdf
+    integrates to the GGX mass of half-vectors within 45 deg of n."""
+    L = O.lib()
+    m = _disney(diffuse=(0.7, 0.6, 0.5), metalness=1.0, roughness=rough, anisotropic=aniso)
+    V = O.f3((0.0, 0.0, 1.0))
+    total = _hemisphere_integral(lambda l: L.orc_bsdf_pdf(C.byref(m), O.f3(l), V, 2), 900, 96)
+    aspect = math.sqrt(1 - 0.9 * aniso)
+    ax, ay = max(0.001, rough ** 2 / aspect), max(0.001, rough ** 2 * aspect)
+    expected = _aniso_inside(ax, ay)
+    assert abs(total - expected) < 0.01 * expected, (total, expected)
+
+
+def test_disney_anisotropy_stretches_along_the_first_tangent():
+    """ax = alpha / aspect belongs to the first tangent (+x for n = +z): the
+    pdf ratio of half-vectors tilted along x and along y is the GGX ratio
+    ((s^2/ay^2 + c^2) / (s^2/ax^2 + c^2))^2, which exceeds 1."""
+    L = O.lib()
+    rough, aniso = 0.5, 0.9
+    m = _disney(diffuse=(1.0, 1.0, 1.0), metalness=1.0, roughness=rough, anisotropic=aniso)
+    aspect = math.sqrt(1 - 0.9 * aniso)
+    ax, ay = rough ** 2 / aspect, rough ** 2 * aspect
+    th = 0.15
+    s, c = math.sin(th), math.cos(th)
+    V = O.f3((0.0, 0.0, 1.0))
+    Lx = (2 * c * s, 0.0, 2 * c * c - 1)  # reflect(-n, H) with H = (s, 0, c)
+    Ly = (0.0, 2 * c * s, 2 * c * c - 1)
+    px = L.orc_bsdf_pdf(C.byref(m), O.f3(Lx), V, 2)
+    py = L.orc_bsdf_pdf(C.byref(m), O.f3(Ly), V, 2)
+    expected = ((s * s / ay ** 2 + c * c) / (s * s / ax ** 2 + c * c)) ** 2
+    assert expected > 1.5
+    assert abs(px / py - expected) < 1e-3 * expected
+
+
+def test_disney_white_metal_loses_little_energy():
+    """White metal (F = 1) at normal view and low roughness: the directional
+    albedo of GGX with separable Smith masking is just below 1."""
+    L = O.lib()
+    m = _disney(diffuse=(1.0, 1.0, 1.0), metalness=1.0, roughness=0.3)
+    V = O.f3((0.0, 0.0, 1.0))
+    out = (C.c_float * 3)()
+
+    def f(l):
+        L.orc_bsdf_eval(C.byref(m), O.f3(l), V, 2, out)
+        return out[1]
+
+    albedo = _hemisphere_integral(f, 900, 32)
+    assert 0.97 < albedo < 1.002, albedo
+
+
+def test_disney_clearcoat_pdf_and_gtr1_mass():
+    """Only the clearcoat lobe (black base, ior 1 so the specular lobe has
+    zero weight, metallic 0).  GTR1 is normalised, and its mass within 45 deg
+    follows from the inverse CDF the sampler uses,
+    cos^2 = (1 - a2^(1-u)) / (1 - a2):  P = 1 - log((1 + a2) / 2) / log(a2)."""
+    L = O.lib()
+    for gloss in (0.0, 0.6):
+        m = _disney(ior=1.0, clearcoat=1.0, clearcoatGloss=gloss)
+        a = 0.1 * (1 - gloss) + 0.001 * gloss
+        a2 = a * a
+        V = O.f3((0.0, 0.0, 1.0))
+        total = _hemisphere_integral(lambda l: L.orc_bsdf_pdf(C.byref(m), O.f3(l), V, 2), 1600, 16)
+        expected = 1 - math.log((1 + a2) / 2) / math.log(a2)
+        assert abs(total - expected) < 0.01, (gloss, total, expected)
+
+
+def _sample_histogram_matches_pdf(m, Vdir, n=60000, nt=8, npf=8, marginal=False):
+    """Counts of sampled directions in (cos theta, phi) bins against the pdf
+    integrated over each bin; samples with pdf 0 (below the horizon) are
+    rejected by the shader and are excluded.  marginal: compare the cos theta
+    marginal only.  Returns the accepted directions."""
+    L = O.lib()
+    seed = C.c_uint32(12345)
+    w, d, pdf, fl = (C.c_float * 3)(), (C.c_float * 3)(), C.c_float(), C.c_uint32()
+    V = O.f3(Vdir)
+    dirs = []
+    for _ in range(n):
+        L.orc_bsdf_sample(C.byref(m), V, C.byref(seed), w, d, C.byref(pdf), C.byref(fl))
+        if pdf.value > 0:
+            dirs.append(tuple(d))
+    dirs = np.array(dirs)
+    ct = np.clip(dirs[:, 2], 0, 1)
+    ph = np.mod(np.arctan2(dirs[:, 1], dirs[:, 0]), 2 * np.pi)
+    obs, _, _ = np.histogram2d(ct, ph, bins=[nt, npf], range=[[0, 1], [0, 2 * np.pi]])
+    # expected: integrate the pdf over each bin in (cos theta, phi), where dw = dcos dphi
+    sub = 24
+    exp = np.zeros((nt, npf))
+    for i in range(nt):
+        for j in range(npf):
+            cs = (i + (np.arange(sub) + 0.5) / sub) / nt
+            ps = (j + (np.arange(sub) + 0.5) / sub) / npf * 2 * np.pi
+            acc = 0.0
+            for cc in cs:
+                st = math.sqrt(max(0.0, 1 - cc * cc))
+                for p in ps:
+                    acc += L.orc_bsdf_pdf(C.byref(m), O.f3((st * math.cos(p), st * math.sin(p), cc)), V, 2)
+            exp[i, j] = acc * (1.0 / nt / sub) * (2 * np.pi / npf / sub) * n
+    if marginal:
+        obs, exp = obs.sum(1), exp.sum(1)
+    big = exp > 50
+    z = (obs[big] - exp[big]) / np.sqrt(exp[big])
+    assert np.all(np.abs(z) < 5.5), (obs, exp)
+    assert abs(obs.sum() - exp.sum()) < 5 * math.sqrt(n) + 0.01 * n
+    return dirs
+
+
+def test_disney_metal_sampler_draws_the_visible_normal_pdf():
+    """Heitz's visible-normal sampler (JCGT 2018) reflected about H must be
+    distributed as the pdf eval reports: G1(V) D(H) / (4 V.z)."""
+    m = _disney(diffuse=(0.9, 0.9, 0.9), metalness=1.0, roughness=0.6, anisotropic=0.5)
+    _sample_histogram_matches_pdf(m, (0.5, 0.2, math.sqrt(1 - 0.29)))
+
+
+def test_disney_clearcoat_sampler_draws_the_gtr1_pdf():
+    """SampleGTR1 takes phi and cos theta_h from the same number r1 (r2 is
+    unused, brdf_disney.rchit:61-74, as written): the cos theta marginal is
+    GTR1's, and phi is fixed by theta: phi = 2 pi u with
+    u = 1 - log(1 - cos^2 theta_h (1 - a2)) / log(a2)."""
+    m = _disney(ior=1.0, clearcoat=1.0, clearcoatGloss=0.0)
+    dirs = _sample_histogram_matches_pdf(m, (0.0, 0.0, 1.0), marginal=True)
+    a2 = 0.1 ** 2
+    # V = n: L = reflect(-n, H), so cos theta_L = 2 cos^2 theta_h - 1 and phi_L = phi_h
+    c2 = (dirs[:, 2] + 1) / 2
+    u = 1 - np.log(1 - c2 * (1 - a2)) / np.log(a2)
+    ph = np.mod(np.arctan2(dirs[:, 1], dirs[:, 0]), 2 * np.pi)
+    err = np.abs(np.mod(ph - 2 * np.pi * u + np.pi, 2 * np.pi) - np.pi)
+    keep = (dirs[:, 2] < 0.9999) & (u > 1e-3)  # phi is ill-defined at the pole
+    assert np.percentile(err[keep], 99) < 2e-3, np.percentile(err[keep], 99)
+
+
+def test_disney_diffuse_closed_forms():
+    """ior 1 and metallic 0 leave the diffuse lobe alone (F0 = 0, and the
+    dielectric Fresnel at eta = 1 is 0).  At normal view and light the Burley
+    factors are 1 (f = base / pi); the Hanrahan-Krueger subsurface term there
+    is 1.25 (1 (1/2 - 1/2) + 1/2) = 0.625; for V = L at grazing cos theta = u,
+    cos theta_d = 1 and f = base / pi (1 + (Fd90 - 1) F)^2 with
+    Fd90 = 1/2 + 2 alpha, F = (1 - u)^5; the pdf is the cosine pdf."""
+    L = O.lib()
+    out = (C.c_float * 3)()
+    n = O.f3((0.0, 0.0, 1.0))
+    base = (0.8, 0.5, 0.3)
+    m = _disney(diffuse=base, ior=1.0, roughness=0.7)
+    L.orc_bsdf_eval(C.byref(m), n, n, 2, out)
+    np.testing.assert_allclose(list(out), np.array(base) / math.pi, rtol=1e-5)
+    assert abs(L.orc_bsdf_pdf(C.byref(m), n, n, 2) - 1 / math.pi) < 1e-6
+    ms = _disney(diffuse=base, ior=1.0, roughness=0.7, subsurface=1.0)
+    L.orc_bsdf_eval(C.byref(ms), n, n, 2, out)
+    np.testing.assert_allclose(list(out), 0.625 * np.array(base) / math.pi, rtol=1e-5)
+    u = 0.2
+    g = (math.sqrt(1 - u * u), 0.0, u)
+    L.orc_bsdf_eval(C.byref(m), O.f3(g), O.f3(g), 2, out)
+    alpha = 0.7 ** 2
+    F = (1 - u) ** 5
+    expected = np.array(base) / math.pi * (1 + (0.5 + 2 * alpha - 1) * F) ** 2 * u
+    np.testing.assert_allclose(list(out), expected, rtol=1e-4)
+    # sheen adds F_H sheen Csheen with F_H = (1 - cos theta_d)^5: zero when V = L
+    mh = _disney(diffuse=base, ior=1.0, roughness=0.7, sheen=1.0)
+    L.orc_bsdf_eval(C.byref(mh), O.f3(g), O.f3(g), 2, out)
+    np.testing.assert_allclose(list(out), expected, rtol=1e-4)
+    # eval needs EArea; delta lights get zero
+    L.orc_bsdf_eval(C.byref(m), n, n, 1, out)
+    assert list(out) == [0, 0, 0]
