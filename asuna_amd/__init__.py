@@ -42,6 +42,22 @@ class GpuMaterial(C.Structure):
                 ("tangentTextureId", C.c_int32), ("opacityTextureId", C.c_int32), ("type", C.c_uint32)]
 
 
+class GpuSunAndSky(C.Structure):
+    _fields_ = [("rgb_unit_conversion", C.c_float * 3), ("multiplier", C.c_float), ("haze", C.c_float),
+                ("redblueshift", C.c_float), ("saturation", C.c_float), ("horizon_height", C.c_float),
+                ("ground_color", C.c_float * 3), ("horizon_blur", C.c_float), ("night_color", C.c_float * 3),
+                ("sun_disk_intensity", C.c_float), ("sun_direction", C.c_float * 3), ("sun_disk_scale", C.c_float),
+                ("sun_glow_intensity", C.c_float), ("y_is_up", C.c_int32), ("physically_scaled_sun", C.c_int32),
+                ("in_use", C.c_int32)]
+
+
+def sun_and_sky_defaults():
+    """The sun & sky record a scene starts with (scene.cpp:112-129; in_use 0)."""
+    s = GpuSunAndSky()
+    lib().ash_sun_and_sky_defaults(C.byref(s))
+    return s
+
+
 class GpuLight(C.Structure):
     _fields_ = [("type", C.c_int32), ("position", C.c_float * 3), ("direction", C.c_float * 3),
                 ("radiance", C.c_float * 3), ("u", C.c_float * 3), ("v", C.c_float * 3),
@@ -210,6 +226,8 @@ def lib():
         "ash_upload_scene": (C.c_int, [vp, P(SceneDesc)]),
         "ash_build_accel": (C.c_int, [vp, P(BuildOpts), P(BuildInfo)]),
         "ash_set_film": (C.c_int, [vp, C.c_uint32, C.c_uint32]),
+        "ash_sun_and_sky_defaults": (None, [P(GpuSunAndSky)]),
+        "ash_set_sun_and_sky": (C.c_int, [vp, P(GpuSunAndSky)]),
         "ash_render": (C.c_int, [vp, P(GpuCamera), P(GpuPushConstantRaytrace), C.c_int32, C.c_int32,
                                  P(RenderOpts), P(Stats)]),
         "ash_render_shots": (C.c_int, [vp, C.c_uint32, P(GpuCamera), P(GpuPushConstantRaytrace), C.c_int32,
@@ -420,6 +438,9 @@ class Renderer:
         self._check(lib().ash_build_accel(self._c, C.byref(opts), C.byref(info)),
                     "build_accel")
         return info.as_dict()
+
+    def set_sun_and_sky(self, sky):
+        self._check(lib().ash_set_sun_and_sky(self._c, C.byref(sky)), "set_sun_and_sky")
 
     def set_film(self, w, h):
         self._check(lib().ash_set_film(self._c, w, h), "set_film")

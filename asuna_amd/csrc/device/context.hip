@@ -18,6 +18,8 @@ static_assert(sizeof(GpuMaterial) == 132, "GpuMaterial must be 132 bytes (materi
 static_assert(sizeof(GpuLight) == 76, "GpuLight must be 76 bytes (light.h)");
 static_assert(sizeof(GpuCamera) == 224, "GpuCamera must be 224 bytes (camera.h)");
 static_assert(sizeof(GpuPushConstantRaytrace) == 84, "push constants must be 84 bytes (pushconstant.h)");
+static_assert(sizeof(GpuSunAndSky) == 96, "GpuSunAndSky must be 96 bytes (sun_and_sky.h)");
+static_assert(offsetof(GpuSunAndSky, in_use) == 92, "GpuSunAndSky.in_use offset");
 static_assert(offsetof(GpuMaterial, type) == 128, "GpuMaterial.type offset");
 static_assert(offsetof(GpuLight, doubleSide) == 72, "GpuLight.doubleSide offset");
 static_assert(offsetof(GpuCamera, type) == 208, "GpuCamera.type offset");
@@ -85,6 +87,7 @@ struct ash_ctx {
   DBuf<float2> marg, cond;
   uint32_t envW = 1, envH = 1;
   bool hasEnv = false;
+  DBuf<DevSky> sky;
   uint32_t numTris = 0, numInst = 0;
   uint32_t maxDepth = 0;
   float4* nodes = nullptr;
@@ -123,7 +126,7 @@ struct ash_ctx {
     inst.release(), idx.release(), instTriOffset.release(), mats.release(), lights.release(), texTab.release();
     marg.release(), cond.release(), shotTab.release(), wInfo.release(), wHitI.release();
     wQ0.release(), wQ1.release(), wQ2.release(), wSpill.release(), wSpillT.release(), wCtr.release(), wCnt64.release();
-    unitBuf.release();
+    unitBuf.release(), sky.release();
     for (auto& g : graphs) releaseGraph(g);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -197,6 +200,7 @@ static SceneView viewOf(const ash_ctx* c) {
   v.env.h = c->envH;
   v.marg = c->marg.p;
   v.cond = c->cond.p;
+  v.sky = c->sky.p;
   return v;
 }
 
@@ -554,6 +558,34 @@ void ash_destroy(ash_ctx* c) {
 
 const char* ash_last_error(const ash_ctx* c) { return c ? c->err.c_str() : "null context"; }
 
+void ash_sun_and_sky_defaults(GpuSunAndSky* o) {
+  if (!o) return;
+  // Scene::init, scene.cpp:112-129
+  *o = GpuSunAndSky{{1.f, 1.f, 1.f}, 0.0000101320f, 0.0f, 0.0f, 1.0f, 0.0f, {0.4f, 0.4f, 0.4f}, 0.1f,
+                    {0.0f, 0.0f, 0.01f}, 0.8f, {0.00f, 0.78f, 0.62f}, 5.0f, 1.0f, 1, 1, 0};
+}
+
+// Writes the record and derives its direction-independent terms on the
+// device (stream-ordered after the context's pending work).
+static int setSky(ash_ctx* c, const GpuSunAndSky& ss) {
+  if (!c->sky.p) MALLOC(c, c->sky, 1);
+  DevSky k;
+  std::memset(&k, 0, sizeof k);
+  k.ss = ss;
+  HIPC(c, hipMemcpyAsync(c->sky.p, &k, sizeof k, hipMemcpyHostToDevice, c->stream));
+  ashd::launchSkyPrepare(c->sky.p, c->stream);
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipStreamSynchronize(c->stream));
+  return ASH_OK;
+}
+
+int ash_set_sun_and_sky(ash_ctx* c, const GpuSunAndSky* sky) {
+  if (!c || !sky) return ASH_E_INVALID;
+  if (!c->hasScene) return fail(c, ASH_E_STATE, "no scene uploaded");
+  HIPC(c, hipSetDevice(c->device));
+  return setSky(c, *sky);
+}
+
 int ash_upload_scene(ash_ctx* c, const ash_scene_desc* d) {
   if (!c || !d) return ASH_E_INVALID;
   HIPC(c, hipSetDevice(c->device));
@@ -676,6 +708,12 @@ int ash_upload_scene(ash_ctx* c, const ash_scene_desc* d) {
     HIPC(c, hipMemcpyAsync(c->env.p, &z, sizeof z, hipMemcpyHostToDevice, s));
     HIPC(c, hipMemcpyAsync(c->marg.p, &z2, sizeof z2, hipMemcpyHostToDevice, s));
     HIPC(c, hipMemcpyAsync(c->cond.p, &z2, sizeof z2, hipMemcpyHostToDevice, s));
+  }
+  {  // every scene starts from the default sun & sky (in_use 0)
+    GpuSunAndSky ss;
+    ash_sun_and_sky_defaults(&ss);
+    int rc = setSky(c, ss);
+    if (rc != ASH_OK) return rc;
   }
   HIPC(c, hipStreamSynchronize(s));
   c->numTris = uint32_t(tri);

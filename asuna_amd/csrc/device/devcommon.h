@@ -143,6 +143,13 @@ __device__ __forceinline__ f3 cosineHemisphere(float u0, float u1) {
   return f3{dx, dy, z};
 }
 __device__ __forceinline__ float cosinePdf(float c) { return c <= 0.0f ? 0.0f : c * kInvPi; }
+// uniformSampleSphere, math.glsl:133-138 (pdf 1 / (4 pi))
+__device__ __forceinline__ f3 uniformSphere(float u0, float u1) {
+  float z = 1.0f - 2.0f * u0;
+  float r = sqrtf(fmaxf(0.0f, 1.0f - z * z));
+  float phi = kTwoPi * u1;
+  return f3{r * cosf(phi), r * sinf(phi), z};
+}
 // powerHeuristic, math.glsl:184-189
 __device__ __forceinline__ float powerHeuristic(float a, float b) {
   a = a * a;

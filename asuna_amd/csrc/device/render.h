@@ -17,6 +17,24 @@ struct ShotParams {
   float4* images;  // 9 planes of width*height float4
 };
 
+// Sun & sky record plus its direction-independent terms (sun direction
+// after the horizon clamp, unit scale, sun colours, ground colour), which
+// kSkyPrepare computes once per record on the device.
+struct DevSky {
+  GpuSunAndSky ss;
+  float sun[3];      // tweaked sun direction, clamped to z >= 0.001
+  float realSun[3];  // tweaked sun direction before the clamp
+  float rgbScale[3];
+  float sunUp[3];    // calc_sun_color at the local haze (view above the horizon)
+  float sunDown[3];  // calc_sun_color at haze 2 (view at or below the horizon)
+  float ground[3];   // ground colour term of the below-horizon branch
+  float factor;      // night factor of the sky tint (1 by day)
+  float haze, sat, hh;
+  float diskScale, glowScale;  // calc_physical_scale (1, 1 when not physically scaled)
+  int32_t inUse;
+  int32_t pad;
+};
+
 struct SceneView {
   const float4* nodes;   // BVH2 (always built)
   const float4* nodes4;  // 4-wide BVH, or null to traverse the BVH2
@@ -32,6 +50,7 @@ struct SceneView {
   DevImage env;
   const float2* marg;  // H entries (column 0 of the marginal table)
   const float2* cond;  // W*H entries
+  const DevSky* sky;   // sun & sky (inUse 0: env map / background)
 };
 
 // Wavefront state for up to `capacity` paths (path numbering: WorkUnit).
@@ -101,6 +120,8 @@ void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t maxRays, uin
                   hipStream_t s);
 void launchAccumulate(const BatchDesc& b, const Wavefront& w, hipStream_t s);
 void launchResetCounters(const Wavefront& w, int cur, hipStream_t s);
+// Fills the derived terms of *sky from sky->ss (one thread).
+void launchSkyPrepare(DevSky* sky, hipStream_t s);
 void launchFirstHit(const SceneView& sc, const ShotParams* shot, const Wavefront& w, ash_hit* out, uint32_t P,
                     hipStream_t s);
 

@@ -887,6 +887,225 @@ __device__ __forceinline__ f3 evalEnv(const SceneView& sc, const float* envT, fl
   return intensity * xyz(tex2D(sc.env, u, v));
 }
 
+// Sun & sky, utils/sun_and_sky.glsl:20-533 (NVIDIA's MDL sun & sky:
+// Preetham/Perez sky, attenuated sun disk with glow, ground below the
+// horizon).  The file's own M_PI is 3.1415926535f.
+constexpr float kSkyPi = 3.1415926535f;
+__device__ __forceinline__ float skySmoothstep(float e0, float e1, float x) {
+  float t = clampf((x - e0) / (e1 - e0), 0.0f, 1.0f);
+  return t * t * (3.0f - 2.0f * t);
+}
+__device__ __forceinline__ f3 skyTweak(f3 d, int yUp, float hh) {  // arch_vectortweak
+  f3 o = yUp == 1 ? f3{d.x, d.z, d.y} : d;
+  if (hh != 0.0f) {
+    o.z -= hh;
+    o = makeNormal(o);
+  }
+  return o;
+}
+__device__ __forceinline__ float skyPerez(float A, float B, float C, float D, float E, float ct, float gamma,
+                                          float cg, float ts, float cts) {
+  return ((1.0f + A * expf(B / ct)) * (1.0f + C * expf(D * gamma) + E * cg * cg)) /
+         ((1.0f + A * expf(B / 1.0f)) * (1.0f + C * expf(D * ts) + E * cts * cts));
+}
+// calc_env_color with sky_luminance and sky_color_xyz, sun_and_sky.glsl:134-244
+__device__ f3 skyEnvColor(f3 sun, f3 dir, float T) {
+  const float cts = sun.z, ts = acosf(cts), ct = dir.z;
+  float chi = (4.0f / 9.0f - T / 120.0f) * (kSkyPi - 2.0f * ts);
+  float lum = 1000.0f * ((4.0453f * T - 4.9710f) * tanf(chi) - 0.2155f * T + 2.4192f);
+  float cg = dot(sun, dir);
+  float cgL = cg < 0.0f ? 0.0f : cg;  // sky_luminance clamps below at 0, sky_color_xyz does not
+  if (cgL > 1.0f) cgL = 2.0f - cgL;
+  if (cg > 1.0f) cg = 2.0f - cg;
+  lum *= skyPerez(0.178721f * T - 1.463037f, -0.355402f * T + 0.427494f, -0.022669f * T + 5.325056f,
+                  0.120647f * T - 2.577052f, -0.066967f * T + 0.370275f, ct, acosf(cgL), cgL, ts, cts);
+  float gamma = acosf(cg);
+  float t2 = T * T, ts2 = ts * ts, ts3 = ts2 * ts;
+  float zx = (0.001650f * ts3 - 0.003742f * ts2 + 0.002088f * ts + 0.0f) * t2 +
+             (-0.029028f * ts3 + 0.063773f * ts2 - 0.032020f * ts + 0.003948f) * T +
+             (0.116936f * ts3 - 0.211960f * ts2 + 0.060523f * ts + 0.258852f);
+  float zy = (0.002759f * ts3 - 0.006105f * ts2 + 0.003162f * ts + 0.0f) * t2 +
+             (-0.042149f * ts3 + 0.089701f * ts2 - 0.041536f * ts + 0.005158f) * T +
+             (0.153467f * ts3 - 0.267568f * ts2 + 0.066698f * ts + 0.266881f);
+  float x = skyPerez(-0.019257f * T - (0.29f - powf(cts, 0.5f) * 0.09f), -0.066513f * T + 0.000818f,
+                     -0.000417f * T + 0.212479f, -0.064097f * T - 0.898875f, -0.003251f * T + 0.045178f, ct, gamma,
+                     cg, ts, cts);
+  float y = skyPerez(-0.016698f * T - 0.260787f, -0.094958f * T + 0.009213f, -0.007928f * T + 0.210230f,
+                     -0.044050f * T - 1.653694f, -0.010922f * T + 0.052919f, ct, gamma, cg, ts, cts);
+  x = zx * ((x * 1.0f) + (1.0f - 1.0f));
+  y = zy * ((y * 1.0f) + (1.0f - 1.0f));
+  float X = (x / y) * lum, Y = lum, Z = ((1.0f - x - y) / y) * lum;
+  f3 c = f3{3.241f * X - 1.537f * Y - 0.499f * Z, -0.969f * X + 1.876f * Y + 0.042f * Z,
+            0.056f * X - 0.204f * Y + 1.057f * Z};
+  return c * kSkyPi;
+}
+// calc_sun_color, sun_and_sky.glsl:108-131
+__device__ f3 skySunColor(f3 sd, float T) {
+  if (!(sd.z > 0.0f)) return mk1(0.f);
+  float m = 1.0f / (sd.z + 0.15f * powf(93.885f - acosf(sd.z) * 180.0f / kSkyPi, -1.253f));
+  float beta = 0.04608f * T - 0.04586f;
+  const float ko[3] = {12.0f, 8.5f, 0.9f}, wl[3] = {0.610f, 0.550f, 0.470f};
+  const float sol[3] = {1.0f * 127500.0f / 0.9878f, 0.992f * 127500.0f / 0.9878f, 0.911f * 127500.0f / 0.9878f};
+  float o[3];
+  for (int k = 0; k < 3; ++k) {
+    float ta = expf(-m * beta * powf(wl[k], -1.3f));
+    float to = expf(-m * ko[k] * 0.0035f);
+    float tr = expf(-m * 0.008735f * powf(wl[k], -4.08f));
+    o[k] = tr * ta * to * sol[k];
+  }
+  return f3{o[0], o[1], o[2]};
+}
+
+// The direction-independent part of sun_and_sky (one thread per record).
+__global__ void kSkyPrepare(DevSky* k) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const GpuSunAndSky ss = k->ss;
+  k->hh = ss.horizon_height / 10.0f;
+  float haze = 2.0f + ss.haze;
+  if (haze < 2.0f) haze = 2.0f;
+  k->haze = haze;
+  {  // tweak_saturation, sun_and_sky.glsl:264-277
+    float sat = ss.saturation, low = powf(sat, 3.0f), out = 1.0f;
+    if (sat <= 1.0f) {
+      float h = clampf((haze - 2.0f) / 15.0f, 0.0f, 1.0f);
+      h = powf(h, 3.0f);
+      out = sat * (1.0f - h) + low * h;
+    }
+    k->sat = out;
+  }
+  f3 rs = ld3(ss.rgb_unit_conversion);
+  if (luminance(rs) < 0.0f) rs = mk1(1.0f / 80000.0f);
+  rs = rs * ss.multiplier;
+  f3 sun = skyTweak(makeNormal(ld3(ss.sun_direction)), ss.y_is_up, k->hh);
+  f3 realSun = sun;
+  float factor = 1.0f;
+  if (sun.z < 0.001f) {
+    if (sun.z < 0.0f) {  // night_brightness_adjustment, sun_and_sky.glsl:398-405
+      const float lmt = 0.30901699437494742410229341718282f;
+      if (sun.z <= -lmt) {
+        factor = 0.0f;
+      } else {
+        float f = (sun.z + lmt) / lmt;
+        f *= f;
+        f *= f;
+        factor = f;
+      }
+    }
+    sun.z = 0.001f;
+    sun = makeNormal(sun);
+  }
+  f3 up = skySunColor(sun, haze), dn = skySunColor(sun, 2.0f);
+  // calc_irrad, sun_and_sky.glsl:246-262: 5 x 5 stratified cosine directions
+  // (mi_reflection_dir_diffuse_x about +z), float loop counters.
+  f3 irr = mk1(0.f);
+  for (float u = 1.0f / 10.0f; u < 1.0f; u += 1.0f / 5.0f)
+    for (float v = 1.0f / 10.0f; v < 1.0f; v += 1.0f / 5.0f) {
+      float lx = 2.0f * u - 1.0f, ly = 2.0f * v - 1.0f, r, phi;
+      const float q = kSkyPi / 4.0f;
+      if (lx == 0.0f && ly == 0.0f) r = 0.0f, phi = 0.0f;
+      else if (lx > -ly) {
+        if (lx > ly) r = lx, phi = q * (1.0f + ly / lx);
+        else r = ly, phi = q * (3.0f - lx / ly);
+      } else {
+        if (lx < ly) r = -lx, phi = q * (5.0f + ly / lx);
+        else r = -ly, phi = q * (7.0f - lx / ly);
+      }
+      float x = r * cosf(phi), y = r * sinf(phi);
+      float z2 = 1.0f - x * x - y * y;
+      float z = z2 > 0.0f ? sqrtf(z2) : 0.0f;
+      // xyz2dir about n = +z: u = (1, 0, 0), v = n x u = (0, 1, 0)
+      f3 d = x * f3{1.0f, 0.0f, 0.0f} + y * f3{0.0f, 1.0f, 0.0f} + z * f3{0.0f, 0.0f, 1.0f};
+      irr = irr + skyEnvColor(sun, d, 2.0f);
+    }
+  irr = irr / 25.0f;
+  f3 ground = ld3(ss.ground_color) * (irr + dn * sun.z) * rs;
+  if (factor < 1.0f) ground = ground * factor;
+  float sd = 1.0f, sg = 1.0f;
+  if (ss.physically_scaled_sun == 1) {  // calc_physical_scale, sun_and_sky.glsl:315-396
+    float diskR = 0.00465f * ss.sun_disk_scale, glowR = diskR * 10.0f;
+    float glowInt = ss.sun_glow_intensity * ((4.0f * kSkyPi) - (24.0f * kSkyPi) / (glowR * glowR) +
+                                             (24.0f * kSkyPi) * sinf(glowR) / (glowR * glowR * glowR));
+    float target = ss.sun_disk_intensity * kSkyPi;
+    float maxGlow = 0.5f * target;
+    if (glowInt > maxGlow) {
+      sg *= maxGlow / glowInt;
+      target -= maxGlow;
+    } else {
+      target -= glowInt;
+    }
+    float area = 2.0f * kSkyPi * (1.0f - cosf(diskR));
+    float targetI = target / area;
+    float actual = ss.sun_disk_intensity * 100.0f * (1.0f * area) / area;
+    sd = targetI == 0.0f ? 0.0f : targetI / actual;
+  }
+  for (int i = 0; i < 3; ++i) {
+    k->sun[i] = (&sun.x)[i];
+    k->realSun[i] = (&realSun.x)[i];
+    k->rgbScale[i] = (&rs.x)[i];
+    k->sunUp[i] = (&up.x)[i];
+    k->sunDown[i] = (&dn.x)[i];
+    k->ground[i] = (&ground.x)[i];
+  }
+  k->factor = factor;
+  k->diskScale = sd;
+  k->glowScale = sg;
+  k->inUse = ss.in_use;
+}
+
+// sun_and_sky, sun_and_sky.glsl:407-531, per direction.
+__device__ f3 skyEval(const DevSky& k, f3 inDir) {
+  const GpuSunAndSky& ss = k.ss;
+  if (ss.multiplier <= 0.0f) return mk1(0.f);
+  f3 dir = skyTweak(inDir, ss.y_is_up, k.hh);
+  const float down = dir.z;
+  const f3 realDir = dir;
+  if (dir.z < 0.001f) {
+    dir.z = 0.001f;
+    dir = makeNormal(dir);
+  }
+  f3 tint = mk1(0.f);
+  if (k.factor > 0.0f) {
+    tint = skyEnvColor(ld3(k.sun), dir, k.haze);
+    if (k.factor < 1.0f) tint = tint * k.factor;
+  }
+  if (ss.sun_disk_intensity > 0.0f && ss.sun_disk_scale > 0.0f) {
+    float angle = acosf(dot(realDir, ld3(k.realSun)));
+    float radius = 0.00465f * ss.sun_disk_scale * 10.0f;
+    if (angle < radius) {
+      float f = (1.0f - angle / radius) * 10.0f;
+      f = powf(f / 10.0f, 3.0f) * 2.0f * ss.sun_glow_intensity * k.glowScale +
+          skySmoothstep(8.5f, 9.5f + (k.haze / 50.0f), f) * 100.0f * ss.sun_disk_intensity * k.diskScale;
+      tint = tint + (down > 0.0f ? ld3(k.sunUp) : ld3(k.sunDown)) * f;
+    }
+  }
+  f3 out = tint * ld3(k.rgbScale);
+  float nightF = 1.0f;
+  if (down <= 0.0f) {
+    f3 dc = ld3(k.ground);
+    float blur = ss.horizon_blur / 10.0f;
+    if (blur > 0.0f) {
+      float dn = -down / blur;
+      if (dn > 1.0f) dn = 1.0f;
+      dn = skySmoothstep(0.0f, 1.0f, dn);
+      out = out * (1.0f - dn) + dc * dn;
+      nightF = 1.0f - dn;
+    } else {
+      out = dc;
+      nightF = 0.0f;
+    }
+  }
+  {  // arch_colortweak, sun_and_sky.glsl:291-313
+    float I = luminance(out), sat = k.sat, red = ss.redblueshift;
+    out = sat <= 0.0f ? mk1(I) : out * sat + mk1(I * (1.0f - sat));
+    out = out * f3{1.0f + red, 1.0f, 1.0f - red};
+  }
+  if (nightF > 0.0f) {
+    f3 night = ld3(ss.night_color) * nightF;
+    out = f3{fmaxf(out.x, night.x), fmaxf(out.y, night.y), fmaxf(out.z, night.z)};
+  }
+  return out * kSkyPi;
+}
+
 // sampleRectLight / sampleTriangleLight / samplePointLight /
 // sampleDistantLight, sample_light.glsl:10-82
 __device__ __forceinline__ f3 sampleOneLight(float r0, float r1, const GpuLight& L, f3 p, LSample& s) {
@@ -945,7 +1164,8 @@ __device__ f3 sampleLights(PathCtx& c, f3 pos, f3 nrm, bool& visible, LSample& l
   const GpuPushConstantRaytrace& pc = c.sp->pc;
   bool allowDoubleSide = false;
   f3 radiance = mk1(0.f);
-  bool hasEnv = pc.hasEnvMap == 1, hasLight = pc.numLights > 0;
+  const bool sky = c.sc->sky->inUse == 1;
+  bool hasEnv = pc.hasEnvMap == 1 || sky, hasLight = pc.numLights > 0;
   float envSel, anaSel;
   if (hasEnv && hasLight) envSel = anaSel = 0.5f;
   else if (hasEnv) envSel = 1.f, anaSel = 0.f;
@@ -960,10 +1180,16 @@ __device__ f3 sampleLights(PathCtx& c, f3 pos, f3 nrm, bool& visible, LSample& l
     l.dist = kInf;
     l.n = -makeNormal(c.rd);
     float r0 = rnd(c.seed), r1 = rnd(c.seed);
-    f3 Ldir;
+    f3 Ldir, rad;
     float pdf;
-    f3 rad = sampleEnv(*c.sc, r0, r1, c.sp->cam.envTransform, pc.envMapResolution[0], pc.envMapResolution[1],
-                       pc.envMapIntensity, Ldir, pdf);
+    if (sky) {  // sampleSunSky, sample_light.glsl:137-141
+      Ldir = uniformSphere(r0, r1);
+      pdf = kInv4Pi;
+      rad = skyEval(*c.sc->sky, Ldir);
+    } else {
+      rad = sampleEnv(*c.sc, r0, r1, c.sp->cam.envTransform, pc.envMapResolution[0], pc.envMapResolution[1],
+                      pc.envMapIntensity, Ldir, pdf);
+    }
     l.d = Ldir;
     l.pdf = pdf;
     radiance = rad / envSel;
@@ -1785,11 +2011,13 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
   if (hi.x < 0) {
     // raytrace.default.rmiss:24-55
     c.stop = true;
-    f3 env = sc.hasEnv && pc.hasEnvMap == 1 ? evalEnv(sc, sp.cam.envTransform, pc.envMapIntensity, c.rd)
-                                            : ld3(pc.bgColor);
+    const bool sky = sc.sky->inUse == 1;
+    f3 env = sky ? skyEval(*sc.sky, c.rd)
+                 : (sc.hasEnv && pc.hasEnvMap == 1 ? evalEnv(sc, sp.cam.envTransform, pc.envMapIntensity, c.rd)
+                                                   : ld3(pc.bgColor));
     float mis = 1.0f;
     if (c.depth != 1 && isNonSpecular(c.bflags)) {
-      float envPdf = (sc.hasEnv && pc.hasEnvMap == 1)
+      float envPdf = (!sky && sc.hasEnv && pc.hasEnvMap == 1)
                          ? pdfEnv(sc, sp.cam.envTransform, pc.envMapResolution[0], pc.envMapResolution[1], c.rd)
                          : kInv4Pi;
       mis = powerHeuristic(c.bpdf, envPdf);
@@ -1990,6 +2218,8 @@ void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t, uint32_t gr
 void launchAccumulate(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
   kAccumulate<<<(b.nAcc + 255) / 256, 256, 0, s>>>(b, w);
 }
+void launchSkyPrepare(DevSky* sky, hipStream_t s) { kSkyPrepare<<<1, 64, 0, s>>>(sky); }
+
 void launchResetCounters(const Wavefront& w, int cur, hipStream_t s) { kResetCounters<<<1, 64, 0, s>>>(w, cur); }
 void launchFirstHit(const SceneView& sc, const ShotParams* shot, const Wavefront& w, ash_hit* out, uint32_t P,
                     hipStream_t s) {

@@ -145,6 +145,30 @@ typedef struct GpuPushConstantRaytrace {
   int32_t uvOutChannel;
 } GpuPushConstantRaytrace;
 
+/* GpuSunAndSky, src/shared/sun_and_sky.h:6-25 (96 B): the procedural sky
+ * (NVIDIA's MDL sun & sky, utils/sun_and_sky.glsl).  When in_use == 1 it
+ * replaces the env map and background in the miss shader
+ * (raytrace.default.rmiss:29-46) and is sampled uniformly over the sphere in
+ * NEE (rchit_layouts.glsl:98-116, sample_light.glsl:137-141). */
+typedef struct GpuSunAndSky {
+  float rgb_unit_conversion[3];
+  float multiplier;
+  float haze;
+  float redblueshift;
+  float saturation;
+  float horizon_height;
+  float ground_color[3];
+  float horizon_blur;
+  float night_color[3];
+  float sun_disk_intensity;
+  float sun_direction[3];
+  float sun_disk_scale;
+  float sun_glow_intensity;
+  int32_t y_is_up;
+  int32_t physically_scaled_sun;
+  int32_t in_use;
+} GpuSunAndSky;
+
 /* Number of RGBA32F output images, src/shared/binding.h:70.
  * image 0 = running weighted mean of radiance, images 1..7 = multi-channel
  * AOVs (channel c -> image c+1), image 8 = sum of filter weights
@@ -288,6 +312,16 @@ const char* ash_last_error(const ash_ctx* ctx);
 /* Upload a scene (Scene::submit, scene.cpp:26-85).  Replaces any previous one.
  * Env-map importance tables are built here (EnvMap::EnvMap, texture.cpp:144-226). */
 int ash_upload_scene(ash_ctx* ctx, const ash_scene_desc* scene);
+
+/* The defaults Scene::init writes (scene.cpp:112-129; in_use = 0). */
+void ash_sun_and_sky_defaults(GpuSunAndSky* out);
+
+/* Set the sun & sky record of the uploaded scene (replaces the GUI's edit of
+ * Scene::getSunsky and its uniform upload, scene.cpp:437,
+ * pipeline_graphics.cpp:100-103).  ash_upload_scene resets it to the
+ * defaults, as each Scene starts from them.  Takes effect for later renders,
+ * including ones replayed from cached graphs. */
+int ash_set_sun_and_sky(ash_ctx* ctx, const GpuSunAndSky* sky);
 
 /* Build the acceleration structure on the device (replaces
  * createBottomLevelAS/createTopLevelAS, pipeline_raytrace.cpp:107-147). */

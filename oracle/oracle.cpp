@@ -324,6 +324,7 @@ struct orc_scene {
   std::vector<Img> textures;
   bool hasEnv = false;
   Img env, marg, cond;
+  GpuSunAndSky sky{};  // in_use 0 until orc_set_sun_and_sky
   // Flattened world-space triangles (9 floats) with (instance, primitive).
   std::vector<float> tri;
   std::vector<int32_t> triInst, triPrim;
@@ -816,13 +817,256 @@ template <class R> struct Payload {
   R firstT = 0;
 };
 
+
+// ------------------------------------------------------------ sun & sky
+// utils/sun_and_sky.glsl:20-533 (NVIDIA's MDL sun & sky: Preetham/Perez
+// sky, attenuated sun disk with glow, ground plane below the horizon).  The
+// file defines its own M_PI = 3.1415926535f.
+template <class R> struct Sky {
+  static constexpr R MPI = R(3.1415926535f);
+};
+template <class R> R skyLum(V3<R> c) { return R(0.2126) * c.x + R(0.7152) * c.y + R(0.0722) * c.z; }
+template <class R> R smoothstepR(R e0, R e1, R x) {
+  R t = std::fmin(std::fmax((x - e0) / (e1 - e0), R(0)), R(1));
+  return t * t * (R(3) - R(2) * t);
+}
+// xyz2dir, sun_and_sky.glsl:33-57
+template <class R> V3<R> skyXyz2dir(V3<R> n, R x, R y, R z) {
+  V3<R> u;
+  if (std::fabs(n.x) < std::fabs(n.y)) u = V3<R>(0, -n.z, n.y);
+  else u = V3<R>(n.z, 0, -n.x);
+  u = makeNormal(u);
+  V3<R> v = cross(n, u);
+  return x * u + y * v + z * n;
+}
+// mi_lib_square_to_disk + mi_reflection_dir_diffuse_x, sun_and_sky.glsl:59-106
+template <class R> V3<R> skyDiffuseDir(V3<R> n, R sx, R sy) {
+  R lx = R(2) * sx - R(1), ly = R(2) * sy - R(1), r, phi;
+  const R q = Sky<R>::MPI / R(4);
+  if (lx == R(0) && ly == R(0)) {
+    r = 0, phi = 0;
+  } else if (lx > -ly) {
+    if (lx > ly) r = lx, phi = q * (R(1) + ly / lx);
+    else r = ly, phi = q * (R(3) - lx / ly);
+  } else {
+    if (lx < ly) r = -lx, phi = q * (R(5) + ly / lx);
+    else r = -ly, phi = q * (R(7) - lx / ly);
+  }
+  R x = r * std::cos(phi), y = r * std::sin(phi);
+  R z2 = R(1) - x * x - y * y;
+  R z = z2 > R(0) ? std::sqrt(z2) : R(0);
+  return skyXyz2dir(n, x, y, z);
+}
+// calc_sun_color, sun_and_sky.glsl:108-131
+template <class R> V3<R> skySunColor(V3<R> sd, R turbidity) {
+  V3<R> c(0);
+  const R ko[3] = {R(12.0), R(8.5), R(0.9)}, wl[3] = {R(0.610), R(0.550), R(0.470)};
+  const R sol[3] = {R(1.0) * R(127500) / R(0.9878), R(0.992) * R(127500) / R(0.9878),
+                    R(0.911) * R(127500) / R(0.9878)};
+  if (sd.z > R(0)) {
+    R m = R(1) / (sd.z + R(0.15) * std::pow(R(93.885) - std::acos(sd.z) * R(180) / Sky<R>::MPI, R(-1.253)));
+    R beta = R(0.04608) * turbidity - R(0.04586), alpha = R(1.3), l = R(0.0035);
+    R out[3];
+    for (int k = 0; k < 3; ++k) {
+      R ta = std::exp(-m * beta * std::pow(wl[k], -alpha));
+      R to = std::exp(-m * ko[k] * l);
+      R tr = std::exp(-m * R(0.008735) * std::pow(wl[k], R(-4.08)));
+      out[k] = tr * ta * to * sol[k];
+    }
+    c = V3<R>(out[0], out[1], out[2]);
+  }
+  return c;
+}
+// Perez distribution ratio F(theta, gamma) / F(0, theta_s).
+template <class R> R skyPerez(R A, R B, R C, R D, R E, R cosTheta, R gamma, R cosGamma, R thetaSun, R cosThetaSun) {
+  return ((R(1) + A * std::exp(B / cosTheta)) * (R(1) + C * std::exp(D * gamma) + E * cosGamma * cosGamma)) /
+         ((R(1) + A * std::exp(B / R(1))) *
+          (R(1) + C * std::exp(D * thetaSun) + E * cosThetaSun * cosThetaSun));
+}
+// sky_color_xyz, sun_and_sky.glsl:134-199
+template <class R> V3<R> skyColorXyz(V3<R> dir, V3<R> sun, R T, R lum) {
+  R cg = dot(sun, dir);
+  if (cg > R(1)) cg = R(2) - cg;
+  R gamma = std::acos(cg), ct = dir.z, cts = sun.z, ts = std::acos(cts);
+  R t2 = T * T, ts2 = ts * ts, ts3 = ts2 * ts;
+  R zx = (R(0.001650) * ts3 - R(0.003742) * ts2 + R(0.002088) * ts + R(0)) * t2 +
+         (R(-0.029028) * ts3 + R(0.063773) * ts2 - R(0.032020) * ts + R(0.003948)) * T +
+         (R(0.116936) * ts3 - R(0.211960) * ts2 + R(0.060523) * ts + R(0.258852));
+  R zy = (R(0.002759) * ts3 - R(0.006105) * ts2 + R(0.003162) * ts + R(0)) * t2 +
+         (R(-0.042149) * ts3 + R(0.089701) * ts2 - R(0.041536) * ts + R(0.005158)) * T +
+         (R(0.153467) * ts3 - R(0.267568) * ts2 + R(0.066698) * ts + R(0.266881));
+  R x = skyPerez(R(-0.019257) * T - (R(0.29) - std::pow(cts, R(0.5)) * R(0.09)), R(-0.066513) * T + R(0.000818),
+                 R(-0.000417) * T + R(0.212479), R(-0.064097) * T - R(0.898875), R(-0.003251) * T + R(0.045178), ct,
+                 gamma, cg, ts, cts);
+  R y = skyPerez(R(-0.016698) * T - R(0.260787), R(-0.094958) * T + R(0.009213), R(-0.007928) * T + R(0.210230),
+                 R(-0.044050) * T - R(1.653694), R(-0.010922) * T + R(0.052919), ct, gamma, cg, ts, cts);
+  const R sat = 1;
+  x = zx * ((x * sat) + (R(1) - sat));
+  y = zy * ((y * sat) + (R(1) - sat));
+  return V3<R>((x / y) * lum, lum, ((R(1) - x - y) / y) * lum);
+}
+// sky_luminance, sun_and_sky.glsl:201-226
+template <class R> R skyLuminance(V3<R> dir, V3<R> sun, R T) {
+  R cg = dot(sun, dir);
+  if (cg < R(0)) cg = 0;
+  if (cg > R(1)) cg = R(2) - cg;
+  R gamma = std::acos(cg), ct = dir.z, cts = sun.z, ts = std::acos(cts);
+  return skyPerez(R(0.178721) * T - R(1.463037), R(-0.355402) * T + R(0.427494), R(-0.022669) * T + R(5.325056),
+                  R(0.120647) * T - R(2.577052), R(-0.066967) * T + R(0.370275), ct, gamma, cg, ts, cts);
+}
+// calc_env_color, sun_and_sky.glsl:228-244
+template <class R> V3<R> skyEnvColor(V3<R> sun, V3<R> dir, R T) {
+  R ts = std::acos(sun.z);
+  R chi = (R(4.0) / R(9.0) - T / R(120.0)) * (Sky<R>::MPI - R(2) * ts);
+  R lum = R(1000.0) * ((R(4.0453) * T - R(4.9710)) * std::tan(chi) - R(0.2155) * T + R(2.4192));
+  lum *= skyLuminance(dir, sun, T);
+  V3<R> X = skyColorXyz(dir, sun, T, lum);
+  V3<R> c(R(3.241) * X.x - R(1.537) * X.y - R(0.499) * X.z, R(-0.969) * X.x + R(1.876) * X.y + R(0.042) * X.z,
+          R(0.056) * X.x - R(0.204) * X.y + R(1.057) * X.z);
+  return c * Sky<R>::MPI;
+}
+// calc_irrad, sun_and_sky.glsl:246-262: 5x5 stratified cosine directions;
+// the loop counters are GLSL floats.
+template <class R> V3<R> skyIrrad(V3<R> sun, R haze) {
+  V3<R> acc(0);
+  for (float u = 1.f / 10.f; u < 1.f; u += 1.f / 5.f)
+    for (float v = 1.f / 10.f; v < 1.f; v += 1.f / 5.f)
+      acc = acc + skyEnvColor(sun, skyDiffuseDir(V3<R>(0, 0, 1), R(u), R(v)), haze);
+  return acc / R(25.0);
+}
+// tweak_saturation, sun_and_sky.glsl:264-277
+template <class R> R skyTweakSaturation(R sat, R haze) {
+  R low = std::pow(sat, R(3));
+  if (sat <= R(1)) {
+    R h = (haze - R(2)) / R(15);
+    h = std::fmin(std::fmax(h, R(0)), R(1));
+    h = std::pow(h, R(3));
+    return sat * (R(1) - h) + low * h;
+  }
+  return 1;
+}
+// arch_vectortweak, sun_and_sky.glsl:279-289
+template <class R> V3<R> skyVectorTweak(V3<R> d, int yUp, R hh) {
+  V3<R> o = yUp == 1 ? V3<R>(d.x, d.z, d.y) : d;
+  if (hh != R(0)) {
+    o.z -= hh;
+    o = makeNormal(o);
+  }
+  return o;
+}
+// arch_colortweak, sun_and_sky.glsl:291-313 (its clamp of boosted
+// saturation writes a value nothing reads).
+template <class R> V3<R> skyColorTweak(V3<R> tint, R sat, R red) {
+  R I = skyLum(tint);
+  V3<R> o = sat <= R(0) ? V3<R>(I) : tint * sat + V3<R>(I * (R(1) - sat));
+  return o * V3<R>(R(1) + red, R(1), R(1) - red);
+}
+// calc_physical_scale, sun_and_sky.glsl:315-396: (disk scale, glow scale).
+template <class R> void skyPhysicalScale(R diskScale, R glow, R diskInt, R& sdScale, R& sgScale) {
+  const R PI = Sky<R>::MPI;
+  R diskR = R(0.00465) * diskScale, glowR = diskR * R(10);
+  R glowInt = glow * ((R(4) * PI) - (R(24) * PI) / (glowR * glowR) + (R(24) * PI) * std::sin(glowR) / (glowR * glowR * glowR));
+  R target = diskInt * PI;
+  sgScale = 1;
+  R maxGlow = R(0.5) * target;
+  if (glowInt > maxGlow) {
+    sgScale *= maxGlow / glowInt;
+    target -= maxGlow;
+  } else {
+    target -= glowInt;
+  }
+  R area = R(2) * PI * (R(1) - std::cos(diskR));
+  R targetI = target / area;
+  R actual = diskInt * R(100) * (R(1) * area) / area;
+  sdScale = targetI == R(0) ? R(0) : targetI / actual;
+}
+// night_brightness_adjustment, sun_and_sky.glsl:398-405
+template <class R> R skyNight(V3<R> sd) {
+  const R lmt = R(0.30901699437494742410229341718282);
+  if (sd.z <= -lmt) return 0;
+  R f = (sd.z + lmt) / lmt;
+  f *= f;
+  f *= f;
+  return f;
+}
+// sun_and_sky, sun_and_sky.glsl:407-531
+template <class R> V3<R> sunAndSky(const GpuSunAndSky& ss, V3<R> inDir) {
+  R factor = 1, nightFactor = 1;
+  V3<R> rgbScale = load3<R>(ss.rgb_unit_conversion);
+  R hh = R(ss.horizon_height) / R(10);
+  V3<R> dir = skyVectorTweak(inDir, ss.y_is_up, hh);
+  R haze = R(2) + R(ss.haze);
+  if (haze < R(2)) haze = 2;
+  R sat = skyTweakSaturation(R(ss.saturation), haze);
+  if (skyLum(rgbScale) < R(0)) rgbScale = V3<R>(R(1) / R(80000));
+  rgbScale = rgbScale * R(ss.multiplier);
+  if (R(ss.multiplier) <= R(0)) return V3<R>(0);
+  R down = dir.z;
+  V3<R> realDir = dir;
+  if (dir.z < R(0.001)) {
+    dir.z = R(0.001);
+    dir = makeNormal(dir);
+  }
+  V3<R> sun = skyVectorTweak(makeNormal(load3<R>(ss.sun_direction)), ss.y_is_up, hh);
+  V3<R> realSun = sun;
+  if (sun.z < R(0.001)) {
+    if (sun.z < R(0)) factor = skyNight(sun);
+    sun.z = R(0.001);
+    sun = makeNormal(sun);
+  }
+  V3<R> tint(0);
+  if (factor > R(0)) {
+    tint = skyEnvColor(sun, dir, haze);
+    if (factor < R(1)) tint = tint * factor;
+  }
+  V3<R> sunColor = skySunColor(sun, down > R(0) ? haze : R(2));
+  if (R(ss.sun_disk_intensity) > R(0) && R(ss.sun_disk_scale) > R(0)) {
+    R angle = std::acos(dot(realDir, realSun));
+    R radius = R(0.00465) * R(ss.sun_disk_scale) * R(10);
+    if (angle < radius) {
+      R sd = 1, sg = 1;
+      if (ss.physically_scaled_sun == 1)
+        skyPhysicalScale(R(ss.sun_disk_scale), R(ss.sun_glow_intensity), R(ss.sun_disk_intensity), sd, sg);
+      R f = (R(1) - angle / radius) * R(10);
+      f = std::pow(f / R(10), R(3)) * R(2) * R(ss.sun_glow_intensity) * sg +
+          smoothstepR(R(8.5), R(9.5) + (haze / R(50)), f) * R(100) * R(ss.sun_disk_intensity) * sd;
+      tint = tint + sunColor * f;
+    }
+  }
+  V3<R> out = tint * rgbScale;
+  if (down <= R(0)) {
+    V3<R> irrad = skyIrrad(sun, R(2));
+    V3<R> dc = load3<R>(ss.ground_color) * (irrad + sunColor * sun.z) * rgbScale;
+    if (factor < R(1)) dc = dc * factor;
+    R blur = R(ss.horizon_blur) / R(10);
+    if (blur > R(0)) {
+      R dn = -down / blur;
+      if (dn > R(1)) dn = 1;
+      dn = smoothstepR(R(0), R(1), dn);
+      out = out * (R(1) - dn) + dc * dn;
+      nightFactor = R(1) - dn;
+    } else {
+      out = dc;
+      nightFactor = 0;
+    }
+  }
+  out = skyColorTweak(out, sat, R(ss.redblueshift));
+  V3<R> res = out;
+  if (nightFactor > R(0)) {
+    V3<R> night = load3<R>(ss.night_color) * nightFactor;
+    res = V3<R>(std::fmax(res.x, night.x), std::fmax(res.y, night.y), std::fmax(res.z, night.z));
+  }
+  return res * Sky<R>::MPI;
+}
+
 template <class R>
 V3<R> sampleLights(const RCtx& c, Payload<R>& pl, V3<R> pos, V3<R> nrm, bool& visible, LRec<R>& l) {
   // rchit_layouts.glsl:118-167
   const GpuPushConstantRaytrace& pc = *c.pc;
   bool allowDoubleSide = false;
   V3<R> radiance(0);
-  bool hasEnv = pc.hasEnvMap == 1;
+  const bool sky = c.s->sky.in_use == 1;
+  bool hasEnv = pc.hasEnvMap == 1 || sky;
   bool hasLight = pc.numLights > 0;
   R envSel, anaSel;
   if (hasEnv && hasLight) envSel = anaSel = R(0.5);
@@ -838,8 +1082,15 @@ V3<R> sampleLights(const RCtx& c, Payload<R>& pl, V3<R> pos, V3<R> nrm, bool& vi
     R r0 = rnd<R>(pl.seed), r1 = rnd<R>(pl.seed);
     V3<R> L;
     R pdf;
-    V3<R> rad = sampleEnvmap(*c.s, r0, r1, c.cam->envTransform, R(pc.envMapResolution[0]),
-                             R(pc.envMapResolution[1]), R(pc.envMapIntensity), L, pdf);
+    V3<R> rad;
+    if (sky) {  // sampleSunSky, sample_light.glsl:137-141
+      L = uniformSampleSphere(r0, r1);
+      pdf = K<R>::INV_4PI;
+      rad = sunAndSky(c.s->sky, L);
+    } else {
+      rad = sampleEnvmap(*c.s, r0, r1, c.cam->envTransform, R(pc.envMapResolution[0]), R(pc.envMapResolution[1]),
+                         R(pc.envMapIntensity), L, pdf);
+    }
     l.d = L;
     l.pdf = pdf;
     radiance = rad / envSel;
@@ -2048,14 +2299,19 @@ template <class R> void missShader(const RCtx& c, Payload<R>& pl) {
   const GpuPushConstantRaytrace& pc = *c.pc;
   pl.stop = true;
   V3<R> env, d = pl.rd;
-  if (pc.hasEnvMap == 1)
+  const bool sky = c.s->sky.in_use == 1;
+  if (sky)
+    env = sunAndSky(c.s->sky, d);
+  else if (pc.hasEnvMap == 1)
     env = evalEnvmap(*c.s, c.cam->envTransform, R(pc.envMapIntensity), d);
   else
     env = load3<R>(pc.bgColor);
   R mis = 1;
   if (pl.depth != 1 && isNonSpecular(pl.bflags)) {
     R envPdf;
-    if (pc.hasEnvMap == 1)
+    if (sky)
+      envPdf = K<R>::INV_4PI;
+    else if (pc.hasEnvMap == 1)
       envPdf = pdfEnvmap(*c.s, c.cam->envTransform, R(pc.envMapResolution[0]), R(pc.envMapResolution[1]), d);
     else
       envPdf = K<R>::INV_4PI;
@@ -2713,6 +2969,22 @@ float orc_bsdf_pdf(const GpuMaterial* m, const float Lp[3], const float Vp[3], u
 }
 
 float orc_conductor_reflectance(float eta, float k, float cosThetaI) { return conductorReflectance(eta, k, cosThetaI); }
+
+void orc_set_sun_and_sky(orc_scene* s, const GpuSunAndSky* sky) { s->sky = *sky; }
+void orc_sun_and_sky(const GpuSunAndSky* sky, const float dir[3], int fp64, float out[3]) {
+  if (fp64) {
+    V3<double> r = sunAndSky(*sky, V3<double>(dir[0], dir[1], dir[2]));
+    out[0] = float(r.x), out[1] = float(r.y), out[2] = float(r.z);
+  } else {
+    V3<float> r = sunAndSky(*sky, load3<float>(dir));
+    out[0] = r.x, out[1] = r.y, out[2] = r.z;
+  }
+}
+void orc_sky_physical_scale(float disk_scale, float glow_intensity, float disk_intensity, float out[2]) {
+  float a, b;
+  skyPhysicalScale(disk_scale, glow_intensity, disk_intensity, a, b);
+  out[0] = a, out[1] = b;
+}
 
 void orc_bsdf_sample(const GpuMaterial* m, const float Vp[3], uint32_t* seed, float w[3], float dir[3], float* pdf,
                      uint32_t* flags) {

@@ -100,6 +100,12 @@ void orc_bsdf_eval(const GpuMaterial* m, const float L[3], const float V[3], uin
 float orc_bsdf_pdf(const GpuMaterial* m, const float L[3], const float V[3], uint32_t flags);
 /* Shirley's conductor Fresnel (brdf_conductor.rchit:14-34). */
 float orc_conductor_reflectance(float eta, float k, float cos_theta_i);
+/* Sun & sky (utils/sun_and_sky.glsl): set the scene's record; evaluate the
+ * sky radiance toward dir (FP32, or FP64 when fp64 != 0); the physical sun
+ * scales (disk, glow) of calc_physical_scale. */
+void orc_set_sun_and_sky(orc_scene* s, const GpuSunAndSky* sky);
+void orc_sun_and_sky(const GpuSunAndSky* sky, const float dir[3], int fp64, float out[3]);
+void orc_sky_physical_scale(float disk_scale, float glow_intensity, float disk_intensity, float out[2]);
 /* Sample: draws from *seed in reference order; returns weight (f*cos, or
  * f*cos/pdf where the shader divides later is NOT applied), direction,
  * pdf, flags. */

@@ -63,6 +63,9 @@ def lib():
         "orc_sample_envmap": (None, [vp, F, C.c_float, C.c_float, C.c_float, F, F, F]),
         "orc_pdf_envmap": (C.c_float, [vp, F, F]),
         "orc_eval_envmap": (None, [vp, F, C.c_float, F, F]),
+        "orc_set_sun_and_sky": (None, [vp, P(A.GpuSunAndSky)]),
+        "orc_sun_and_sky": (None, [P(A.GpuSunAndSky), F, C.c_int, F]),
+        "orc_sky_physical_scale": (None, [C.c_float, C.c_float, C.c_float, F]),
     }
     for n, (r, a) in sigs.items():
         f = getattr(L, n)
@@ -110,6 +113,9 @@ class OracleScene:
         hit = A.Hit()
         r = lib().orc_trace_closest(self._s, f3(o), f3(d), tmin, tmax, C.byref(hit), int(brute))
         return r, hit
+
+    def set_sun_and_sky(self, sky):
+        lib().orc_set_sun_and_sky(self._s, C.byref(sky))
 
     def trace_any(self, o, d, tmin, tmax, brute=False):
         return lib().orc_trace_any(self._s, f3(o), f3(d), tmin, tmax, int(brute))

@@ -287,6 +287,51 @@ def test_disney_scene_matches_oracle(renderer, tmp_path):
     assert abs(st["extension_rays"] - ost["extension_rays"]) / ost["extension_rays"] < 2e-3
 
 
+def test_sun_and_sky_scene_matches_oracle(renderer, tmp_path):
+    """The procedural sun & sky in place of the background: miss radiance
+    (sky, sun disk and glow, ground below the horizon) and uniform-sphere NEE
+    toward it, in the open box with a pbr floor; the record resets to
+    in_use 0 on the next upload."""
+    d = tmp_path / "sky"
+    d.mkdir()
+    sp = A.generate_scene(0, str(d))
+    s = A.HostScene(sp)
+    renderer.upload(s)
+    renderer.build()
+    sky = A.sun_and_sky_defaults()
+    sky.in_use = 1
+    sky.sun_direction[:] = (0.4, 0.5, -0.6)
+    sky.haze = 1.0
+    sky.horizon_blur = 0.5
+    sky.multiplier = 2e-5
+    renderer.set_sun_and_sky(sky)
+    cam, pc = s.shot(0)
+    w, h = s.film
+    renderer.set_film(w, h)
+    spp = 16
+    st = renderer.render(cam, pc, 0, spp)
+    g = renderer.read(0)
+    osc = O.OracleScene(s.desc)
+    osc.set_sun_and_sky(sky)
+    sq = np.zeros((h, w, 4), np.float32)
+    o, ost = osc.render(cam, pc, w, h, 0, spp, sq=sq)
+    frac_bad = _three_sigma(g, o[0], sq, spp)
+    mg, mo = g[..., :3].mean((0, 1)), o[0][..., :3].mean((0, 1))
+    log(f"sun & sky: outside 3 sigma {frac_bad:.5f}, means {mg} vs {mo}")
+    assert frac_bad <= 0.01
+    np.testing.assert_allclose(mg, mo, rtol=1e-2)
+    assert abs(st["shadow_rays"] - ost["shadow_rays"]) / ost["shadow_rays"] < 2e-3
+    # a new upload starts from the defaults again (in_use 0): the background colour is back
+    renderer.upload(s)
+    renderer.build()
+    renderer.set_film(w, h)
+    renderer.render(cam, pc, 0, 2)
+    base = renderer.read(0)
+    osc2 = O.OracleScene(s.desc)
+    o2, _ = osc2.render(cam, pc, w, h, 0, 2)
+    np.testing.assert_allclose(base[..., :3].mean((0, 1)), o2[0][..., :3].mean((0, 1)), rtol=2e-2)
+
+
 def test_env_tables_on_device_equal_oracle(renderer, spheres_small):
     s, _ = _load(renderer, spheres_small)
     env = np.ctypeslib.as_array(s.desc.envmap.rgba, (s.desc.envmap.height, s.desc.envmap.width, 4)).copy()

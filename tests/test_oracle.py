@@ -838,3 +838,109 @@ def test_disney_diffuse_closed_forms():
     # eval needs EArea; delta lights get zero
     L.orc_bsdf_eval(C.byref(m), n, n, 1, out)
     assert list(out) == [0, 0, 0]
+
+
+# --------------------------------------------------------------- sun & sky
+def _sky(**kw):
+    s = A.sun_and_sky_defaults()
+    s.in_use = 1
+    for k, v in kw.items():
+        if isinstance(v, (tuple, list)):
+            getattr(s, k)[:] = v
+        else:
+            setattr(s, k, v)
+    return s
+
+
+def _sky_eval(s, d, fp64=0):
+    out = (C.c_float * 3)()
+    O.lib().orc_sun_and_sky(C.byref(s), O.f3(d), fp64, out)
+    return np.array(out, np.float64)
+
+
+def test_sun_and_sky_defaults_are_the_scene_defaults():
+    """scene.cpp:112-129."""
+    s = A.sun_and_sky_defaults()
+    assert list(s.rgb_unit_conversion) == [1, 1, 1] and abs(s.multiplier - 0.0000101320) < 1e-12
+    assert s.haze == 0 and s.redblueshift == 0 and s.saturation == 1 and s.horizon_height == 0
+    np.testing.assert_allclose(list(s.ground_color), [0.4] * 3, rtol=1e-7)
+    assert abs(s.horizon_blur - 0.1) < 1e-7
+    np.testing.assert_allclose(list(s.night_color), [0, 0, 0.01], rtol=1e-7)
+    assert abs(s.sun_disk_intensity - 0.8) < 1e-7 and s.sun_disk_scale == 5 and s.sun_glow_intensity == 1
+    np.testing.assert_allclose(list(s.sun_direction), [0, 0.78, 0.62], rtol=1e-7)
+    assert s.y_is_up == 1 and s.physically_scaled_sun == 1 and s.in_use == 0
+
+
+def test_sky_night_is_the_night_colour_and_zero_multiplier_is_black():
+    """Sun below -18 deg (z <= -sin 18 deg): the sky tint is 0 and the
+    result is pi x night colour; a non-positive multiplier returns 0."""
+    s = _sky(sun_direction=(0.0, -0.5, 0.866))
+    r = _sky_eval(s, (0.0, 1.0, 0.0))
+    np.testing.assert_allclose(r, np.float32(3.1415926535) * np.array([0, 0, 0.01], np.float32), rtol=1e-6)
+    s0 = _sky(multiplier=0.0)
+    assert list(_sky_eval(s0, (0.2, 0.9, 0.1))) == [0, 0, 0]
+
+
+def test_sky_is_symmetric_about_the_sun_vertical_plane():
+    s = _sky(sun_direction=(0.3, 0.6, 0.0), haze=1.5, redblueshift=0.1, saturation=0.8)
+    rng = np.random.default_rng(3)
+    for _ in range(50):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        a = _sky_eval(s, tuple(d))
+        b = _sky_eval(s, (d[0], d[1], -d[2]))
+        np.testing.assert_array_equal(a, b)
+
+
+def test_sky_zenith_is_preetham_zenith_luminance():
+    """At the zenith both Perez ratios are exactly 1, so the sky XYZ there is
+    Preetham's zenith value: Y = 1000 Yz with Yz = (4.0453 T - 4.9710) tan chi
+    - 0.2155 T + 2.4192, chi = (4/9 - T/120)(pi - 2 theta_s) (Preetham, Shirley
+    and Smits 1999, appendix A.2), turbidity T = 2 + haze.  Recovered through
+    the inverse of the XYZ->RGB matrix and the two factors of pi; a clear sky
+    there is blue."""
+    M = np.array([[3.241, -1.537, -0.499], [-0.969, 1.876, 0.042], [0.056, -0.204, 1.057]])
+    for haze, elev in [(0.0, 40.0), (2.0, 25.0), (4.0, 60.0)]:
+        e = math.radians(elev)
+        s = _sky(sun_direction=(math.cos(e), math.sin(e), 0.0), haze=haze, sun_disk_intensity=0.0)
+        rgb = _sky_eval(s, (0.0, 1.0, 0.0), fp64=1)
+        pi = float(np.float32(3.1415926535))
+        xyz = np.linalg.solve(M, rgb / (pi * pi * s.multiplier))
+        T = 2.0 + haze
+        ts = math.pi / 2 - e
+        chi = (4 / 9 - T / 120) * (math.pi - 2 * ts)
+        Yz = (4.0453 * T - 4.9710) * math.tan(chi) - 0.2155 * T + 2.4192
+        assert abs(xyz[1] - 1000 * Yz) < 1e-4 * 1000 * Yz, (xyz, Yz)
+        assert rgb[2] > rgb[1] > rgb[0] > 0
+
+
+def test_sky_physical_sun_scale_uses_the_glow_integral():
+    """calc_physical_scale's flattened glow integral equals the integral its
+    comment states, 2 pi int_0^R (1 - x/R)^3 2 g sin x dx (R = glow radius),
+    and the glow is capped at half of pi x disk intensity."""
+    from scipy.integrate import quad
+    L = O.lib()
+    out = (C.c_float * 2)()
+    for scale, glow, disk in [(5.0, 1.0, 0.8), (40.0, 0.05, 1.0)]:
+        Rd = 0.00465 * scale
+        Rg = Rd * 10
+        G = 2 * math.pi * quad(lambda x: (1 - x / Rg) ** 3 * 2 * glow * math.sin(x), 0, Rg, epsabs=1e-14)[0]
+        L.orc_sky_physical_scale(scale, glow, disk, out)
+        area = 2 * math.pi * (1 - math.cos(Rd))
+        if G <= 0.5 * disk * math.pi:
+            assert abs(out[1] - 1.0) < 1e-7
+            expected = (disk * math.pi - G) / area / (disk * 100)
+        else:
+            assert abs(out[1] - 0.5 * disk * math.pi / G) < 2e-3 * out[1]
+            expected = 0.5 * disk * math.pi / area / (disk * 100)
+        assert abs(out[0] - expected) < 2e-3 * expected, (scale, out[0], expected)
+
+
+def test_sky_fp32_and_fp64_restatements_agree():
+    s = _sky(sun_direction=(0.2, 0.5, 0.4), haze=1.0, horizon_height=0.5, saturation=1.5, redblueshift=-0.1)
+    rng = np.random.default_rng(5)
+    for _ in range(100):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        a, b = _sky_eval(s, tuple(d)), _sky_eval(s, tuple(d), fp64=1)
+        np.testing.assert_allclose(a, b, rtol=2e-3, atol=1e-9)
