@@ -144,18 +144,19 @@ class BuildInfo(C.Structure):
 class RenderOpts(C.Structure):
     _fields_ = [("queue_paths", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("timing", C.c_uint32),
                 ("depthOutChannel", C.c_int32), ("max_bounce_iters", C.c_uint32), ("count_traversal", C.c_uint32),
-                ("no_graphs", C.c_uint32), ("lds_stack", C.c_uint32)]
+                ("no_graphs", C.c_uint32), ("lds_stack", C.c_uint32), ("traversal", C.c_uint32)]
 
     @staticmethod
     def make(queue_paths=0, blocks_per_cu=0, timing=False, depth_channel=-1, max_bounce_iters=0,
-             count_traversal=False, graphs=True, lds_stack=0):
+             count_traversal=False, graphs=True, lds_stack=0, traversal=0):
         return RenderOpts(queue_paths, blocks_per_cu, int(bool(timing)), depth_channel, max_bounce_iters,
-                          int(bool(count_traversal)), 0 if graphs else 1, lds_stack)
+                          int(bool(count_traversal)), 0 if graphs else 1, lds_stack, traversal)
 
 
 class TuneResult(C.Structure):
     _fields_ = [("queue_paths", C.c_uint32), ("blocks_per_cu", C.c_uint32), ("candidates", C.c_uint32),
-                ("lds_stack", C.c_uint32), ("best_ms", C.c_double), ("worst_ms", C.c_double)]
+                ("lds_stack", C.c_uint32), ("traversal", C.c_uint32), ("pad", C.c_uint32),
+                ("best_ms", C.c_double), ("worst_ms", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -447,20 +448,22 @@ class Renderer:
         self.width, self.height = w, h
 
     def render(self, cam, pc, first_frame=0, n_frames=1, queue_paths=0, timing=False, depth_channel=-1,
-               max_bounce_iters=0, count_traversal=False, blocks_per_cu=0, graphs=True, lds_stack=0) -> dict:
+               max_bounce_iters=0, count_traversal=False, blocks_per_cu=0, graphs=True, lds_stack=0,
+               traversal=0) -> dict:
         opts = RenderOpts.make(queue_paths, blocks_per_cu, timing, depth_channel, max_bounce_iters, count_traversal,
-                               graphs, lds_stack)
+                               graphs, lds_stack, traversal)
         st = Stats()
         self._check(lib().ash_render(self._c, C.byref(cam), C.byref(pc), first_frame, n_frames, C.byref(opts),
                                      C.byref(st)), "render")
         return st.as_dict()
 
     def render_shots(self, cams, pcs, n_frames, queue_paths=0, timing=False, blocks_per_cu=0, graphs=True,
-                     count_traversal=False, lds_stack=0) -> dict:
+                     count_traversal=False, lds_stack=0, traversal=0) -> dict:
         n = len(cams)
         ca = (GpuCamera * n)(*cams)
         pa = (GpuPushConstantRaytrace * n)(*pcs)
-        opts = RenderOpts.make(queue_paths, blocks_per_cu, timing, -1, 0, count_traversal, graphs, lds_stack)
+        opts = RenderOpts.make(queue_paths, blocks_per_cu, timing, -1, 0, count_traversal, graphs, lds_stack,
+                               traversal)
         st = Stats()
         self._check(lib().ash_render_shots(self._c, n, ca, pa, n_frames, C.byref(opts), C.byref(st)),
                     "render_shots")

@@ -52,7 +52,7 @@ struct GraphKey {
   Wavefront w;
   const ShotParams* tab;
   ShotParams one;
-  uint32_t grid, iters, n, single, lds;
+  uint32_t grid, iters, n, single, lds, scheme;
 };
 
 struct GraphEntry {
@@ -219,7 +219,7 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   MALLOC(c, c->wQ0, cap);
   MALLOC(c, c->wQ1, cap);
   MALLOC(c, c->wQ2, cap);
-  MALLOC(c, c->wCtr, 16);
+  MALLOC(c, c->wCtr, kNumCtr);
   MALLOC(c, c->wCnt64, 8);
   Wavefront& w = c->wf;
   w.capacity = cap;
@@ -238,7 +238,7 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   w.shD = c->wShD.p;
   w.ctr = c->wCtr.p;
   w.cnt64 = c->wCnt64.p;
-  HIPC(c, hipMemsetAsync(c->wCtr.p, 0, 16 * sizeof(uint32_t), c->stream));
+  HIPC(c, hipMemsetAsync(c->wCtr.p, 0, kNumCtr * sizeof(uint32_t), c->stream));
   HIPC(c, hipMemsetAsync(c->wCnt64.p, 0, 8 * sizeof(unsigned long long), c->stream));
   return ASH_OK;
 }
@@ -276,13 +276,13 @@ static int checkPc(ash_ctx* c, const GpuPushConstantRaytrace* pc) {
 // The shading grid covers the wavefront capacity (threads past the live
 // count exit), so one captured loop serves every batch size.
 static void launchBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid,
-                          uint32_t lds) {
+                          uint32_t lds, uint32_t scheme) {
   int cur = 0;
   for (uint32_t it = 0; it < iters; ++it) {
     launchResetCounters(c->wf, cur, c->stream);
-    launchExtend(sc, c->wf, cur, lds, grid, false, c->stream);
+    launchExtend(sc, c->wf, cur, lds, scheme, grid, false, c->stream);
     launchShade(sc, b, c->wf, cur, c->wf.capacity, c->stream);
-    launchShadow(sc, c->wf, b.n, grid, false, c->stream);
+    launchShadow(sc, c->wf, scheme, grid, false, c->stream);
     cur = 1 - cur;
   }
 }
@@ -291,7 +291,7 @@ static void launchBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, u
 // first use of a key; a few keys are cached).  The loop holds kernel nodes
 // only.  Returns false when capture is unavailable (eager launch instead).
 static bool graphBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid,
-                         uint32_t lds) {
+                         uint32_t lds, uint32_t scheme) {
   if (c->graphsBroken) return false;
   GraphKey k;
   std::memset(&k, 0, sizeof k);
@@ -304,6 +304,7 @@ static bool graphBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, ui
   k.n = c->wf.capacity;
   k.single = b.shots == 1;
   k.lds = lds;
+  k.scheme = scheme;
   ++c->graphClock;
   for (auto& g : c->graphs)
     if (g.exec && std::memcmp(&g.key, &k, sizeof k) == 0) {
@@ -318,7 +319,7 @@ static bool graphBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, ui
     c->graphsBroken = true;
     return false;
   }
-  launchBounces(c, sc, b, iters, grid, lds);
+  launchBounces(c, sc, b, iters, grid, lds, scheme);
   hipGraph_t graph = nullptr;
   if (hipStreamEndCapture(c->stream, &graph) != hipSuccess || !graph) {
     c->graphsBroken = true;
@@ -348,19 +349,20 @@ static int runBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, uint32_t grid,
   bool eager = timing || count || (o && o->no_graphs);
   uint32_t iters = o && o->max_bounce_iters ? o->max_bounce_iters : uint32_t(std::max(maxDepth, 1));
   uint32_t lds = o && o->lds_stack == uint32_t(kStackLdsMin) ? uint32_t(kStackLdsMin) : uint32_t(kStackLds);
+  uint32_t scheme = o ? o->traversal : uint32_t(kSchemeXcdSlices);
   launchRaygen(b, c->wf, s);
   st->batches++;
   st->extend_launches += iters;
   st->shadow_launches += iters;
-  if (!eager && graphBounces(c, sc, b, iters, grid, lds)) {
+  if (!eager && graphBounces(c, sc, b, iters, grid, lds, scheme)) {
     st->graph_launches++;
   } else if (!timing) {
     int cur = 0;
     for (uint32_t it = 0; it < iters; ++it) {
       launchResetCounters(c->wf, cur, s);
-      launchExtend(sc, c->wf, cur, lds, grid, count, s);
+      launchExtend(sc, c->wf, cur, lds, scheme, grid, count, s);
       launchShade(sc, b, c->wf, cur, b.n, s);
-      launchShadow(sc, c->wf, b.n, grid, count, s);
+      launchShadow(sc, c->wf, scheme, grid, count, s);
       cur = 1 - cur;
     }
   } else {
@@ -380,9 +382,9 @@ static int runBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, uint32_t grid,
     int cur = 0, rc = ASH_OK;
     for (uint32_t it = 0; it < iters && rc == ASH_OK; ++it) {
       launchResetCounters(c->wf, cur, s);
-      rc = timed(&st->extend_ms, [&] { launchExtend(sc, c->wf, cur, lds, grid, count, s); });
+      rc = timed(&st->extend_ms, [&] { launchExtend(sc, c->wf, cur, lds, scheme, grid, count, s); });
       if (rc == ASH_OK) rc = timed(&st->shade_ms, [&] { launchShade(sc, b, c->wf, cur, b.n, s); });
-      if (rc == ASH_OK) rc = timed(&st->shadow_ms, [&] { launchShadow(sc, c->wf, b.n, grid, count, s); });
+      if (rc == ASH_OK) rc = timed(&st->shadow_ms, [&] { launchShadow(sc, c->wf, scheme, grid, count, s); });
       cur = 1 - cur;
     }
     (void)hipEventDestroy(ev[0]);
@@ -459,6 +461,7 @@ static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots
   if (bpc > 64) return fail(c, ASH_E_INVALID, "blocks_per_cu above 64");
   if (o && o->lds_stack && o->lds_stack != uint32_t(kStackLdsMin) && o->lds_stack != uint32_t(kStackLds))
     return fail(c, ASH_E_INVALID, "lds_stack must be 0, %d or %d", kStackLdsMin, kStackLds);
+  if (o && o->traversal > uint32_t(kSchemeRefill)) return fail(c, ASH_E_INVALID, "traversal must be 0, 1 or 2");
   std::vector<std::vector<WorkUnit>> plan;
   planBatches(nShots, P, firstFrame, nFrames, Q, plan);
   uint32_t cap = 0;
@@ -837,35 +840,52 @@ int ash_autotune(ash_ctx* c, uint32_t nShots, const GpuCamera* cams, const GpuPu
   ash_render_opts o = *io;
   o.timing = 0;
   o.count_traversal = 0;
+  if (o.traversal > uint32_t(kSchemeRefill)) return fail(c, ASH_E_INVALID, "traversal must be 0, 1 or 2");
   double best = 1e300, worst = 0;
   uint32_t bq = 0, bb = 0, n = 0;
-  uint32_t bl = 0;
+  uint32_t bl = 0, bs = o.traversal;
+  auto probe = [&](double& ms) -> int {
+    ash_stats st;
+    for (int rep = 0; rep < 2; ++rep) {
+      int rc = nShots == 1 ? ash_render(c, cams, pcs, 0, probeFrames, &o, &st)
+                           : ash_render_shots(c, nShots, cams, pcs, probeFrames, &o, &st);
+      if (rc) return rc;
+      ms = st.render_ms;
+    }
+    ++n;
+    worst = std::max(worst, ms);
+    return ASH_OK;
+  };
   for (uint32_t lq = 18; lq <= 22; ++lq)
     for (uint32_t bpc : {4u, 8u, 16u})
       for (uint32_t lds : {uint32_t(kStackLdsMin), uint32_t(kStackLds)}) {
         o.queue_paths = 1u << lq;
         o.blocks_per_cu = bpc;
         o.lds_stack = lds;
-        ash_stats st;
         double ms = 0;
-        for (int rep = 0; rep < 2; ++rep) {
-          int rc = nShots == 1 ? ash_render(c, cams, pcs, 0, probeFrames, &o, &st)
-                               : ash_render_shots(c, nShots, cams, pcs, probeFrames, &o, &st);
-          if (rc) return rc;
-          ms = st.render_ms;
-        }
-        ++n;
-        worst = std::max(worst, ms);
+        if (int rc = probe(ms)) return rc;
         if (ms < best) best = ms, bq = o.queue_paths, bb = bpc, bl = lds;
       }
+  o.queue_paths = bq, o.blocks_per_cu = bb, o.lds_stack = bl;
+  const uint32_t first = o.traversal;
+  for (uint32_t sch = 0; sch <= uint32_t(kSchemeRefill); ++sch) {
+    if (sch == first) continue;
+    o.traversal = sch;
+    double ms = 0;
+    if (int rc = probe(ms)) return rc;
+    if (ms < best) best = ms, bs = sch;
+  }
   io->queue_paths = bq;
   io->blocks_per_cu = bb;
   io->lds_stack = bl;
+  io->traversal = bs;
   if (res) {
     res->queue_paths = bq;
     res->blocks_per_cu = bb;
     res->candidates = n;
     res->lds_stack = bl;
+    res->traversal = bs;
+    res->pad = 0;
     res->best_ms = best;
     res->worst_ms = worst;
   }
@@ -936,7 +956,7 @@ int ash_first_hit(ash_ctx* c, const GpuCamera* cam, ash_hit* out) {
   SceneView sc = viewOf(c);
   launchRaygen(b, c->wf, c->stream);
   launchResetCounters(c->wf, 0, c->stream);
-  launchExtend(sc, c->wf, 0, kStackLds, grid, false, c->stream);
+  launchExtend(sc, c->wf, 0, kStackLds, kSchemeXcdSlices, grid, false, c->stream);
   launchFirstHit(sc, c->shotTab.p, c->wf, dout.p, P, c->stream);
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(out, dout.p, P * sizeof(ash_hit), hipMemcpyDeviceToHost, c->stream));

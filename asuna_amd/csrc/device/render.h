@@ -68,7 +68,9 @@ struct Wavefront {
   float4* shO;  // shadow rays: origin.xyz, tmax
   float4* shD;  // direction.xyz, path id
   uint32_t* ctr;  // [0],[1] queue counts, [2] shadow count, [3] extend fetch,
-                  // [4] shadow fetch, [8] second-group shading queue count
+                  // [4] shadow fetch, [8] second-group shading queue count,
+                  // [kXcdCtr + 32 j]: fetch counters of the per-XCD queue slices
+                  // (j < 8 extend, j >= 8 shadow), one 128-byte line each
   int* spill;       // traversal stack spill: spillCap entries per persistent thread
   float* spillT;    // entry distances of the spilled entries (4-wide closest hit)
   uint32_t spillThreads;
@@ -107,16 +109,26 @@ constexpr int kStackLds = 16;    // per-thread traversal stack entries in LDS (d
 constexpr int kStackLdsMin = 8;  // smallest LDS share (the spill covers the rest)
 constexpr int kSpill = 48;       // minimum further entries spilled to global memory
 constexpr int kTraceBlock = 256;
+constexpr int kXcds = 8;          // XCDs of the device; workgroup b runs on XCD b % 8
+constexpr int kXcdCtr = 32;       // first per-slice fetch counter in Wavefront::ctr
+constexpr int kNumCtr = kXcdCtr + 2 * kXcds * 32;
+
+// Ray fetch scheme of the persistent traversal kernels.
+enum TraceScheme : uint32_t {
+  kSchemeXcdSlices = 0,  // wave fetch from the wave's own XCD slice of the queue first
+  kSchemeQueueOrder = 1, // wave fetch in queue order
+  kSchemeRefill = 2,     // per-lane refill (4-wide tree only)
+};
 
 // Host-side launchers (render.hip).
 void launchRaygen(const BatchDesc& b, const Wavefront& w, hipStream_t s);
 // ldsStack: traversal stack entries per thread kept in LDS (8 or 16) by the
 // 4-wide closest-hit kernel
-void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t grid,
-                  bool count, hipStream_t s);
+void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t scheme,
+                  uint32_t grid, bool count, hipStream_t s);
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
                  hipStream_t s);
-void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t maxRays, uint32_t grid, bool count,
+void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t scheme, uint32_t grid, bool count,
                   hipStream_t s);
 void launchAccumulate(const BatchDesc& b, const Wavefront& w, hipStream_t s);
 void launchResetCounters(const Wavefront& w, int cur, hipStream_t s);

@@ -507,23 +507,70 @@ __device__ __forceinline__ void initStack(Stack& st, int* lds, float* ldsT, cons
   st.cap = NL + int(w.spillCap);
 }
 
-template <bool COUNT, bool WIDE>
+// Work split of the persistent traversal kernels: a wave takes 64
+// consecutive queue entries per atomic fetch.  XCD: the queue is cut into
+// one contiguous slice per XCD; a wave drains the slice of its own XCD
+// first, then helps with the others, so each XCD's L2 serves one run of the
+// pixel-ordered queue instead of rays from the whole image.
+struct ChunkFetch {
+  uint32_t* ctr;  // first fetch counter (queue order) or first slice counter (XCD)
+  uint32_t count;
+  uint32_t slice, home, k;
+  uint32_t lo, hi;
+  __device__ __forceinline__ void slicesFor(uint32_t s) {
+    lo = s * slice;
+    hi = lo < count ? min(count, lo + slice) : lo;
+  }
+};
+template <bool XCD>
+__device__ __forceinline__ void chunkInit(ChunkFetch& f, uint32_t* ctr, uint32_t count) {
+  f.ctr = ctr;
+  f.count = count;
+  f.k = 0;
+  if (XCD) {
+    f.slice = (count + kXcds - 1) / kXcds;
+    f.home = blockIdx.x % kXcds;
+    f.slicesFor(f.home);
+  } else {
+    f.slice = count;
+    f.home = 0;
+    f.lo = 0;
+    f.hi = count;
+  }
+}
+// Next chunk start for the calling wave (all lanes); false when the queue is drained.
+template <bool XCD>
+__device__ __forceinline__ bool chunkNext(ChunkFetch& f, int lane, uint32_t& start) {
+  for (;;) {
+    uint32_t s = XCD ? (f.home + f.k) % kXcds : 0u;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(f.ctr + (XCD ? s * 32u : 0u), 64u);
+    base = __shfl(base, 0);
+    if (base < f.hi - f.lo) {
+      start = f.lo + base;
+      return true;
+    }
+    if (!XCD || ++f.k == kXcds) return false;
+    f.slicesFor((f.home + f.k) % kXcds);
+  }
+}
+
+template <bool COUNT, bool WIDE, bool XCD, int NL>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w, int cur) {
-  __shared__ int lds[kStackLds * kTraceBlock];
-  __shared__ float ldsT[WIDE ? kStackLds * kTraceBlock : 1];
+  __shared__ int lds[NL * kTraceBlock];
+  __shared__ float ldsT[WIDE ? NL * kTraceBlock : 1];
   Stack st;
-  initStack<WIDE>(st, lds, ldsT, w);
+  initStack<WIDE, NL>(st, lds, ldsT, w);
   const uint32_t count = w.ctr[cur];
   const int* q = w.queue[cur];
   int lane = __lane_id();
   uint32_t nNodes = 0, nTris = 0;
-  for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&w.ctr[3], 64u);
-    base = __shfl(base, 0);
-    if (base >= count) break;
-    uint32_t i = base + uint32_t(lane);
-    if (i < count) {
+  ChunkFetch f;
+  chunkInit<XCD>(f, XCD ? &w.ctr[kXcdCtr] : &w.ctr[3], count);
+  uint32_t start;
+  while (chunkNext<XCD>(f, lane, start)) {
+    uint32_t i = start + uint32_t(lane);
+    if (i < f.hi) {
       int pid = q[i];
       TRay r;
       setupRay(r, xyz(w.ro[pid]), xyz(w.rd[pid]));
@@ -546,7 +593,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w
 }
 
 // ---------------------------------------------------------------- shadow
-template <bool COUNT, bool WIDE>
+template <bool COUNT, bool WIDE, bool XCD>
 __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w) {
   __shared__ int lds[kStackLds * kTraceBlock];
   Stack st;
@@ -554,13 +601,12 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w
   const uint32_t count = w.ctr[2];
   int lane = __lane_id();
   uint32_t nNodes = 0, nTris = 0;
-  for (;;) {
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&w.ctr[4], 64u);
-    base = __shfl(base, 0);
-    if (base >= count) break;
-    uint32_t i = base + uint32_t(lane);
-    if (i < count) {
+  ChunkFetch f;
+  chunkInit<XCD>(f, XCD ? &w.ctr[kXcdCtr + kXcds * 32] : &w.ctr[4], count);
+  uint32_t start;
+  while (chunkNext<XCD>(f, lane, start)) {
+    uint32_t i = start + uint32_t(lane);
+    if (i < f.hi) {
       float4 so = w.shO[i], sd = w.shD[i];
       TRay r;
       setupRay(r, xyz(so), xyz(sd));
@@ -2106,6 +2152,7 @@ __global__ void kResetCounters(Wavefront w, int cur) {
     w.ctr[4] = 0;
     w.ctr[8] = 0;
   }
+  if (threadIdx.x < 2 * kXcds) w.ctr[kXcdCtr + threadIdx.x * 32] = 0;
 }
 
 __global__ void kStatShadow(Wavefront w) {
@@ -2180,19 +2227,29 @@ __global__ __launch_bounds__(256) void kFirstHitState(SceneView sc, const ShotPa
 void launchRaygen(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
   kRaygen<<<(b.n + 255) / 256, 256, 0, s>>>(b, w);
 }
-void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t grid, bool count,
-                  hipStream_t s) {
-  bool wide = sc.nodes4 != nullptr;
-  bool shallow = ldsStack == 8;
-  if (count) {
-    if (!wide) kExtend<true, false><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
-    else if (shallow) kExtendWide<true, 8><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
-    else kExtendWide<true, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+template <bool COUNT>
+static void extendFor(const SceneView& sc, const Wavefront& w, int cur, bool shallow, uint32_t scheme, uint32_t grid,
+                      hipStream_t s) {
+  const bool wide = sc.nodes4 != nullptr, xcd = scheme == kSchemeXcdSlices;
+  if (!wide) {
+    if (xcd) kExtend<COUNT, false, true, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtend<COUNT, false, false, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+  } else if (scheme == kSchemeRefill) {
+    if (shallow) kExtendWide<COUNT, kStackLdsMin><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtendWide<COUNT, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+  } else if (xcd) {
+    if (shallow) kExtend<COUNT, true, true, kStackLdsMin><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtend<COUNT, true, true, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
   } else {
-    if (!wide) kExtend<false, false><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
-    else if (shallow) kExtendWide<false, 8><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
-    else kExtendWide<false, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    if (shallow) kExtend<COUNT, true, false, kStackLdsMin><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtend<COUNT, true, false, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
   }
+}
+void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t scheme,
+                  uint32_t grid, bool count, hipStream_t s) {
+  const bool shallow = ldsStack == uint32_t(kStackLdsMin);
+  if (count) extendFor<true>(sc, w, cur, shallow, scheme, grid, s);
+  else extendFor<false>(sc, w, cur, shallow, scheme, grid, s);
 }
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
                  hipStream_t s) {
@@ -2205,15 +2262,19 @@ void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, in
   }
   kStatShadow<<<1, 64, 0, s>>>(w);
 }
-void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t, uint32_t grid, bool count, hipStream_t s) {
-  bool wide = sc.nodes4 != nullptr;
-  if (count) {
-    if (wide) kShadowWide<true><<<grid, kTraceBlock, 0, s>>>(sc, w);
-    else kShadow<true, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
-  } else {
-    if (wide) kShadowWide<false><<<grid, kTraceBlock, 0, s>>>(sc, w);
-    else kShadow<false, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
-  }
+template <bool COUNT>
+static void shadowFor(const SceneView& sc, const Wavefront& w, uint32_t scheme, uint32_t grid, hipStream_t s) {
+  const bool wide = sc.nodes4 != nullptr, xcd = scheme == kSchemeXcdSlices;
+  if (wide && scheme == kSchemeRefill) kShadowWide<COUNT><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  else if (wide && xcd) kShadow<COUNT, true, true><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  else if (wide) kShadow<COUNT, true, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  else if (xcd) kShadow<COUNT, false, true><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  else kShadow<COUNT, false, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
+}
+void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t scheme, uint32_t grid, bool count,
+                  hipStream_t s) {
+  if (count) shadowFor<true>(sc, w, scheme, grid, s);
+  else shadowFor<false>(sc, w, scheme, grid, s);
 }
 void launchAccumulate(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
   kAccumulate<<<(b.nAcc + 255) / 256, 256, 0, s>>>(b, w);

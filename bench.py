@@ -64,6 +64,9 @@ def main():
     ap.add_argument("--branching", type=int, choices=[2, 4], default=4)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--traversal", type=int, choices=[0, 1, 2], default=0,
+                    help="ray fetch of the traversal kernels when not autotuned: 0 per-XCD queue slices, "
+                         "1 queue order, 2 per-lane refill")
     ap.add_argument("--copy-probe", action="store_true",
                     help="run a 256 MiB device-to-device copy first (calibrates rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -115,23 +118,23 @@ def main():
     dch = scene.state.depth_out_channel
 
     tune = None
-    qp, bpc, lds = args.queue_paths, 0, 0
+    qp, bpc, lds, trav = args.queue_paths, 0, 0, args.traversal
     if not args.no_autotune and not args.queue_paths:
-        opts, tune = r.autotune(list(cams), list(pcs), probe_frames=1)
-        qp, bpc, lds = opts.queue_paths, opts.blocks_per_cu, opts.lds_stack
+        opts, tune = r.autotune(list(cams), list(pcs), probe_frames=1, traversal=args.traversal)
+        qp, bpc, lds, trav = opts.queue_paths, opts.blocks_per_cu, opts.lds_stack, opts.traversal
         log(f"[bench] autotune: {tune}")
     graphs = not args.no_graphs
 
     def step():
         if cfg_id == 2:
             return r.render_shots(list(cams), list(pcs), spp, queue_paths=qp, blocks_per_cu=bpc, graphs=graphs,
-                                  lds_stack=lds)
+                                  lds_stack=lds, traversal=trav)
         r.set_film(w, h)
         return r.render(cams[0], pcs[0], 0, spp, queue_paths=qp, blocks_per_cu=bpc, depth_channel=dch,
-                        graphs=graphs, lds_stack=lds)
+                        graphs=graphs, lds_stack=lds, traversal=trav)
 
     # calibration: traversal work per ray (not timed)
-    cal = r.render(cams[0], pcs[0], 0, 1, count_traversal=True, depth_channel=dch)
+    cal = r.render(cams[0], pcs[0], 0, 1, count_traversal=True, depth_channel=dch, traversal=trav)
     ext_rays = max(1, cal["extension_rays"])
     nodes_per_ray = cal["ext_node_visits"] / ext_rays
     tris_per_ray = cal["ext_tri_tests"] / ext_rays
@@ -175,11 +178,12 @@ def main():
 
     # per-kernel timing pass (HIP events around every launch; not the timed steps)
     if cfg_id == 2:
-        tp = r.render_shots(list(cams), list(pcs), spp, queue_paths=qp, blocks_per_cu=bpc, timing=True, lds_stack=lds)
+        tp = r.render_shots(list(cams), list(pcs), spp, queue_paths=qp, blocks_per_cu=bpc, timing=True, lds_stack=lds,
+                            traversal=trav)
     else:
         r.set_film(w, h)
         tp = r.render(cams[0], pcs[0], 0, spp, queue_paths=qp, blocks_per_cu=bpc, timing=True, depth_channel=dch,
-                      lds_stack=lds)
+                      lds_stack=lds, traversal=trav)
     ext_ms_per_launch = tp["extend_ms"] / max(1, tp["extend_launches"])
     rays_per_launch = tp["extension_rays"] / max(1, tp["extend_launches"])
     bytes_per_launch = rays_per_launch * bytes_per_ray

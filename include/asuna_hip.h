@@ -288,6 +288,8 @@ typedef struct ash_render_opts {
   uint32_t count_traversal;  /* 1 = count node visits / triangle tests (slower) */
   uint32_t no_graphs;        /* 1 = launch the bounce loop eagerly instead of replaying a HIP graph */
   uint32_t lds_stack;        /* closest-hit stack entries per thread held in LDS: 8 or 16 (0 = 16) */
+  uint32_t traversal;        /* ray fetch of the traversal kernels: 0 = per-XCD slices of the queue
+                                (default), 1 = queue order, 2 = per-lane refill (4-wide tree) */
 } ash_render_opts;
 
 /* ------------------------------------------------------------ error codes */
@@ -350,22 +352,27 @@ int ash_render_shots(ash_ctx* ctx, uint32_t n_shots, const GpuCamera* cams,
 int ash_read_shot_output(ash_ctx* ctx, uint32_t shot, int32_t image, float* host_rgba);
 
 /* Autotuner (no counterpart in the reference, whose dispatch shape is fixed
- * at one thread per pixel): sweeps the wavefront batch size queue_paths over
- * 2^18..2^22, the persistent traversal grid blocks_per_cu over {4, 8, 16} and
- * the LDS share of the closest-hit stack over {8, 16} on a probe render of `probe_frames` frames of the given shots (each
- * candidate rendered twice, the second timed), then writes the fastest pair
- * into `inout`.  The probe renders overwrite the shot outputs. */
+ * at one thread per pixel): on a probe render of `probe_frames` frames of
+ * the given shots (each candidate rendered twice, the second timed) it
+ * sweeps the wavefront batch size queue_paths over 2^18..2^22, the
+ * persistent traversal grid blocks_per_cu over {4, 8, 16} and the LDS share
+ * of the closest-hit stack over {8, 16} with the traversal fetch scheme of
+ * `inout`, then tries the other fetch schemes at the best of those, and
+ * writes the fastest setting into `inout`.  The probe renders overwrite the
+ * shot outputs. */
 typedef struct ash_tune_result {
   uint32_t queue_paths;
   uint32_t blocks_per_cu;
   uint32_t candidates;
   uint32_t lds_stack;
-  double best_ms;  /* probe render time of the chosen pair */
+  uint32_t traversal;
+  uint32_t pad;
+  double best_ms;  /* probe render time of the chosen setting */
   double worst_ms; /* slowest candidate */
 } ash_tune_result;
 int ash_autotune(ash_ctx* ctx, uint32_t n_shots, const GpuCamera* cams, const GpuPushConstantRaytrace* pcs,
                  int32_t probe_frames, ash_render_opts* inout, ash_tune_result* result);
-/* The three parameters are written into `inout` (queue_paths, blocks_per_cu, lds_stack). */
+/* Written into `inout`: queue_paths, blocks_per_cu, lds_stack, traversal. */
 
 /* Copy output image `image` (0..8) to host RGBA32F (w*h*4 floats). */
 int ash_read_output(ash_ctx* ctx, int32_t image, float* host_rgba);

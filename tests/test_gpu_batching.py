@@ -70,17 +70,50 @@ def test_batch_plans_are_equivalent(renderer, multishot_scene):
             np.testing.assert_array_equal(a, b)
 
 
+@pytest.mark.parametrize("branching", [2, 4])
+def test_traversal_fetch_schemes_render_identically(renderer, spheres_small, branching):
+    """Per-XCD queue slices, queue order and per-lane refill change only
+    which wave traces which ray: every image is bit-identical, and the
+    traversal work counted is the same."""
+    s = A.HostScene(spheres_small)
+    renderer.upload(s)
+    renderer.build(branching=branching)
+    cam, pc = s.shot(0)
+    w, h = s.film
+    imgs, stats = [], []
+    for trav in (0, 1, 2):
+        for lds in (8, 16):
+            renderer.set_film(w, h)
+            st = renderer.render(cam, pc, 0, 3, traversal=trav, lds_stack=lds, count_traversal=True,
+                                 blocks_per_cu=4)
+            imgs.append([renderer.read(i) for i in (0, 1, 2, 8)])
+            stats.append(st)
+            log(f"branching {branching} traversal {trav} lds {lds}: nodes {st['ext_node_visits']} "
+                f"tris {st['ext_tri_tests']}")
+    for other in imgs[1:]:
+        for a, b in zip(imgs[0], other):
+            np.testing.assert_array_equal(a, b)
+    for st in stats[1:]:
+        assert st["extension_rays"] == stats[0]["extension_rays"] and st["shadow_rays"] == stats[0]["shadow_rays"]
+    # graph replays with each scheme equal the eager render
+    for trav in (0, 1, 2):
+        renderer.set_film(w, h)
+        renderer.render(cam, pc, 0, 3, traversal=trav, graphs=True)
+        np.testing.assert_array_equal(renderer.read(0), imgs[0][0])
+
+
 def test_autotune_picks_a_candidate_and_keeps_results(renderer, multishot_scene):
     s = _load(renderer, multishot_scene)
     cams, pcs = zip(*[s.shot(i) for i in range(8)])
     opts, rep = renderer.autotune(list(cams), list(pcs), probe_frames=1)
     log(f"autotune: {rep}")
-    assert rep["candidates"] == 30
+    assert rep["candidates"] == 32
     assert opts.queue_paths in [1 << k for k in range(18, 23)] and opts.blocks_per_cu in (4, 8, 16)
     assert opts.lds_stack in (8, 16) and rep["lds_stack"] == opts.lds_stack
+    assert opts.traversal in (0, 1, 2) and rep["traversal"] == opts.traversal
     assert 0 < rep["best_ms"] <= rep["worst_ms"]
     renderer.render_shots(list(cams), list(pcs), 2, queue_paths=opts.queue_paths, blocks_per_cu=opts.blocks_per_cu,
-                          lds_stack=opts.lds_stack)
+                          lds_stack=opts.lds_stack, traversal=opts.traversal)
     tuned = [renderer.read_shot(i, 0) for i in range(8)]
     renderer.render_shots(list(cams), list(pcs), 2, graphs=False)
     for i in range(8):
