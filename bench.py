@@ -64,6 +64,9 @@ def main():
     ap.add_argument("--branching", type=int, choices=[2, 4], default=4)
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
+    ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent traversal blocks per CU when not autotuned")
+    ap.add_argument("--lds-stack", type=int, choices=[0, 8, 16], default=0,
+                    help="closest-hit stack entries in LDS when not autotuned")
     ap.add_argument("--traversal", type=int, choices=[0, 1, 2], default=0,
                     help="ray fetch of the traversal kernels when not autotuned: 0 per-XCD queue slices, "
                          "1 queue order, 2 per-lane refill")
@@ -118,9 +121,12 @@ def main():
     dch = scene.state.depth_out_channel
 
     tune = None
-    qp, bpc, lds, trav = args.queue_paths, 0, 0, args.traversal
+    qp, bpc, lds, trav = args.queue_paths, args.blocks_per_cu, args.lds_stack, args.traversal
     if not args.no_autotune and not args.queue_paths:
-        opts, tune = r.autotune(list(cams), list(pcs), probe_frames=1, traversal=args.traversal)
+        # probe with enough frames for several wavefront batches at the largest queue size
+        probe = max(1, min(spp, (4 << 22) // (w * h * nshots)))
+        opts, tune = r.autotune(list(cams), list(pcs), probe_frames=probe, traversal=args.traversal)
+        tune["probe_frames"] = probe
         qp, bpc, lds, trav = opts.queue_paths, opts.blocks_per_cu, opts.lds_stack, opts.traversal
         log(f"[bench] autotune: {tune}")
     graphs = not args.no_graphs

@@ -268,11 +268,11 @@ def test_disney_scene_matches_oracle(renderer, tmp_path):
     js["lights"].append({"type": "point", "radiance": [1.0, 1.0, 1.0], "position": [0.3, 0.6, 0.2]})
     json.dump(js, open(sp, "w"))
     s = A.HostScene(sp)
-    m = s.desc.materials
-    m[2].subsurface, m[2].sheen, m[2].sheenTint, m[2].specularTint = 0.8, 1.0, 0.5, 0.7
-    m[3].clearcoat, m[3].clearcoatGloss, m[3].anisotropic = 1.0, 0.7, 0.8
-    m[5].clearcoat, m[5].clearcoatGloss, m[5].ior = 0.5, 0.2, 1.8
-    assert all(m[k].type == 9 for k in range(6))
+    m = s.desc.materials  # record 0 is the loader's default material: JSON material k is record k + 1
+    m[3].subsurface, m[3].sheen, m[3].sheenTint, m[3].specularTint = 0.8, 1.0, 0.5, 0.7
+    m[4].clearcoat, m[4].clearcoatGloss, m[4].anisotropic = 1.0, 0.7, 0.8
+    m[6].clearcoat, m[6].clearcoatGloss, m[6].ior = 0.5, 0.2, 1.8
+    assert all(m[k + 1].type == 9 for k in range(6))
     renderer.upload(s)
     renderer.build()
     spp = 16
