@@ -280,6 +280,23 @@ __global__ void kReorderTris(uint32_t n, const uint32_t* sortedIds, const float4
   for (int k = 0; k < 3; ++k) tris[3 * size_t(j) + k] = wtri[3 * size_t(t) + k];
 }
 
+// Shading records in leaf order: the object-space vertex pairs (position +
+// u, normal + v) of the three corners of each triangle, so a hit reads one
+// contiguous 96-byte record instead of an index triple and three vertices.
+__global__ void kShadeTris(uint32_t n, const uint32_t* sortedIds, BuildArgs a, const float4* wtri, float4* shade) {
+  uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  uint32_t t = sortedIds[j];
+  uint32_t inst = uint32_t(__float_as_int(wtri[3 * size_t(t)].w));
+  uint32_t prim = uint32_t(__float_as_int(wtri[3 * size_t(t) + 1].w));
+  const DevInstance& in = a.inst[inst];
+  for (int k = 0; k < 3; ++k) {
+    uint32_t vi = in.vbase + a.indices[in.ibase + prim * 3 + k];
+    shade[6 * size_t(j) + 2 * k] = a.vertices[2 * size_t(vi)];
+    shade[6 * size_t(j) + 2 * k + 1] = a.vertices[2 * size_t(vi) + 1];
+  }
+}
+
 // Root for scenes the radix tree cannot express (0 or 1 triangle, or the
 // whole tree collapsed into one leaf): child 0 = one leaf, child 1 = empty.
 __global__ void kSingleLeafRoot(uint32_t n, const float4* boxLo, const float4* boxHi, const uint32_t* sortedIds,
@@ -416,6 +433,78 @@ __global__ void kValidate4(const float4* nodes, uint32_t numNodes, uint32_t numT
   }
 }
 
+// Quantized copy of a 4-wide node (64 bytes): per axis the origin (the
+// smallest child plane) and a power-of-two scale 2^e, the smallest that
+// spans the largest child plane in 255 steps; each child plane as a byte,
+// rounded outwards: the lower plane q is the largest with origin + q * 2^e
+// <= plane, the upper the smallest with origin + q * 2^e >= plane, both
+// evaluated in float exactly as the traversal evaluates them (the product
+// is exact, the sum rounds once).  Layout: (origin.xyz, biased exponents
+// in bytes 0..2), (lo.x, hi.x, lo.y, hi.y bytes of children 0..3),
+// (lo.z, hi.z, ref0, ref1), (ref2, ref3, 0, 0).
+__device__ __forceinline__ float pow2f(int e) { return __uint_as_float(uint32_t(e + 127) << 23); }
+__global__ void kQuantize4(const float4* n4, uint32_t numNodes, float4* nq) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= numNodes) return;
+  const float4* src = n4 + size_t(i) * 8;
+  float pl[3][2][4];  // [axis][lo/hi][child]
+  for (int ax = 0; ax < 3; ++ax)
+    for (int side = 0; side < 2; ++side) {
+      float4 v = src[2 * ax + side];
+      pl[ax][side][0] = v.x, pl[ax][side][1] = v.y, pl[ax][side][2] = v.z, pl[ax][side][3] = v.w;
+    }
+  float4 rf = src[6];
+  int ref[4] = {__float_as_int(rf.x), __float_as_int(rf.y), __float_as_int(rf.z), __float_as_int(rf.w)};
+  float origin[3];
+  uint32_t qlo[3], qhi[3], ebits = 0;
+  for (int ax = 0; ax < 3; ++ax) {
+    float o = FLT_MAX, m = -FLT_MAX;
+    bool any = false;
+    for (int c = 0; c < 4; ++c)
+      if (ref[c] != kEmptyRef) {
+        o = fminf(o, pl[ax][0][c]);
+        m = fmaxf(m, pl[ax][1][c]);
+        any = true;
+      }
+    if (!any) o = m = 0.f;
+    double ext = double(m) - double(o);
+    int e = -126;
+    if (ext > 0.0) e = max(-126, min(127, int(ceil(log2(ext / 255.0)))));
+    uint32_t pk = 0, ph = 0;
+    for (;; ++e) {
+      const float sc = pow2f(e);
+      bool ok = true;
+      pk = ph = 0;
+      for (int c = 0; c < 4 && ok; ++c) {
+        if (ref[c] == kEmptyRef) continue;
+        const float lo = pl[ax][0][c], hi = pl[ax][1][c];
+        double fl = floor((double(lo) - double(o)) / double(sc));
+        uint32_t ql = fl <= 0.0 ? 0u : (fl >= 255.0 ? 255u : uint32_t(fl));
+        while (ql > 0 && o + float(ql) * sc > lo) --ql;
+        double fh = ceil((double(hi) - double(o)) / double(sc));
+        uint32_t qh = fh <= 0.0 ? 0u : (fh >= 256.0 ? 256u : uint32_t(fh));
+        while (qh < 256u && o + float(qh) * sc < hi) ++qh;
+        if (qh > 255u) {
+          ok = false;
+        } else {
+          pk |= ql << (8 * c);
+          ph |= qh << (8 * c);
+        }
+      }
+      if (ok || e >= 127) break;
+    }
+    origin[ax] = o;
+    qlo[ax] = pk;
+    qhi[ax] = ph;
+    ebits |= uint32_t(e + 127) << (8 * ax);
+  }
+  float4* d = nq + size_t(i) * 4;
+  d[0] = make_float4(origin[0], origin[1], origin[2], __uint_as_float(ebits));
+  d[1] = make_float4(__uint_as_float(qlo[0]), __uint_as_float(qhi[0]), __uint_as_float(qlo[1]), __uint_as_float(qhi[1]));
+  d[2] = make_float4(__uint_as_float(qlo[2]), __uint_as_float(qhi[2]), rf.x, rf.y);
+  d[3] = make_float4(rf.z, rf.w, 0.f, 0.f);
+}
+
 template <class T> struct DevBuf {
   T* p = nullptr;
   ~DevBuf() {
@@ -532,7 +621,9 @@ int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_
     kSingleLeafRoot<<<1, 1, 0, s>>>(n, boxLo.p, boxHi.p, sorted, r.nodes);
   }
   HIPB(hipMalloc(&r.tris, nn * 3 * sizeof(float4)));
+  HIPB(hipMalloc(&r.shade, nn * 6 * sizeof(float4)));
   if (n) kReorderTris<<<grid, kBlock, 0, s>>>(n, sorted, wtri.p, r.tris);
+  if (n) kShadeTris<<<grid, kBlock, 0, s>>>(n, sorted, a, wtri.p, r.shade);
   HIPB(hipGetLastError());
   {
     DevBuf<unsigned> refcnt, bad;
@@ -594,6 +685,11 @@ int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_
     if (nbad) {
       std::snprintf(err, errLen, "bvh: 4-wide structural validation failed (%u violations)", nbad);
       return ASH_E_INVALID;
+    }
+    if (a.quantize) {
+      HIPB(hipMalloc(&r.nodesQ, size_t(r.numNodes4) * 4 * sizeof(float4)));
+      kQuantize4<<<gv, kBlock, 0, s>>>(r.nodes4, r.numNodes4, r.nodesQ);
+      HIPB(hipGetLastError());
     }
   }
   HIPB(hipEventRecord(e1, s));

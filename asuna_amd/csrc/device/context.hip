@@ -92,7 +92,9 @@ struct ash_ctx {
   uint32_t maxDepth = 0;
   float4* nodes = nullptr;
   float4* nodes4 = nullptr;  // 4-wide tree when built with branching 4
+  float4* nodesQ = nullptr;  // its quantized 64-byte copy (node format 2)
   float4* tris = nullptr;
+  float4* shadeTris = nullptr;  // per leaf-ordered triangle: its three object-space vertex pairs
   uint32_t stackNeed = 0;    // traversal stack entries the current tree can need
   uint64_t nodeCount = 0;
   // film (single shot)
@@ -119,7 +121,9 @@ struct ash_ctx {
   ~ash_ctx() {
     if (nodes) (void)hipFree(nodes);
     if (nodes4) (void)hipFree(nodes4);
+    if (nodesQ) (void)hipFree(nodesQ);
     if (tris) (void)hipFree(tris);
+    if (shadeTris) (void)hipFree(shadeTris);
     for (auto& t : texData) t.release();
     for (auto* b : {&verts, &env, &images, &shotImages, &wRo, &wRd, &wThr, &wRad, &wLd, &wHitF, &wShO, &wShD})
       b->release();
@@ -186,7 +190,9 @@ static SceneView viewOf(const ash_ctx* c) {
   std::memset(&v, 0, sizeof v);
   v.nodes = c->nodes;
   v.nodes4 = c->nodes4;
+  v.nodesQ = c->nodesQ;
   v.tris = c->tris;
+  v.shadeTris = c->shadeTris;
   v.inst = c->inst.p;
   v.verts = c->verts.p;
   v.idx = c->idx.p;
@@ -731,7 +737,9 @@ int ash_build_accel(ash_ctx* c, const ash_build_opts* o, ash_build_info* info) {
   HIPC(c, hipSetDevice(c->device));
   if (c->nodes) (void)hipFree(c->nodes), c->nodes = nullptr;
   if (c->nodes4) (void)hipFree(c->nodes4), c->nodes4 = nullptr;
+  if (c->nodesQ) (void)hipFree(c->nodesQ), c->nodesQ = nullptr;
   if (c->tris) (void)hipFree(c->tris), c->tris = nullptr;
+  if (c->shadeTris) (void)hipFree(c->shadeTris), c->shadeTris = nullptr;
   c->hasAccel = false;
   BuildArgs a;
   a.inst = c->inst.p;
@@ -744,18 +752,26 @@ int ash_build_accel(ash_ctx* c, const ash_build_opts* o, ash_build_info* info) {
   a.quality = o ? o->quality : 1;
   a.branching = o && o->branching ? o->branching : 4;
   if (a.branching != 2 && a.branching != 4) return fail(c, ASH_E_INVALID, "branching must be 2 or 4");
+  if (c->numTris >= (1u << 28)) return fail(c, ASH_E_UNSUPPORTED, "more than 2^28 - 1 triangles");
+  uint32_t fmt = o ? o->node_format : 0;
+  if (fmt > 2) return fail(c, ASH_E_INVALID, "node_format must be 0, 1 or 2");
+  a.quantize = a.branching == 4 && fmt == 2;
   BuildResult r;
   char err[512] = {0};
   int rc = buildBvh(a, c->stream, r, err, sizeof err);
   if (rc) {
     if (r.nodes) (void)hipFree(r.nodes);
     if (r.nodes4) (void)hipFree(r.nodes4);
+    if (r.nodesQ) (void)hipFree(r.nodesQ);
     if (r.tris) (void)hipFree(r.tris);
+    if (r.shade) (void)hipFree(r.shade);
     return fail(c, rc, "%s", err);
   }
   c->nodes = r.nodes;
   c->nodes4 = r.nodes4;
+  c->nodesQ = r.nodesQ;
   c->tris = r.tris;
+  c->shadeTris = r.shade;
   c->nodeCount = r.numNodes;
   c->maxDepth = r.maxDepth;
   // A binary walk stacks at most one entry per level, a 4-wide walk three.
@@ -765,9 +781,11 @@ int ash_build_accel(ash_ctx* c, const ash_build_opts* o, ash_build_info* info) {
     info->num_triangles = c->numTris;
     info->num_nodes = r.numNodes;
     info->max_depth = r.maxDepth;
+    info->node_bytes = r.nodesQ ? 64u : (r.nodes4 ? 128u : 64u);
     info->build_ms = r.ms;
     info->sah_cost = r.sahCost;
-    info->device_bytes = uint64_t(r.numNodes) * 64 + uint64_t(r.numNodes4) * 128 + uint64_t(c->numTris) * 48 +
+    info->device_bytes = uint64_t(r.numNodes) * 64 + uint64_t(r.numNodes4) * (r.nodesQ ? 192 : 128) +
+                         uint64_t(c->numTris) * (48 + 96) +
                          c->verts.n * 16 + c->idx.n * 4;
     info->num_wide_nodes = r.numNodes4;
     info->wide_depth = r.depth4;

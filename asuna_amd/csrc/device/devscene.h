@@ -54,12 +54,15 @@ struct BuildArgs {
   uint32_t leafMax;   // 1..8
   uint32_t quality;   // 0 = LBVH (one triangle per leaf), 1 = SAH leaf collapse
   uint32_t branching; // 2 or 4
+  uint32_t quantize;  // 1: also emit quantized 64-byte 4-wide nodes (branching 4)
 };
 
 struct BuildResult {
   float4* nodes = nullptr;   // device, 4 float4 per binary node
   float4* nodes4 = nullptr;  // device, 8 float4 per 4-wide node (branching 4)
+  float4* nodesQ = nullptr;  // device, 4 float4 per quantized 4-wide node (same ids and refs as nodes4)
   float4* tris = nullptr;    // device, 3 float4 per triangle
+  float4* shade = nullptr;   // device, 6 float4 per triangle (leaf order): object-space vertex pairs
   uint32_t numNodes = 0;
   uint32_t maxDepth = 0;
   uint32_t numNodes4 = 0;

@@ -38,7 +38,9 @@ struct DevSky {
 struct SceneView {
   const float4* nodes;   // BVH2 (always built)
   const float4* nodes4;  // 4-wide BVH, or null to traverse the BVH2
+  const float4* nodesQ;  // quantized 64-byte 4-wide nodes, or null (same refs as nodes4)
   const float4* tris;
+  const float4* shadeTris;  // 6 float4 per triangle in leaf order (object-space vertex pairs)
   const DevInstance* inst;
   const float4* verts;
   const uint32_t* idx;

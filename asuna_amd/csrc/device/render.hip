@@ -205,6 +205,41 @@ __device__ __forceinline__ void testNode4(const float4* np, const TRay& r, float
   cswap(k, 1, 2);
 }
 
+// The same test on a quantized 64-byte node: per-axis origin and power-of-
+// two scale, 8-bit child planes rounded outwards by the builder (each
+// dequantized box contains its exact box, computed with the expression
+// used here), child refs, empty slots masked by their ref.  Because the
+// slab arithmetic is monotone in the planes, a dequantized box is entered
+// whenever its exact box is, so the triangles tested include every one the
+// exact-box walks test.
+__device__ __forceinline__ void testNode4Q(const float4* np, const TRay& r, float tmin, float lim, Kids& k) {
+  const uint4* up = reinterpret_cast<const uint4*>(np);
+  const float4 h0 = np[0];
+  const uint4 q1 = up[1], q2 = up[2], q3 = up[3];
+  const uint32_t e = __float_as_uint(h0.w);
+  const float sx = __uint_as_float((e & 0xffu) << 23), sy = __uint_as_float(((e >> 8) & 0xffu) << 23),
+              sz = __uint_as_float(((e >> 16) & 0xffu) << 23);
+  k.c[0] = int(q2.z), k.c[1] = int(q2.w), k.c[2] = int(q3.x), k.c[3] = int(q3.y);
+  const float kMiss = 3.0e38f;
+  k.n = 0;
+#pragma unroll
+  for (int c = 0; c < 4; ++c) {
+    const uint32_t sh = 8u * uint32_t(c);
+    float lx = h0.x + float((q1.x >> sh) & 0xffu) * sx, hx = h0.x + float((q1.y >> sh) & 0xffu) * sx;
+    float ly = h0.y + float((q1.z >> sh) & 0xffu) * sy, hy = h0.y + float((q1.w >> sh) & 0xffu) * sy;
+    float lz = h0.z + float((q2.x >> sh) & 0xffu) * sz, hz = h0.z + float((q2.y >> sh) & 0xffu) * sz;
+    float t;
+    bool h = slab(lx, hx, ly, hy, lz, hz, r, tmin, lim, t) && k.c[c] != kEmptyRef;
+    k.t[c] = h ? t : kMiss;
+    k.n += h;
+  }
+  cswap(k, 0, 1);
+  cswap(k, 2, 3);
+  cswap(k, 0, 2);
+  cswap(k, 1, 3);
+  cswap(k, 1, 2);
+}
+
 // Closest hit in [tmin, tmax]; ties at equal t go to the smaller global id.
 // Returns the leaf position of the hit triangle or -1.
 template <bool COUNT>
@@ -325,7 +360,8 @@ __device__ bool traceAny(const SceneView& sc, const TRay& r, float tmin, float t
 // current closest hit.  The drop test is the slab test's own far clamp
 // (tn <= lim * (1 + 2 gamma_3)), so the triangles tested include every one
 // the binary walk tests and the result is identical (same tie rule).
-template <bool COUNT>
+// QUANT: walk the quantized 64-byte nodes instead of the 128-byte ones.
+template <bool COUNT, bool QUANT>
 __device__ int traceClosest4(const SceneView& sc, const TRay& r, float tmin, float tmax, Stack& st, float& bt,
                              float& bu, float& bv, uint32_t& nNodes, uint32_t& nTris) {
   int best = -1, bestGid = 0x7fffffff;
@@ -337,7 +373,8 @@ __device__ int traceClosest4(const SceneView& sc, const TRay& r, float tmin, flo
       if (node != kEmptyRef) {
         if (COUNT) ++nNodes;
         Kids k;
-        testNode4(sc.nodes4 + size_t(node) * 8, r, tmin, best >= 0 ? bt : tmax, k);
+        if (QUANT) testNode4Q(sc.nodesQ + size_t(node) * 4, r, tmin, best >= 0 ? bt : tmax, k);
+        else testNode4(sc.nodes4 + size_t(node) * 8, r, tmin, best >= 0 ? bt : tmax, k);
         if (k.n > 0) {
           if (k.n > 3) st.push(k.c[3], k.t[3]);
           if (k.n > 2) st.push(k.c[2], k.t[2]);
@@ -381,7 +418,7 @@ __device__ int traceClosest4(const SceneView& sc, const TRay& r, float tmin, flo
   return best;
 }
 
-template <bool COUNT>
+template <bool COUNT, bool QUANT>
 __device__ bool traceAny4(const SceneView& sc, const TRay& r, float tmin, float tmax, Stack& st, uint32_t& nNodes,
                           uint32_t& nTris) {
   st.sp = 0;
@@ -391,7 +428,8 @@ __device__ bool traceAny4(const SceneView& sc, const TRay& r, float tmin, float 
       if (node != kEmptyRef) {
         if (COUNT) ++nNodes;
         Kids k;
-        testNode4(sc.nodes4 + size_t(node) * 8, r, tmin, tmax, k);
+        if (QUANT) testNode4Q(sc.nodesQ + size_t(node) * 4, r, tmin, tmax, k);
+        else testNode4(sc.nodes4 + size_t(node) * 8, r, tmin, tmax, k);
         if (k.n > 0) {
           if (k.n > 3) st.push(k.c[3]);
           if (k.n > 2) st.push(k.c[2]);
@@ -414,6 +452,19 @@ __device__ bool traceAny4(const SceneView& sc, const TRay& r, float tmin, float 
     node = st.pop();
   }
   return false;
+}
+
+// Pop for the closest-hit walks: the next stacked entry whose entry
+// distance is not beyond the current hit (kDoneRef when none is left).
+constexpr int32_t kDoneRef = -0x7fffffff - 1;  // walk finished (leaf refs stay above: first < 2^28)
+
+__device__ __forceinline__ int popClosest(Stack& st, int best, float bt) {
+  while (st.sp > 0) {
+    float tn;
+    int c = st.pop(tn);
+    if (best < 0 || tn <= bt * 1.0000004f) return c;
+  }
+  return kDoneRef;
 }
 
 // -------------------------------------------------------------- raygen
@@ -555,12 +606,13 @@ __device__ __forceinline__ bool chunkNext(ChunkFetch& f, int lane, uint32_t& sta
   }
 }
 
-template <bool COUNT, bool WIDE, bool XCD, int NL>
+// TREE: 0 binary, 1 4-wide, 2 quantized 4-wide.
+template <bool COUNT, int TREE, bool XCD, int NL>
 __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w, int cur) {
   __shared__ int lds[NL * kTraceBlock];
-  __shared__ float ldsT[WIDE ? NL * kTraceBlock : 1];
+  __shared__ float ldsT[TREE ? NL * kTraceBlock : 1];
   Stack st;
-  initStack<WIDE, NL>(st, lds, ldsT, w);
+  initStack<TREE != 0, NL>(st, lds, ldsT, w);
   const uint32_t count = w.ctr[cur];
   const int* q = w.queue[cur];
   int lane = __lane_id();
@@ -575,7 +627,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w
       TRay r;
       setupRay(r, xyz(w.ro[pid]), xyz(w.rd[pid]));
       float t = 0.f, u = 0.f, v = 0.f;
-      int pos = WIDE ? traceClosest4<COUNT>(sc, r, kMinimum, kInf, st, t, u, v, nNodes, nTris)
+      int pos = TREE ? traceClosest4<COUNT, TREE == 2>(sc, r, kMinimum, kInf, st, t, u, v, nNodes, nTris)
                      : traceClosest<COUNT>(sc, r, kMinimum, kInf, st, t, u, v, nNodes, nTris);
       if (pos >= 0) {
         const float4* tp = sc.tris + size_t(pos) * 3;
@@ -592,8 +644,167 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w
   }
 }
 
+// Persistent closest-hit walk with dynamic ray fetch (4-wide float nodes).
+// Each lane walks its own ray in rounds of the speculative while-while form
+// (Aila and Laine 2009): it descends through inner nodes until it stands at
+// a leaf; a lane that reaches a leaf while holding none keeps it aside and
+// descends further until every lane of the wave holds one; then the wave
+// tests the leaves together, so node visits and triangle tests are not
+// interleaved in one loop iteration.  Idle lanes take new rays (from the
+// XCD slices, as the wave fetch does) as soon as REFILL of them are idle,
+// so a wave does not wait for its slowest ray.  Visit order, culling and
+// tie rule are those of traceClosest4 (speculation only delays the
+// culling), so the closest hit is the same.
+struct DynState {
+  int node, post, best, bestGid;
+  float bt, bu, bv;
+};
+
+template <bool COUNT>
+__device__ __forceinline__ void dynLeaf(const SceneView& sc, const TRay& r, DynState& s, int ref, uint32_t& nT) {
+  uint32_t first, cnt;
+  decodeLeaf(ref, first, cnt);
+  for (uint32_t k = first; k < first + cnt; ++k) {
+    const float4* tp = sc.tris + size_t(k) * 3;
+    float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+    if (COUNT) ++nT;
+    float t, u, v;
+    if (hitTri(r, p0, p1, p2, kMinimum, s.best >= 0 ? s.bt : kInf, t, u, v)) {
+      int gid = __float_as_int(p2.w);
+      if (s.best < 0 || t < s.bt || (t == s.bt && gid < s.bestGid)) {
+        s.bt = t;
+        s.bu = u;
+        s.bv = v;
+        s.best = int(k);
+        s.bestGid = gid;
+      }
+    }
+  }
+}
+
+// One descend-and-test round; true when the ray has finished.
+template <bool COUNT>
+__device__ __forceinline__ bool dynRound(const SceneView& sc, const TRay& r, DynState& s, Stack& st, uint32_t& nN,
+                                         uint32_t& nT) {
+  while (s.node >= 0) {
+    if (s.node != kEmptyRef) {
+      if (COUNT) ++nN;
+      Kids k;
+      testNode4(sc.nodes4 + size_t(s.node) * 8, r, kMinimum, s.best >= 0 ? s.bt : kInf, k);
+      if (k.n > 0) {
+        if (k.n > 3) st.push(k.c[3], k.t[3]);
+        if (k.n > 2) st.push(k.c[2], k.t[2]);
+        if (k.n > 1) st.push(k.c[1], k.t[1]);
+        s.node = k.c[0];
+      } else {
+        s.node = popClosest(st, s.best, s.bt);
+      }
+    } else {
+      s.node = popClosest(st, s.best, s.bt);
+    }
+    if (s.node < 0 && s.node != kDoneRef && s.post == kDoneRef) {
+      s.post = s.node;
+      s.node = popClosest(st, s.best, s.bt);
+    }
+    if (__ballot(s.post == kDoneRef) == 0) break;
+  }
+  if (s.post != kDoneRef) {
+    dynLeaf<COUNT>(sc, r, s, s.post, nT);
+    s.post = kDoneRef;
+  }
+  if (s.node < 0 && s.node != kDoneRef) {
+    dynLeaf<COUNT>(sc, r, s, s.node, nT);
+    s.node = popClosest(st, s.best, s.bt);
+  }
+  return s.node == kDoneRef && s.post == kDoneRef;
+}
+
+// Hands the next queue entries of the wave's slices to its idle lanes (all
+// lanes call it); `drained` once every slice has been handed out.
+__device__ __forceinline__ bool fetchIdle(ChunkFetch& f, bool idle, int lane, bool& drained, uint32_t& idx) {
+  const uint64_t m = __ballot(idle);
+  const uint32_t n = uint32_t(__popcll(m));
+  if (n == 0 || drained) return false;
+  const uint32_t rank = uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+  for (;;) {
+    const uint32_t sl = (f.home + f.k) % kXcds;
+    uint32_t base = 0;
+    if (lane == 0) base = atomicAdd(f.ctr + sl * 32u, n);
+    base = __shfl(base, 0);
+    const uint32_t len = f.hi - f.lo;
+    if (base < len) {
+      idx = f.lo + base + rank;
+      bool got = idle && idx < f.hi;
+      if (base + n >= len) {
+        if (++f.k == kXcds) drained = true;
+        else f.slicesFor((f.home + f.k) % kXcds);
+      }
+      return got;
+    }
+    if (++f.k == kXcds) {
+      drained = true;
+      return false;
+    }
+    f.slicesFor((f.home + f.k) % kXcds);
+  }
+}
+
+template <bool COUNT, int NL, int REFILL>
+__global__ __launch_bounds__(kTraceBlock) void kExtendDyn(SceneView sc, Wavefront w, int cur) {
+  __shared__ int lds[NL * kTraceBlock];
+  __shared__ float ldsT[NL * kTraceBlock];
+  Stack st;
+  initStack<true, NL>(st, lds, ldsT, w);
+  const uint32_t count = w.ctr[cur];
+  const int* q = w.queue[cur];
+  const int lane = __lane_id();
+  uint32_t nNodes = 0, nTris = 0;
+  ChunkFetch f;
+  chunkInit<true>(f, &w.ctr[kXcdCtr], count);
+  bool active = false, drained = false;
+  int pid = -1;
+  TRay r;
+  DynState s;
+  for (;;) {
+    const uint32_t nIdle = uint32_t(__popcll(__ballot(!active)));
+    if (!drained && nIdle >= uint32_t(REFILL)) {
+      uint32_t i = 0;
+      if (fetchIdle(f, !active, lane, drained, i)) {
+        pid = q[i];
+        setupRay(r, xyz(w.ro[pid]), xyz(w.rd[pid]));
+        s.node = 0;
+        s.post = kDoneRef;
+        s.best = -1;
+        s.bestGid = 0x7fffffff;
+        s.bt = kInf;
+        s.bu = s.bv = 0.f;
+        st.sp = 0;
+        active = true;
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (drained) break;
+      continue;
+    }
+    if (active && dynRound<COUNT>(sc, r, s, st, nNodes, nTris)) {
+      active = false;
+      if (s.best >= 0) {
+        const float4* tp = sc.tris + size_t(s.best) * 3;
+        w.hitF[pid] = make_float4(s.bt, s.bu, s.bv, 0.f);
+        w.hitI[pid] = make_int4(__float_as_int(tp[0].w), __float_as_int(tp[1].w), __float_as_int(tp[2].w), s.best);
+      } else {
+        w.hitI[pid] = make_int4(-1, -1, -1, -1);
+      }
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&w.cnt64[0], (unsigned long long)nNodes);
+    atomicAdd(&w.cnt64[1], (unsigned long long)nTris);
+  }
+}
+
 // ---------------------------------------------------------------- shadow
-template <bool COUNT, bool WIDE, bool XCD>
+template <bool COUNT, int TREE, bool XCD>
 __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w) {
   __shared__ int lds[kStackLds * kTraceBlock];
   Stack st;
@@ -611,7 +822,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w
       TRay r;
       setupRay(r, xyz(so), xyz(sd));
       int pid = __float_as_int(sd.w);
-      bool occluded = WIDE ? traceAny4<COUNT>(sc, r, 0.0f, so.w, st, nNodes, nTris)
+      bool occluded = TREE ? traceAny4<COUNT, TREE == 2>(sc, r, 0.0f, so.w, st, nNodes, nTris)
                            : traceAny<COUNT>(sc, r, 0.0f, so.w, st, nNodes, nTris);
       if (!occluded) {
         float4 L = w.rad[pid], d = w.ld[pid];
@@ -850,14 +1061,13 @@ struct Hit {
 };
 
 // getHitState + configureShadingFrame, rchit_layouts.glsl:61-95.
-__device__ __forceinline__ void hitState(const SceneView& sc, int inst, int prim, float bu, float bv, f3 rayDir,
+// `leaf`: the triangle's position in leaf order, whose shading record holds
+// the same vertex data as the mesh's index triple and vertices.
+__device__ __forceinline__ void hitState(const SceneView& sc, int inst, int leaf, float bu, float bv, f3 rayDir,
                                          uint32_t useFaceNormal, Hit& h) {
   const DevInstance& in = sc.inst[inst];
-  const uint32_t* ix = sc.idx + in.ibase + uint32_t(prim) * 3;
-  uint32_t i0 = in.vbase + ix[0], i1 = in.vbase + ix[1], i2 = in.vbase + ix[2];
-  float4 a0 = sc.verts[2 * size_t(i0)], b0 = sc.verts[2 * size_t(i0) + 1];
-  float4 a1 = sc.verts[2 * size_t(i1)], b1 = sc.verts[2 * size_t(i1) + 1];
-  float4 a2 = sc.verts[2 * size_t(i2)], b2 = sc.verts[2 * size_t(i2) + 1];
+  const float4* sr = sc.shadeTris + size_t(leaf) * 6;
+  float4 a0 = sr[0], b0 = sr[1], a1 = sr[2], b1 = sr[3], a2 = sr[4], b2 = sr[5];
   f3 ba = f3{1.0f - bu - bv, bu, bv};
   h.lightId = in.light;
   h.uvx = a0.w * ba.x + a1.w * ba.y + a2.w * ba.z;
@@ -2072,7 +2282,7 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
   } else {
     float4 hf = w.hitF[pid];
     Hit h;
-    hitState(sc, hi.x, hi.y, hf.y, hf.z, c.rd, pc.useFaceNormal, h);
+    hitState(sc, hi.x, hi.w, hf.y, hf.z, c.rd, pc.useFaceNormal, h);
     bool aov = frame0 && c.depth == 1;
     if (aov && sp.depthChannel >= 0) writeAov(c, sp.depthChannel, mk1(hf.x), pix);
     if (h.lightId >= 0) {
@@ -2210,7 +2420,7 @@ __global__ __launch_bounds__(256) void kFirstHitState(SceneView sc, const ShotPa
   } else {
     float4 hf = w.hitF[p];
     Hit h;
-    hitState(sc, hi.x, hi.y, hf.y, hf.z, xyz(w.rd[p]), 0u, h);
+    hitState(sc, hi.x, hi.w, hf.y, hf.z, xyz(w.rd[p]), 0u, h);
     o.t = hf.x;
     o.u = hf.y;
     o.v = hf.z;
@@ -2227,22 +2437,39 @@ __global__ __launch_bounds__(256) void kFirstHitState(SceneView sc, const ShotPa
 void launchRaygen(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
   kRaygen<<<(b.n + 255) / 256, 256, 0, s>>>(b, w);
 }
+template <bool COUNT, int TREE>
+static void extendWave(const SceneView& sc, const Wavefront& w, int cur, bool shallow, bool xcd, uint32_t grid,
+                       hipStream_t s) {
+  if (xcd) {
+    if (shallow) kExtend<COUNT, TREE, true, kStackLdsMin><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtend<COUNT, TREE, true, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+  } else {
+    if (shallow) kExtend<COUNT, TREE, false, kStackLdsMin><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtend<COUNT, TREE, false, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+  }
+}
+// Closest-hit kernel per tree and fetch scheme: per-XCD slices walk float
+// 4-wide nodes with dynamic fetch (kExtendDyn), quantized nodes with wave
+// fetch; queue order uses wave fetch; the per-lane refill scheme walks the
+// 128-byte nodes (kept beside the quantized ones).
+constexpr int kDynRefill = 32;
 template <bool COUNT>
 static void extendFor(const SceneView& sc, const Wavefront& w, int cur, bool shallow, uint32_t scheme, uint32_t grid,
                       hipStream_t s) {
   const bool wide = sc.nodes4 != nullptr, xcd = scheme == kSchemeXcdSlices;
   if (!wide) {
-    if (xcd) kExtend<COUNT, false, true, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
-    else kExtend<COUNT, false, false, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    if (xcd) kExtend<COUNT, 0, true, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtend<COUNT, 0, false, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
   } else if (scheme == kSchemeRefill) {
     if (shallow) kExtendWide<COUNT, kStackLdsMin><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
     else kExtendWide<COUNT, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+  } else if (sc.nodesQ) {
+    extendWave<COUNT, 2>(sc, w, cur, shallow, xcd, grid, s);
   } else if (xcd) {
-    if (shallow) kExtend<COUNT, true, true, kStackLdsMin><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
-    else kExtend<COUNT, true, true, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    if (shallow) kExtendDyn<COUNT, kStackLdsMin, kDynRefill><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtendDyn<COUNT, kStackLds, kDynRefill><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
   } else {
-    if (shallow) kExtend<COUNT, true, false, kStackLdsMin><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
-    else kExtend<COUNT, true, false, kStackLds><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    extendWave<COUNT, 1>(sc, w, cur, shallow, false, grid, s);
   }
 }
 void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t scheme,
@@ -2264,12 +2491,14 @@ void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, in
 }
 template <bool COUNT>
 static void shadowFor(const SceneView& sc, const Wavefront& w, uint32_t scheme, uint32_t grid, hipStream_t s) {
-  const bool wide = sc.nodes4 != nullptr, xcd = scheme == kSchemeXcdSlices;
+  const bool wide = sc.nodes4 != nullptr, quant = sc.nodesQ != nullptr, xcd = scheme == kSchemeXcdSlices;
   if (wide && scheme == kSchemeRefill) kShadowWide<COUNT><<<grid, kTraceBlock, 0, s>>>(sc, w);
-  else if (wide && xcd) kShadow<COUNT, true, true><<<grid, kTraceBlock, 0, s>>>(sc, w);
-  else if (wide) kShadow<COUNT, true, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
-  else if (xcd) kShadow<COUNT, false, true><<<grid, kTraceBlock, 0, s>>>(sc, w);
-  else kShadow<COUNT, false, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  else if (quant && xcd) kShadow<COUNT, 2, true><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  else if (quant) kShadow<COUNT, 2, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  else if (wide && xcd) kShadow<COUNT, 1, true><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  else if (wide) kShadow<COUNT, 1, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  else if (xcd) kShadow<COUNT, 0, true><<<grid, kTraceBlock, 0, s>>>(sc, w);
+  else kShadow<COUNT, 0, false><<<grid, kTraceBlock, 0, s>>>(sc, w);
 }
 void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t scheme, uint32_t grid, bool count,
                   hipStream_t s) {

@@ -15,8 +15,9 @@ images in HBM.  Scene upload and BVH build happen before the timed region.
 Prints one JSON line:
   value        Mpaths/s over the timed steps (camera samples per second)
   roofline     HBM roofline of the closest-hit traversal kernel: algorithmic
-               bytes per launch (node visits x node bytes (112 B for the 4-wide
-               tree, 64 B binary) + triangle tests x 48 B +
+               bytes per launch (node visits x node bytes (64 B for the
+               quantized 4-wide node, 112 B for the float 4-wide node, 64 B
+               binary) + triangle tests x 48 B +
                68 B ray/hit I/O per ray, counted in a calibration render) over
                its average launch time (HIP events on the render stream)
   cpu_baseline the CPU oracle (the same estimator, FP32, own SAH BVH) on the
@@ -36,9 +37,9 @@ sys.path.insert(0, HERE)
 
 METRIC = "Mpaths/s & Mrays/s at 1920x1080; p50/p99 job latency at equal relMSE"
 HBM_PEAK_GBS = 8000.0
-# bytes a node visit reads: binary node 64 B; 4-wide node 7 x 16 B (6 box
-# planes + child refs) of its 128 B line
-NODE_BYTES = {2: 64, 4: 112}
+# bytes a node visit reads: binary node 64 B; float 4-wide node 7 x 16 B (6
+# box planes + child refs) of its 128 B line; quantized 4-wide node 64 B
+NODE_BYTES = {(2, 64): 64, (4, 128): 112, (4, 64): 64}
 TRI_BYTES, RAY_IO_BYTES = 48, 68
 
 
@@ -62,6 +63,8 @@ def main():
     ap.add_argument("--queue-paths", type=int, default=0)
     ap.add_argument("--scene-dir", default="")
     ap.add_argument("--branching", type=int, choices=[2, 4], default=4)
+    ap.add_argument("--node-format", type=int, choices=[0, 1, 2], default=0,
+                    help="4-wide nodes: 0 default (float), 1 float 128 B, 2 quantized 64 B")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent traversal blocks per CU when not autotuned")
@@ -115,7 +118,7 @@ def main():
     t0 = time.time()
     r.upload(scene)
     t_up = time.time() - t0
-    binfo = r.build(branching=args.branching)
+    binfo = r.build(branching=args.branching, node_format=args.node_format)
     log(f"[bench] upload {t_up:.2f}s, BVH {binfo}")
     r.set_film(w, h)
     dch = scene.state.depth_out_channel
@@ -144,7 +147,8 @@ def main():
     ext_rays = max(1, cal["extension_rays"])
     nodes_per_ray = cal["ext_node_visits"] / ext_rays
     tris_per_ray = cal["ext_tri_tests"] / ext_rays
-    bytes_per_ray = nodes_per_ray * NODE_BYTES[binfo["branching"]] + tris_per_ray * TRI_BYTES + RAY_IO_BYTES
+    node_bytes = NODE_BYTES[(binfo["branching"], binfo["node_bytes"])]
+    bytes_per_ray = nodes_per_ray * node_bytes + tris_per_ray * TRI_BYTES + RAY_IO_BYTES
     log(f"[bench] calibration: {nodes_per_ray:.2f} nodes, {tris_per_ray:.2f} tris per closest-hit ray "
         f"-> {bytes_per_ray:.0f} B/ray")
 
@@ -195,14 +199,16 @@ def main():
     bytes_per_launch = rays_per_launch * bytes_per_ray
     achieved = bytes_per_launch / (ext_ms_per_launch * 1e-3) / 1e9
     traffic = None
-    tfile = os.path.join(HERE, "profiles", f"traffic_extend_{args.config}_bvh{binfo['branching']}.json")
+    tree = f"bvh{binfo['branching']}" + ("q" if binfo["branching"] == 4 and binfo["node_bytes"] == 64 else "")
+    tfile = os.path.join(HERE, "profiles", f"traffic_extend_{args.config}_{tree}.json")
     if os.path.exists(tfile) and args.scale == 0:
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                 "kernel": "kExtend (closest-hit traversal)", "avg_launch_ms": round(ext_ms_per_launch, 4),
-                "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 3),
+                "bytes_per_ray": round(bytes_per_ray, 1), "node_bytes": node_bytes,
+                "nodes_per_ray": round(nodes_per_ray, 3),
                 "tris_per_ray": round(tris_per_ray, 3), "rays_per_launch": round(rays_per_launch)}
 
     cpu = None

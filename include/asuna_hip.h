@@ -262,14 +262,16 @@ typedef struct ash_build_opts {
   uint32_t leaf_max_tris; /* max triangles per leaf after collapse (default 4) */
   uint32_t quality;       /* 0 = LBVH, 1 = LBVH + SAH leaf collapse (default 1) */
   uint32_t branching;     /* 2 = binary BVH, 4 = 4-wide BVH traversed by the kernels (default 4) */
-  uint32_t pad;
+  uint32_t node_format;   /* 4-wide nodes the wave-fetch kernels walk: 0 = default (float),
+                             1 = 128-byte float boxes, 2 = 64-byte quantized boxes (8-bit planes on a
+                             per-node power-of-two grid, rounded outwards: the same closest hits) */
 } ash_build_opts;
 
 typedef struct ash_build_info {
   uint64_t num_triangles;
   uint64_t num_nodes;
   uint32_t max_depth;
-  uint32_t pad;
+  uint32_t node_bytes; /* bytes of one node of the tree the kernels walk (64 binary, 128 or 64 4-wide) */
   double build_ms;
   double sah_cost;
   uint64_t device_bytes; /* nodes + triangles + shading data */
@@ -289,7 +291,9 @@ typedef struct ash_render_opts {
   uint32_t no_graphs;        /* 1 = launch the bounce loop eagerly instead of replaying a HIP graph */
   uint32_t lds_stack;        /* closest-hit stack entries per thread held in LDS: 8 or 16 (0 = 16) */
   uint32_t traversal;        /* ray fetch of the traversal kernels: 0 = per-XCD slices of the queue
-                                (default), 1 = queue order, 2 = per-lane refill (4-wide tree) */
+                                (default; closest hit on float 4-wide nodes with dynamic per-lane fetch
+                                and speculative while-while rounds), 1 = wave fetch in queue order,
+                                2 = per-lane refill, one node or triangle per step (4-wide tree) */
 } ash_render_opts;
 
 /* ------------------------------------------------------------ error codes */

@@ -72,9 +72,10 @@ def test_batch_plans_are_equivalent(renderer, multishot_scene):
 
 @pytest.mark.parametrize("branching", [2, 4])
 def test_traversal_fetch_schemes_render_identically(renderer, spheres_small, branching):
-    """Per-XCD queue slices, queue order and per-lane refill change only
-    which wave traces which ray: every image is bit-identical, and the
-    traversal work counted is the same."""
+    """Per-XCD queue slices (dynamic per-lane fetch, speculative
+    while-while rounds), wave fetch in queue order and per-lane refill
+    change only which lane traces which ray and when leaves are tested:
+    every image is bit-identical and the ray counts are the same."""
     s = A.HostScene(spheres_small)
     renderer.upload(s)
     renderer.build(branching=branching)

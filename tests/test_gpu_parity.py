@@ -31,10 +31,10 @@ def renderer():
     r.close()
 
 
-def _load(r, path, branching=4):
+def _load(r, path, branching=4, node_format=0):
     s = A.HostScene(path)
     r.upload(s)
-    info = r.build(branching=branching)
+    info = r.build(branching=branching, node_format=node_format)
     w, h = s.film
     r.set_film(w, h)
     return s, info
@@ -75,10 +75,11 @@ def test_first_hit_spheres_vs_oracle_bvh(renderer, spheres_small):
     assert dn <= 1e-5
 
 
-@pytest.mark.parametrize("branching", [2, 4])
-def test_first_hit_oblique_cameras(renderer, spheres_small, branching):
-    """Random poses (rays at every angle to the BVH slabs), binary and 4-wide trees."""
-    s, _ = _load(renderer, spheres_small, branching)
+@pytest.mark.parametrize("branching,node_format", [(2, 0), (4, 1), (4, 2)])
+def test_first_hit_oblique_cameras(renderer, spheres_small, branching, node_format):
+    """Random poses (rays at every angle to the BVH slabs): binary, float
+    4-wide and quantized 4-wide trees."""
+    s, _ = _load(renderer, spheres_small, branching, node_format)
     osc = O.OracleScene(s.desc)
     rng = np.random.default_rng(11)
     w, h = s.film
@@ -91,10 +92,11 @@ def test_first_hit_oblique_cameras(renderer, spheres_small, branching):
         assert prim >= 0.9999 and rel_t.max(initial=0) <= 1e-5 and dn <= 1e-5
 
 
-def test_wide_and_binary_trees_render_identically(renderer, spheres_small):
+@pytest.mark.parametrize("node_format", [1, 2])
+def test_wide_and_binary_trees_render_identically(renderer, spheres_small, node_format):
     """The 4-wide walk (nearest-first, distance-culled stack) finds the same
     closest hit as the binary walk, so whole renders are bit-identical; it
-    visits fewer nodes."""
+    visits fewer nodes.  Quantized nodes (boxes rounded outwards) too."""
     s, i2 = _load(renderer, spheres_small, 2)
     cam, pc = s.shot(0)
     w, h = s.film
@@ -102,7 +104,7 @@ def test_wide_and_binary_trees_render_identically(renderer, spheres_small):
     st2 = renderer.render(cam, pc, 0, 4, count_traversal=True)
     img2 = [renderer.read(i) for i in (0, 1, 2, 8)]
     hit2 = renderer.first_hit(cam)
-    s, i4 = _load(renderer, spheres_small, 4)
+    s, i4 = _load(renderer, spheres_small, 4, node_format)
     renderer.set_film(w, h)
     st4 = renderer.render(cam, pc, 0, 4, count_traversal=True)
     img4 = [renderer.read(i) for i in (0, 1, 2, 8)]
@@ -111,6 +113,7 @@ def test_wide_and_binary_trees_render_identically(renderer, spheres_small):
         f"{st4['ext_node_visits'] / st4['extension_rays']:.2f}")
     assert i2["branching"] == 2 and i2["num_wide_nodes"] == 0
     assert i4["branching"] == 4 and 0 < i4["num_wide_nodes"] < i4["num_nodes"]
+    assert i4["node_bytes"] == (128 if node_format == 1 else 64) and i2["node_bytes"] == 64
     assert i4["wide_depth"] < i4["max_depth"]
     for a, b in zip(img2, img4):
         np.testing.assert_array_equal(a, b)
