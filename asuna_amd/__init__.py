@@ -433,7 +433,7 @@ class Renderer:
     def upload_desc(self, desc: SceneDesc):
         self._check(lib().ash_upload_scene(self._c, C.byref(desc)), "upload_scene")
 
-    def build(self, leaf_max_tris=4, quality=1, branching=4, node_format=0) -> dict:
+    def build(self, leaf_max_tris=4, quality=2, branching=4, node_format=0) -> dict:
         info = BuildInfo()
         opts = BuildOpts(leaf_max_tris, quality, branching, node_format)
         self._check(lib().ash_build_accel(self._c, C.byref(opts), C.byref(info)),

@@ -175,6 +175,147 @@ __device__ __forceinline__ float halfArea(float4 lo, float4 hi) {
   return dx * dy + dy * dz + dz * dx;
 }
 
+// ----------------------------------------------------------------- PLOC
+// Parallel locally-ordered clustering (Meister and Bittner 2018): clusters
+// kept in Morton order; each finds, among the kPlocRadius clusters on either
+// side, the one whose merged box has the smallest surface area (ties to the
+// smaller index, so the globally best pair is always mutual); mutual pairs
+// merge into a new internal node; the survivors are compacted in order.
+// Internal ids are handed out from n - 2 down, so the root is node 0.
+constexpr int kPlocRadius = 16;
+
+__global__ void kPlocInit(uint32_t n, const uint32_t* sortedIds, const float4* boxLo, const float4* boxHi, int* cNode,
+                          float4* cLo, float4* cHi) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint32_t t = sortedIds[i];
+  cNode[i] = ~int(i);
+  cLo[i] = boxLo[t];
+  cHi[i] = boxHi[t];
+}
+
+__global__ __launch_bounds__(kBlock) void kPlocNearest(uint32_t m, const float4* cLo, const float4* cHi, int* nn) {
+  __shared__ float4 sl[kBlock + 2 * kPlocRadius], sh[kBlock + 2 * kPlocRadius];
+  const int base = int(blockIdx.x * blockDim.x) - kPlocRadius;
+  for (int k = int(threadIdx.x); k < kBlock + 2 * kPlocRadius; k += kBlock) {
+    int g = base + k;
+    if (g >= 0 && g < int(m)) {
+      sl[k] = cLo[g];
+      sh[k] = cHi[g];
+    }
+  }
+  __syncthreads();
+  const int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= int(m)) return;
+  const int li = i - base;
+  const float4 a = sl[li], b = sh[li];
+  float best = FLT_MAX;
+  int bj = -1;
+  const int j0 = max(0, i - kPlocRadius), j1 = min(int(m) - 1, i + kPlocRadius);
+  for (int j = j0; j <= j1; ++j) {
+    if (j == i) continue;
+    const float4 c = sl[j - base], d = sh[j - base];
+    float4 lo = make_float4(fminf(a.x, c.x), fminf(a.y, c.y), fminf(a.z, c.z), 0.f);
+    float4 hi = make_float4(fmaxf(b.x, d.x), fmaxf(b.y, d.y), fmaxf(b.z, d.z), 0.f);
+    float cost = halfArea(lo, hi);
+    if (cost < best) best = cost, bj = j;  // ascending j: ties keep the smaller index
+  }
+  nn[i] = bj;
+}
+
+__global__ void kPlocMerge(uint32_t m, uint32_t n, const int* nn, const int* cNode, const float4* cLo, const float4* cHi,
+                           int* oNode, float4* oLo, float4* oHi, uint32_t* valid, int* left, int* right, int* parent,
+                           int* leafParent, uint32_t* nodeCnt, unsigned* idCounter) {
+  int i = int(blockIdx.x * blockDim.x + threadIdx.x);
+  if (i >= int(m)) return;
+  int j = nn[i];
+  bool mutual = j >= 0 && nn[j] == i;
+  if (mutual && i > j) {  // absorbed by its partner
+    valid[i] = 0;
+    return;
+  }
+  valid[i] = 1;
+  if (!mutual) {
+    oNode[i] = cNode[i];
+    oLo[i] = cLo[i];
+    oHi[i] = cHi[i];
+    return;
+  }
+  int id = int(n - 2) - int(atomicAdd(idCounter, 1u));
+  int c0 = cNode[i], c1 = cNode[j];
+  left[id] = c0;
+  right[id] = c1;
+  uint32_t cnt = 0;
+  for (int c : {c0, c1}) {
+    if (c >= 0) {
+      parent[c] = id;
+      cnt += nodeCnt[c];
+    } else {
+      leafParent[~c] = id;
+      cnt += 1u;
+    }
+  }
+  nodeCnt[id] = cnt;
+  float4 a = cLo[i], b = cHi[i], c = cLo[j], d = cHi[j];
+  oNode[i] = id;
+  oLo[i] = make_float4(fminf(a.x, c.x), fminf(a.y, c.y), fminf(a.z, c.z), 0.f);
+  oHi[i] = make_float4(fmaxf(b.x, d.x), fmaxf(b.y, d.y), fmaxf(b.z, d.z), 0.f);
+}
+
+__global__ void kPlocCompact(uint32_t m, const uint32_t* valid, const uint32_t* offs, const int* oNode,
+                             const float4* oLo, const float4* oHi, int* cNode, float4* cLo, float4* cHi) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m || !valid[i]) return;
+  uint32_t o = offs[i];
+  cNode[o] = oNode[i];
+  cLo[o] = oLo[i];
+  cHi[o] = oHi[i];
+}
+
+// Leaves in depth-first order, so each subtree covers one run of positions:
+// the new position of a leaf is the number of leaves left of it, summed
+// over the ancestors whose right subtree holds it.
+__global__ void kPlocLeafOrder(uint32_t n, const int* left, const int* right, const int* parent, const int* leafParent,
+                               const uint32_t* nodeCnt, const uint32_t* sortedIds, uint32_t* posOf,
+                               uint32_t* newSorted, int* newLeafParent) {
+  uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int child = ~int(p);
+  int node = leafParent[p];
+  uint32_t pos = 0;
+  while (node >= 0) {
+    if (right[node] == child) {
+      int l = left[node];
+      pos += l >= 0 ? nodeCnt[l] : 1u;
+    }
+    child = node;
+    node = parent[node];
+  }
+  posOf[p] = pos;
+  newSorted[pos] = sortedIds[p];
+  newLeafParent[pos] = leafParent[p];
+}
+
+// Leaf range of each internal node (its leftmost leaf's new position and
+// its leaf count); runs before the child refs are relabelled.
+__global__ void kPlocRanges(uint32_t nInternal, const int* left, const uint32_t* posOf, const uint32_t* nodeCnt,
+                            int2* range) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nInternal) return;
+  int k = int(i);
+  while (left[k] >= 0) k = left[k];
+  uint32_t first = posOf[~left[k]];
+  range[i] = make_int2(int(first), int(first + nodeCnt[i] - 1));
+}
+
+__global__ void kPlocRelabel(uint32_t nInternal, int* left, int* right, const uint32_t* posOf) {
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nInternal) return;
+  int l = left[i], r = right[i];
+  if (l < 0) left[i] = ~int(posOf[~l]);
+  if (r < 0) right[i] = ~int(posOf[~r]);
+}
+
 // Bottom-up pass: the second child to arrive at a node computes it.
 __global__ void kRefit(int n, const uint32_t* sortedIds, const float4* boxLo, const float4* boxHi, const int* left,
                        const int* right, const int* parent, const int* leafParent, const int2* range,
@@ -582,7 +723,70 @@ int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_
     HIPB(hipMalloc(&newIdx.p, ni * sizeof(uint32_t)));
     HIPB(hipMemsetAsync(visit.p, 0, ni * sizeof(unsigned), s));
     unsigned gi = (nInternal + kBlock - 1) / kBlock;
-    kKarras<<<gi, kBlock, 0, s>>>(keys2.p, int(n), left.p, right.p, parent.p, leafParent.p, range.p);
+    if (a.quality == 2) {
+      // PLOC hierarchy over the Morton order, then depth-first leaf order
+      DevBuf<int> cNode[2], nnb, leafParent2;
+      DevBuf<float4> cLo[2], cHi[2];
+      DevBuf<uint32_t> valid, offs, nodeCnt, posOf, sorted2;
+      DevBuf<unsigned> idCounter;
+      for (int b2 = 0; b2 < 2; ++b2) {
+        HIPB(hipMalloc(&cNode[b2].p, nn * sizeof(int)));
+        HIPB(hipMalloc(&cLo[b2].p, nn * sizeof(float4)));
+        HIPB(hipMalloc(&cHi[b2].p, nn * sizeof(float4)));
+      }
+      HIPB(hipMalloc(&nnb.p, nn * sizeof(int)));
+      HIPB(hipMalloc(&valid.p, nn * sizeof(uint32_t)));
+      HIPB(hipMalloc(&offs.p, nn * sizeof(uint32_t)));
+      HIPB(hipMalloc(&nodeCnt.p, ni * sizeof(uint32_t)));
+      HIPB(hipMalloc(&posOf.p, nn * sizeof(uint32_t)));
+      HIPB(hipMalloc(&sorted2.p, nn * sizeof(uint32_t)));
+      HIPB(hipMalloc(&leafParent2.p, nn * sizeof(int)));
+      HIPB(hipMalloc(&idCounter.p, sizeof(unsigned)));
+      HIPB(hipMemsetAsync(idCounter.p, 0, sizeof(unsigned), s));
+      int rootParent = -1;
+      HIPB(hipMemcpyAsync(parent.p, &rootParent, sizeof(int), hipMemcpyHostToDevice, s));
+      kPlocInit<<<grid, kBlock, 0, s>>>(n, ids2.p, boxLo.p, boxHi.p, cNode[0].p, cLo[0].p, cHi[0].p);
+      size_t stb = 0;
+      HIPB(hipcub::DeviceScan::ExclusiveSum(nullptr, stb, valid.p, offs.p, int(n), s));
+      DevBuf<char> stmp;
+      HIPB(hipMalloc(&stmp.p, stb));
+      uint32_t m = n, iters = 0;
+      while (m > 1) {
+        unsigned gm = (m + kBlock - 1) / kBlock;
+        kPlocNearest<<<gm, kBlock, 0, s>>>(m, cLo[0].p, cHi[0].p, nnb.p);
+        kPlocMerge<<<gm, kBlock, 0, s>>>(m, n, nnb.p, cNode[0].p, cLo[0].p, cHi[0].p, cNode[1].p, cLo[1].p, cHi[1].p,
+                                          valid.p, left.p, right.p, parent.p, leafParent.p, nodeCnt.p, idCounter.p);
+        HIPB(hipcub::DeviceScan::ExclusiveSum(stmp.p, stb, valid.p, offs.p, int(m), s));
+        kPlocCompact<<<gm, kBlock, 0, s>>>(m, valid.p, offs.p, cNode[1].p, cLo[1].p, cHi[1].p, cNode[0].p, cLo[0].p,
+                                            cHi[0].p);
+        uint32_t lastOff = 0, lastValid = 0;
+        HIPB(hipMemcpyAsync(&lastOff, offs.p + m - 1, 4, hipMemcpyDeviceToHost, s));
+        HIPB(hipMemcpyAsync(&lastValid, valid.p + m - 1, 4, hipMemcpyDeviceToHost, s));
+        HIPB(hipStreamSynchronize(s));
+        uint32_t next = lastOff + lastValid;
+        if (next >= m || ++iters > 100000) {
+          std::snprintf(err, errLen, "bvh: clustering made no progress (%u clusters)", m);
+          return ASH_E_INVALID;
+        }
+        m = next;
+      }
+      unsigned used = 0;
+      HIPB(hipMemcpyAsync(&used, idCounter.p, sizeof used, hipMemcpyDeviceToHost, s));
+      HIPB(hipStreamSynchronize(s));
+      if (used != nInternal) {
+        std::snprintf(err, errLen, "bvh: clustering produced %u of %u internal nodes", used, nInternal);
+        return ASH_E_INVALID;
+      }
+      kPlocLeafOrder<<<grid, kBlock, 0, s>>>(n, left.p, right.p, parent.p, leafParent.p, nodeCnt.p, ids2.p, posOf.p,
+                                             sorted2.p, leafParent2.p);
+      kPlocRanges<<<gi, kBlock, 0, s>>>(nInternal, left.p, posOf.p, nodeCnt.p, range.p);
+      kPlocRelabel<<<gi, kBlock, 0, s>>>(nInternal, left.p, right.p, posOf.p);
+      HIPB(hipMemcpyAsync(ids2.p, sorted2.p, nn * sizeof(uint32_t), hipMemcpyDeviceToDevice, s));
+      HIPB(hipMemcpyAsync(leafParent.p, leafParent2.p, nn * sizeof(int), hipMemcpyDeviceToDevice, s));
+      HIPB(hipGetLastError());
+    } else {
+      kKarras<<<gi, kBlock, 0, s>>>(keys2.p, int(n), left.p, right.p, parent.p, leafParent.p, range.p);
+    }
     kRefit<<<grid, kBlock, 0, s>>>(int(n), sorted, boxLo.p, boxHi.p, left.p, right.p, parent.p, leafParent.p,
                                    range.p, visit.p, nodeLo.p, nodeHi.p, cost.p, collapsed.p, a.leafMax, a.quality);
     kEmitFlags<<<gi, kBlock, 0, s>>>(int(nInternal), parent.p, collapsed.p, emitted.p, maxDepth.p);

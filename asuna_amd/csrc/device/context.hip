@@ -749,7 +749,8 @@ int ash_build_accel(ash_ctx* c, const ash_build_opts* o, ash_build_info* info) {
   a.indices = c->idx.p;
   a.numTris = c->numTris;
   a.leafMax = o && o->leaf_max_tris ? o->leaf_max_tris : 4;
-  a.quality = o ? o->quality : 1;
+  a.quality = o ? o->quality : 2;
+  if (a.quality > 2) return fail(c, ASH_E_INVALID, "quality must be 0, 1 or 2");
   a.branching = o && o->branching ? o->branching : 4;
   if (a.branching != 2 && a.branching != 4) return fail(c, ASH_E_INVALID, "branching must be 2 or 4");
   if (c->numTris >= (1u << 28)) return fail(c, ASH_E_UNSUPPORTED, "more than 2^28 - 1 triangles");

@@ -97,16 +97,20 @@ __device__ __forceinline__ bool hitTri(const TRay& r, float4 p0, float4 p1, floa
   return true;
 }
 
-// Conservative slab test (per-endpoint NaN ignored; far bound scaled by
-// 1 + 2*gamma_3, Ize 2013 "Robust BVH Ray Traversal").
+// Conservative slab test (far bound scaled by 1 + 2*gamma_3, Ize 2013
+// "Robust BVH Ray Traversal").  Near and far planes are chosen by the sign
+// of the inverse direction; an endpoint that is NaN (a ray parallel to a
+// plane it starts on) is ignored by the min/max chains, so such a ray is
+// never culled by that axis, whatever the sign of its zero component.
 __device__ __forceinline__ bool slab(float lx, float hx, float ly, float hy, float lz, float hz, const TRay& r,
                                      float tmin, float tmax, float& tn) {
   float x0 = (lx - r.o.x) * r.inv.x, x1 = (hx - r.o.x) * r.inv.x;
   float y0 = (ly - r.o.y) * r.inv.y, y1 = (hy - r.o.y) * r.inv.y;
   float z0 = (lz - r.o.z) * r.inv.z, z1 = (hz - r.o.z) * r.inv.z;
-  float nx = x0 > x1 ? x1 : x0, fx = x0 > x1 ? x0 : x1;
-  float ny = y0 > y1 ? y1 : y0, fy = y0 > y1 ? y0 : y1;
-  float nz = z0 > z1 ? z1 : z0, fz = z0 > z1 ? z0 : z1;
+  const bool sx = r.inv.x < 0.0f, sy = r.inv.y < 0.0f, sz = r.inv.z < 0.0f;
+  float nx = sx ? x1 : x0, fx = sx ? x0 : x1;
+  float ny = sy ? y1 : y0, fy = sy ? y0 : y1;
+  float nz = sz ? z1 : z0, fz = sz ? z0 : z1;
   tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
   float tf = fminf(fminf(fx, fy), fminf(fz, tmax)) * 1.0000004f;
   return tn <= tf;
@@ -170,14 +174,13 @@ struct Kids {
   int n;
 };
 __device__ __forceinline__ void cswap(Kids& k, int a, int b) {
-  if (k.t[b] < k.t[a]) {
-    float tt = k.t[a];
-    k.t[a] = k.t[b];
-    k.t[b] = tt;
-    int cc = k.c[a];
-    k.c[a] = k.c[b];
-    k.c[b] = cc;
-  }
+  const bool sw = k.t[b] < k.t[a];
+  const float ta = k.t[a], tb = k.t[b];
+  const int ca = k.c[a], cb = k.c[b];
+  k.t[a] = sw ? tb : ta;
+  k.t[b] = sw ? ta : tb;
+  k.c[a] = sw ? cb : ca;
+  k.c[b] = sw ? ca : cb;
 }
 __device__ __forceinline__ void testNode4(const float4* np, const TRay& r, float tmin, float lim, Kids& k) {
   float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5], rf = np[6];

@@ -173,7 +173,7 @@ int ash_run_offline(const char* scenePath, const char* outName, const ash_offlin
   ash_scene_get_desc(s, &d);
   if ((rc = ash_upload_scene(ctx, &d))) return bail(rc);
   ash_build_info bi;
-  ash_build_opts bo{4, 1, 4, 0};
+  ash_build_opts bo{4, 2, 4, 0};
   if ((rc = ash_build_accel(ctx, &bo, &bi))) return bail(rc);
   uint32_t W = s->camera.width, H = s->camera.height;
   std::vector<float> px(size_t(W) * H * 4);

@@ -63,6 +63,8 @@ def main():
     ap.add_argument("--queue-paths", type=int, default=0)
     ap.add_argument("--scene-dir", default="")
     ap.add_argument("--branching", type=int, choices=[2, 4], default=4)
+    ap.add_argument("--quality", type=int, choices=[0, 1, 2], default=2,
+                    help="BVH builder: 0 LBVH, 1 LBVH + SAH leaf collapse, 2 PLOC + SAH leaf collapse")
     ap.add_argument("--node-format", type=int, choices=[0, 1, 2], default=0,
                     help="4-wide nodes: 0 default (float), 1 float 128 B, 2 quantized 64 B")
     ap.add_argument("--no-autotune", action="store_true")
@@ -118,7 +120,7 @@ def main():
     t0 = time.time()
     r.upload(scene)
     t_up = time.time() - t0
-    binfo = r.build(branching=args.branching, node_format=args.node_format)
+    binfo = r.build(branching=args.branching, node_format=args.node_format, quality=args.quality)
     log(f"[bench] upload {t_up:.2f}s, BVH {binfo}")
     r.set_film(w, h)
     dch = scene.state.depth_out_channel

@@ -260,7 +260,9 @@ typedef struct ash_stats {
 
 typedef struct ash_build_opts {
   uint32_t leaf_max_tris; /* max triangles per leaf after collapse (default 4) */
-  uint32_t quality;       /* 0 = LBVH, 1 = LBVH + SAH leaf collapse (default 1) */
+  uint32_t quality;       /* 0 = LBVH, 1 = LBVH + SAH leaf collapse, 2 = PLOC (locally-ordered clustering
+                             over the Morton order, radius 16) + SAH leaf collapse; a null opts pointer
+                             selects 2 */
   uint32_t branching;     /* 2 = binary BVH, 4 = 4-wide BVH traversed by the kernels (default 4) */
   uint32_t node_format;   /* 4-wide nodes the wave-fetch kernels walk: 0 = default (float),
                              1 = 128-byte float boxes, 2 = 64-byte quantized boxes (8-bit planes on a

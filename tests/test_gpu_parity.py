@@ -123,6 +123,33 @@ def test_wide_and_binary_trees_render_identically(renderer, spheres_small, node_
     assert st4["ext_node_visits"] < st2["ext_node_visits"]
 
 
+@pytest.mark.parametrize("branching", [2, 4])
+def test_ploc_and_lbvh_trees_render_identically(renderer, spheres_small, branching):
+    """The clustered (PLOC) tree and the LBVH hold the same triangles under
+    different boxes; the conservative walks find the same closest and any
+    hits, so renders and first hits are bit-identical, and the clustered
+    tree (lower SAH cost) is walked with fewer node visits."""
+    s = A.HostScene(spheres_small)
+    cam, pc = s.shot(0)
+    w, h = s.film
+    out = {}
+    for q in (1, 2):
+        renderer.upload(s)
+        info = renderer.build(quality=q, branching=branching)
+        renderer.set_film(w, h)
+        st = renderer.render(cam, pc, 0, 4, count_traversal=True)
+        out[q] = (info, st, [renderer.read(i) for i in (0, 1, 2, 8)], renderer.first_hit(cam))
+        log(f"branching {branching} quality {q}: sah {info['sah_cost']:.3f} nodes/ray "
+            f"{st['ext_node_visits'] / st['extension_rays']:.2f} shadow nodes/ray "
+            f"{st['shadow_node_visits'] / max(1, st['shadow_rays']):.2f}")
+    for a, b in zip(out[1][2], out[2][2]):
+        np.testing.assert_array_equal(a, b)
+    for k in ("t", "prim_id", "inst_id", "normal"):
+        np.testing.assert_array_equal(out[1][3][k], out[2][3][k])
+    assert out[2][0]["sah_cost"] < out[1][0]["sah_cost"]
+    assert out[2][1]["ext_node_visits"] < out[1][1]["ext_node_visits"]
+
+
 def _radiance_parity(r, s, spp, crop=None, precision=0):
     cam, pc = s.shot(0)
     w, h = s.film
