@@ -2306,8 +2306,8 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
       }
     }
   }
-  // Counters for Mrays/s (extension rays of this bounce, NEE rays).
-  waveCount64(&w.cnt64[4], true);
+  // Delta-lobe NEE rays (counted, not traced); the extension and shadow
+  // ray counts are added per bounce by kStatRays.
   waveCount64(&w.cnt64[6], c.fromDielectric);
   // NEE shadow ray: maxDist = dist - 2*EPS (rgen:117-122)
   int si = waveAppend(&w.ctr[2], c.shadow);
@@ -2368,8 +2368,13 @@ __global__ void kResetCounters(Wavefront w, int cur) {
   if (threadIdx.x < 2 * kXcds) w.ctr[kXcdCtr + threadIdx.x * 32] = 0;
 }
 
-__global__ void kStatShadow(Wavefront w) {
-  if (threadIdx.x == 0) w.cnt64[5] += w.ctr[2];
+// Ray counts of the bounce: every queued path traced one extension ray,
+// every appended shadow ray is traced.
+__global__ void kStatRays(Wavefront w, int cur) {
+  if (threadIdx.x == 0) {
+    w.cnt64[4] += w.ctr[cur];
+    w.cnt64[5] += w.ctr[2];
+  }
 }
 
 // Ordered fold of the batch's frames into the output images
@@ -2490,7 +2495,7 @@ void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, in
     kShade<false><<<(maxRays + 255) / 256, 256, 0, s>>>(sc, b, w, cur);
     kShadeGroup2<false><<<512, 256, 0, s>>>(sc, b, w, cur);
   }
-  kStatShadow<<<1, 64, 0, s>>>(w);
+  kStatRays<<<1, 64, 0, s>>>(w, cur);
 }
 template <bool COUNT>
 static void shadowFor(const SceneView& sc, const Wavefront& w, uint32_t scheme, uint32_t grid, hipStream_t s) {
