@@ -63,7 +63,7 @@ __global__ void kWorldTris(BuildArgs a, float4* wtri, float4* boxLo, float4* box
     p[k] = xform(in.o2w, a.vertices[2 * size_t(vi)]);
   }
   wtri[3 * size_t(g) + 0] = make_float4(p[0].x, p[0].y, p[0].z, __int_as_float(int(lo)));
-  wtri[3 * size_t(g) + 1] = make_float4(p[1].x, p[1].y, p[1].z, __int_as_float(int(prim)));
+  wtri[3 * size_t(g) + 1] = make_float4(p[1].x, p[1].y, p[1].z, __uint_as_float(in.shade));
   wtri[3 * size_t(g) + 2] = make_float4(p[2].x, p[2].y, p[2].z, __int_as_float(int(g)));
   boxLo[g] = make_float4(fminf(p[0].x, fminf(p[1].x, p[2].x)), fminf(p[0].y, fminf(p[1].y, p[2].y)),
                          fminf(p[0].z, fminf(p[1].z, p[2].z)), 0.f);
@@ -429,8 +429,8 @@ __global__ void kShadeTris(uint32_t n, const uint32_t* sortedIds, BuildArgs a, c
   if (j >= n) return;
   uint32_t t = sortedIds[j];
   uint32_t inst = uint32_t(__float_as_int(wtri[3 * size_t(t)].w));
-  uint32_t prim = uint32_t(__float_as_int(wtri[3 * size_t(t) + 1].w));
   const DevInstance& in = a.inst[inst];
+  uint32_t prim = t - in.triBase;  // t is the global triangle id
   for (int k = 0; k < 3; ++k) {
     uint32_t vi = in.vbase + a.indices[in.ibase + prim * 3 + k];
     shade[6 * size_t(j) + 2 * k] = a.vertices[2 * size_t(vi)];

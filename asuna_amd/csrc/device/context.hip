@@ -79,7 +79,7 @@ struct ash_ctx {
   DBuf<DevInstance> inst;
   DBuf<float4> verts;
   DBuf<uint32_t> idx, instTriOffset;
-  DBuf<GpuMaterial> mats;
+  DBuf<DevMaterial> mats;
   DBuf<GpuLight> lights;
   std::vector<DBuf<float4>> texData;
   DBuf<DevImage> texTab;
@@ -600,6 +600,8 @@ int ash_upload_scene(ash_ctx* c, const ash_scene_desc* d) {
   HIPC(c, hipSetDevice(c->device));
   c->hasScene = c->hasAccel = false;
   if (d->num_materials == 0 || d->num_lights == 0) return fail(c, ASH_E_INVALID, "material 0 and light 0 (dummies) are required");
+  if (d->num_materials > kShadeIdMask || d->num_lights > kShadeIdMask)
+    return fail(c, ASH_E_UNSUPPORTED, "more than 2^30 - 1 materials or lights");
   // flatten meshes
   std::vector<uint32_t> vbase(d->num_meshes), ibase(d->num_meshes);
   size_t nv = 0, ni = 0;
@@ -648,6 +650,8 @@ int ash_upload_scene(ash_ctx* c, const ash_scene_desc* d) {
     di.material = id.light_id < 0 ? id.material_id : 0;
     di.light = id.light_id;
     di.ntri = d->meshes[id.mesh_id].num_indices / 3;
+    if (id.light_id >= 0) di.shade = kShadeLight | uint32_t(id.light_id);
+    else di.shade = id.material_id | (isGroup2Type(d->materials[id.material_id].type) ? kShadeSide : 0u);
     di.triBase = uint32_t(tri);
     off[i] = uint32_t(tri);
     tri += di.ntri;
@@ -675,7 +679,12 @@ int ash_upload_scene(ash_ctx* c, const ash_scene_desc* d) {
   HIPC(c, hipMemcpyAsync(c->idx.p, hi.data(), hi.size() * 4, hipMemcpyHostToDevice, s));
   HIPC(c, hipMemcpyAsync(c->inst.p, hinst.data(), hinst.size() * sizeof(DevInstance), hipMemcpyHostToDevice, s));
   HIPC(c, hipMemcpyAsync(c->instTriOffset.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, s));
-  HIPC(c, hipMemcpyAsync(c->mats.p, d->materials, d->num_materials * sizeof(GpuMaterial), hipMemcpyHostToDevice, s));
+  std::vector<DevMaterial> hmats(d->num_materials);
+  for (uint32_t k = 0; k < d->num_materials; ++k) {
+    std::memset(&hmats[k], 0, sizeof(DevMaterial));
+    hmats[k].m = d->materials[k];
+  }
+  HIPC(c, hipMemcpyAsync(c->mats.p, hmats.data(), hmats.size() * sizeof(DevMaterial), hipMemcpyHostToDevice, s));
   HIPC(c, hipMemcpyAsync(c->lights.p, d->lights, d->num_lights * sizeof(GpuLight), hipMemcpyHostToDevice, s));
   for (auto& t : c->texData) t.release();
   c->texData.assign(d->num_textures, DBuf<float4>());

@@ -27,9 +27,29 @@ struct DevInstance {
   int32_t light;      // light id, -1 if not an emitter
   uint32_t triBase;   // global id of the instance's first triangle
   uint32_t ntri;
-  uint32_t pad[2];
+  uint32_t shade;     // shade word, carried by each triangle record into the hit buffer
+  uint32_t pad;
 };
+// Shade word of an instance: bit 31 = surface of the second shading group,
+// bit 30 = emitter; bits 0..29 = light id (emitter) or material id.
+constexpr uint32_t kShadeSide = 0x80000000u, kShadeLight = 0x40000000u, kShadeIdMask = 0x3fffffffu;
+
+// Device copy of a material, padded to nine 16-byte words so that one can
+// be fetched with nine aligned vector loads.
+struct alignas(16) DevMaterial {
+  GpuMaterial m;
+  uint32_t pad[3];
+};
+static_assert(sizeof(DevMaterial) == 144, "DevMaterial layout");
 static_assert(sizeof(DevInstance) == 128, "DevInstance layout");
+
+// Surface plugins compiled into the second shading kernel (mirror,
+// conductor, rough conductor, plastic, phong, kang18, disney).
+__host__ __device__ inline bool isGroup2Type(uint32_t t) {
+  return t == MaterialTypeBrdfMirror || t == MaterialTypeBrdfConductor || t == MaterialTypeBrdfRoughConductor ||
+         t == MaterialTypeBrdfPlastic || t == MaterialTypeBrdfPhong || t == MaterialTypeBrdfKang18 ||
+         t == MaterialTypeBrdfDisney;
+}
 
 // Leaf reference: ~((first << 3) | (count - 1)), count in [1, 8].
 constexpr int kLeafMax = 8;

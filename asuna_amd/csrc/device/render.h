@@ -44,7 +44,7 @@ struct SceneView {
   const DevInstance* inst;
   const float4* verts;
   const uint32_t* idx;
-  const GpuMaterial* mats;
+  const DevMaterial* mats;
   const GpuLight* lights;
   const DevImage* textures;
   uint32_t numTextures;

@@ -1066,13 +1066,15 @@ struct Hit {
 // getHitState + configureShadingFrame, rchit_layouts.glsl:61-95.
 // `leaf`: the triangle's position in leaf order, whose shading record holds
 // the same vertex data as the mesh's index triple and vertices.
-__device__ __forceinline__ void hitState(const SceneView& sc, int inst, int leaf, float bu, float bv, f3 rayDir,
-                                         uint32_t useFaceNormal, Hit& h) {
+// `shade`: the instance's shade word (emitter or material id), carried by
+// the hit so that the material can be fetched without the instance record.
+__device__ __forceinline__ void hitState(const SceneView& sc, int inst, uint32_t shade, int leaf, float bu, float bv,
+                                         f3 rayDir, uint32_t useFaceNormal, Hit& h) {
   const DevInstance& in = sc.inst[inst];
   const float4* sr = sc.shadeTris + size_t(leaf) * 6;
   float4 a0 = sr[0], b0 = sr[1], a1 = sr[2], b1 = sr[3], a2 = sr[4], b2 = sr[5];
   f3 ba = f3{1.0f - bu - bv, bu, bv};
-  h.lightId = in.light;
+  h.lightId = (shade & kShadeLight) ? int(shade & kShadeIdMask) : -1;
   h.uvx = a0.w * ba.x + a1.w * ba.y + a2.w * ba.z;
   h.uvy = b0.w * ba.x + b1.w * ba.y + b2.w * ba.z;
   f3 op = xyz(a0) * ba.x + xyz(a1) * ba.y + xyz(a2) * ba.z;
@@ -1085,7 +1087,7 @@ __device__ __forceinline__ void hitState(const SceneView& sc, int inst, int leaf
              n.x * q[2] + n.y * q[6] + n.z * q[10]};
   h.N = makeNormal(nw);
   h.V = makeNormal(-rayDir);
-  h.mat = in.material;
+  h.mat = shade & kShadeIdMask;
   h.inst = inst;
   if (useFaceNormal == 1) h.N = mk1(0.f);  // geoN is never assigned (rchit_layouts.glsl:62)
   basis(h.N, h.X, h.Y);
@@ -1822,11 +1824,6 @@ __device__ f3 dsnSample(float r1, float r2, f3 Vw, f3 N, f3 X, f3 Y, const DsnMa
 // rough plastic, dielectric; true: mirror, conductor, rough conductor,
 // plastic, phong, kang18, disney) so that each shading kernel keeps the
 // register budget of one group.
-__device__ __forceinline__ bool isGroup2(uint32_t t) {
-  return t == MaterialTypeBrdfMirror || t == MaterialTypeBrdfConductor || t == MaterialTypeBrdfRoughConductor ||
-         t == MaterialTypeBrdfPlastic || t == MaterialTypeBrdfPhong || t == MaterialTypeBrdfKang18 ||
-         t == MaterialTypeBrdfDisney;
-}
 
 template <bool GROUP2>
 __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t pix, bool aov) {
@@ -2241,9 +2238,12 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
 // surface plugin; appends the NEE shadow ray and the continuing path.
 // ONE: the batch holds a single shot whose constants come from the kernel
 // arguments (scalar loads) instead of the per-shot table.
+// sMat: this thread's 144-byte LDS slot; the hit's material is fetched
+// into it with nine vector loads issued together, so the plugins read
+// their inputs from LDS instead of waiting on one global load per field.
 template <bool ONE, bool GROUP2>
 __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur,
-                                         int pid) {
+                                         int pid, float4* sMat) {
   uint4 info = w.info[pid];
   uint32_t shot = info.w & 0xffffu;
   const ShotParams& sp = ONE ? b.one : b.shotTab[shot];
@@ -2285,13 +2285,23 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
   } else {
     float4 hf = w.hitF[pid];
     Hit h;
-    hitState(sc, hi.x, hi.w, hf.y, hf.z, c.rd, pc.useFaceNormal, h);
+    hitState(sc, hi.x, uint32_t(hi.y), hi.w, hf.y, hf.z, c.rd, pc.useFaceNormal, h);
     bool aov = frame0 && c.depth == 1;
     if (aov && sp.depthChannel >= 0) writeAov(c, sp.depthChannel, mk1(hf.x), pix);
     if (h.lightId >= 0) {
       hitLight(c, h.lightId, h.pos);
     } else {
-      const GpuMaterial& m = sc.mats[h.mat];
+      // The second group keeps the global reads: its kernel's registers
+      // cannot hold the staging copy without losing a wave of occupancy.
+      if (!GROUP2) {
+        const float4* src = reinterpret_cast<const float4*>(sc.mats + h.mat);
+        float4 mv[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) mv[k] = src[k];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sMat[k] = mv[k];
+      }
+      const GpuMaterial& m = GROUP2 ? sc.mats[h.mat].m : *reinterpret_cast<const GpuMaterial*>(sMat);
       uint32_t type = m.type;
       if (type == MaterialTypeBrdfEmissive) {
         // brdf_emissive.rchit:12-26
@@ -2337,16 +2347,14 @@ __global__ __launch_bounds__(256) void kShade(SceneView sc, BatchDesc b, Wavefro
   const int pid = valid ? w.queue[cur][i] : -1;
   bool side = false;
   if (valid) {
-    int inst = w.hitI[pid].x;
-    if (inst >= 0) {
-      const DevInstance& in = sc.inst[inst];
-      side = in.light < 0 && isGroup2(sc.mats[in.material].type);
-    }
+    int2 hi = *reinterpret_cast<const int2*>(&w.hitI[pid]);
+    side = hi.x >= 0 && (uint32_t(hi.y) & kShadeSide);  // the hit carries its instance's shade word
   }
   int si = waveAppend(&w.ctr[8], side);
   if (side) w.queue[2][si] = pid;
   if (!valid || side) return;
-  shadeOne<ONE, false>(sc, b, w, cur, pid);
+  __shared__ float4 sMat[256 * 9];
+  shadeOne<ONE, false>(sc, b, w, cur, pid, sMat + threadIdx.x * 9);
 }
 
 // Second plugin group, grid-stride over the side queue.
@@ -2354,7 +2362,7 @@ template <bool ONE>
 __global__ __launch_bounds__(256) void kShadeGroup2(SceneView sc, BatchDesc b, Wavefront w, int cur) {
   const uint32_t count = w.ctr[8];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-    shadeOne<ONE, true>(sc, b, w, cur, w.queue[2][i]);
+    shadeOne<ONE, true>(sc, b, w, cur, w.queue[2][i], nullptr);
 }
 
 __global__ void kResetCounters(Wavefront w, int cur) {
@@ -2428,12 +2436,12 @@ __global__ __launch_bounds__(256) void kFirstHitState(SceneView sc, const ShotPa
   } else {
     float4 hf = w.hitF[p];
     Hit h;
-    hitState(sc, hi.x, hi.w, hf.y, hf.z, xyz(w.rd[p]), 0u, h);
+    hitState(sc, hi.x, uint32_t(hi.y), hi.w, hf.y, hf.z, xyz(w.rd[p]), 0u, h);
     o.t = hf.x;
     o.u = hf.y;
     o.v = hf.z;
     o.inst_id = hi.x;
-    o.prim_id = hi.y;
+    o.prim_id = hi.z - int(sc.inst[hi.x].triBase);  // global id minus the instance's first triangle
     o.normal[0] = h.N.x, o.normal[1] = h.N.y, o.normal[2] = h.N.z;
     o.pos[0] = h.pos.x, o.pos[1] = h.pos.y, o.pos[2] = h.pos.z;
   }
