@@ -601,7 +601,7 @@ int ash_upload_scene(ash_ctx* c, const ash_scene_desc* d) {
   c->hasScene = c->hasAccel = false;
   if (d->num_materials == 0 || d->num_lights == 0) return fail(c, ASH_E_INVALID, "material 0 and light 0 (dummies) are required");
   if (d->num_materials > kShadeIdMask || d->num_lights > kShadeIdMask)
-    return fail(c, ASH_E_UNSUPPORTED, "more than 2^30 - 1 materials or lights");
+    return fail(c, ASH_E_UNSUPPORTED, "more than 2^24 - 1 materials or lights");
   // flatten meshes
   std::vector<uint32_t> vbase(d->num_meshes), ibase(d->num_meshes);
   size_t nv = 0, ni = 0;
@@ -651,7 +651,10 @@ int ash_upload_scene(ash_ctx* c, const ash_scene_desc* d) {
     di.light = id.light_id;
     di.ntri = d->meshes[id.mesh_id].num_indices / 3;
     if (id.light_id >= 0) di.shade = kShadeLight | uint32_t(id.light_id);
-    else di.shade = id.material_id | (isGroup2Type(d->materials[id.material_id].type) ? kShadeSide : 0u);
+    else {
+      const uint32_t t = d->materials[id.material_id].type;
+      di.shade = id.material_id | ((t & kShadeTypeMask) << kShadeTypeShift) | (isGroup2Type(t) ? kShadeSide : 0u);
+    }
     di.triBase = uint32_t(tri);
     off[i] = uint32_t(tri);
     tri += di.ntri;

@@ -31,8 +31,10 @@ struct DevInstance {
   uint32_t pad;
 };
 // Shade word of an instance: bit 31 = surface of the second shading group,
-// bit 30 = emitter; bits 0..29 = light id (emitter) or material id.
-constexpr uint32_t kShadeSide = 0x80000000u, kShadeLight = 0x40000000u, kShadeIdMask = 0x3fffffffu;
+// bit 30 = emitter, bits 24..28 = material type; bits 0..23 = light id
+// (emitter) or material id.
+constexpr uint32_t kShadeSide = 0x80000000u, kShadeLight = 0x40000000u, kShadeIdMask = 0x00ffffffu;
+constexpr uint32_t kShadeTypeShift = 24, kShadeTypeMask = 0x1fu;
 
 // Device copy of a material, padded to nine 16-byte words so that one can
 // be fetched with nine aligned vector loads.

@@ -2339,22 +2339,49 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
 
 // Shades the queue; paths whose surface belongs to the second plugin group
 // are moved to the side queue (queue[2], count ctr[8]) for kShadeGroup2.
+// Each block first sorts its paths by what they hit (miss, emitter, material
+// type; counting sort in LDS), so that its waves run one plugin branch each
+// instead of every branch their paths need.
+constexpr int kShadeKeys = 16;  // 0 miss, 1 emitter, 2 + material type; kShadeKeys - 1 = not shaded here
 template <bool ONE>
 __global__ __launch_bounds__(256) void kShade(SceneView sc, BatchDesc b, Wavefront w, int cur) {
+  __shared__ float4 sMat[256 * 9];
+  __shared__ int sPid[256];
+  __shared__ uint32_t sCnt[kShadeKeys];
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t count = w.ctr[cur];
   const bool valid = i < count;
   const int pid = valid ? w.queue[cur][i] : -1;
   bool side = false;
+  uint32_t key = kShadeKeys - 1;
   if (valid) {
-    int2 hi = *reinterpret_cast<const int2*>(&w.hitI[pid]);
-    side = hi.x >= 0 && (uint32_t(hi.y) & kShadeSide);  // the hit carries its instance's shade word
+    int2 hi = *reinterpret_cast<const int2*>(&w.hitI[pid]);  // the hit carries its instance's shade word
+    const uint32_t sw = uint32_t(hi.y);
+    side = hi.x >= 0 && (sw & kShadeSide);
+    key = hi.x < 0 ? 0u : (sw & kShadeLight) ? 1u : 2u + ((sw >> kShadeTypeShift) & kShadeTypeMask);
+    if (side || key >= uint32_t(kShadeKeys - 1)) key = kShadeKeys - 1;
   }
   int si = waveAppend(&w.ctr[8], side);
   if (side) w.queue[2][si] = pid;
-  if (!valid || side) return;
-  __shared__ float4 sMat[256 * 9];
-  shadeOne<ONE, false>(sc, b, w, cur, pid, sMat + threadIdx.x * 9);
+  if (threadIdx.x < kShadeKeys) sCnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t rank = key < uint32_t(kShadeKeys - 1) ? atomicAdd(&sCnt[key], 1u) : 0u;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t run = 0;
+    for (int k = 0; k < kShadeKeys - 1; ++k) {
+      uint32_t c = sCnt[k];
+      sCnt[k] = run;
+      run += c;
+    }
+    sCnt[kShadeKeys - 1] = run;
+  }
+  __syncthreads();
+  if (key < uint32_t(kShadeKeys - 1)) sPid[sCnt[key] + rank] = pid;
+  const uint32_t total = sCnt[kShadeKeys - 1];
+  __syncthreads();
+  if (threadIdx.x >= total) return;
+  shadeOne<ONE, false>(sc, b, w, cur, sPid[threadIdx.x], sMat + threadIdx.x * 9);
 }
 
 // Second plugin group, grid-stride over the side queue.
