@@ -2343,10 +2343,11 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
 // type; counting sort in LDS), so that its waves run one plugin branch each
 // instead of every branch their paths need.
 constexpr int kShadeKeys = 16;  // 0 miss, 1 emitter, 2 + material type; kShadeKeys - 1 = not shaded here
+constexpr int kShadeBlock = 512;
 template <bool ONE>
-__global__ __launch_bounds__(256) void kShade(SceneView sc, BatchDesc b, Wavefront w, int cur) {
-  __shared__ float4 sMat[256 * 9];
-  __shared__ int sPid[256];
+__global__ __launch_bounds__(kShadeBlock) void kShade(SceneView sc, BatchDesc b, Wavefront w, int cur) {
+  __shared__ float4 sMat[kShadeBlock * 9];
+  __shared__ int sPid[kShadeBlock];
   __shared__ uint32_t sCnt[kShadeKeys];
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const uint32_t count = w.ctr[cur];
@@ -2524,10 +2525,10 @@ void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t lds
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
                  hipStream_t s) {
   if (b.shots == 1) {
-    kShade<true><<<(maxRays + 255) / 256, 256, 0, s>>>(sc, b, w, cur);
+    kShade<true><<<(maxRays + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, s>>>(sc, b, w, cur);
     kShadeGroup2<true><<<512, 256, 0, s>>>(sc, b, w, cur);
   } else {
-    kShade<false><<<(maxRays + 255) / 256, 256, 0, s>>>(sc, b, w, cur);
+    kShade<false><<<(maxRays + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, s>>>(sc, b, w, cur);
     kShadeGroup2<false><<<512, 256, 0, s>>>(sc, b, w, cur);
   }
   kStatRays<<<1, 64, 0, s>>>(w, cur);
