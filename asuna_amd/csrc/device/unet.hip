@@ -107,52 +107,80 @@ __device__ inline float loadIn1(const ConvArgs& a, uint32_t n, int y, int x, uin
 }
 
 // ------------------------------------------------------------- forward
-// Block: 4 waves x 16 output pixels; each wave computes all output channels.
+// Block: 4 waves, each computing PG groups of 16 output pixels for all NT
+// 16-wide output channel tiles; a wave loads each weight fragment once per
+// (tap, 32-channel step) and uses it for its PG pixel groups.
+template <int NT, int PG>
 __global__ __launch_bounds__(256) void kConvFwd(ConvArgs a) {
   const uint32_t P = a.N * a.H * a.W;
   const uint32_t lane = threadIdx.x & 63, q = lane >> 4, r16 = lane & 15;
-  const uint32_t p0 = blockIdx.x * 64 + (threadIdx.x >> 6) * 16;
-  const uint32_t p = p0 + r16;
-  const bool pin = p < P;
-  uint32_t n = 0;
-  int y = 0, x = 0;
-  if (pin) pixelOf(p, a.H, a.W, n, y, x);
-  const uint32_t nt = a.Cout / 16;
-  f32x4 acc[kMaxTiles];
-  for (int t = 0; t < kMaxTiles; ++t) acc[t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  const uint32_t p0 = (blockIdx.x * 4 + (threadIdx.x >> 6)) * (16 * PG);
+  uint32_t n[PG];
+  int y[PG], x[PG];
+  bool pin[PG];
+#pragma unroll
+  for (int g = 0; g < PG; ++g) {
+    const uint32_t p = p0 + 16 * g + r16;
+    pin[g] = p < P;
+    n[g] = 0, y[g] = 0, x[g] = 0;
+    if (pin[g]) pixelOf(p, a.H, a.W, n[g], y[g], x[g]);
+  }
+  f32x4 acc[PG][NT];
+#pragma unroll
+  for (int g = 0; g < PG; ++g)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[g][t] = f32x4{0.f, 0.f, 0.f, 0.f};
   for (int tap = 0; tap < 9; ++tap) {
-    const int yy = y + tap / 3 - 1, xx = x + tap % 3 - 1;
+    const int dy = tap / 3 - 1, dx = tap % 3 - 1;
     const bf16* wt = a.Wf + size_t(tap) * a.Cout * a.Cin;
     for (uint32_t c0 = 0; c0 < a.Cin; c0 += 32) {
-      bf16x8 A;
-      if (pin) {
-        A = loadIn(a, n, yy, xx, c0 + 8 * q);
-      } else {
-        for (int i = 0; i < 8; ++i) A[i] = bf16(0.0f);
-      }
       const bf16* wr = wt + c0 + 8 * q;
+      bf16x8 B[NT];
 #pragma unroll
-      for (int t = 0; t < kMaxTiles; ++t) {
-        if (uint32_t(t) < nt) {
-          bf16x8 B = *reinterpret_cast<const bf16x8*>(wr + size_t(16 * t + r16) * a.Cin);
-          acc[t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B, acc[t], 0, 0, 0);
+      for (int t = 0; t < NT; ++t) B[t] = *reinterpret_cast<const bf16x8*>(wr + size_t(16 * t + r16) * a.Cin);
+#pragma unroll
+      for (int g = 0; g < PG; ++g) {
+        bf16x8 A;
+        if (pin[g]) {
+          A = loadIn(a, n[g], y[g] + dy, x[g] + dx, c0 + 8 * q);
+        } else {
+          for (int i = 0; i < 8; ++i) A[i] = bf16(0.0f);
         }
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[g][t], 0, 0, 0);
       }
     }
   }
 #pragma unroll
-  for (int t = 0; t < kMaxTiles; ++t) {
-    if (uint32_t(t) >= nt) continue;
-    const uint32_t co = 16 * t + r16;
-    const float b = a.bias[co];
-    for (int r = 0; r < 4; ++r) {
-      const uint32_t pp = p0 + 4 * q + r;
-      if (pp >= P) continue;
-      float v = acc[t][r] + b;
-      if (a.relu) v = fmaxf(v, 0.f);
-      if (a.Y) a.Y[size_t(pp) * a.Cout + co] = bf16(v);
-      if (a.Yf) a.Yf[size_t(pp) * a.Cout + co] = v;
+  for (int g = 0; g < PG; ++g)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const uint32_t co = 16 * t + r16;
+      const float b = a.bias[co];
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t pp = p0 + 16 * g + 4 * q + r;
+        if (pp >= P) continue;
+        float v = acc[g][t][r] + b;
+        if (a.relu) v = fmaxf(v, 0.f);
+        if (a.Y) a.Y[size_t(pp) * a.Cout + co] = bf16(v);
+        if (a.Yf) a.Yf[size_t(pp) * a.Cout + co] = v;
+      }
     }
+}
+
+template <int NT>
+static void convFwdFor(const ConvArgs& a, size_t P, hipStream_t s) {
+  // large levels: 4 pixel groups per wave; small ones keep one so the grid still fills the chip
+  if (P >= (size_t(1) << 18)) kConvFwd<NT, 4><<<uint32_t((P + 255) / 256), 256, 0, s>>>(a);
+  else kConvFwd<NT, 1><<<uint32_t((P + 63) / 64), 256, 0, s>>>(a);
+}
+static bool launchConvFwd(const ConvArgs& a, size_t P, hipStream_t s) {
+  switch (a.Cout) {
+    case 32: convFwdFor<2>(a, P, s); return true;
+    case 64: convFwdFor<4>(a, P, s); return true;
+    case 96: convFwdFor<6>(a, P, s); return true;
+    case 128: convFwdFor<8>(a, P, s); return true;
+    default: return false;
   }
 }
 
@@ -689,7 +717,7 @@ static int forwardPass(ash_denoiser* d) {
     if (st.out) a.Y = st.out->a.p;
     else a.Yf = d->outF.p;
     size_t P = size_t(N) * H * W;
-    kConvFwd<<<blocks(P, 64), 256, 0, d->stream>>>(a);
+    if (!launchConvFwd(a, P, d->stream)) return dfail(d, ASH_E_INVALID, "layer %d: %u output channels", st.layer, a.Cout);
   }
   DH(d, hipGetLastError());
   return ASH_OK;
