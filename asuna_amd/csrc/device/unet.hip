@@ -170,8 +170,10 @@ __global__ __launch_bounds__(256) void kConvFwd(ConvArgs a) {
 
 template <int NT>
 static void convFwdFor(const ConvArgs& a, size_t P, hipStream_t s) {
-  // large levels: 4 pixel groups per wave; small ones keep one so the grid still fills the chip
-  if (P >= (size_t(1) << 18)) kConvFwd<NT, 4><<<uint32_t((P + 255) / 256), 256, 0, s>>>(a);
+  // more pixel groups per wave on larger levels (and narrower layers); small
+  // levels keep one so that the grid still fills the chip
+  if (P >= (size_t(1) << 20) && NT <= 2) kConvFwd<NT, 4><<<uint32_t((P + 255) / 256), 256, 0, s>>>(a);
+  else if (P >= (size_t(1) << 16)) kConvFwd<NT, 2><<<uint32_t((P + 127) / 128), 256, 0, s>>>(a);
   else kConvFwd<NT, 1><<<uint32_t((P + 63) / 64), 256, 0, s>>>(a);
 }
 static bool launchConvFwd(const ConvArgs& a, size_t P, hipStream_t s) {
