@@ -168,8 +168,87 @@ __global__ __launch_bounds__(256) void kConvFwd(ConvArgs a) {
     }
 }
 
+// Tiled forward convolution for the large levels: a block stages an
+// (8 + 2) x (TC + 2) pixel input tile with its halo (all input channels,
+// zeros outside the image) in LDS with coalesced 16-byte loads, then each
+// of its four waves computes 2 rows x TC columns of output pixels (TC / 8
+// groups of 16) for all NT output tiles, reading A fragments from LDS.
+// Pixel stride in LDS: Cin + 8 BF16, which spreads the 16 lanes of a
+// fragment over the banks.
+constexpr int kTileRows = 8;
+template <int NT, int TC>
+__global__ __launch_bounds__(256) void kConvFwdTile(ConvArgs a) {
+  extern __shared__ __align__(16) bf16 tile[];
+  const uint32_t Cin = a.Cin, stride = Cin + 8;
+  const uint32_t tilesX = (a.W + TC - 1) / TC, tilesY = (a.H + kTileRows - 1) / kTileRows;
+  const uint32_t tx0 = (blockIdx.x % tilesX) * TC, ty0 = ((blockIdx.x / tilesX) % tilesY) * kTileRows;
+  const uint32_t n = blockIdx.x / (tilesX * tilesY);
+  // stage: (kTileRows + 2) x (TC + 2) pixels x Cin channels, 8 channels per thread-step
+  const uint32_t chunks = Cin / 8, tw = TC + 2, total = (kTileRows + 2) * tw * chunks;
+  for (uint32_t k = threadIdx.x; k < total; k += 256) {
+    const uint32_t c8 = k % chunks, pix = k / chunks;
+    const int yy = int(ty0) - 1 + int(pix / tw), xx = int(tx0) - 1 + int(pix % tw);
+    *reinterpret_cast<bf16x8*>(tile + pix * stride + 8 * c8) = loadIn(a, n, yy, xx, 8 * c8);
+  }
+  __syncthreads();
+  constexpr int G = TC / 8;  // 16-pixel groups per wave: 2 rows x TC columns
+  const uint32_t lane = threadIdx.x & 63, q = lane >> 4, r16 = lane & 15, wv = threadIdx.x >> 6;
+  f32x4 acc[G][NT];
+#pragma unroll
+  for (int g = 0; g < G; ++g)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[g][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (int tap = 0; tap < 9; ++tap) {
+    const int dy = tap / 3, dx = tap % 3;  // tile coordinates are offset by the halo
+    const bf16* wt = a.Wf + size_t(tap) * a.Cout * Cin;
+    for (uint32_t c0 = 0; c0 < Cin; c0 += 32) {
+      const bf16* wr = wt + c0 + 8 * q;
+      bf16x8 B[NT];
+#pragma unroll
+      for (int t = 0; t < NT; ++t) B[t] = *reinterpret_cast<const bf16x8*>(wr + size_t(16 * t + r16) * Cin);
+#pragma unroll
+      for (int g = 0; g < G; ++g) {
+        const uint32_t row = 2 * wv + uint32_t(g) / (TC / 16), col = (uint32_t(g) % (TC / 16)) * 16 + r16;
+        const bf16x8 A =
+            *reinterpret_cast<const bf16x8*>(tile + ((row + dy) * tw + col + dx) * stride + c0 + 8 * q);
+#pragma unroll
+        for (int t = 0; t < NT; ++t) acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[g][t], 0, 0, 0);
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < G; ++g) {
+    const uint32_t y = ty0 + 2 * wv + uint32_t(g) / (TC / 16);
+    if (y >= a.H) continue;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const uint32_t co = 16 * t + r16;
+      const float b = a.bias[co];
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t x = tx0 + (uint32_t(g) % (TC / 16)) * 16 + 4 * q + r;
+        if (x >= a.W) continue;
+        const size_t pp = (size_t(n) * a.H + y) * a.W + x;
+        float v = acc[g][t][r] + b;
+        if (a.relu) v = fmaxf(v, 0.f);
+        if (a.Y) a.Y[pp * a.Cout + co] = bf16(v);
+        if (a.Yf) a.Yf[pp * a.Cout + co] = v;
+      }
+    }
+  }
+}
+
 template <int NT>
 static void convFwdFor(const ConvArgs& a, size_t P, hipStream_t s) {
+  if (P >= (size_t(1) << 18) && a.Cin <= 128) {
+    // large levels: the LDS-tiled kernel (32-column tiles up to 64 input channels, else 16)
+    const bool wide = a.Cin <= 64;
+    const uint32_t TC = wide ? 32 : 16;
+    const uint32_t tiles = a.N * ((a.H + kTileRows - 1) / kTileRows) * ((a.W + TC - 1) / TC);
+    const size_t lds = size_t(kTileRows + 2) * (TC + 2) * (a.Cin + 8) * sizeof(bf16);
+    if (wide) kConvFwdTile<NT, 32><<<tiles, 256, lds, s>>>(a);
+    else kConvFwdTile<NT, 16><<<tiles, 256, lds, s>>>(a);
+    return;
+  }
   // more pixel groups per wave on larger levels (and narrower layers); small
   // levels keep one so that the grid still fills the chip
   if (P >= (size_t(1) << 20) && NT <= 2) kConvFwd<NT, 4><<<uint32_t((P + 255) / 256), 256, 0, s>>>(a);
