@@ -197,6 +197,7 @@ static SceneView viewOf(const ash_ctx* c) {
   v.verts = c->verts.p;
   v.idx = c->idx.p;
   v.mats = c->mats.p;
+  v.numMats = uint32_t(c->mats.n);
   v.lights = c->lights.p;
   v.textures = c->texTab.p;
   v.numTextures = uint32_t(c->texTab.n);

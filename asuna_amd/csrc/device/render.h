@@ -45,6 +45,7 @@ struct SceneView {
   const float4* verts;
   const uint32_t* idx;
   const DevMaterial* mats;
+  uint32_t numMats;
   const GpuLight* lights;
   const DevImage* textures;
   uint32_t numTextures;

@@ -2238,99 +2238,164 @@ __device__ void shadeSurface(PathCtx& c, Hit& h, const GpuMaterial& m, uint32_t 
 // surface plugin; appends the NEE shadow ray and the continuing path.
 // ONE: the batch holds a single shot whose constants come from the kernel
 // arguments (scalar loads) instead of the per-shot table.
-// sMat: this thread's 144-byte LDS slot; the hit's material is fetched
-// into it with nine vector loads issued together, so the plugins read
-// their inputs from LDS instead of waiting on one global load per field.
-template <bool ONE, bool GROUP2>
-__device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur,
-                                         int pid, float4* sMat) {
-  uint4 info = w.info[pid];
-  uint32_t shot = info.w & 0xffffu;
-  const ShotParams& sp = ONE ? b.one : b.shotTab[shot];
-  const GpuPushConstantRaytrace& pc = sp.pc;
-  float4 thr = w.thr[pid], rad = w.rad[pid];
-  PathCtx c;
-  c.sc = &sc;
-  c.sp = &sp;
-  c.seed = info.x;
-  c.depth = info.y & 0xffffu;
-  c.bflags = info.y >> 16;
-  c.ro = xyz(w.ro[pid]);
-  c.rd = xyz(w.rd[pid]);
-  c.T = xyz(thr);
-  c.bpdf = thr.w;
-  c.L = xyz(rad);
-  c.stop = false;
-  c.shadow = false;
-  c.fromDielectric = false;
-  c.Ld = mk1(0.f);
-  int4 hi = w.hitI[pid];
-  uint32_t pix = info.z;
-  bool frame0 = (info.w >> 16) & 1u;
-  if (hi.x < 0) {
-    // raytrace.default.rmiss:24-55
-    c.stop = true;
-    const bool sky = sc.sky->inUse == 1;
-    f3 env = sky ? skyEval(*sc.sky, c.rd)
-                 : (sc.hasEnv && pc.hasEnvMap == 1 ? evalEnv(sc, sp.cam.envTransform, pc.envMapIntensity, c.rd)
-                                                   : ld3(pc.bgColor));
-    float mis = 1.0f;
-    if (c.depth != 1 && isNonSpecular(c.bflags)) {
-      float envPdf = (!sky && sc.hasEnv && pc.hasEnvMap == 1)
-                         ? pdfEnv(sc, sp.cam.envTransform, pc.envMapResolution[0], pc.envMapResolution[1], c.rd)
-                         : kInv4Pi;
-      mis = powerHeuristic(c.bpdf, envPdf);
+// MATS says where the plugins read the hit's material from:
+//   kMatGlobal  global memory, field by field;
+//   kMatStaged  sMat is this thread's 144-byte LDS slot; the material is
+//               fetched into it with nine vector loads issued together;
+//   kMatTable   sMat holds the scene's whole material table, copied into
+//               LDS by the block (scenes with at most kMatTableMax materials).
+enum MatSource { kMatGlobal = 0, kMatStaged = 1, kMatTable = 2 };
+constexpr uint32_t kMatTableMax = 128;
+constexpr int kShadeKeys = 16;  // 0 miss, 1 emitter, 2 + material type; kShadeKeys - 1 = not shaded here
+constexpr int kShadeBlock = 512;
+constexpr int kShadeWaves = kShadeBlock / 64;
+
+// Block-wide appends to the shadow-ray stream and the next queue, plus the
+// delta-lobe NEE count: one atomic per counter and block instead of one per
+// wave (device-scope atomics on one address are resolved beyond the XCD's
+// L2 and serialise). Every thread of the block must call it.
+struct BlockAppend {
+  uint32_t n[3][kShadeWaves];
+  uint32_t base[2];
+};
+__device__ __forceinline__ void blockAppend(BlockAppend& s, uint32_t* ctrA, bool pa, uint32_t* ctrB, bool pb,
+                                            unsigned long long* cnt, bool pn, int& ia, int& ib) {
+  const uint64_t ma = __ballot(pa), mb = __ballot(pb), mn = __ballot(pn);
+  const uint32_t lane = __lane_id(), wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    s.n[0][wave] = uint32_t(__popcll(ma));
+    s.n[1][wave] = uint32_t(__popcll(mb));
+    s.n[2][wave] = uint32_t(__popcll(mn));
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    const uint32_t k = threadIdx.x;
+    uint32_t run = 0;
+    for (int v = 0; v < kShadeWaves; ++v) {
+      const uint32_t c = s.n[k][v];
+      s.n[k][v] = run;
+      run += c;
     }
-    c.L = c.L + c.T * env * mis;
-  } else {
-    float4 hf = w.hitF[pid];
-    Hit h;
-    hitState(sc, hi.x, uint32_t(hi.y), hi.w, hf.y, hf.z, c.rd, pc.useFaceNormal, h);
-    bool aov = frame0 && c.depth == 1;
-    if (aov && sp.depthChannel >= 0) writeAov(c, sp.depthChannel, mk1(hf.x), pix);
-    if (h.lightId >= 0) {
-      hitLight(c, h.lightId, h.pos);
+    if (k == 2) {
+      if (run) atomicAdd(cnt, (unsigned long long)run);
     } else {
-      // The second group keeps the global reads: its kernel's registers
-      // cannot hold the staging copy without losing a wave of occupancy.
-      if (!GROUP2) {
-        const float4* src = reinterpret_cast<const float4*>(sc.mats + h.mat);
-        float4 mv[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) mv[k] = src[k];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) sMat[k] = mv[k];
-      }
-      const GpuMaterial& m = GROUP2 ? sc.mats[h.mat].m : *reinterpret_cast<const GpuMaterial*>(sMat);
-      uint32_t type = m.type;
-      if (type == MaterialTypeBrdfEmissive) {
-        // brdf_emissive.rchit:12-26
-        c.stop = true;
-        f3 e = ld3(m.radiance);
-        if (m.radianceTextureId >= 0) e = ld3(m.radianceFactor) * xyz(tex2D(sc.textures[m.radianceTextureId], h.uvx, h.uvy));
-        if (pc.ignoreEmissive == 0) c.L = c.L + e * c.T;
-      } else if (type < MaterialTypeNum) {
-        shadeSurface<GROUP2>(c, h, m, pix, aov);
-      } else {
-        c.stop = true;
-      }
+      s.base[k] = run ? atomicAdd(k == 0 ? ctrA : ctrB, run) : 0u;
     }
   }
-  // Delta-lobe NEE rays (counted, not traced); the extension and shadow
-  // ray counts are added per bounce by kStatRays.
-  waveCount64(&w.cnt64[6], c.fromDielectric);
-  // NEE shadow ray: maxDist = dist - 2*EPS (rgen:117-122)
-  int si = waveAppend(&w.ctr[2], c.shadow);
+  __syncthreads();
+  const uint64_t below = lane ? ((~0ull) >> (64 - lane)) : 0ull;
+  ia = pa ? int(s.base[0] + s.n[0][wave] + uint32_t(__popcll(ma & below))) : -1;
+  ib = pb ? int(s.base[1] + s.n[1][wave] + uint32_t(__popcll(mb & below))) : -1;
+}
+
+template <bool ONE, bool GROUP2, int MATS, bool BLOCK>
+__device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur,
+                                         int pid, float4* sMat, BlockAppend* sApp) {
+  const bool active = pid >= 0;
+  PathCtx c;
+  c.stop = true;
+  c.shadow = false;
+  c.fromDielectric = false;
+  uint4 info = make_uint4(0u, 0u, 0u, 0u);
+  float4 rad = make_float4(0.f, 0.f, 0.f, 0.f);
+  bool cont = false;
+  if (active) {
+    info = w.info[pid];
+    uint32_t shot = info.w & 0xffffu;
+    const ShotParams& sp = ONE ? b.one : b.shotTab[shot];
+    const GpuPushConstantRaytrace& pc = sp.pc;
+    float4 thr = w.thr[pid];
+    rad = w.rad[pid];
+    c.sc = &sc;
+    c.sp = &sp;
+    c.seed = info.x;
+    c.depth = info.y & 0xffffu;
+    c.bflags = info.y >> 16;
+    c.ro = xyz(w.ro[pid]);
+    c.rd = xyz(w.rd[pid]);
+    c.T = xyz(thr);
+    c.bpdf = thr.w;
+    c.L = xyz(rad);
+    c.stop = false;
+    c.shadow = false;
+    c.fromDielectric = false;
+    c.Ld = mk1(0.f);
+    int4 hi = w.hitI[pid];
+    uint32_t pix = info.z;
+    bool frame0 = (info.w >> 16) & 1u;
+    if (hi.x < 0) {
+      // raytrace.default.rmiss:24-55
+      c.stop = true;
+      const bool sky = sc.sky->inUse == 1;
+      f3 env = sky ? skyEval(*sc.sky, c.rd)
+                   : (sc.hasEnv && pc.hasEnvMap == 1 ? evalEnv(sc, sp.cam.envTransform, pc.envMapIntensity, c.rd)
+                                                     : ld3(pc.bgColor));
+      float mis = 1.0f;
+      if (c.depth != 1 && isNonSpecular(c.bflags)) {
+        float envPdf = (!sky && sc.hasEnv && pc.hasEnvMap == 1)
+                           ? pdfEnv(sc, sp.cam.envTransform, pc.envMapResolution[0], pc.envMapResolution[1], c.rd)
+                           : kInv4Pi;
+        mis = powerHeuristic(c.bpdf, envPdf);
+      }
+      c.L = c.L + c.T * env * mis;
+    } else {
+      float4 hf = w.hitF[pid];
+      Hit h;
+      hitState(sc, hi.x, uint32_t(hi.y), hi.w, hf.y, hf.z, c.rd, pc.useFaceNormal, h);
+      bool aov = frame0 && c.depth == 1;
+      if (aov && sp.depthChannel >= 0) writeAov(c, sp.depthChannel, mk1(hf.x), pix);
+      if (h.lightId >= 0) {
+        hitLight(c, h.lightId, h.pos);
+      } else {
+        if (MATS == kMatStaged) {
+          const float4* src = reinterpret_cast<const float4*>(sc.mats + h.mat);
+          float4 mv[9];
+  #pragma unroll
+          for (int k = 0; k < 9; ++k) mv[k] = src[k];
+  #pragma unroll
+          for (int k = 0; k < 9; ++k) sMat[k] = mv[k];
+        }
+        const GpuMaterial& m = MATS == kMatGlobal   ? sc.mats[h.mat].m
+                               : MATS == kMatTable ? *reinterpret_cast<const GpuMaterial*>(sMat + 9 * h.mat)
+                                                   : *reinterpret_cast<const GpuMaterial*>(sMat);
+        uint32_t type = m.type;
+        if (type == MaterialTypeBrdfEmissive) {
+          // brdf_emissive.rchit:12-26
+          c.stop = true;
+          f3 e = ld3(m.radiance);
+          if (m.radianceTextureId >= 0) e = ld3(m.radianceFactor) * xyz(tex2D(sc.textures[m.radianceTextureId], h.uvx, h.uvy));
+          if (pc.ignoreEmissive == 0) c.L = c.L + e * c.T;
+        } else if (type < MaterialTypeNum) {
+          shadeSurface<GROUP2>(c, h, m, pix, aov);
+        } else {
+          c.stop = true;
+        }
+      }
+    }
+    uint32_t nd = c.depth + 1;
+    cont = !c.stop && int32_t(nd) <= pc.maxPathDepth;
+    c.depth = c.stop ? c.depth : nd;
+  }
+  // Appends: the NEE shadow ray (maxDist = dist - 2*EPS, rgen:117-122), the
+  // continuing path, and the count of delta-lobe NEE rays (counted, not
+  // traced; the extension and shadow ray counts are added per bounce by
+  // kStatRays).
+  int si, ni;
+  if (BLOCK) {
+    blockAppend(*sApp, &w.ctr[2], c.shadow, &w.ctr[1 - cur], cont, &w.cnt64[6], c.fromDielectric, si, ni);
+  } else {
+    waveCount64(&w.cnt64[6], c.fromDielectric);
+    si = waveAppend(&w.ctr[2], c.shadow);
+    ni = waveAppend(&w.ctr[1 - cur], cont);
+  }
   if (si >= 0) {
     w.shO[si] = f4(c.shO, c.shDist - 2.0f * kEps);
     w.shD[si] = f4(c.shD, __int_as_float(pid));
     w.ld[pid] = f4(c.Ld, 0.f);
   }
-  uint32_t nd = c.depth + 1;
-  bool cont = !c.stop && int32_t(nd) <= pc.maxPathDepth;
-  int ni = waveAppend(&w.ctr[1 - cur], cont);
   if (ni >= 0) w.queue[1 - cur][ni] = pid;
-  w.info[pid] = make_uint4(c.seed, (c.stop ? c.depth : nd) | (c.bflags << 16), info.z, info.w);
+  if (!active) return;
+  w.info[pid] = make_uint4(c.seed, c.depth | (c.bflags << 16), info.z, info.w);
   w.thr[pid] = f4(c.T, c.bpdf);
   w.rad[pid] = f4(c.L, rad.w);
   w.ro[pid] = f4(c.ro, 0.f);
@@ -2342,15 +2407,22 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
 // Each block first sorts its paths by what they hit (miss, emitter, material
 // type; counting sort in LDS), so that its waves run one plugin branch each
 // instead of every branch their paths need.
-constexpr int kShadeKeys = 16;  // 0 miss, 1 emitter, 2 + material type; kShadeKeys - 1 = not shaded here
-constexpr int kShadeBlock = 512;
-template <bool ONE>
+// TABLE: the scene has at most kMatTableMax materials and the block keeps
+// all of them in LDS (18 KiB); otherwise each thread stages its own hit's
+// material (72 KiB per block, which halves the resident waves).
+template <bool ONE, bool TABLE>
 __global__ __launch_bounds__(kShadeBlock) void kShade(SceneView sc, BatchDesc b, Wavefront w, int cur) {
-  __shared__ float4 sMat[kShadeBlock * 9];
+  __shared__ float4 sMat[TABLE ? kMatTableMax * 9 : kShadeBlock * 9];
   __shared__ int sPid[kShadeBlock];
   __shared__ uint32_t sCnt[kShadeKeys];
-  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  __shared__ BlockAppend sApp;
   const uint32_t count = w.ctr[cur];
+  if (blockIdx.x * kShadeBlock >= count) return;  // whole block past the queue
+  if (TABLE) {
+    const float4* src = reinterpret_cast<const float4*>(sc.mats);
+    for (uint32_t k = threadIdx.x; k < 9 * sc.numMats; k += kShadeBlock) sMat[k] = src[k];
+  }
+  uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < count;
   const int pid = valid ? w.queue[cur][i] : -1;
   bool side = false;
@@ -2381,8 +2453,9 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(SceneView sc, BatchDesc b,
   if (key < uint32_t(kShadeKeys - 1)) sPid[sCnt[key] + rank] = pid;
   const uint32_t total = sCnt[kShadeKeys - 1];
   __syncthreads();
-  if (threadIdx.x >= total) return;
-  shadeOne<ONE, false>(sc, b, w, cur, sPid[threadIdx.x], sMat + threadIdx.x * 9);
+  const int mine = threadIdx.x < total ? sPid[threadIdx.x] : -1;
+  if (TABLE) shadeOne<ONE, false, kMatTable, true>(sc, b, w, cur, mine, sMat, &sApp);
+  else shadeOne<ONE, false, kMatStaged, true>(sc, b, w, cur, mine, sMat + threadIdx.x * 9, &sApp);
 }
 
 // Second plugin group, grid-stride over the side queue.
@@ -2390,7 +2463,7 @@ template <bool ONE>
 __global__ __launch_bounds__(256) void kShadeGroup2(SceneView sc, BatchDesc b, Wavefront w, int cur) {
   const uint32_t count = w.ctr[8];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-    shadeOne<ONE, true>(sc, b, w, cur, w.queue[2][i], nullptr);
+    shadeOne<ONE, true, kMatGlobal, false>(sc, b, w, cur, w.queue[2][i], nullptr, nullptr);
 }
 
 __global__ void kResetCounters(Wavefront w, int cur) {
@@ -2524,11 +2597,15 @@ void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t lds
 }
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
                  hipStream_t s) {
+  const uint32_t blocks = (maxRays + kShadeBlock - 1) / kShadeBlock;
+  const bool table = sc.numMats <= kMatTableMax;
   if (b.shots == 1) {
-    kShade<true><<<(maxRays + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, s>>>(sc, b, w, cur);
+    if (table) kShade<true, true><<<blocks, kShadeBlock, 0, s>>>(sc, b, w, cur);
+    else kShade<true, false><<<blocks, kShadeBlock, 0, s>>>(sc, b, w, cur);
     kShadeGroup2<true><<<512, 256, 0, s>>>(sc, b, w, cur);
   } else {
-    kShade<false><<<(maxRays + kShadeBlock - 1) / kShadeBlock, kShadeBlock, 0, s>>>(sc, b, w, cur);
+    if (table) kShade<false, true><<<blocks, kShadeBlock, 0, s>>>(sc, b, w, cur);
+    else kShade<false, false><<<blocks, kShadeBlock, 0, s>>>(sc, b, w, cur);
     kShadeGroup2<false><<<512, 256, 0, s>>>(sc, b, w, cur);
   }
   kStatRays<<<1, 64, 0, s>>>(w, cur);
