@@ -316,67 +316,77 @@ __global__ void kPlocRelabel(uint32_t nInternal, int* left, int* right, const ui
   if (r < 0) right[i] = ~int(posOf[~r]);
 }
 
-// Bottom-up pass: the second child to arrive at a node computes it.
-__global__ void kRefit(int n, const uint32_t* sortedIds, const float4* boxLo, const float4* boxHi, const int* left,
-                       const int* right, const int* parent, const int* leafParent, const int2* range,
-                       unsigned* visit, float4* nodeLo, float4* nodeHi, float* cost, uint8_t* collapsed,
-                       uint32_t leafMax, uint32_t quality) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= n) return;
-  int node = leafParent[k];
-  while (node >= 0) {
-    __threadfence();
-    if (atomicAdd(&visit[node], 1u) == 0u) return;  // first arrival stops
-    __threadfence();
-    int c[2] = {left[node], right[node]};
-    float4 lo = make_float4(FLT_MAX, FLT_MAX, FLT_MAX, 0.f), hi = make_float4(-FLT_MAX, -FLT_MAX, -FLT_MAX, 0.f);
-    float childCost = 0.f;
-    for (int q = 0; q < 2; ++q) {
-      float4 a, b;
-      float cc;
-      if (c[q] < 0) {
-        uint32_t t = sortedIds[~c[q]];
-        a = boxLo[t];
-        b = boxHi[t];
-        cc = kCostTri * halfArea(a, b);
-      } else {
-        const volatile float4* vl = nodeLo + c[q];
-        const volatile float4* vh = nodeHi + c[q];
-        a = make_float4(vl->x, vl->y, vl->z, 0.f);
-        b = make_float4(vh->x, vh->y, vh->z, 0.f);
-        cc = ((const volatile float*)cost)[c[q]];
-      }
-      lo = make_float4(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), 0.f);
-      hi = make_float4(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), 0.f);
-      childCost += cc;
-    }
-    float area = halfArea(lo, hi);
-    int2 rg = range[node];
-    uint32_t cnt = uint32_t(rg.y - rg.x + 1);
-    float splitCost = kCostNode * area + childCost;
-    float leafCost = kCostTri * area * float(cnt);
-    bool col = quality > 0 && cnt <= leafMax && leafCost <= splitCost;
-    nodeLo[node] = lo;
-    nodeHi[node] = hi;
-    cost[node] = col ? leafCost : splitCost;
-    collapsed[node] = col ? 1 : 0;
-    node = parent[node];
+// Bottom-up boxes, SAH costs and leaf-collapse decisions, one tree level
+// per launch (deepest first): the launch boundary orders a node's reads
+// after its children's writes, so no device-scope fences are needed (on
+// this chip each one writes back or invalidates the XCD's L2).
+// level[i] = depth of internal node i (root 0); maxLevel = deepest level
+// (nInternal or more only if the parent links hold a cycle).
+__global__ void kNodeLevel(int nInternal, const int* parent, int* level, unsigned* maxLevel) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  int d = 0;
+  if (i < nInternal) {
+    for (int p = parent[i]; p >= 0 && d < nInternal; p = parent[p]) ++d;  // bounded: a cycle ends the walk
+    level[i] = d;
   }
+  for (int off = 32; off > 0; off >>= 1) d = max(d, __shfl_xor(d, off));
+  if (__lane_id() == 0 && d > 0) atomicMax(maxLevel, unsigned(d));
+}
+
+__global__ void kRefitLevel(int nInternal, int lv, const int* level, const uint32_t* sortedIds, const float4* boxLo,
+                            const float4* boxHi, const int* left, const int* right, const int2* range, float4* nodeLo,
+                            float4* nodeHi, float* cost, uint8_t* collapsed, uint32_t leafMax, uint32_t quality) {
+  int node = blockIdx.x * blockDim.x + threadIdx.x;
+  if (node >= nInternal || level[node] != lv) return;
+  int c[2] = {left[node], right[node]};
+  float4 lo = make_float4(FLT_MAX, FLT_MAX, FLT_MAX, 0.f), hi = make_float4(-FLT_MAX, -FLT_MAX, -FLT_MAX, 0.f);
+  float childCost = 0.f;
+  for (int q = 0; q < 2; ++q) {
+    float4 a, b;
+    float cc;
+    if (c[q] < 0) {
+      uint32_t t = sortedIds[~c[q]];
+      a = boxLo[t];
+      b = boxHi[t];
+      cc = kCostTri * halfArea(a, b);
+    } else {
+      a = nodeLo[c[q]];
+      b = nodeHi[c[q]];
+      cc = cost[c[q]];
+    }
+    lo = make_float4(fminf(lo.x, a.x), fminf(lo.y, a.y), fminf(lo.z, a.z), 0.f);
+    hi = make_float4(fmaxf(hi.x, b.x), fmaxf(hi.y, b.y), fmaxf(hi.z, b.z), 0.f);
+    childCost += cc;
+  }
+  float area = halfArea(lo, hi);
+  int2 rg = range[node];
+  uint32_t cnt = uint32_t(rg.y - rg.x + 1);
+  float splitCost = kCostNode * area + childCost;
+  float leafCost = kCostTri * area * float(cnt);
+  bool col = quality > 0 && cnt <= leafMax && leafCost <= splitCost;
+  nodeLo[node] = lo;
+  nodeHi[node] = hi;
+  cost[node] = col ? leafCost : splitCost;
+  collapsed[node] = col ? 1 : 0;
 }
 
 // emitted[i] = internal node i survives (neither it nor an ancestor collapsed).
 __global__ void kEmitFlags(int nInternal, const int* parent, const uint8_t* collapsed, uint32_t* emitted,
                            unsigned* maxDepth) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nInternal) return;
-  bool e = !collapsed[i];
-  unsigned depth = 1;
-  for (int p = parent[i]; p >= 0 && e; p = parent[p]) {
-    if (collapsed[p]) e = false;
-    ++depth;
+  unsigned deepest = 0;  // one atomic per wave
+  if (i < nInternal) {
+    bool e = !collapsed[i];
+    unsigned depth = 1;
+    for (int p = parent[i]; p >= 0 && e; p = parent[p]) {
+      if (collapsed[p]) e = false;
+      ++depth;
+    }
+    emitted[i] = e ? 1u : 0u;
+    if (e) deepest = depth;
   }
-  emitted[i] = e ? 1u : 0u;
-  if (e) atomicMax(maxDepth, depth);
+  for (int off = 32; off > 0; off >>= 1) deepest = max(deepest, unsigned(__shfl_xor(int(deepest), off)));
+  if (__lane_id() == 0 && deepest > 0) atomicMax(maxDepth, deepest);
 }
 
 __device__ __forceinline__ int32_t childRef(int c, const int2* range, const uint8_t* collapsed, const uint32_t* newIdx) {
@@ -674,7 +684,8 @@ int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_
   DevBuf<uint32_t> ids, ids2, emitted, newIdx;
   DevBuf<int> left, right, parent, leafParent;
   DevBuf<int2> range;
-  DevBuf<unsigned> visit, maxDepth;
+  DevBuf<unsigned> maxLevel, maxDepth;
+  DevBuf<int> level;
   DevBuf<float> cost;
   DevBuf<uint8_t> collapsed;
   DevBuf<char> tmp;
@@ -714,14 +725,15 @@ int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_
     HIPB(hipMalloc(&parent.p, ni * sizeof(int)));
     HIPB(hipMalloc(&leafParent.p, nn * sizeof(int)));
     HIPB(hipMalloc(&range.p, ni * sizeof(int2)));
-    HIPB(hipMalloc(&visit.p, ni * sizeof(unsigned)));
+    HIPB(hipMalloc(&level.p, ni * sizeof(int)));
+    HIPB(hipMalloc(&maxLevel.p, sizeof(unsigned)));
     HIPB(hipMalloc(&nodeLo.p, ni * sizeof(float4)));
     HIPB(hipMalloc(&nodeHi.p, ni * sizeof(float4)));
     HIPB(hipMalloc(&cost.p, ni * sizeof(float)));
     HIPB(hipMalloc(&collapsed.p, ni));
     HIPB(hipMalloc(&emitted.p, ni * sizeof(uint32_t)));
     HIPB(hipMalloc(&newIdx.p, ni * sizeof(uint32_t)));
-    HIPB(hipMemsetAsync(visit.p, 0, ni * sizeof(unsigned), s));
+    HIPB(hipMemsetAsync(maxLevel.p, 0, sizeof(unsigned), s));
     unsigned gi = (nInternal + kBlock - 1) / kBlock;
     if (a.quality == 2) {
       // PLOC hierarchy over the Morton order, then depth-first leaf order
@@ -787,8 +799,17 @@ int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_
     } else {
       kKarras<<<gi, kBlock, 0, s>>>(keys2.p, int(n), left.p, right.p, parent.p, leafParent.p, range.p);
     }
-    kRefit<<<grid, kBlock, 0, s>>>(int(n), sorted, boxLo.p, boxHi.p, left.p, right.p, parent.p, leafParent.p,
-                                   range.p, visit.p, nodeLo.p, nodeHi.p, cost.p, collapsed.p, a.leafMax, a.quality);
+    kNodeLevel<<<gi, kBlock, 0, s>>>(int(nInternal), parent.p, level.p, maxLevel.p);
+    unsigned deepest = 0;
+    HIPB(hipMemcpyAsync(&deepest, maxLevel.p, sizeof deepest, hipMemcpyDeviceToHost, s));
+    HIPB(hipStreamSynchronize(s));
+    if (deepest >= nInternal) {
+      std::snprintf(err, errLen, "bvh: tree deeper (%u) than its node count", deepest);
+      return ASH_E_INVALID;
+    }
+    for (int lv = int(deepest); lv >= 0; --lv)
+      kRefitLevel<<<gi, kBlock, 0, s>>>(int(nInternal), lv, level.p, sorted, boxLo.p, boxHi.p, left.p, right.p,
+                                        range.p, nodeLo.p, nodeHi.p, cost.p, collapsed.p, a.leafMax, a.quality);
     kEmitFlags<<<gi, kBlock, 0, s>>>(int(nInternal), parent.p, collapsed.p, emitted.p, maxDepth.p);
     size_t tb = 0;
     HIPB(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, emitted.p, newIdx.p, int(nInternal), s));
