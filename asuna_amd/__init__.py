@@ -164,7 +164,8 @@ class TuneResult(C.Structure):
 
 class ServerOpts(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("queue_paths", C.c_uint32), ("blocks_per_cu", C.c_uint32),
-                ("max_jobs_per_batch", C.c_uint32), ("batches_ahead", C.c_uint32)]
+                ("max_jobs_per_batch", C.c_uint32), ("batches_ahead", C.c_uint32), ("policy", C.c_uint32),
+                ("pad", C.c_uint32)]
 
 
 class JobDesc(C.Structure):
@@ -513,14 +514,18 @@ def run_offline(scene: str, out: str, gpu_id: int = 0, output_scanline: bool = F
 
 
 # ------------------------------------------------------------ render server
+SERVE_ROUND_ROBIN, SERVE_OLDEST_FIRST = 0, 1  # ash_server_opts.policy
+
+
 class Server:
     """Resident render server (include/asuna_server.h): scenes stay in HBM,
-    jobs are admitted by an HBM budget and co-batched per scene."""
+    jobs are admitted by an HBM budget and co-batched per scene.  policy:
+    SERVE_ROUND_ROBIN or SERVE_OLDEST_FIRST (batch fill order)."""
 
     def __init__(self, device=0, hbm_budget_bytes=0, queue_paths=0, blocks_per_cu=0, max_jobs_per_batch=0,
-                 batches_ahead=0):
+                 batches_ahead=0, policy=0):
         self._s = C.c_void_p()
-        o = ServerOpts(hbm_budget_bytes, queue_paths, blocks_per_cu, max_jobs_per_batch, batches_ahead)
+        o = ServerOpts(hbm_budget_bytes, queue_paths, blocks_per_cu, max_jobs_per_batch, batches_ahead, policy, 0)
         rc = lib().ash_server_create(device, C.byref(o), C.byref(self._s))
         if rc != ASH_OK:
             self._s = None
@@ -533,7 +538,7 @@ class Server:
             msg = lib().ash_server_last_error(self._s).decode()
             raise AsunaError(f"{what} failed ({ERRORS.get(rc, rc)}): {msg}")
 
-    def add_scene(self, scene: "HostScene", leaf_max_tris=4, quality=1, branching=4) -> int:
+    def add_scene(self, scene: "HostScene", leaf_max_tris=4, quality=2, branching=4) -> int:
         sid = C.c_uint32()
         self._check(lib().ash_server_add_scene(self._s, C.byref(scene.desc),
                                                C.byref(BuildOpts(leaf_max_tris, quality, branching, 0)),

@@ -151,7 +151,7 @@ static int launchBatch(ash_server* s, SceneSlot& sc, bool& launched) {
   std::vector<Job*> batchJobs;
   const size_t N = sc.active.size();
   for (size_t k = 0; k < N && n < Q && nu < maxJobs; ++k) {
-    Job* j = sc.active[(sc.rr + k) % N];
+    Job* j = sc.active[s->o.policy == ASH_SERVE_OLDEST_FIRST ? k : (sc.rr + k) % N];
     uint32_t room = Q - n;
     WorkUnit u;
     std::memset(&u, 0, sizeof u);
@@ -326,7 +326,7 @@ int ash_server_create(int device, const ash_server_opts* opts, ash_server** out)
   if (!s->o.batches_ahead) s->o.batches_ahead = 2;
   s->o.batches_ahead = std::min<uint32_t>(s->o.batches_ahead, kRing - 1);
   if (s->o.queue_paths < 4096 || s->o.queue_paths > (1u << 26) || s->o.blocks_per_cu > 64 ||
-      s->o.max_jobs_per_batch > 65535) {
+      s->o.max_jobs_per_batch > 65535 || s->o.policy > ASH_SERVE_OLDEST_FIRST) {
     delete s;
     return ASH_E_INVALID;
   }

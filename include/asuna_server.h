@@ -33,7 +33,16 @@ typedef struct ash_server_opts {
   uint32_t blocks_per_cu;    /* persistent traversal blocks per CU (0 = 8) */
   uint32_t max_jobs_per_batch; /* shot-table entries per batch (0 = 256) */
   uint32_t batches_ahead;    /* batches enqueued per scene before waiting (0 = 2, max 3) */
+  uint32_t policy;           /* ASH_SERVE_* batch fill order */
+  uint32_t pad;
 } ash_server_opts;
+
+/* Batch fill order of a scene's admitted jobs.  ROUND_ROBIN: each batch
+ * starts at the next job in turn (short jobs finish early, long ones share
+ * the device with every newcomer); OLDEST_FIRST: each batch takes all it can
+ * from the oldest job and backfills the rest from younger ones (long jobs
+ * keep their place in line). */
+enum { ASH_SERVE_ROUND_ROBIN = 0, ASH_SERVE_OLDEST_FIRST = 1 };
 
 typedef struct ash_job_desc {
   uint32_t scene;   /* id returned by ash_server_add_scene */

@@ -37,6 +37,11 @@ def test_pod_layouts_match_the_reference_structs():
     assert C.sizeof(A.GpuSunAndSky) == 96 and A.GpuSunAndSky.in_use.offset == 92  # sun_and_sky.h
 
 
+def test_server_opts_layout():
+    # ash_server_opts (asuna_server.h): u64 budget, six u32 fields
+    assert C.sizeof(A.ServerOpts) == 32 and A.ServerOpts.policy.offset == 24
+
+
 def test_version_string():
     assert b"gfx950" in A.lib().ash_version()
 
