@@ -280,18 +280,45 @@ static int checkPc(ash_ctx* c, const GpuPushConstantRaytrace* pc) {
   return ASH_OK;
 }
 
+// The bounce loop of a batch.  On the default traversal scheme the shadow
+// rays of each bounce are traced in the same persistent launch as the
+// extension rays of the next one (extend, then per bounce: shade, fused
+// trace; the last bounce's shadow rays alone); otherwise each bounce is
+// extend, shade, shadow.  `shadeRays` sizes the shading grid.
+static void launchLoop(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid,
+                       uint32_t lds, uint32_t scheme, uint32_t shadeRays) {
+  hipStream_t s = c->stream;
+  int cur = 0;
+  if (!canFuseTrace(sc, scheme)) {
+    for (uint32_t it = 0; it < iters; ++it) {
+      launchResetCounters(c->wf, cur, s);
+      launchExtend(sc, c->wf, cur, lds, scheme, grid, false, s);
+      launchShade(sc, b, c->wf, cur, shadeRays, s);
+      launchShadow(sc, c->wf, scheme, grid, false, s);
+      cur = 1 - cur;
+    }
+    return;
+  }
+  launchResetCounters(c->wf, 0, s);
+  launchExtend(sc, c->wf, 0, lds, scheme, grid, false, s);
+  for (uint32_t it = 0; it < iters; ++it) {
+    if (it > 0) launchResetShade(c->wf, cur, s);
+    launchShade(sc, b, c->wf, cur, shadeRays, s);
+    if (it + 1 < iters) {
+      launchResetExtend(c->wf, s);
+      launchTrace(sc, c->wf, 1 - cur, lds, grid, s);
+    } else {
+      launchShadow(sc, c->wf, scheme, grid, false, s);
+    }
+    cur = 1 - cur;
+  }
+}
+
 // The shading grid covers the wavefront capacity (threads past the live
 // count exit), so one captured loop serves every batch size.
 static void launchBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid,
                           uint32_t lds, uint32_t scheme) {
-  int cur = 0;
-  for (uint32_t it = 0; it < iters; ++it) {
-    launchResetCounters(c->wf, cur, c->stream);
-    launchExtend(sc, c->wf, cur, lds, scheme, grid, false, c->stream);
-    launchShade(sc, b, c->wf, cur, c->wf.capacity, c->stream);
-    launchShadow(sc, c->wf, scheme, grid, false, c->stream);
-    cur = 1 - cur;
-  }
+  launchLoop(c, sc, b, iters, grid, lds, scheme, c->wf.capacity);
 }
 
 // Replays the bounce loop of a batch from a captured HIP graph (captured on
@@ -363,6 +390,8 @@ static int runBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, uint32_t grid,
   st->shadow_launches += iters;
   if (!eager && graphBounces(c, sc, b, iters, grid, lds, scheme)) {
     st->graph_launches++;
+  } else if (!timing && !count) {
+    launchLoop(c, sc, b, iters, grid, lds, scheme, b.n);
   } else if (!timing) {
     int cur = 0;
     for (uint32_t it = 0; it < iters; ++it) {

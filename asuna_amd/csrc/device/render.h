@@ -135,6 +135,14 @@ void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t scheme, uint
                   hipStream_t s);
 void launchAccumulate(const BatchDesc& b, const Wavefront& w, hipStream_t s);
 void launchResetCounters(const Wavefront& w, int cur, hipStream_t s);
+// Resets of the fused loop: the counts shading appends to (and the shadow
+// fetch counters), and the extension fetch counters.
+void launchResetShade(const Wavefront& w, int cur, hipStream_t s);
+void launchResetExtend(const Wavefront& w, hipStream_t s);
+// Shadow rays of the current bounce and extension rays of queue `cur` in
+// one persistent launch (default traversal scheme on the float 4-wide tree).
+bool canFuseTrace(const SceneView& sc, uint32_t scheme);
+void launchTrace(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t grid, hipStream_t s);
 // Fills the derived terms of *sky from sky->ss (one thread).
 void launchSkyPrepare(DevSky* sky, hipStream_t s);
 void launchFirstHit(const SceneView& sc, const ShotParams* shot, const Wavefront& w, ash_hit* out, uint32_t P,

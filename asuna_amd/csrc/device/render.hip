@@ -752,16 +752,12 @@ __device__ __forceinline__ bool fetchIdle(ChunkFetch& f, bool idle, int lane, bo
   }
 }
 
-template <bool COUNT, int NL, int REFILL>
-__global__ __launch_bounds__(kTraceBlock) void kExtendDyn(SceneView sc, Wavefront w, int cur) {
-  __shared__ int lds[NL * kTraceBlock];
-  __shared__ float ldsT[NL * kTraceBlock];
-  Stack st;
-  initStack<true, NL>(st, lds, ldsT, w);
+// Persistent closest-hit loop over queue `cur` (per-lane dynamic fetch).
+template <bool COUNT, int REFILL>
+__device__ __forceinline__ void extendDynLoop(const SceneView& sc, const Wavefront& w, int cur, Stack& st, int lane,
+                                              uint32_t& nNodes, uint32_t& nTris) {
   const uint32_t count = w.ctr[cur];
   const int* q = w.queue[cur];
-  const int lane = __lane_id();
-  uint32_t nNodes = 0, nTris = 0;
   ChunkFetch f;
   chunkInit<true>(f, &w.ctr[kXcdCtr], count);
   bool active = false, drained = false;
@@ -800,6 +796,16 @@ __global__ __launch_bounds__(kTraceBlock) void kExtendDyn(SceneView sc, Wavefron
       }
     }
   }
+}
+
+template <bool COUNT, int NL, int REFILL>
+__global__ __launch_bounds__(kTraceBlock) void kExtendDyn(SceneView sc, Wavefront w, int cur) {
+  __shared__ int lds[NL * kTraceBlock];
+  __shared__ float ldsT[NL * kTraceBlock];
+  Stack st;
+  initStack<true, NL>(st, lds, ldsT, w);
+  uint32_t nNodes = 0, nTris = 0;
+  extendDynLoop<COUNT, REFILL>(sc, w, cur, st, __lane_id(), nNodes, nTris);
   if (COUNT) {
     atomicAdd(&w.cnt64[0], (unsigned long long)nNodes);
     atomicAdd(&w.cnt64[1], (unsigned long long)nTris);
@@ -807,14 +813,13 @@ __global__ __launch_bounds__(kTraceBlock) void kExtendDyn(SceneView sc, Wavefron
 }
 
 // ---------------------------------------------------------------- shadow
+// Persistent any-hit loop over the shadow-ray stream: an unoccluded ray
+// adds its path's pending NEE radiance.  TREE: 0 binary, 1 4-wide,
+// 2 quantized 4-wide.
 template <bool COUNT, int TREE, bool XCD>
-__global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w) {
-  __shared__ int lds[kStackLds * kTraceBlock];
-  Stack st;
-  initStack<false>(st, lds, nullptr, w);
+__device__ __forceinline__ void shadowLoop(const SceneView& sc, const Wavefront& w, Stack& st, int lane,
+                                           uint32_t& nNodes, uint32_t& nTris) {
   const uint32_t count = w.ctr[2];
-  int lane = __lane_id();
-  uint32_t nNodes = 0, nTris = 0;
   ChunkFetch f;
   chunkInit<XCD>(f, XCD ? &w.ctr[kXcdCtr + kXcds * 32] : &w.ctr[4], count);
   uint32_t start;
@@ -833,10 +838,39 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w
       }
     }
   }
+}
+
+template <bool COUNT, int TREE, bool XCD>
+__global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w) {
+  __shared__ int lds[kStackLds * kTraceBlock];
+  Stack st;
+  initStack<false>(st, lds, nullptr, w);
+  uint32_t nNodes = 0, nTris = 0;
+  shadowLoop<COUNT, TREE, XCD>(sc, w, st, __lane_id(), nNodes, nTris);
   if (COUNT) {
     atomicAdd(&w.cnt64[2], (unsigned long long)nNodes);
     atomicAdd(&w.cnt64[3], (unsigned long long)nTris);
   }
+}
+
+// The shadow rays of one bounce and the extension rays of the next in one
+// persistent launch (they touch disjoint path state: the shadow pass adds
+// to rad, the extension pass writes the hit records).  A wave that runs out
+// of extension rays goes on to shadow rays while the others finish, so the
+// latency tail of each pass (its slowest rays, when few waves still have
+// work) overlaps the other pass instead of idling the device twice.
+template <int NL, int REFILL>
+__global__ __launch_bounds__(kTraceBlock) void kTraceDyn(SceneView sc, Wavefront w, int cur) {
+  __shared__ int lds[NL * kTraceBlock];
+  __shared__ float ldsT[NL * kTraceBlock];
+  const int lane = __lane_id();
+  uint32_t nNodes = 0, nTris = 0;
+  Stack st;
+  initStack<true, NL>(st, lds, ldsT, w);
+  extendDynLoop<false, REFILL>(sc, w, cur, st, lane, nNodes, nTris);
+  Stack ss;
+  initStack<false, NL>(ss, lds, nullptr, w);
+  shadowLoop<false, 1, true>(sc, w, ss, lane, nNodes, nTris);
 }
 
 // ------------------------------------------- 4-wide walk with ray refill
@@ -2466,15 +2500,20 @@ __global__ __launch_bounds__(256) void kShadeGroup2(SceneView sc, BatchDesc b, W
     shadeOne<ONE, true, kMatGlobal, false>(sc, b, w, cur, w.queue[2][i], nullptr, nullptr);
 }
 
+// EXT: the extension fetch counters; SHADE: the counts shading appends to
+// (next queue 1 - cur, shadow rays, side queue) and the shadow fetch counters.
+template <bool EXT, bool SHADE>
 __global__ void kResetCounters(Wavefront w, int cur) {
   if (threadIdx.x == 0) {
-    w.ctr[1 - cur] = 0;
-    w.ctr[2] = 0;
-    w.ctr[3] = 0;
-    w.ctr[4] = 0;
-    w.ctr[8] = 0;
+    if (SHADE) {
+      w.ctr[1 - cur] = 0;
+      w.ctr[2] = 0;
+      w.ctr[4] = 0;
+      w.ctr[8] = 0;
+    }
+    if (EXT) w.ctr[3] = 0;
   }
-  if (threadIdx.x < 2 * kXcds) w.ctr[kXcdCtr + threadIdx.x * 32] = 0;
+  if (threadIdx.x < 2 * kXcds && (threadIdx.x < kXcds ? EXT : SHADE)) w.ctr[kXcdCtr + threadIdx.x * 32] = 0;
 }
 
 // Ray counts of the bounce: every queued path traced one extension ray,
@@ -2631,7 +2670,16 @@ void launchAccumulate(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
 }
 void launchSkyPrepare(DevSky* sky, hipStream_t s) { kSkyPrepare<<<1, 64, 0, s>>>(sky); }
 
-void launchResetCounters(const Wavefront& w, int cur, hipStream_t s) { kResetCounters<<<1, 64, 0, s>>>(w, cur); }
+void launchResetCounters(const Wavefront& w, int cur, hipStream_t s) { kResetCounters<true, true><<<1, 64, 0, s>>>(w, cur); }
+void launchResetShade(const Wavefront& w, int cur, hipStream_t s) { kResetCounters<false, true><<<1, 64, 0, s>>>(w, cur); }
+void launchResetExtend(const Wavefront& w, hipStream_t s) { kResetCounters<true, false><<<1, 64, 0, s>>>(w, 0); }
+bool canFuseTrace(const SceneView& sc, uint32_t scheme) {
+  return scheme == kSchemeXcdSlices && sc.nodes4 != nullptr && sc.nodesQ == nullptr;
+}
+void launchTrace(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t grid, hipStream_t s) {
+  if (ldsStack == uint32_t(kStackLdsMin)) kTraceDyn<kStackLdsMin, kDynRefill><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+  else kTraceDyn<kStackLds, kDynRefill><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+}
 void launchFirstHit(const SceneView& sc, const ShotParams* shot, const Wavefront& w, ash_hit* out, uint32_t P,
                     hipStream_t s) {
   kFirstHitState<<<(P + 255) / 256, 256, 0, s>>>(sc, shot, w, out, P);
