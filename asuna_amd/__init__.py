@@ -120,7 +120,10 @@ class Stats(C.Structure):
                 ("shadow_ms", C.c_double), ("shade_ms", C.c_double), ("extend_launches", C.c_uint64),
                 ("shadow_launches", C.c_uint64), ("ext_node_visits", C.c_uint64), ("ext_tri_tests", C.c_uint64),
                 ("shadow_node_visits", C.c_uint64), ("shadow_tri_tests", C.c_uint64), ("batches", C.c_uint64),
-                ("graph_launches", C.c_uint64)]
+                ("graph_launches", C.c_uint64), ("trace_launches", C.c_uint64), ("trace_ms", C.c_double),
+                ("trace_ext_rays", C.c_uint64), ("trace_shadow_rays", C.c_uint64),
+                ("trace_ext_node_visits", C.c_uint64), ("trace_ext_tri_tests", C.c_uint64),
+                ("trace_shadow_node_visits", C.c_uint64), ("trace_shadow_tri_tests", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -227,7 +227,7 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   MALLOC(c, c->wQ1, cap);
   MALLOC(c, c->wQ2, cap);
   MALLOC(c, c->wCtr, kNumCtr);
-  MALLOC(c, c->wCnt64, 8);
+  MALLOC(c, c->wCnt64, kNumCnt64);
   Wavefront& w = c->wf;
   w.capacity = cap;
   w.ro = c->wRo.p;
@@ -246,7 +246,7 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   w.ctr = c->wCtr.p;
   w.cnt64 = c->wCnt64.p;
   HIPC(c, hipMemsetAsync(c->wCtr.p, 0, kNumCtr * sizeof(uint32_t), c->stream));
-  HIPC(c, hipMemsetAsync(c->wCnt64.p, 0, 8 * sizeof(unsigned long long), c->stream));
+  HIPC(c, hipMemsetAsync(c->wCnt64.p, 0, kNumCnt64 * sizeof(unsigned long long), c->stream));
   return ASH_OK;
 }
 
@@ -285,40 +285,72 @@ static int checkPc(ash_ctx* c, const GpuPushConstantRaytrace* pc) {
 // extension rays of the next one (extend, then per bounce: shade, fused
 // trace; the last bounce's shadow rays alone); otherwise each bounce is
 // extend, shade, shadow.  `shadeRays` sizes the shading grid.
-static void launchLoop(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid,
-                       uint32_t lds, uint32_t scheme, uint32_t shadeRays) {
+// `count` selects the counting kernels; with `st` set, every traversal and
+// shading launch is bracketed by HIP events and its time added to st.
+static int launchLoop(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid,
+                      uint32_t lds, uint32_t scheme, uint32_t shadeRays, bool count, ash_stats* st) {
   hipStream_t s = c->stream;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  if (st) {
+    HIPC(c, hipEventCreate(&ev[0]));
+    HIPC(c, hipEventCreate(&ev[1]));
+  }
+  int rc = ASH_OK;
+  auto timed = [&](double* acc, auto&& fn) {
+    if (rc != ASH_OK) return;
+    if (!st) {
+      fn();
+      return;
+    }
+    float ms = 0;
+    if (hipEventRecord(ev[0], s) != hipSuccess) rc = ASH_E_HIP;
+    fn();
+    if (rc == ASH_OK && (hipEventRecord(ev[1], s) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                         hipEventElapsedTime(&ms, ev[0], ev[1]) != hipSuccess))
+      rc = ASH_E_HIP;
+    *acc += ms;
+  };
+  double scratch = 0;
+  double* tExt = st ? &st->extend_ms : &scratch;
+  double* tShade = st ? &st->shade_ms : &scratch;
+  double* tShadow = st ? &st->shadow_ms : &scratch;
+  double* tTrace = st ? &st->trace_ms : &scratch;
   int cur = 0;
   if (!canFuseTrace(sc, scheme)) {
     for (uint32_t it = 0; it < iters; ++it) {
       launchResetCounters(c->wf, cur, s);
-      launchExtend(sc, c->wf, cur, lds, scheme, grid, false, s);
-      launchShade(sc, b, c->wf, cur, shadeRays, s);
-      launchShadow(sc, c->wf, scheme, grid, false, s);
+      timed(tExt, [&] { launchExtend(sc, c->wf, cur, lds, scheme, grid, count, s); });
+      timed(tShade, [&] { launchShade(sc, b, c->wf, cur, shadeRays, s); });
+      timed(tShadow, [&] { launchShadow(sc, c->wf, scheme, grid, count, s); });
       cur = 1 - cur;
     }
-    return;
-  }
-  launchResetCounters(c->wf, 0, s);
-  launchExtend(sc, c->wf, 0, lds, scheme, grid, false, s);
-  for (uint32_t it = 0; it < iters; ++it) {
-    if (it > 0) launchResetShade(c->wf, cur, s);
-    launchShade(sc, b, c->wf, cur, shadeRays, s);
-    if (it + 1 < iters) {
-      launchResetExtend(c->wf, s);
-      launchTrace(sc, c->wf, 1 - cur, lds, grid, s);
-    } else {
-      launchShadow(sc, c->wf, scheme, grid, false, s);
+  } else {
+    launchResetCounters(c->wf, 0, s);
+    timed(tExt, [&] { launchExtend(sc, c->wf, 0, lds, scheme, grid, count, s); });
+    for (uint32_t it = 0; it < iters; ++it) {
+      if (it > 0) launchResetShade(c->wf, cur, s);
+      timed(tShade, [&] { launchShade(sc, b, c->wf, cur, shadeRays, s); });
+      if (it + 1 < iters) {
+        launchResetExtend(c->wf, s);
+        timed(tTrace, [&] { launchTrace(sc, c->wf, 1 - cur, lds, grid, count, s); });
+      } else {
+        timed(tShadow, [&] { launchShadow(sc, c->wf, scheme, grid, count, s); });
+      }
+      cur = 1 - cur;
     }
-    cur = 1 - cur;
   }
+  if (st) {
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+  }
+  return rc;
 }
 
 // The shading grid covers the wavefront capacity (threads past the live
 // count exit), so one captured loop serves every batch size.
 static void launchBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid,
                           uint32_t lds, uint32_t scheme) {
-  launchLoop(c, sc, b, iters, grid, lds, scheme, c->wf.capacity);
+  (void)launchLoop(c, sc, b, iters, grid, lds, scheme, c->wf.capacity, false, nullptr);
 }
 
 // Replays the bounce loop of a batch from a captured HIP graph (captured on
@@ -386,46 +418,19 @@ static int runBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, uint32_t grid,
   uint32_t scheme = o ? o->traversal : uint32_t(kSchemeXcdSlices);
   launchRaygen(b, c->wf, s);
   st->batches++;
-  st->extend_launches += iters;
-  st->shadow_launches += iters;
+  if (canFuseTrace(sc, scheme)) {
+    st->extend_launches += 1;
+    st->shadow_launches += 1;
+    st->trace_launches += iters - 1;
+  } else {
+    st->extend_launches += iters;
+    st->shadow_launches += iters;
+  }
   if (!eager && graphBounces(c, sc, b, iters, grid, lds, scheme)) {
     st->graph_launches++;
-  } else if (!timing && !count) {
-    launchLoop(c, sc, b, iters, grid, lds, scheme, b.n);
-  } else if (!timing) {
-    int cur = 0;
-    for (uint32_t it = 0; it < iters; ++it) {
-      launchResetCounters(c->wf, cur, s);
-      launchExtend(sc, c->wf, cur, lds, scheme, grid, count, s);
-      launchShade(sc, b, c->wf, cur, b.n, s);
-      launchShadow(sc, c->wf, scheme, grid, count, s);
-      cur = 1 - cur;
-    }
   } else {
-    hipEvent_t ev[2];
-    HIPC(c, hipEventCreate(&ev[0]));
-    HIPC(c, hipEventCreate(&ev[1]));
-    auto timed = [&](double* acc, auto&& fn) -> int {
-      HIPC(c, hipEventRecord(ev[0], s));
-      fn();
-      HIPC(c, hipEventRecord(ev[1], s));
-      HIPC(c, hipEventSynchronize(ev[1]));
-      float ms = 0;
-      HIPC(c, hipEventElapsedTime(&ms, ev[0], ev[1]));
-      *acc += ms;
-      return ASH_OK;
-    };
-    int cur = 0, rc = ASH_OK;
-    for (uint32_t it = 0; it < iters && rc == ASH_OK; ++it) {
-      launchResetCounters(c->wf, cur, s);
-      rc = timed(&st->extend_ms, [&] { launchExtend(sc, c->wf, cur, lds, scheme, grid, count, s); });
-      if (rc == ASH_OK) rc = timed(&st->shade_ms, [&] { launchShade(sc, b, c->wf, cur, b.n, s); });
-      if (rc == ASH_OK) rc = timed(&st->shadow_ms, [&] { launchShadow(sc, c->wf, scheme, grid, count, s); });
-      cur = 1 - cur;
-    }
-    (void)hipEventDestroy(ev[0]);
-    (void)hipEventDestroy(ev[1]);
-    if (rc) return rc;
+    int rc = launchLoop(c, sc, b, iters, grid, lds, scheme, b.n, count, timing ? st : nullptr);
+    if (rc) return fail(c, rc, "bounce loop: HIP event failure");
   }
   launchAccumulate(b, c->wf, s);
   HIPC(c, hipGetLastError());
@@ -522,7 +527,7 @@ static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots
   ash_stats local{};
   ash_stats* S = st ? st : &local;
   std::memset(S, 0, sizeof *S);
-  HIPC(c, hipMemsetAsync(c->wf.cnt64, 0, 8 * sizeof(unsigned long long), c->stream));
+  HIPC(c, hipMemsetAsync(c->wf.cnt64, 0, kNumCnt64 * sizeof(unsigned long long), c->stream));
   hipEvent_t e0, e1;
   HIPC(c, hipEventCreate(&e0));
   HIPC(c, hipEventCreate(&e1));
@@ -552,7 +557,7 @@ static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
   if (rc) return rc;
-  unsigned long long cnt[8];
+  unsigned long long cnt[kNumCnt64];
   HIPC(c, hipMemcpy(cnt, c->wf.cnt64, sizeof cnt, hipMemcpyDeviceToHost));
   S->render_ms = ms;
   S->paths = uint64_t(P) * nShots * uint64_t(nFrames);
@@ -564,6 +569,12 @@ static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots
     S->ext_tri_tests = cnt[1];
     S->shadow_node_visits = cnt[2];
     S->shadow_tri_tests = cnt[3];
+    S->trace_ext_node_visits = cnt[7];
+    S->trace_ext_tri_tests = cnt[8];
+    S->trace_shadow_node_visits = cnt[9];
+    S->trace_shadow_tri_tests = cnt[10];
+    S->trace_ext_rays = cnt[11];
+    S->trace_shadow_rays = cnt[12];
   }
   return ASH_OK;
 }

@@ -79,7 +79,9 @@ struct Wavefront {
   uint32_t spillThreads;
   uint32_t spillCap;
   unsigned long long* cnt64;  // [0] ext nodes, [1] ext tris, [2] shadow nodes, [3] shadow tris,
-                              // [4] extension rays, [5] shadow rays, [6] dielectric shadow rays
+                              // [4] extension rays, [5] shadow rays, [6] dielectric shadow rays,
+                              // [7..12] of the fused launches: ext nodes, ext tris, shadow nodes,
+                              // shadow tris, ext rays, shadow rays (kNumCnt64 in all)
 };
 
 // A work unit: frames [f0, f0 + nf) of the linear pixel run [pix0, pix0 + np)
@@ -115,6 +117,7 @@ constexpr int kTraceBlock = 256;
 constexpr int kXcds = 8;          // XCDs of the device; workgroup b runs on XCD b % 8
 constexpr int kXcdCtr = 32;       // first per-slice fetch counter in Wavefront::ctr
 constexpr int kNumCtr = kXcdCtr + 2 * kXcds * 32;
+constexpr int kNumCnt64 = 16;
 
 // Ray fetch scheme of the persistent traversal kernels.
 enum TraceScheme : uint32_t {
@@ -142,7 +145,8 @@ void launchResetExtend(const Wavefront& w, hipStream_t s);
 // Shadow rays of the current bounce and extension rays of queue `cur` in
 // one persistent launch (default traversal scheme on the float 4-wide tree).
 bool canFuseTrace(const SceneView& sc, uint32_t scheme);
-void launchTrace(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t grid, hipStream_t s);
+void launchTrace(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t grid, bool count,
+                 hipStream_t s);
 // Fills the derived terms of *sky from sky->ss (one thread).
 void launchSkyPrepare(DevSky* sky, hipStream_t s);
 void launchFirstHit(const SceneView& sc, const ShotParams* shot, const Wavefront& w, ash_hit* out, uint32_t P,

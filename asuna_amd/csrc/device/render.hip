@@ -859,18 +859,32 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w
 // of extension rays goes on to shadow rays while the others finish, so the
 // latency tail of each pass (its slowest rays, when few waves still have
 // work) overlaps the other pass instead of idling the device twice.
-template <int NL, int REFILL>
+// COUNT: traversal work is added to the totals (cnt64[0..3]) and to the
+// fused launches' own counters (cnt64[7..12]: extension nodes, triangles,
+// shadow nodes, triangles, extension rays, shadow rays).
+template <bool COUNT, int NL, int REFILL>
 __global__ __launch_bounds__(kTraceBlock) void kTraceDyn(SceneView sc, Wavefront w, int cur) {
   __shared__ int lds[NL * kTraceBlock];
   __shared__ float ldsT[NL * kTraceBlock];
   const int lane = __lane_id();
-  uint32_t nNodes = 0, nTris = 0;
+  uint32_t eN = 0, eT = 0, sN = 0, sT = 0;
+  if (COUNT && blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicAdd(&w.cnt64[11], (unsigned long long)w.ctr[cur]);
+    atomicAdd(&w.cnt64[12], (unsigned long long)w.ctr[2]);
+  }
   Stack st;
   initStack<true, NL>(st, lds, ldsT, w);
-  extendDynLoop<false, REFILL>(sc, w, cur, st, lane, nNodes, nTris);
+  extendDynLoop<COUNT, REFILL>(sc, w, cur, st, lane, eN, eT);
   Stack ss;
   initStack<false, NL>(ss, lds, nullptr, w);
-  shadowLoop<false, 1, true>(sc, w, ss, lane, nNodes, nTris);
+  shadowLoop<COUNT, 1, true>(sc, w, ss, lane, sN, sT);
+  if (COUNT) {
+    const unsigned long long v[4] = {eN, eT, sN, sT};
+    for (int k = 0; k < 4; ++k) {
+      atomicAdd(&w.cnt64[k], v[k]);
+      atomicAdd(&w.cnt64[7 + k], v[k]);
+    }
+  }
 }
 
 // ------------------------------------------- 4-wide walk with ray refill
@@ -2676,9 +2690,16 @@ void launchResetExtend(const Wavefront& w, hipStream_t s) { kResetCounters<true,
 bool canFuseTrace(const SceneView& sc, uint32_t scheme) {
   return scheme == kSchemeXcdSlices && sc.nodes4 != nullptr && sc.nodesQ == nullptr;
 }
-void launchTrace(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t grid, hipStream_t s) {
-  if (ldsStack == uint32_t(kStackLdsMin)) kTraceDyn<kStackLdsMin, kDynRefill><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
-  else kTraceDyn<kStackLds, kDynRefill><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+void launchTrace(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t grid, bool count,
+                 hipStream_t s) {
+  const bool shallow = ldsStack == uint32_t(kStackLdsMin);
+  if (count) {
+    if (shallow) kTraceDyn<true, kStackLdsMin, kDynRefill><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kTraceDyn<true, kStackLds, kDynRefill><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+  } else {
+    if (shallow) kTraceDyn<false, kStackLdsMin, kDynRefill><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kTraceDyn<false, kStackLds, kDynRefill><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+  }
 }
 void launchFirstHit(const SceneView& sc, const ShotParams* shot, const Wavefront& w, ash_hit* out, uint32_t P,
                     hipStream_t s) {

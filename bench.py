@@ -14,12 +14,15 @@ images in HBM.  Scene upload and BVH build happen before the timed region.
 
 Prints one JSON line:
   value        Mpaths/s over the timed steps (camera samples per second)
-  roofline     HBM roofline of the closest-hit traversal kernel: algorithmic
-               bytes per launch (node visits x node bytes (64 B for the
-               quantized 4-wide node, 112 B for the float 4-wide node, 64 B
-               binary) + triangle tests x 48 B +
-               68 B ray/hit I/O per ray, counted in a calibration render) over
-               its average launch time (HIP events on the render stream)
+  roofline     HBM roofline of the dominant traversal kernel: on the default
+               scheme the fused trace launch (shadow rays of one bounce and
+               extension rays of the next), otherwise the closest-hit kernel.
+               Algorithmic bytes per launch = node visits x node bytes (64 B
+               for the quantized 4-wide node, 112 B for the float 4-wide
+               node, 64 B binary) + triangle tests x 48 B + ray I/O (68 B per
+               extension ray, 48 B per shadow ray), counted in a calibration
+               render of the same batches; over the kernel's average launch
+               time (HIP events on the render stream)
   cpu_baseline the CPU oracle (the same estimator, FP32, own SAH BVH) on the
                host cores over a bounded sample of the same workload
 """
@@ -41,6 +44,8 @@ HBM_PEAK_GBS = 8000.0
 # box planes + child refs) of its 128 B line; quantized 4-wide node 64 B
 NODE_BYTES = {(2, 64): 64, (4, 128): 112, (4, 64): 64}
 TRI_BYTES, RAY_IO_BYTES = 48, 68
+# shadow ray I/O: origin+tmax, direction+path id, and the pending NEE radiance
+SHADOW_IO_BYTES = 48
 
 
 def log(*a):
@@ -145,8 +150,14 @@ def main():
         return r.render(cams[0], pcs[0], 0, spp, queue_paths=qp, blocks_per_cu=bpc, depth_channel=dch,
                         graphs=graphs, lds_stack=lds, traversal=trav)
 
-    # calibration: traversal work per ray (not timed)
-    cal = r.render(cams[0], pcs[0], 0, 1, count_traversal=True, depth_channel=dch, traversal=trav)
+    # calibration: traversal work of the same batches as a timed step (counting kernels, not timed)
+    if cfg_id == 2:
+        cal = r.render_shots(list(cams), list(pcs), spp, queue_paths=qp, blocks_per_cu=bpc, count_traversal=True,
+                             lds_stack=lds, traversal=trav)
+    else:
+        r.set_film(w, h)
+        cal = r.render(cams[0], pcs[0], 0, spp, queue_paths=qp, blocks_per_cu=bpc, count_traversal=True,
+                       depth_channel=dch, lds_stack=lds, traversal=trav)
     ext_rays = max(1, cal["extension_rays"])
     nodes_per_ray = cal["ext_node_visits"] / ext_rays
     tris_per_ray = cal["ext_tri_tests"] / ext_rays
@@ -197,22 +208,43 @@ def main():
         r.set_film(w, h)
         tp = r.render(cams[0], pcs[0], 0, spp, queue_paths=qp, blocks_per_cu=bpc, timing=True, depth_channel=dch,
                       lds_stack=lds, traversal=trav)
-    ext_ms_per_launch = tp["extend_ms"] / max(1, tp["extend_launches"])
-    rays_per_launch = tp["extension_rays"] / max(1, tp["extend_launches"])
-    bytes_per_launch = rays_per_launch * bytes_per_ray
-    achieved = bytes_per_launch / (ext_ms_per_launch * 1e-3) / 1e9
-    traffic = None
     tree = f"bvh{binfo['branching']}" + ("q" if binfo["branching"] == 4 and binfo["node_bytes"] == 64 else "")
-    tfile = os.path.join(HERE, "profiles", f"traffic_extend_{args.config}_{tree}.json")
+    fused = tp["trace_launches"] > 0 and cal["trace_launches"] == tp["trace_launches"]
+    if fused:
+        # kTraceDyn: the shadow rays of bounce k and the extension rays of bounce k + 1
+        n = cal["trace_launches"]
+        ext_b = (cal["trace_ext_node_visits"] * node_bytes + cal["trace_ext_tri_tests"] * TRI_BYTES
+                 + cal["trace_ext_rays"] * RAY_IO_BYTES)
+        sh_b = (cal["trace_shadow_node_visits"] * node_bytes + cal["trace_shadow_tri_tests"] * TRI_BYTES
+                + cal["trace_shadow_rays"] * SHADOW_IO_BYTES)
+        bytes_per_launch = (ext_b + sh_b) / n
+        ms_per_launch = tp["trace_ms"] / n
+        kernel, tkey = "kTraceDyn (fused shadow + next-bounce closest-hit traversal)", "trace"
+        per_launch = {"ext_rays_per_launch": round(cal["trace_ext_rays"] / n),
+                      "shadow_rays_per_launch": round(cal["trace_shadow_rays"] / n),
+                      "ext_nodes_per_ray": round(cal["trace_ext_node_visits"] / max(1, cal["trace_ext_rays"]), 3),
+                      "ext_tris_per_ray": round(cal["trace_ext_tri_tests"] / max(1, cal["trace_ext_rays"]), 3),
+                      "shadow_nodes_per_ray": round(cal["trace_shadow_node_visits"] /
+                                                    max(1, cal["trace_shadow_rays"]), 3),
+                      "shadow_tris_per_ray": round(cal["trace_shadow_tri_tests"] / max(1, cal["trace_shadow_rays"]), 3)}
+    else:
+        ms_per_launch = tp["extend_ms"] / max(1, tp["extend_launches"])
+        rays_per_launch = tp["extension_rays"] / max(1, tp["extend_launches"])
+        bytes_per_launch = rays_per_launch * bytes_per_ray
+        kernel, tkey = "kExtend (closest-hit traversal)", "extend"
+        per_launch = {"rays_per_launch": round(rays_per_launch)}
+    achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
+    traffic = None
+    tfile = os.path.join(HERE, "profiles", f"traffic_{tkey}_{args.config}_{tree}.json")
     if os.path.exists(tfile) and args.scale == 0:
         with open(tfile) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": "kExtend (closest-hit traversal)", "avg_launch_ms": round(ext_ms_per_launch, 4),
-                "bytes_per_ray": round(bytes_per_ray, 1), "node_bytes": node_bytes,
-                "nodes_per_ray": round(nodes_per_ray, 3),
-                "tris_per_ray": round(tris_per_ray, 3), "rays_per_launch": round(rays_per_launch)}
+                "kernel": kernel, "avg_launch_ms": round(ms_per_launch, 4),
+                "bytes_per_launch": round(bytes_per_launch), "node_bytes": node_bytes,
+                "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 3),
+                "tris_per_ray": round(tris_per_ray, 3), **per_launch}
 
     cpu = None
     if not args.no_cpu_baseline:

@@ -256,6 +256,14 @@ typedef struct ash_stats {
   uint64_t shadow_tri_tests;
   uint64_t batches;        /* wavefront batches */
   uint64_t graph_launches; /* bounce loops replayed from a captured HIP graph */
+  /* Fused trace launches (default traversal scheme): one launch traces the
+   * shadow rays of a bounce and the extension rays of the next; the first
+   * bounce's extension rays and the last bounce's shadow rays keep their own
+   * launches (counted in extend_launches / shadow_launches). */
+  uint64_t trace_launches;
+  double trace_ms;         /* their kernel time, if timing enabled */
+  uint64_t trace_ext_rays, trace_shadow_rays; /* rays they traced, when count_traversal = 1 */
+  uint64_t trace_ext_node_visits, trace_ext_tri_tests, trace_shadow_node_visits, trace_shadow_tri_tests;
 } ash_stats;
 
 typedef struct ash_build_opts {
