@@ -2,6 +2,7 @@
 #pragma once
 
 #include "render.h"
+#include "../../../include/asuna_denoise.h"
 
 struct ash_ctx;
 
@@ -20,5 +21,10 @@ ShotParams ctxMakeShot(const GpuCamera* cam, const GpuPushConstantRaytrace* pc, 
 const char* ctxError(ash_ctx* c);
 // Film size of the single-shot output images (0 x 0 before ash_set_film).
 void ctxFilm(ash_ctx* c, uint32_t* w, uint32_t* h);
+
+// ash_denoise_planes without the wait: enqueues the denoise on the
+// denoiser's stream between the caller's events `start` and `done`.
+int denoisePlanesAsync(ash_denoiser* d, const void* radiance, const void* albedo, const void* normal,
+                       const void* depth, uint32_t W, uint32_t H, void* out, hipEvent_t start, hipEvent_t done);
 
 }  // namespace ashd

@@ -80,6 +80,11 @@ struct Job {
   double tSubmit = 0, tStart = -1, tDone = -1;
   double denoiseMs = 0;
   uint32_t planes = ASH_NUM_OUTPUT_IMAGES;  // + 1 denoised plane with ASH_JOB_DENOISE
+  hipEvent_t dnStart = nullptr, dnDone = nullptr;  // brackets of the job's denoise on the denoiser's stream
+  ~Job() {
+    if (dnStart) (void)hipEventDestroy(dnStart);
+    if (dnDone) (void)hipEventDestroy(dnDone);
+  }
 };
 
 struct Batch {
@@ -113,6 +118,7 @@ struct ash_server {
   std::vector<SceneSlot*> scenes;
   std::unordered_map<uint64_t, Job*> jobs;
   std::deque<Job*> pending;
+  std::vector<Job*> denoising;  // jobs whose denoise is enqueued and not yet observed complete
   std::mutex m;
   std::condition_variable cvWork, cvDone;
   std::thread th;
@@ -241,15 +247,27 @@ static void schedulerLoop(ash_server* s) {
           if (--j->inflight == 0 && j->issued) {
             j->state = q == hipSuccess ? ASH_JOB_DONE : ASH_E_HIP;
             if (j->state == ASH_JOB_DONE && (j->d.flags & ASH_JOB_DENOISE)) {
-              // the batch event has completed: the job's planes are final
+              // the batch event has completed: the job's planes are final.  The
+              // denoise is enqueued on the denoiser's stream (it overlaps the
+              // scenes' batches); the job is done when its event completes.
               const size_t P = j->P;
               auto plane = [&](int32_t ch) -> const void* {
                 return ch >= 0 && ch < ASH_NUM_OUTPUT_IMAGES - 1 ? j->sp.images + size_t(ch + 1) * P : nullptr;
               };
-              int rc = ash_denoise_planes(s->den, j->sp.images, plane(j->d.pc.diffuseOutChannel),
-                                          plane(j->d.pc.normalOutChannel), plane(j->d.depth_channel), j->d.width,
-                                          j->d.height, j->sp.images + 9 * P, &j->denoiseMs);
-              if (rc) j->state = rc;
+              int rc = ASH_OK;
+              if ((!j->dnStart && hipEventCreate(&j->dnStart) != hipSuccess) ||
+                  (!j->dnDone && hipEventCreate(&j->dnDone) != hipSuccess))
+                rc = ASH_E_HIP;
+              if (!rc)
+                rc = denoisePlanesAsync(s->den, j->sp.images, plane(j->d.pc.diffuseOutChannel),
+                                        plane(j->d.pc.normalOutChannel), plane(j->d.depth_channel), j->d.width,
+                                        j->d.height, j->sp.images + 9 * P, j->dnStart, j->dnDone);
+              if (!rc) {
+                j->state = ASH_JOB_RUNNING;
+                s->denoising.push_back(j);
+                continue;
+              }
+              j->state = rc;
             }
             j->tDone = s->now();
             s->st.jobs_done++;
@@ -260,6 +278,28 @@ static void schedulerLoop(ash_server* s) {
         progress = true;
         s->cvDone.notify_all();
       }
+    }
+    // retire finished denoises
+    for (size_t k = 0; k < s->denoising.size();) {
+      Job* j = s->denoising[k];
+      hipError_t q = hipEventQuery(j->dnDone);
+      if (q == hipErrorNotReady) {
+        ++k;
+        continue;
+      }
+      float ms = 0;
+      if (q == hipSuccess && hipEventElapsedTime(&ms, j->dnStart, j->dnDone) == hipSuccess) {
+        j->denoiseMs = ms;
+        j->state = ASH_JOB_DONE;
+      } else {
+        j->state = ASH_E_HIP;
+      }
+      j->tDone = s->now();
+      s->st.jobs_done++;
+      s->denoising[k] = s->denoising.back();
+      s->denoising.pop_back();
+      progress = true;
+      s->cvDone.notify_all();
     }
     // admit queued jobs in arrival order while their images fit
     while (!s->pending.empty()) {
@@ -288,6 +328,7 @@ static void schedulerLoop(ash_server* s) {
       }
       any = any || !sc->inflight.empty();
     }
+    any = any || !s->denoising.empty();
     double t = s->now();
     if (any && s->busySince < 0) s->busySince = t;
     if (!any && s->busySince >= 0) {
@@ -306,6 +347,7 @@ static void schedulerLoop(ash_server* s) {
   // drain
   for (SceneSlot* sc : s->scenes)
     for (Batch& bt : sc->inflight) (void)hipEventSynchronize(bt.done);
+  for (Job* j : s->denoising) (void)hipEventSynchronize(j->dnDone);
   s->cvDone.notify_all();
 }
 

@@ -537,15 +537,25 @@ __global__ void kPackW(const float* w, uint32_t Cout, uint32_t Cin, bf16* wf, bf
 template <class T> struct Buf {
   T* p = nullptr;
   size_t n = 0;
+  size_t cap = 0;
   hipError_t alloc(size_t count) {
     release();
-    n = count;
+    n = cap = count;
     return hipMalloc(&p, std::max<size_t>(count, 1) * sizeof(T));
+  }
+  // Keeps the allocation when it is large enough: a shape change then costs
+  // no hipFree, which would wait for the whole device.
+  hipError_t reserve(size_t count) {
+    if (p && count <= cap) {
+      n = count;
+      return hipSuccess;
+    }
+    return alloc(count);
   }
   void release() {
     if (p) (void)hipFree(p);
     p = nullptr;
-    n = 0;
+    n = cap = 0;
   }
 };
 
@@ -697,8 +707,8 @@ static int packWeights(ash_denoiser* d) {
 
 static int allocAct(ash_denoiser* d, Act& t, uint32_t N, uint32_t H, uint32_t W, uint32_t C) {
   t.N = N, t.H = H, t.W = W, t.C = C;
-  DH(d, t.a.alloc(t.count()));
-  DH(d, t.g.alloc(t.count()));
+  DH(d, t.a.reserve(t.count()));
+  DH(d, t.g.reserve(t.count()));
   return ASH_OK;
 }
 
@@ -722,9 +732,9 @@ static int ensureShape(ash_denoiser* d, uint32_t N, uint32_t H, uint32_t W) {
   CK(allocAct(d, d->out, N, H, W, 32));
 #undef CK
   size_t P = size_t(N) * H * W;
-  DH(d, d->upTmp.alloc(P * 128));
-  DH(d, d->outF.alloc(P * 32));
-  DH(d, d->lossPart.alloc(blocks(P * 32)));
+  DH(d, d->upTmp.reserve(P * 128));
+  DH(d, d->outF.reserve(P * 32));
+  DH(d, d->lossPart.reserve(blocks(P * 32)));
   d->N = N, d->H = H, d->W = W;
   return ASH_OK;
 }
@@ -1016,17 +1026,17 @@ int ash_denoiser_train_step(ash_denoiser* d, uint32_t n, uint32_t h, uint32_t w,
   return ASH_OK;
 }
 
-int ash_denoise_planes(ash_denoiser* d, const void* radiance, const void* albedo, const void* normal,
-                       const void* depth, uint32_t W, uint32_t H, void* out, double* ms) {
+}  // extern "C"
+
+namespace ashd {
+int denoisePlanesAsync(ash_denoiser* d, const void* radiance, const void* albedo, const void* normal,
+                       const void* depth, uint32_t W, uint32_t H, void* out, hipEvent_t start, hipEvent_t done) {
   if (!d || !radiance || !out || !W || !H) return ASH_E_INVALID;
   DH(d, hipSetDevice(d->device));
   const uint32_t Hp = (H + 15) & ~15u, Wp = (W + 15) & ~15u;
   int rc = ensureShape(d, 1, Hp, Wp);
   if (rc) return rc;
-  hipEvent_t e0, e1;
-  DH(d, hipEventCreate(&e0));
-  DH(d, hipEventCreate(&e1));
-  DH(d, hipEventRecord(e0, d->stream));
+  DH(d, hipEventRecord(start, d->stream));
   kFeaturesFilm<<<blocks(size_t(Wp) * Hp), 256, 0, d->stream>>>(
       static_cast<const float4*>(radiance), static_cast<const float4*>(albedo), static_cast<const float4*>(normal),
       static_cast<const float4*>(depth), W, H, Wp, Hp, d->x.a.p);
@@ -1034,7 +1044,26 @@ int ash_denoise_planes(ash_denoiser* d, const void* radiance, const void* albedo
   if (rc) return rc;
   kFilmOut<<<blocks(size_t(W) * H), 256, 0, d->stream>>>(d->outF.p, 32, W, H, Wp, static_cast<float4*>(out));
   DH(d, hipGetLastError());
-  DH(d, hipEventRecord(e1, d->stream));
+  DH(d, hipEventRecord(done, d->stream));
+  return ASH_OK;
+}
+}  // namespace ashd
+
+extern "C" {
+
+int ash_denoise_planes(ash_denoiser* d, const void* radiance, const void* albedo, const void* normal,
+                       const void* depth, uint32_t W, uint32_t H, void* out, double* ms) {
+  if (!d || !radiance || !out || !W || !H) return ASH_E_INVALID;
+  DH(d, hipSetDevice(d->device));
+  hipEvent_t e0, e1;
+  DH(d, hipEventCreate(&e0));
+  DH(d, hipEventCreate(&e1));
+  int rc = ashd::denoisePlanesAsync(d, radiance, albedo, normal, depth, W, H, out, e0, e1);
+  if (rc) {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return rc;
+  }
   DH(d, hipEventSynchronize(e1));
   float t = 0;
   DH(d, hipEventElapsedTime(&t, e0, e1));

@@ -58,10 +58,14 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", choices=["c2", "c1", "c3", "c4", "c5"], default="c2")
-    ap.add_argument("--train-iters", type=int, default=1500, help="c4: denoiser training steps")
+    ap.add_argument("--train-iters", type=int, default=2500, help="c4: denoiser training steps")
     ap.add_argument("--jobs", type=int, default=200, help="c5: jobs per arrival rate")
     ap.add_argument("--rates", default="10,50,200", help="c5: Poisson arrival rates (jobs/s)")
     ap.add_argument("--serve-policy", type=int, default=0, help="c5: server batch fill order (0 round robin, 1 oldest first)")
+    ap.add_argument("--matched", default="",
+                    help="c5 at equal relMSE: plain spp -> denoised spp map (e.g. 1:1,4:1,16:2,64:64p, from c4; "
+                         "a p suffix: no denoised spp reaches that relMSE, render plain); the server renders every "
+                         "job at the mapped spp (denoised unless p), the one-at-a-time baseline renders the plain spp")
     ap.add_argument("--scale", type=int, default=0, help="shrink the procedural scene (tests)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-frames", type=int, default=2)
@@ -289,11 +293,12 @@ def main():
 
 def run_denoiser(A, args):
     """Learned denoiser: trained on the GPU on renders of seeded procedural
-    scenes (4-spp radiance + albedo + normal + depth -> 1024-spp radiance,
-    128x128 views), then run on the 1080p procedural scene's 4-spp render.
+    scenes (1, 2, 4 and 8-spp radiance + albedo + normal + depth -> 1024-spp
+    radiance, 128x128 views), then run on the 1080p procedural scene.
     Reports training throughput, 1080p inference time and relMSE against the
-    FP64 oracle at 4096 spp (16 fixed 32x32 crops) for the noisy input, the
-    denoised image and plain 16 / 256-spp renders."""
+    FP64 oracle at 4096 spp (16 fixed 32x32 crops) for plain renders at 1 to
+    256 spp and denoised renders at 1 to 16 spp, and the equal-relMSE map:
+    for each plain spp, the fewest denoised spp that reach its relMSE."""
     rng = np.random.default_rng(0xA5A52026)
     t0 = time.time()
     scenes = []
@@ -316,14 +321,15 @@ def run_denoiser(A, args):
             eye = eye / np.linalg.norm(eye) * rng.uniform(2.0, 6.0)
             eye[1] = abs(eye[1]) + 0.3
             cam = A.camera_lookat(res, res, float(rng.uniform(30, 75)), tuple(eye), tuple(rng.uniform(-1, 1, 3) * [1, 0.3, 1]))
-            r.set_film(res, res)
-            r.render(cam, pc, 0, 4, depth_channel=st.depth_out_channel)
-            xs.append(np.concatenate([r.read(0)[..., :3], r.read(pc.diffuseOutChannel + 1)[..., :3],
-                                      r.read(pc.normalOutChannel + 1)[..., :3],
-                                      r.read(st.depth_out_channel + 1)[..., :1]], -1))
+            for spp_in in (1, 2, 4, 8):
+                r.set_film(res, res)
+                r.render(cam, pc, 0, spp_in, depth_channel=st.depth_out_channel)
+                xs.append(np.concatenate([r.read(0)[..., :3], r.read(pc.diffuseOutChannel + 1)[..., :3],
+                                          r.read(pc.normalOutChannel + 1)[..., :3],
+                                          r.read(st.depth_out_channel + 1)[..., :1]], -1))
             r.set_film(res, res)
             r.render(cam, pc, 0, spp_ref)
-            ys.append(r.read(0)[..., :3])
+            ys += [r.read(0)[..., :3]] * 4
         r.close()
     X, Y = np.stack(xs), np.stack(ys)
     log(f"[den] {len(X)} training views rendered in {time.time() - t0:.1f}s")
@@ -362,26 +368,35 @@ def run_denoiser(A, args):
         return float(np.mean((x - g["ref"]) ** 2 / (g["ref"] ** 2 + 1e-2)))
 
     out = {}
-    for spp in (4, 16, 256):
+    for spp in (1, 2, 4, 8, 16, 64, 256):
         r.set_film(w, h)
         st_r = r.render(cam, pc, 0, spp, depth_channel=st.depth_out_channel)
-        out[f"plain_{spp}spp"] = {"relmse": round(relmse(r.read(0)), 5), "render_ms": round(st_r["render_ms"], 2)}
-        if spp == 4:
+        if spp in (1, 4, 16, 64, 256):
+            out[f"plain_{spp}spp"] = {"relmse": round(relmse(r.read(0)), 5), "render_ms": round(st_r["render_ms"], 2)}
+        if spp <= 16:
             den.denoise_film(r, pc.diffuseOutChannel, pc.normalOutChannel, st.depth_out_channel, 7)  # warm
             ms = min(den.denoise_film(r, pc.diffuseOutChannel, pc.normalOutChannel, st.depth_out_channel, 7)
                      for _ in range(3))
-            out["denoised_4spp"] = {"relmse": round(relmse(r.read(7)), 5), "denoise_ms": round(ms, 2),
-                                    "total_ms": round(ms + st_r["render_ms"], 2)}
+            out[f"denoised_{spp}spp"] = {"relmse": round(relmse(r.read(7)), 5), "denoise_ms": round(ms, 2),
+                                         "total_ms": round(ms + st_r["render_ms"], 2)}
+    # equal relMSE: for each plain spp, the fewest denoised spp whose relMSE is no higher
+    matched = {}
+    for sp in (1, 4, 16, 64, 256):
+        ok = [d for d in (1, 2, 4, 8, 16) if out[f"denoised_{d}spp"]["relmse"] <= out[f"plain_{sp}spp"]["relmse"]]
+        if ok:
+            d = min(ok)
+            matched[str(sp)] = {"denoised_spp": d, "plain_ms": out[f"plain_{sp}spp"]["render_ms"],
+                                "denoised_ms": out[f"denoised_{d}spp"]["total_ms"]}
     flop = 181_000 * 1920 * 1088  # about 181 kFLOP per pixel (3x3 convs, all levels)
     res_line = {"metric": "denoiser: relMSE vs FP64 4096-spp oracle and 1080p inference time",
                 "config": {"workload": "U-Net 32/48/64/96/128, BF16 MFMA; trained on 96 views of 6 seeded "
-                                       "procedural scenes (4 spp -> 1024 spp, 128x128); inference on the "
+                                       "procedural scenes (1/2/4/8 spp -> 1024 spp, 128x128); inference on the "
                                        "1920x1080 procedural scene (5.24M triangles)",
                            "train_iters": args.train_iters, "batch": f"{nb}x{crop}x{crop}"},
                 "train_step_ms": round(t_train / args.train_iters * 1e3, 2),
                 "train_loss_first_last": [round(float(np.mean(losses[:50])), 4), round(float(np.mean(losses[-50:])), 4)],
                 "inference_tflops": round(flop / (out["denoised_4spp"]["denoise_ms"] * 1e-3) / 1e12, 1),
-                "results": out}
+                "results": out, "equal_relmse": matched}
     den.close()
     r.close()
     print(json.dumps(res_line), flush=True)
@@ -394,7 +409,10 @@ def run_serving(A, args):
     {1, 4, 16, 64}; scene uniform.  The server co-batches the jobs of a
     scene and overlaps scenes; the baseline renders the same jobs one at a
     time in arrival order (latency = FIFO queueing with the measured render
-    times)."""
+    times).  Default: both columns do the same work (half of the jobs
+    denoised).  --matched: equal relMSE -- the server renders every job at
+    the denoised spp that c4 measured to match the job's plain spp and
+    denoises it; the baseline renders the plain spp."""
     import threading
     rng = np.random.default_rng(0xA5A52026)
     scenes = []
@@ -409,6 +427,10 @@ def run_serving(A, args):
     log(f"[serve] 8 scenes loaded in {time.time() - t0:.1f}s: {[s.state.num_triangles for s in scenes]}")
     res = [(256, 256), (512, 512), (1024, 1024), (1920, 1080)]
     rates = [float(x) for x in args.rates.split(",")]
+    matched = None
+    if args.matched:
+        matched = {int(a): (int(b.rstrip("p")), not b.endswith("p"))
+                   for a, b in (kv.split(":") for kv in args.matched.split(","))}
 
     def make_jobs(n):
         jobs = []
@@ -421,7 +443,11 @@ def run_serving(A, args):
             eye[1] = abs(eye[1]) + 0.5
             cam = A.camera_lookat(w, h, float(rng.uniform(35, 70)), tuple(eye), (0.0, 0.3, 0.0))
             _, pc = scenes[sid].shot(0)
-            jobs.append((sid, cam, pc, w, h, spp, bool(rng.integers(0, 2))))
+            dn = bool(rng.integers(0, 2))
+            if matched:
+                jobs.append((sid, cam, pc, w, h, matched[spp][0], matched[spp][1], spp, False))
+            else:
+                jobs.append((sid, cam, pc, w, h, spp, dn, spp, dn))
         return jobs
 
     srv = A.Server(0, hbm_budget_bytes=16 << 30, queue_paths=1 << 21, policy=args.serve_policy)
@@ -451,7 +477,10 @@ def run_serving(A, args):
                                   "arrivals; resolution {256^2,512^2,1024^2,1920x1080} w {.4,.3,.2,.1}; "
                                   "spp {1,4,16,64}; denoise flag on 50 % of jobs (U-Net on the job's planes)",
                       "jobs_per_rate": args.jobs, "queue_paths": 1 << 21,
-                      "policy": ["round robin", "oldest first"][args.serve_policy]},
+                      "policy": ["round robin", "oldest first"][args.serve_policy],
+                      "mode": ("equal relMSE: server renders plain spp S at the spp map " + args.matched +
+                               " (denoised; p = plain) measured by c4; baseline renders S plain") if matched else
+                              "same work in both columns (denoise flag on half of the jobs)"},
            "rates": []}
     for lam in rates:
         jobs = make_jobs(args.jobs)
@@ -481,7 +510,7 @@ def run_serving(A, args):
                   "p99_ms": round(float(np.percentile(lat, 99)), 1), "mpaths_per_s": round(paths / span / 1e6, 1)}
         # baseline: the same jobs rendered one at a time (FIFO)
         service = []
-        for sid, cam, pc, w, h, spp, dn in jobs:
+        for sid, cam, pc, w, h, _, _, spp, dn in jobs:
             r = rs[sid]
             dch = scenes[sid].state.depth_out_channel
             ts = time.perf_counter()
