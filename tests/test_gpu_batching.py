@@ -52,6 +52,37 @@ def test_graph_replays_equal_eager(renderer, spheres_small):
     assert st["extension_rays"] == st0["extension_rays"] and st["shadow_rays"] == st0["shadow_rays"]
 
 
+def test_fused_trace_launch_counts(renderer, spheres_small):
+    """The default scheme traces each bounce's shadow rays with the next
+    bounce's extension rays in one launch: same images and ray counts as the
+    separate launches, and the fused launches' own counters add up (every
+    path's first extension ray is traced before the first fused launch)."""
+    s = _load(renderer, spheres_small)
+    cam, pc = s.shot(0)
+    pc.maxPathDepth = 4
+    kw = dict(queue_paths=1 << 16, count_traversal=True)
+    renderer.set_film(*s.film)
+    fused = renderer.render(cam, pc, 0, 2, traversal=0, **kw)
+    img_f = [renderer.read(i) for i in (0, 1, 8)]
+    renderer.set_film(*s.film)
+    sep = renderer.render(cam, pc, 0, 2, traversal=1, **kw)
+    img_s = [renderer.read(i) for i in (0, 1, 8)]
+    log(f"fused: {fused}")
+    for a, b in zip(img_f, img_s):
+        np.testing.assert_array_equal(a, b)
+    assert fused["extension_rays"] == sep["extension_rays"] and fused["shadow_rays"] == sep["shadow_rays"]
+    assert sep["trace_launches"] == 0
+    assert fused["batches"] > 1 and fused["extend_launches"] == fused["batches"]
+    assert fused["trace_launches"] == fused["batches"] * (pc.maxPathDepth - 1)
+    assert fused["trace_ext_rays"] == fused["extension_rays"] - fused["paths"]
+    assert 0 < fused["trace_shadow_rays"] <= fused["shadow_rays"]
+    assert 0 < fused["trace_ext_node_visits"] < fused["ext_node_visits"]
+    assert 0 < fused["trace_shadow_node_visits"] <= fused["shadow_node_visits"]
+    renderer.set_film(*s.film)
+    tm = renderer.render(cam, pc, 0, 2, traversal=0, queue_paths=1 << 16, timing=True)
+    assert tm["trace_ms"] > 0 and tm["trace_launches"] == fused["trace_launches"]
+
+
 def test_batch_plans_are_equivalent(renderer, multishot_scene):
     """64 shots x 2 frames split three ways: several frames per batch, groups
     of whole shots per batch, and pixel runs of one shot per batch."""

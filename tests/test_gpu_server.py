@@ -32,10 +32,11 @@ def _jobs(scenes, rng):
     return out
 
 
-def test_server_jobs_equal_single_renders(box_scene, spheres_small):
+@pytest.mark.parametrize("policy", [A.SERVE_ROUND_ROBIN, A.SERVE_OLDEST_FIRST])
+def test_server_jobs_equal_single_renders(box_scene, spheres_small, policy):
     scenes = [A.HostScene(box_scene), A.HostScene(spheres_small)]
     # a small batch so that jobs share batches and large jobs span several
-    srv = A.Server(0, hbm_budget_bytes=64 << 20, queue_paths=1 << 16, batches_ahead=2)
+    srv = A.Server(0, hbm_budget_bytes=64 << 20, queue_paths=1 << 16, batches_ahead=2, policy=policy)
     ids = [srv.add_scene(s) for s in scenes]
     assert ids == [0, 1]
     jobs = _jobs(scenes, np.random.default_rng(5))
