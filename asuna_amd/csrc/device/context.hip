@@ -496,7 +496,7 @@ static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots
     maxDepth = std::max(maxDepth, hostShots[s].pc.maxPathDepth);
     if (hostShots[s].width * hostShots[s].height != P) return fail(c, ASH_E_INVALID, "shots must share one film size");
   }
-  uint64_t Q = o && o->queue_paths ? o->queue_paths : (1u << 22);
+  uint64_t Q = o && o->queue_paths ? o->queue_paths : (1u << 25);
   if (Q > (1u << 30)) return fail(c, ASH_E_INVALID, "queue_paths above 2^30");
   uint32_t bpc = o && o->blocks_per_cu ? o->blocks_per_cu : 8;
   if (bpc > 64) return fail(c, ASH_E_INVALID, "blocks_per_cu above 64");
@@ -916,6 +916,7 @@ int ash_autotune(ash_ctx* c, uint32_t nShots, const GpuCamera* cams, const GpuPu
   double best = 1e300, worst = 0;
   uint32_t bq = 0, bb = 0, n = 0;
   uint32_t bl = 0, bs = o.traversal;
+  uint64_t batches = 0;
   auto probe = [&](double& ms) -> int {
     ash_stats st;
     for (int rep = 0; rep < 2; ++rep) {
@@ -924,11 +925,14 @@ int ash_autotune(ash_ctx* c, uint32_t nShots, const GpuCamera* cams, const GpuPu
       if (rc) return rc;
       ms = st.render_ms;
     }
+    batches = st.batches;
     ++n;
     worst = std::max(worst, ms);
     return ASH_OK;
   };
-  for (uint32_t lq = 18; lq <= 22; ++lq)
+  // queue sizes from 2^18 up; once the probe fits in one batch, larger
+  // queues would launch the same work
+  for (uint32_t lq = 18; lq <= 26; ++lq) {
     for (uint32_t bpc : {4u, 8u, 16u})
       for (uint32_t lds : {uint32_t(kStackLdsMin), uint32_t(kStackLds)}) {
         o.queue_paths = 1u << lq;
@@ -938,6 +942,8 @@ int ash_autotune(ash_ctx* c, uint32_t nShots, const GpuCamera* cams, const GpuPu
         if (int rc = probe(ms)) return rc;
         if (ms < best) best = ms, bq = o.queue_paths, bb = bpc, bl = lds;
       }
+    if (batches <= 1) break;
+  }
   o.queue_paths = bq, o.blocks_per_cu = bb, o.lds_stack = bl;
   const uint32_t first = o.traversal;
   for (uint32_t sch = 0; sch <= uint32_t(kSchemeRefill); ++sch) {

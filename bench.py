@@ -138,8 +138,9 @@ def main():
     tune = None
     qp, bpc, lds, trav = args.queue_paths, args.blocks_per_cu, args.lds_stack, args.traversal
     if not args.no_autotune and not args.queue_paths:
-        # probe with enough frames for several wavefront batches at the largest queue size
-        probe = max(1, min(spp, (4 << 22) // (w * h * nshots)))
+        # probe with the frames of a whole step when they fit in 2^25 paths, so that every queue
+        # size is tried on the batch split the timed steps will use
+        probe = max(1, min(spp, (1 << 25) // (w * h * nshots)))
         opts, tune = r.autotune(list(cams), list(pcs), probe_frames=probe, traversal=args.traversal)
         tune["probe_frames"] = probe
         qp, bpc, lds, trav = opts.queue_paths, opts.blocks_per_cu, opts.lds_stack, opts.traversal

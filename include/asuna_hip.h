@@ -292,7 +292,8 @@ typedef struct ash_build_info {
 
 /* Render options beyond the push constants. */
 typedef struct ash_render_opts {
-  uint32_t queue_paths;    /* paths per wavefront batch, the tile size of the work split (0 = 2^22) */
+  uint32_t queue_paths;    /* paths per wavefront batch, the tile size of the work split (0 = 2^25; the
+                              wavefront is sized by the largest batch a render needs) */
   uint32_t blocks_per_cu;  /* persistent traversal blocks (256 threads) per CU (0 = 8) */
   uint32_t timing;         /* 1 = time each stage with HIP events */
   int32_t depthOutChannel; /* extension: AOV channel receiving first-hit t, -1 = none */
@@ -368,7 +369,8 @@ int ash_read_shot_output(ash_ctx* ctx, uint32_t shot, int32_t image, float* host
 /* Autotuner (no counterpart in the reference, whose dispatch shape is fixed
  * at one thread per pixel): on a probe render of `probe_frames` frames of
  * the given shots (each candidate rendered twice, the second timed) it
- * sweeps the wavefront batch size queue_paths over 2^18..2^22, the
+ * sweeps the wavefront batch size queue_paths over 2^18, 2^19, ... up to
+ * the first size at which the probe fits in one batch (at most 2^26), the
  * persistent traversal grid blocks_per_cu over {4, 8, 16} and the LDS share
  * of the closest-hit stack over {8, 16} with the traversal fetch scheme of
  * `inout`, then tries the other fetch schemes at the best of those, and

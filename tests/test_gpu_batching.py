@@ -139,8 +139,11 @@ def test_autotune_picks_a_candidate_and_keeps_results(renderer, multishot_scene)
     cams, pcs = zip(*[s.shot(i) for i in range(8)])
     opts, rep = renderer.autotune(list(cams), list(pcs), probe_frames=1)
     log(f"autotune: {rep}")
-    assert rep["candidates"] == 32
-    assert opts.queue_paths in [1 << k for k in range(18, 23)] and opts.blocks_per_cu in (4, 8, 16)
+    # queue sizes 2^18, 2^19, ... up to the first that holds the whole probe in one batch
+    w, h = s.film
+    top = next(k for k in range(18, 27) if (1 << k) >= w * h * 8 or k == 26)
+    assert rep["candidates"] == 6 * (top - 17) + 2
+    assert opts.queue_paths in [1 << k for k in range(18, top + 1)] and opts.blocks_per_cu in (4, 8, 16)
     assert opts.lds_stack in (8, 16) and rep["lds_stack"] == opts.lds_stack
     assert opts.traversal in (0, 1, 2) and rep["traversal"] == opts.traversal
     assert 0 < rep["best_ms"] <= rep["worst_ms"]
