@@ -362,7 +362,7 @@ int ash_server_create(int device, const ash_server_opts* opts, ash_server** out)
   s->device = device;
   if (opts) s->o = *opts;
   if (!s->o.hbm_budget_bytes) s->o.hbm_budget_bytes = 8ull << 30;
-  if (!s->o.queue_paths) s->o.queue_paths = 1u << 21;
+  if (!s->o.queue_paths) s->o.queue_paths = 1u << 23;
   if (!s->o.blocks_per_cu) s->o.blocks_per_cu = 8;
   if (!s->o.max_jobs_per_batch) s->o.max_jobs_per_batch = 256;
   if (!s->o.batches_ahead) s->o.batches_ahead = 2;

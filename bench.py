@@ -62,6 +62,7 @@ def main():
     ap.add_argument("--jobs", type=int, default=200, help="c5: jobs per arrival rate")
     ap.add_argument("--rates", default="10,50,200", help="c5: Poisson arrival rates (jobs/s)")
     ap.add_argument("--serve-policy", type=int, default=0, help="c5: server batch fill order (0 round robin, 1 oldest first)")
+    ap.add_argument("--serve-queue", type=int, default=1 << 23, help="c5: server paths per batch")
     ap.add_argument("--matched", default="",
                     help="c5 at equal relMSE: plain spp -> denoised spp map (e.g. 1:1,4:1,16:2,64:64p, from c4; "
                          "a p suffix: no denoised spp reaches that relMSE, render plain); the server renders every "
@@ -451,7 +452,7 @@ def run_serving(A, args):
                 jobs.append((sid, cam, pc, w, h, spp, dn, spp, dn))
         return jobs
 
-    srv = A.Server(0, hbm_budget_bytes=16 << 30, queue_paths=1 << 21, policy=args.serve_policy)
+    srv = A.Server(0, hbm_budget_bytes=16 << 30, queue_paths=args.serve_queue, policy=args.serve_policy)
     den = A.Denoiser(0, seed=1)  # weights do not change the cost; c4 trains one
     srv.set_denoiser(den)
     for s in scenes:
@@ -477,7 +478,7 @@ def run_serving(A, args):
            "config": {"workload": "8 resident scenes (1.2e4, 8.2e4, 1.0e6, 5.2e6 triangles x 2 seeds); Poisson "
                                   "arrivals; resolution {256^2,512^2,1024^2,1920x1080} w {.4,.3,.2,.1}; "
                                   "spp {1,4,16,64}; denoise flag on 50 % of jobs (U-Net on the job's planes)",
-                      "jobs_per_rate": args.jobs, "queue_paths": 1 << 21,
+                      "jobs_per_rate": args.jobs, "queue_paths": args.serve_queue,
                       "policy": ["round robin", "oldest first"][args.serve_policy],
                       "mode": ("equal relMSE: server renders plain spp S at the spp map " + args.matched +
                                " (denoised; p = plain) measured by c4; baseline renders S plain") if matched else

@@ -29,7 +29,7 @@ typedef struct ash_server ash_server;
 
 typedef struct ash_server_opts {
   uint64_t hbm_budget_bytes; /* arena for admitted jobs' output images (0 = 8 GiB) */
-  uint32_t queue_paths;      /* paths per wavefront batch of one scene (0 = 2^21) */
+  uint32_t queue_paths;      /* paths per wavefront batch of one scene (0 = 2^23) */
   uint32_t blocks_per_cu;    /* persistent traversal blocks per CU (0 = 8) */
   uint32_t max_jobs_per_batch; /* shot-table entries per batch (0 = 256) */
   uint32_t batches_ahead;    /* batches enqueued per scene before waiting (0 = 2, max 3) */
