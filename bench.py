@@ -44,6 +44,7 @@ HBM_PEAK_GBS = 8000.0
 # box planes + child refs) of its 128 B line; quantized 4-wide node 64 B
 NODE_BYTES = {(2, 64): 64, (4, 128): 112, (4, 64): 64}
 TRI_BYTES, RAY_IO_BYTES = 48, 68
+L2_BYTES = 4 << 20  # L2 of one XCD
 # shadow ray I/O: origin+tmax, direction+path id, and the pending NEE radiance
 SHADOW_IO_BYTES = 48
 
@@ -251,6 +252,11 @@ def main():
                 "bytes_per_launch": round(bytes_per_launch), "node_bytes": node_bytes,
                 "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 3),
                 "tris_per_ray": round(tris_per_ray, 3), **per_launch}
+    tree_bytes = binfo["num_wide_nodes"] * 128 + binfo["num_triangles"] * TRI_BYTES if binfo["branching"] == 4 \
+        else binfo["num_nodes"] * 64 + binfo["num_triangles"] * TRI_BYTES
+    if tree_bytes < L2_BYTES:
+        roofline["note"] = (f"tree and triangles ({tree_bytes} B) fit in one XCD's 4 MiB L2: the algorithmic bytes "
+                            f"are served from cache, so frac can exceed 1 and does not measure HBM use")
 
     cpu = None
     if not args.no_cpu_baseline:
