@@ -460,6 +460,7 @@ __device__ bool traceAny4(const SceneView& sc, const TRay& r, float tmin, float 
 // Pop for the closest-hit walks: the next stacked entry whose entry
 // distance is not beyond the current hit (kDoneRef when none is left).
 constexpr int32_t kDoneRef = -0x7fffffff - 1;  // walk finished (leaf refs stay above: first < 2^28)
+constexpr int kShadowRefill = 32;  // idle lanes of a wave that trigger a shadow-ray refill
 
 __device__ __forceinline__ int popClosest(Stack& st, int best, float bt) {
   while (st.sp > 0) {
@@ -840,13 +841,121 @@ __device__ __forceinline__ void shadowLoop(const SceneView& sc, const Wavefront&
   }
 }
 
+// Any-hit walk in rounds (4-wide float tree), the counterpart of dynRound:
+// a lane descends until it holds a leaf (one more is postponed while the
+// wave still has lanes at inner nodes), then the leaves are tested.  True
+// when the ray is finished; s.hit tells whether it was occluded.
+struct AnyState {
+  int node, post;
+  bool hit;
+};
+template <bool COUNT>
+__device__ __forceinline__ bool anyLeaf(const SceneView& sc, const TRay& r, float tmax, int ref, uint32_t& nT) {
+  uint32_t first, cnt;
+  decodeLeaf(ref, first, cnt);
+  for (uint32_t k = first; k < first + cnt; ++k) {
+    const float4* tp = sc.tris + size_t(k) * 3;
+    float t, u, v;
+    if (COUNT) ++nT;
+    if (hitTri(r, tp[0], tp[1], tp[2], 0.0f, tmax, t, u, v)) return true;
+  }
+  return false;
+}
+template <bool COUNT>
+__device__ __forceinline__ bool anyRound(const SceneView& sc, const TRay& r, float tmax, AnyState& s, Stack& st,
+                                         uint32_t& nN, uint32_t& nT) {
+  auto next = [&]() { return st.sp > 0 ? st.pop() : kDoneRef; };
+  while (s.node >= 0) {
+    if (s.node != kEmptyRef) {
+      if (COUNT) ++nN;
+      Kids k;
+      testNode4(sc.nodes4 + size_t(s.node) * 8, r, 0.0f, tmax, k);
+      if (k.n > 0) {
+        if (k.n > 3) st.push(k.c[3]);
+        if (k.n > 2) st.push(k.c[2]);
+        if (k.n > 1) st.push(k.c[1]);
+        s.node = k.c[0];
+      } else {
+        s.node = next();
+      }
+    } else {
+      s.node = next();
+    }
+    if (s.node < 0 && s.node != kDoneRef && s.post == kDoneRef) {
+      s.post = s.node;
+      s.node = next();
+    }
+    if (__ballot(s.post == kDoneRef) == 0) break;
+  }
+  if (s.post != kDoneRef) {
+    if (anyLeaf<COUNT>(sc, r, tmax, s.post, nT)) {
+      s.hit = true;
+      return true;
+    }
+    s.post = kDoneRef;
+  }
+  if (s.node < 0 && s.node != kDoneRef) {
+    if (anyLeaf<COUNT>(sc, r, tmax, s.node, nT)) {
+      s.hit = true;
+      return true;
+    }
+    s.node = next();
+  }
+  return s.node == kDoneRef && s.post == kDoneRef;
+}
+
+// Shadow rays with per-lane refill (the fetch of extendDynLoop over the
+// shadow stream's queue slices): a lane whose ray has finished takes the
+// next one as soon as REFILL lanes of its wave are idle, instead of the
+// wave waiting for the slowest of its 64 rays.
+template <bool COUNT, int REFILL>
+__device__ __forceinline__ void shadowDynLoop(const SceneView& sc, const Wavefront& w, Stack& st, int lane,
+                                              uint32_t& nNodes, uint32_t& nTris) {
+  ChunkFetch f;
+  chunkInit<true>(f, &w.ctr[kXcdCtr + kXcds * 32], w.ctr[2]);
+  bool active = false, drained = false;
+  int pid = -1;
+  float tmax = 0.0f;
+  TRay r;
+  AnyState s;
+  for (;;) {
+    const uint32_t nIdle = uint32_t(__popcll(__ballot(!active)));
+    if (!drained && nIdle >= uint32_t(REFILL)) {
+      uint32_t i = 0;
+      if (fetchIdle(f, !active, lane, drained, i)) {
+        const float4 so = w.shO[i], sd = w.shD[i];
+        setupRay(r, xyz(so), xyz(sd));
+        tmax = so.w;
+        pid = __float_as_int(sd.w);
+        s.node = 0;
+        s.post = kDoneRef;
+        s.hit = false;
+        st.sp = 0;
+        active = true;
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (drained) break;
+      continue;
+    }
+    if (active && anyRound<COUNT>(sc, r, tmax, s, st, nNodes, nTris)) {
+      active = false;
+      if (!s.hit) {
+        float4 L = w.rad[pid], d = w.ld[pid];
+        w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
+      }
+    }
+  }
+}
+
 template <bool COUNT, int TREE, bool XCD>
 __global__ __launch_bounds__(kTraceBlock) void kShadow(SceneView sc, Wavefront w) {
   __shared__ int lds[kStackLds * kTraceBlock];
   Stack st;
   initStack<false>(st, lds, nullptr, w);
   uint32_t nNodes = 0, nTris = 0;
-  shadowLoop<COUNT, TREE, XCD>(sc, w, st, __lane_id(), nNodes, nTris);
+  if (TREE == 1 && XCD) shadowDynLoop<COUNT, kShadowRefill>(sc, w, st, __lane_id(), nNodes, nTris);
+  else shadowLoop<COUNT, TREE, XCD>(sc, w, st, __lane_id(), nNodes, nTris);
   if (COUNT) {
     atomicAdd(&w.cnt64[2], (unsigned long long)nNodes);
     atomicAdd(&w.cnt64[3], (unsigned long long)nTris);
@@ -877,7 +986,7 @@ __global__ __launch_bounds__(kTraceBlock) void kTraceDyn(SceneView sc, Wavefront
   extendDynLoop<COUNT, REFILL>(sc, w, cur, st, lane, eN, eT);
   Stack ss;
   initStack<false, NL>(ss, lds, nullptr, w);
-  shadowLoop<COUNT, 1, true>(sc, w, ss, lane, sN, sT);
+  shadowDynLoop<COUNT, kShadowRefill>(sc, w, ss, lane, sN, sT);
   if (COUNT) {
     const unsigned long long v[4] = {eN, eT, sN, sT};
     for (int k = 0; k < 4; ++k) {
