@@ -511,15 +511,32 @@ __device__ __forceinline__ uint32_t findUnit(const WorkUnit* u, uint32_t n, uint
   return lo;
 }
 
+// Pixel of position k of a film's pixel order: 8x8 tiles in row-major
+// order over the largest multiple-of-8 rectangle, then the right strip and
+// the bottom strip row by row.  Work units are runs of this order, so a
+// wave of 64 consecutive paths covers one tile: its camera rays, and the
+// bounces from where they land, start close together.
+__device__ __forceinline__ uint32_t runPixel(uint32_t k, uint32_t W, uint32_t H) {
+  const uint32_t Wt = W & ~7u, Ht = H & ~7u, nFull = Wt * Ht;
+  if (k < nFull) {
+    const uint32_t t = k >> 6, j = k & 63u, tw = Wt >> 3;
+    return ((t / tw) * 8u + (j >> 3)) * W + (t % tw) * 8u + (j & 7u);
+  }
+  const uint32_t r = k - nFull, rw = W - Wt;
+  if (r < rw * Ht) return (r / rw) * W + Wt + r % rw;
+  return Ht * W + (r - rw * Ht);
+}
+
 __global__ __launch_bounds__(256) void kRaygen(BatchDesc b, Wavefront w) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= b.n) return;
   const WorkUnit un = b.units[findUnit<false>(b.units, b.nUnits, p)];
   uint32_t rel = p - un.base;
-  uint32_t fslot = rel / un.np, pix = un.pix0 + rel % un.np;
+  uint32_t fslot = rel / un.np;
   int32_t frame = un.f0 + int32_t(fslot);
   uint32_t shot = un.shot;
   const ShotParams& sp = b.shotTab[shot];
+  const uint32_t pix = runPixel(un.pix0 + rel % un.np, sp.width, sp.height);
   uint32_t x = pix % sp.width, y = pix / sp.width;
   uint32_t seed = xxhash32Seed(x, y, uint32_t(frame));
   float jx, jy;
@@ -2655,8 +2672,8 @@ __global__ __launch_bounds__(256) void kAccumulate(BatchDesc b, Wavefront w) {
   if (t >= b.nAcc) return;
   const WorkUnit un = b.units[findUnit<true>(b.units, b.nUnits, t)];
   uint32_t k = t - un.accBase;
-  uint32_t pix = un.pix0 + k;
   const ShotParams& sp = b.shotTab[un.shot];
+  const uint32_t pix = runPixel(un.pix0 + k, sp.width, sp.height);
   size_t plane = size_t(sp.width) * sp.height;
   float4* img0 = sp.images;
   float4* img8 = sp.images + 8 * plane;
@@ -2708,7 +2725,7 @@ __global__ __launch_bounds__(256) void kFirstHitState(SceneView sc, const ShotPa
     o.normal[0] = h.N.x, o.normal[1] = h.N.y, o.normal[2] = h.N.z;
     o.pos[0] = h.pos.x, o.pos[1] = h.pos.y, o.pos[2] = h.pos.z;
   }
-  out[p] = o;
+  out[runPixel(p, sp->width, sp->height)] = o;
 }
 
 }  // namespace
