@@ -2485,8 +2485,7 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
     c.seed = info.x;
     c.depth = info.y & 0xffffu;
     c.bflags = info.y >> 16;
-    c.ro = xyz(w.ro[pid]);
-    c.rd = xyz(w.rd[pid]);
+    c.rd = xyz(w.rd[pid]);  // the origin is read only for light hits (MIS distance)
     c.T = xyz(thr);
     c.bpdf = thr.w;
     c.L = xyz(rad);
@@ -2519,6 +2518,7 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
       bool aov = frame0 && c.depth == 1;
       if (aov && sp.depthChannel >= 0) writeAov(c, sp.depthChannel, mk1(hf.x), pix);
       if (h.lightId >= 0) {
+        c.ro = xyz(w.ro[pid]);
         hitLight(c, h.lightId, h.pos);
       } else {
         if (MATS == kMatStaged) {
@@ -2569,9 +2569,10 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
   }
   if (ni >= 0) w.queue[1 - cur][ni] = pid;
   if (!active) return;
+  w.rad[pid] = f4(c.L, rad.w);
+  if (!cont) return;  // a finished path is read again only for its radiance
   w.info[pid] = make_uint4(c.seed, c.depth | (c.bflags << 16), info.z, info.w);
   w.thr[pid] = f4(c.T, c.bpdf);
-  w.rad[pid] = f4(c.L, rad.w);
   w.ro[pid] = f4(c.ro, 0.f);
   w.rd[pid] = f4(c.rd, 0.f);
 }
