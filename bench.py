@@ -252,6 +252,13 @@ def main():
                 "bytes_per_launch": round(bytes_per_launch), "node_bytes": node_bytes,
                 "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 3),
                 "tris_per_ray": round(tris_per_ray, 3), **per_launch}
+    if traffic:
+        # HBM bytes rocprofv3 counted per launch (profiles/), over the same launch time
+        roofline["hbm_gbs"] = round(traffic / (ms_per_launch * 1e-3) / 1e9, 1)
+        roofline["hbm_frac"] = round(roofline["hbm_gbs"] / HBM_PEAK_GBS, 4)
+        if achieved > HBM_PEAK_GBS:
+            roofline["note"] = ("frac above 1: the L2 and MALL serve part of the algorithmic bytes; hbm_gbs is the "
+                                "HBM traffic rocprofv3 counted for this kernel over the same launch time")
     tree_bytes = binfo["num_wide_nodes"] * 128 + binfo["num_triangles"] * TRI_BYTES if binfo["branching"] == 4 \
         else binfo["num_nodes"] * 64 + binfo["num_triangles"] * TRI_BYTES
     if tree_bytes < L2_BYTES:
