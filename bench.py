@@ -59,7 +59,8 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", choices=["c2", "c1", "c3", "c4", "c5"], default="c2")
-    ap.add_argument("--train-iters", type=int, default=2500, help="c4: denoiser training steps")
+    ap.add_argument("--train-iters", type=int, default=4000, help="c4: denoiser training steps")
+    ap.add_argument("--train-views", type=int, default=64, help="c4: training views per scene (six scenes)")
     ap.add_argument("--jobs", type=int, default=200, help="c5: jobs per arrival rate")
     ap.add_argument("--rates", default="10,50,200", help="c5: Poisson arrival rates (jobs/s)")
     ap.add_argument("--serve-policy", type=int, default=0, help="c5: server batch fill order (0 round robin, 1 oldest first)")
@@ -331,7 +332,7 @@ def run_denoiser(A, args):
         r.build()
         st = s.state
         _, pc = s.shot(0)
-        for _ in range(16):
+        for _ in range(args.train_views):
             eye = rng.normal(size=3)
             eye = eye / np.linalg.norm(eye) * rng.uniform(2.0, 6.0)
             eye[1] = abs(eye[1]) + 0.3
@@ -404,9 +405,9 @@ def run_denoiser(A, args):
                                 "denoised_ms": out[f"denoised_{d}spp"]["total_ms"]}
     flop = 181_000 * 1920 * 1088  # about 181 kFLOP per pixel (3x3 convs, all levels)
     res_line = {"metric": "denoiser: relMSE vs FP64 4096-spp oracle and 1080p inference time",
-                "config": {"workload": "U-Net 32/48/64/96/128, BF16 MFMA; trained on 96 views of 6 seeded "
-                                       "procedural scenes (1/2/4/8 spp -> 1024 spp, 128x128); inference on the "
-                                       "1920x1080 procedural scene (5.24M triangles)",
+                "config": {"workload": f"U-Net 32/48/64/96/128, BF16 MFMA; trained on {6 * args.train_views} views "
+                                       "of 6 seeded procedural scenes (1/2/4/8 spp -> 1024 spp, 128x128); inference "
+                                       "on the 1920x1080 procedural scene (5.24M triangles)",
                            "train_iters": args.train_iters, "batch": f"{nb}x{crop}x{crop}"},
                 "train_step_ms": round(t_train / args.train_iters * 1e3, 2),
                 "train_loss_first_last": [round(float(np.mean(losses[:50])), 4), round(float(np.mean(losses[-50:])), 4)],
