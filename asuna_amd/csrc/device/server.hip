@@ -157,7 +157,7 @@ static int launchBatch(ash_server* s, SceneSlot& sc, bool& launched) {
   std::vector<Job*> batchJobs;
   const size_t N = sc.active.size();
   for (size_t k = 0; k < N && n < Q && nu < maxJobs; ++k) {
-    Job* j = sc.active[s->o.policy == ASH_SERVE_OLDEST_FIRST ? k : (sc.rr + k) % N];
+    Job* j = sc.active[s->o.policy != ASH_SERVE_ROUND_ROBIN ? k : (sc.rr + k) % N];
     uint32_t room = Q - n;
     WorkUnit u;
     std::memset(&u, 0, sizeof u);
@@ -313,9 +313,25 @@ static void schedulerLoop(ash_server* s) {
       s->pending.pop_front();
       progress = true;
     }
-    // keep every scene's stream fed
+    // keep the scenes' streams fed: all of them, or (OLDEST_GLOBAL) in the
+    // order of their oldest job while the device holds fewer than
+    // 2 x queue_paths paths
+    std::vector<SceneSlot*> order;
+    uint64_t flying = 0, cap = ~0ull;
     for (SceneSlot* sc : s->scenes) {
-      while (sc->inflight.size() < ahead) {
+      order.push_back(sc);
+      for (const Batch& bt : sc->inflight) flying += bt.paths;
+    }
+    if (s->o.policy == ASH_SERVE_OLDEST_GLOBAL) {
+      cap = 2ull * s->o.queue_paths;
+      order.erase(std::remove_if(order.begin(), order.end(), [](SceneSlot* sc) { return sc->active.empty(); }),
+                  order.end());
+      std::stable_sort(order.begin(), order.end(), [](SceneSlot* a, SceneSlot* b) {
+        return a->active.front()->tSubmit < b->active.front()->tSubmit;
+      });
+    }
+    for (SceneSlot* sc : order) {
+      while (sc->inflight.size() < ahead && flying < cap) {
         bool launched = false;
         int rc = launchBatch(s, *sc, launched);
         if (rc) {
@@ -324,10 +340,11 @@ static void schedulerLoop(ash_server* s) {
           break;
         }
         if (!launched) break;
+        flying += sc->inflight.back().paths;
         progress = true;
       }
-      any = any || !sc->inflight.empty();
     }
+    for (SceneSlot* sc : s->scenes) any = any || !sc->inflight.empty();
     any = any || !s->denoising.empty();
     double t = s->now();
     if (any && s->busySince < 0) s->busySince = t;
@@ -368,7 +385,7 @@ int ash_server_create(int device, const ash_server_opts* opts, ash_server** out)
   if (!s->o.batches_ahead) s->o.batches_ahead = 2;
   s->o.batches_ahead = std::min<uint32_t>(s->o.batches_ahead, kRing - 1);
   if (s->o.queue_paths < 4096 || s->o.queue_paths > (1u << 26) || s->o.blocks_per_cu > 64 ||
-      s->o.max_jobs_per_batch > 65535 || s->o.policy > ASH_SERVE_OLDEST_FIRST) {
+      s->o.max_jobs_per_batch > 65535 || s->o.policy > ASH_SERVE_OLDEST_GLOBAL) {
     delete s;
     return ASH_E_INVALID;
   }

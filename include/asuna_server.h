@@ -41,8 +41,11 @@ typedef struct ash_server_opts {
  * starts at the next job in turn (short jobs finish early, long ones share
  * the device with every newcomer); OLDEST_FIRST: each batch takes all it can
  * from the oldest job and backfills the rest from younger ones (long jobs
- * keep their place in line). */
-enum { ASH_SERVE_ROUND_ROBIN = 0, ASH_SERVE_OLDEST_FIRST = 1 };
+ * keep their place in line within their scene); OLDEST_GLOBAL: as
+ * OLDEST_FIRST, and the scenes are fed in the order of their oldest job with
+ * at most 2 x queue_paths paths in flight on the device, so the oldest work
+ * shares the device with little else. */
+enum { ASH_SERVE_ROUND_ROBIN = 0, ASH_SERVE_OLDEST_FIRST = 1, ASH_SERVE_OLDEST_GLOBAL = 2 };
 
 typedef struct ash_job_desc {
   uint32_t scene;   /* id returned by ash_server_add_scene */

@@ -32,7 +32,7 @@ def _jobs(scenes, rng):
     return out
 
 
-@pytest.mark.parametrize("policy", [A.SERVE_ROUND_ROBIN, A.SERVE_OLDEST_FIRST])
+@pytest.mark.parametrize("policy", [A.SERVE_ROUND_ROBIN, A.SERVE_OLDEST_FIRST, A.SERVE_OLDEST_GLOBAL])
 def test_server_jobs_equal_single_renders(box_scene, spheres_small, policy):
     scenes = [A.HostScene(box_scene), A.HostScene(spheres_small)]
     # a small batch so that jobs share batches and large jobs span several
