@@ -61,10 +61,11 @@ def main():
     ap.add_argument("--config", choices=["c2", "c1", "c3", "c4", "c5"], default="c2")
     ap.add_argument("--train-iters", type=int, default=4000, help="c4: denoiser training steps")
     ap.add_argument("--train-views", type=int, default=64, help="c4: training views per scene (six scenes)")
-    ap.add_argument("--jobs", type=int, default=200, help="c5: jobs per arrival rate")
+    ap.add_argument("--jobs", type=int, default=600, help="c5: jobs per arrival rate")
     ap.add_argument("--rates", default="10,50,200", help="c5: Poisson arrival rates (jobs/s)")
     ap.add_argument("--serve-policy", type=int, default=0, help="c5: server batch fill order (0 round robin, 1 oldest first, 2 oldest first across scenes)")
     ap.add_argument("--serve-queue", type=int, default=1 << 23, help="c5: server paths per batch")
+    ap.add_argument("--jobs-log", default="", help="c5: write one JSON line per job (both columns) to this file")
     ap.add_argument("--matched", default="",
                     help="c5 at equal relMSE: plain spp -> denoised spp map (e.g. 1:1,4:1,16:2,64:64p, from c4; "
                          "a p suffix: no denoised spp reaches that relMSE, render plain); the server renders every "
@@ -539,6 +540,14 @@ def run_serving(A, args):
         for a, sv in zip(arrivals, service):
             finish = max(finish, a) + sv
             blat.append((finish - a) * 1e3)
+        if args.jobs_log:
+            with open(args.jobs_log, "a") as f:
+                for k, (sid, _, _, w, h, sspp, sdn, spp, dn) in enumerate(jobs):
+                    f.write(json.dumps({"rate": lam, "job": k, "arrival_ms": round(float(arrivals[k]) * 1e3, 2),
+                                        "scene": sid, "w": w, "h": h, "spp": spp, "server_spp": sspp,
+                                        "server_denoise": sdn, "server_ms": round(lat[k], 2),
+                                        "single_ms": round(blat[k], 2),
+                                        "single_service_ms": round(service[k] * 1e3, 2)}) + "\n")
         single = {"jobs_per_s": round(len(jobs) / max(finish, arrivals[-1]), 2),
                   "p50_ms": round(float(np.percentile(blat, 50)), 1), "p99_ms": round(float(np.percentile(blat, 99)), 1),
                   "busy_s": round(sum(service), 2)}
