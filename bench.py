@@ -467,7 +467,7 @@ def run_serving(A, args):
                 jobs.append((sid, cam, pc, w, h, spp, dn, spp, dn))
         return jobs
 
-    srv = A.Server(0, hbm_budget_bytes=16 << 30, queue_paths=args.serve_queue, policy=args.serve_policy)
+    srv = A.Server(0, hbm_budget_bytes=64 << 30, queue_paths=args.serve_queue, policy=args.serve_policy)
     den = A.Denoiser(0, seed=1)  # weights do not change the cost; c4 trains one
     srv.set_denoiser(den)
     for s in scenes:
@@ -513,15 +513,25 @@ def run_serving(A, args):
                 sid = jb[0]
                 ids[k] = srv.submit(*jb[:6], depth_channel=scenes[sid].state.depth_out_channel, denoise=jb[6])
 
-        th = threading.Thread(target=feeder)
+        lat, paths = [0.0] * len(jobs), [0]
+
+        def collector():
+            # waits for the jobs in submission order and releases each at once, so that finished
+            # jobs do not hold HBM budget while later ones arrive
+            for k in range(len(jobs)):
+                while ids[k] is None:
+                    time.sleep(0.0002)
+                st = srv.wait(ids[k])
+                lat[k] = st["done_ms"] - st["submit_ms"]
+                paths[0] += st["paths"]
+                srv.release(ids[k])
+
+        th, tc = threading.Thread(target=feeder), threading.Thread(target=collector)
         th.start()
+        tc.start()
         th.join()
-        lat, paths = [], 0
-        for k in range(len(jobs)):
-            st = srv.wait(ids[k])
-            lat.append(st["done_ms"] - st["submit_ms"])
-            paths += st["paths"]
-            srv.release(ids[k])
+        tc.join()
+        paths = paths[0]
         span = (time.perf_counter() - base)
         served = {"jobs_per_s": round(len(jobs) / span, 2), "p50_ms": round(float(np.percentile(lat, 50)), 1),
                   "p99_ms": round(float(np.percentile(lat, 99)), 1), "mpaths_per_s": round(paths / span / 1e6, 1)}
