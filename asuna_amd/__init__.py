@@ -517,14 +517,15 @@ def run_offline(scene: str, out: str, gpu_id: int = 0, output_scanline: bool = F
 
 
 # ------------------------------------------------------------ render server
-SERVE_ROUND_ROBIN, SERVE_OLDEST_FIRST, SERVE_OLDEST_GLOBAL = 0, 1, 2  # ash_server_opts.policy
+SERVE_OLDEST_GLOBAL, SERVE_ROUND_ROBIN, SERVE_OLDEST_FIRST = 0, 1, 2  # ash_server_opts.policy
 
 
 class Server:
     """Resident render server (include/asuna_server.h): scenes stay in HBM,
     jobs are admitted by an HBM budget and co-batched per scene.  policy:
-    SERVE_ROUND_ROBIN, SERVE_OLDEST_FIRST or SERVE_OLDEST_GLOBAL (batch fill
-    order, and for the last the order in which scenes get the device)."""
+    SERVE_OLDEST_GLOBAL (default), SERVE_ROUND_ROBIN or SERVE_OLDEST_FIRST
+    (batch fill order, and for the first the order in which scenes get the
+    device)."""
 
     def __init__(self, device=0, hbm_budget_bytes=0, queue_paths=0, blocks_per_cu=0, max_jobs_per_batch=0,
                  batches_ahead=0, policy=0):

@@ -379,13 +379,13 @@ int ash_server_create(int device, const ash_server_opts* opts, ash_server** out)
   s->device = device;
   if (opts) s->o = *opts;
   if (!s->o.hbm_budget_bytes) s->o.hbm_budget_bytes = 8ull << 30;
-  if (!s->o.queue_paths) s->o.queue_paths = 1u << 23;
+  if (!s->o.queue_paths) s->o.queue_paths = 1u << 25;
   if (!s->o.blocks_per_cu) s->o.blocks_per_cu = 8;
   if (!s->o.max_jobs_per_batch) s->o.max_jobs_per_batch = 256;
   if (!s->o.batches_ahead) s->o.batches_ahead = 2;
   s->o.batches_ahead = std::min<uint32_t>(s->o.batches_ahead, kRing - 1);
   if (s->o.queue_paths < 4096 || s->o.queue_paths > (1u << 26) || s->o.blocks_per_cu > 64 ||
-      s->o.max_jobs_per_batch > 65535 || s->o.policy > ASH_SERVE_OLDEST_GLOBAL) {
+      s->o.max_jobs_per_batch > 65535 || s->o.policy > ASH_SERVE_OLDEST_FIRST) {
     delete s;
     return ASH_E_INVALID;
   }

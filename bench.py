@@ -63,8 +63,9 @@ def main():
     ap.add_argument("--train-views", type=int, default=64, help="c4: training views per scene (six scenes)")
     ap.add_argument("--jobs", type=int, default=600, help="c5: jobs per arrival rate")
     ap.add_argument("--rates", default="10,50,200", help="c5: Poisson arrival rates (jobs/s)")
-    ap.add_argument("--serve-policy", type=int, default=0, help="c5: server batch fill order (0 round robin, 1 oldest first, 2 oldest first across scenes)")
-    ap.add_argument("--serve-queue", type=int, default=1 << 23, help="c5: server paths per batch")
+    ap.add_argument("--serve-policy", type=int, default=0,
+                    help="c5: server batch fill order (0 oldest first across scenes, 1 round robin, 2 oldest first)")
+    ap.add_argument("--serve-queue", type=int, default=1 << 25, help="c5: server paths per batch")
     ap.add_argument("--jobs-log", default="", help="c5: write one JSON line per job (both columns) to this file")
     ap.add_argument("--matched", default="",
                     help="c5 at equal relMSE: plain spp -> denoised spp map (e.g. 1:1,4:1,16:2,64:64p, from c4; "
@@ -494,7 +495,7 @@ def run_serving(A, args):
                                   "arrivals; resolution {256^2,512^2,1024^2,1920x1080} w {.4,.3,.2,.1}; "
                                   "spp {1,4,16,64}; denoise flag on 50 % of jobs (U-Net on the job's planes)",
                       "jobs_per_rate": args.jobs, "queue_paths": args.serve_queue,
-                      "policy": ["round robin", "oldest first", "oldest first across scenes"][args.serve_policy],
+                      "policy": ["oldest first across scenes", "round robin", "oldest first"][args.serve_policy],
                       "mode": ("equal relMSE: server renders plain spp S at the spp map " + args.matched +
                                " (denoised; p = plain) measured by c4; baseline renders S plain") if matched else
                               "same work in both columns (denoise flag on half of the jobs)"},

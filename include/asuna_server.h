@@ -29,7 +29,7 @@ typedef struct ash_server ash_server;
 
 typedef struct ash_server_opts {
   uint64_t hbm_budget_bytes; /* arena for admitted jobs' output images (0 = 8 GiB) */
-  uint32_t queue_paths;      /* paths per wavefront batch of one scene (0 = 2^23) */
+  uint32_t queue_paths;      /* paths per wavefront batch of one scene (0 = 2^25) */
   uint32_t blocks_per_cu;    /* persistent traversal blocks per CU (0 = 8) */
   uint32_t max_jobs_per_batch; /* shot-table entries per batch (0 = 256) */
   uint32_t batches_ahead;    /* batches enqueued per scene before waiting (0 = 2, max 3) */
@@ -37,15 +37,15 @@ typedef struct ash_server_opts {
   uint32_t pad;
 } ash_server_opts;
 
-/* Batch fill order of a scene's admitted jobs.  ROUND_ROBIN: each batch
- * starts at the next job in turn (short jobs finish early, long ones share
- * the device with every newcomer); OLDEST_FIRST: each batch takes all it can
- * from the oldest job and backfills the rest from younger ones (long jobs
- * keep their place in line within their scene); OLDEST_GLOBAL: as
- * OLDEST_FIRST, and the scenes are fed in the order of their oldest job with
- * at most 2 x queue_paths paths in flight on the device, so the oldest work
- * shares the device with little else. */
-enum { ASH_SERVE_ROUND_ROBIN = 0, ASH_SERVE_OLDEST_FIRST = 1, ASH_SERVE_OLDEST_GLOBAL = 2 };
+/* Batch fill order.  OLDEST_GLOBAL (default): each batch takes all it can
+ * from its scene's oldest job and backfills the rest from younger ones, and
+ * the scenes are fed in the order of their oldest job with at most
+ * 2 x queue_paths paths in flight on the device, so the oldest work shares
+ * the device with little else; ROUND_ROBIN: each batch of a scene starts at
+ * the next job in turn and every scene keeps its batches in flight (short
+ * jobs finish early, long ones share the device with every newcomer);
+ * OLDEST_FIRST: oldest-first batches, every scene fed. */
+enum { ASH_SERVE_OLDEST_GLOBAL = 0, ASH_SERVE_ROUND_ROBIN = 1, ASH_SERVE_OLDEST_FIRST = 2 };
 
 typedef struct ash_job_desc {
   uint32_t scene;   /* id returned by ash_server_add_scene */
