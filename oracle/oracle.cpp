@@ -2970,6 +2970,18 @@ float orc_bsdf_pdf(const GpuMaterial* m, const float Lp[3], const float Vp[3], u
 
 float orc_conductor_reflectance(float eta, float k, float cosThetaI) { return conductorReflectance(eta, k, cosThetaI); }
 
+// sampleOneLight (sample_light.glsl:10-82) for one light at shading point p.
+void orc_sample_light(const GpuLight* l, float r0, float r1, const float p[3], float rad[3], float d[3], float* dist,
+                      float* pdf, uint32_t* flags) {
+  LRec<float> rec{};
+  V3<float> r = sampleOneLight<float>(r0, r1, *l, load3<float>(p), rec);
+  rad[0] = r.x, rad[1] = r.y, rad[2] = r.z;
+  d[0] = rec.d.x, d[1] = rec.d.y, d[2] = rec.d.z;
+  *dist = rec.dist;
+  *pdf = rec.pdf;
+  *flags = rec.flags;
+}
+
 void orc_set_sun_and_sky(orc_scene* s, const GpuSunAndSky* sky) { s->sky = *sky; }
 void orc_sun_and_sky(const GpuSunAndSky* sky, const float dir[3], int fp64, float out[3]) {
   if (fp64) {

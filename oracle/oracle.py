@@ -64,6 +64,7 @@ def lib():
         "orc_pdf_envmap": (C.c_float, [vp, F, F]),
         "orc_eval_envmap": (None, [vp, F, C.c_float, F, F]),
         "orc_set_sun_and_sky": (None, [vp, P(A.GpuSunAndSky)]),
+        "orc_sample_light": (None, [P(A.GpuLight), C.c_float, C.c_float, F, F, F, F, F, P(C.c_uint32)]),
         "orc_sun_and_sky": (None, [P(A.GpuSunAndSky), F, C.c_int, F]),
         "orc_sky_physical_scale": (None, [C.c_float, C.c_float, C.c_float, F]),
     }

@@ -944,3 +944,92 @@ def test_sky_fp32_and_fp64_restatements_agree():
         d /= np.linalg.norm(d)
         a, b = _sky_eval(s, tuple(d)), _sky_eval(s, tuple(d), fp64=1)
         np.testing.assert_allclose(a, b, rtol=2e-3, atol=1e-9)
+
+
+# ------------------------------------------------ dielectric refraction
+def test_dielectric_refraction_obeys_snell():
+    """bsdf_dielectric.rchit:88-138 as restated: a transmitted direction is
+    unit length, leaves on the other side of the surface, lies in the plane
+    of V and N on the far side of the normal, and obeys Snell's law with
+    eta = 1/ior entering (V above the surface) and ior leaving; a reflected
+    one is V mirrored about N; weight / pdf is eta^2 for transmission and 1
+    for reflection; total internal reflection past the critical angle."""
+    L = O.lib()
+    ior = 1.5
+    m = _mat(10, ior=ior)
+    seed = C.c_uint32(7)
+    w, d, pdf, fl = (C.c_float * 3)(), (C.c_float * 3)(), C.c_float(), C.c_uint32()
+    rng = np.random.default_rng(3)
+    seen = {16: 0, 32: 0}
+    for k in range(4000):
+        ct = rng.uniform(0.05, 1.0) * (1 if k % 2 == 0 else -1)  # both sides of the surface
+        st, ph = math.sqrt(1 - ct * ct), rng.uniform(0, 2 * math.pi)
+        V = np.array([st * math.cos(ph), st * math.sin(ph), ct], np.float32)
+        V /= np.linalg.norm(V)
+        L.orc_bsdf_sample(C.byref(m), O.f3(V), C.byref(seed), w, d, C.byref(pdf), C.byref(fl))
+        D, W = np.array(d, np.float64), np.array(w, np.float64)
+        eta = 1 / ior if V[2] > 0 else ior
+        sin_i = math.hypot(V[0], V[1])
+        assert abs(np.linalg.norm(D) - 1) < 1e-5
+        if fl.value == 16:  # specular reflection
+            np.testing.assert_allclose(D, [-V[0], -V[1], V[2]], atol=2e-6)
+            np.testing.assert_allclose(W / pdf.value, 1.0, rtol=1e-5)
+        else:
+            assert fl.value == 32 and eta * sin_i < 1.0  # no transmission past the critical angle
+            assert D[2] * V[2] < 0
+            np.testing.assert_allclose(math.hypot(D[0], D[1]), eta * sin_i, atol=2e-6)
+            assert abs(np.linalg.det(np.stack([V, D, [0, 0, 1]]))) < 2e-6  # coplanar with V and N
+            assert D[0] * V[0] + D[1] * V[1] <= 1e-7  # tangential part turns to the far side of N
+            np.testing.assert_allclose(W / pdf.value, eta * eta, rtol=1e-5)
+        seen[fl.value] += 1
+    assert seen[16] > 100 and seen[32] > 1000
+    # past the critical angle (inside, ior * sin > 1) every sample reflects
+    V = np.array([0.9, 0.0, -math.sqrt(1 - 0.81)], np.float32)
+    for _ in range(200):
+        L.orc_bsdf_sample(C.byref(m), O.f3(V), C.byref(seed), w, d, C.byref(pdf), C.byref(fl))
+        assert fl.value == 16
+
+
+# ------------------------------------------------ triangle-light warp
+def test_triangle_light_warp_and_pdf():
+    """sample_light.glsl:10-82: a triangle light is sampled at
+    p0 + u*r0 + v*(1 - r0)*r1, which is not area-uniform: its first
+    barycentric coordinate is uniform (an area-uniform warp would put only a
+    quarter of the samples past b1 = 1/2, this one half), and the second,
+    given the first, is uniform on [0, 1 - b1].  The pdf it reports is
+    nevertheless the area-uniform one, dist^2 / (area |cos|) (+ EPS)."""
+    L = O.lib()
+    l = A.GpuLight()
+    l.type = 2
+    l.position[:] = (1.0, 2.0, 3.0)
+    e1, e2 = np.array([2.0, 0.5, 0.0], np.float32), np.array([0.3, 1.5, 0.4], np.float32)
+    l.u[:] = e1.tolist()
+    l.v[:] = e2.tolist()
+    l.radiance[:] = (5.0, 4.0, 3.0)
+    area = 0.5 * np.linalg.norm(np.cross(e1, e2))
+    l.area = float(area)
+    n = np.cross(e1, e2) / np.linalg.norm(np.cross(e1, e2))
+    P = (np.array([1.0, 2.0, 3.0]) + (e1 + e2) / 3 + 2.5 * n).astype(np.float32)
+    rng = np.random.default_rng(11)
+    N = 40000
+    rad, d, dist, pdf, fl = (C.c_float * 3)(), (C.c_float * 3)(), C.c_float(), C.c_float(), C.c_uint32()
+    B = np.empty((N, 2))
+    M = np.stack([e1, e2], 1).astype(np.float64)
+    for k in range(N):
+        r0, r1 = rng.random(), rng.random()
+        L.orc_sample_light(C.byref(l), r0, r1, O.f3(P), rad, d, C.byref(dist), C.byref(pdf), C.byref(fl))
+        q = P + np.array(d, np.float64) * dist.value - np.array([1.0, 2.0, 3.0])
+        B[k] = np.linalg.lstsq(M, q, rcond=None)[0]
+        if k < 2000:
+            assert fl.value == 2 and list(rad) == [5.0, 4.0, 3.0]
+            cos = abs(float(np.dot(n, np.array(d))))
+            np.testing.assert_allclose(pdf.value, dist.value ** 2 / (area * cos + 1e-3), rtol=2e-4)
+    b1, b2 = B[:, 0], B[:, 1]
+    assert b1.min() > -1e-4 and b2.min() > -1e-4 and (b1 + b2).max() < 1 + 1e-4
+    assert abs(np.mean(b1 > 0.5) - 0.5) < 0.012  # area-uniform: 0.25
+    hist = np.histogram(b1, bins=20, range=(0, 1))[0]
+    assert np.sum((hist - N / 20) ** 2 / (N / 20)) < 43.8  # chi-square, 19 dof, p = 0.001
+    t = b2[b1 < 0.98] / (1 - b1[b1 < 0.98])
+    hist = np.histogram(t, bins=20, range=(0, 1))[0]
+    e = len(t) / 20
+    assert np.sum((hist - e) ** 2 / e) < 43.8
