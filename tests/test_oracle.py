@@ -1033,3 +1033,47 @@ def test_triangle_light_warp_and_pdf():
     hist = np.histogram(t, bins=20, range=(0, 1))[0]
     e = len(t) / 20
     assert np.sum((hist - e) ** 2 / e) < 43.8
+
+
+# ------------------------------------------ pbr / rough conductor eval
+def test_pbr_and_rough_conductor_eval_closed_forms():
+    """Cook-Torrance with GGX and Smith masking at V = L = N reduces to
+    F(0) * D(n) / 4 (G = 1, the 1/(4 cos cos) Jacobian at cos = 1), where the
+    shaders' GGX is D(n) = 1 / (pi ax ay + EPS), EPS = 1e-3
+    (brdf_pbr_metalness_roughness.rchit:18-22): pbr (F0 = ((eta-1)/(eta+1))^2 for a dielectric,
+    the albedo for a metal, plus albedo (1 - metal)/pi), rough conductor
+    (kd * the conductor Fresnel at normal incidence, alpha floored at 1e-3).
+    A white metal (pbr, metalness 1, albedo 1) reflects at most all the
+    light: its directional albedo is in (0.9, 1] at roughness 0.3."""
+    L = O.lib()
+    out = (C.c_float * 3)()
+    Nv = O.f3((0.0, 0.0, 1.0))
+    for rough in (0.2, 0.5, 0.8):
+        a = max(rough * rough, 1e-3)
+        m = _mat(3, roughness=rough, metalness=0.0, ior=1.5, diffuse=(0.0, 0.0, 0.0))
+        L.orc_bsdf_eval(C.byref(m), Nv, Nv, 2, out)
+        d0 = 1 / (math.pi * a * a + 1e-3)
+        np.testing.assert_allclose(list(out), [0.04 * d0 / 4] * 3, rtol=2e-5)
+        m = _mat(3, roughness=rough, metalness=1.0, ior=1.5, diffuse=(0.9, 0.5, 0.2))
+        L.orc_bsdf_eval(C.byref(m), Nv, Nv, 2, out)
+        np.testing.assert_allclose(list(out), np.array([0.9, 0.5, 0.2]) * d0 / 4, rtol=2e-5)
+        m = _mat(3, roughness=rough, metalness=0.25, ior=1.5, diffuse=(0.6, 0.6, 0.6))
+        L.orc_bsdf_eval(C.byref(m), Nv, Nv, 2, out)
+        f0 = 0.75 * 0.04 + 0.25 * 0.6
+        np.testing.assert_allclose(out[0], f0 * d0 / 4 + 0.6 * 0.75 / math.pi, rtol=2e-5)
+    n, k = (0.2, 1.1, 1.6), (3.9, 2.6, 2.3)
+    fn = [((ni - 1) ** 2 + ki ** 2) / ((ni + 1) ** 2 + ki ** 2) for ni, ki in zip(n, k)]
+    for ax, ay in ((0.1, 0.1), (0.3, 0.05), (0.0, 0.5)):
+        m = _mat(7, diffuse=(0.8, 0.9, 1.0), radiance=n, radianceFactor=k, anisoAlpha=(ax, ay))
+        L.orc_bsdf_eval(C.byref(m), Nv, Nv, 2, out)
+        ax_, ay_ = max(ax, 1e-3), max(ay, 1e-3)
+        want = np.array([0.8, 0.9, 1.0]) * np.array(fn) / (4 * (math.pi * ax_ * ay_ + 1e-3))
+        np.testing.assert_allclose(list(out), want, rtol=2e-5)
+    m = _mat(3, roughness=0.3, metalness=1.0, ior=1.5, diffuse=(1.0, 1.0, 1.0))
+    V = O.f3((0.0, 0.0, 1.0))
+
+    def f(Ld):
+        L.orc_bsdf_eval(C.byref(m), O.f3(Ld), V, 2, out)
+        return out[0]
+    alb = _hemisphere_integral(f, 200, 200)
+    assert 0.9 < alb <= 1.0 + 2e-3
