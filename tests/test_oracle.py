@@ -1077,3 +1077,26 @@ def test_pbr_and_rough_conductor_eval_closed_forms():
         return out[0]
     alb = _hemisphere_integral(f, 200, 200)
     assert 0.9 < alb <= 1.0 + 2e-3
+
+
+def test_rough_conductor_fresnel_at_n_dot_l():
+    """brdf_rough_conductor.rchit evaluates the conductor Fresnel at N.L (as
+    written), not at H.V: D and G are symmetric in L and V, so swapping them
+    scales f = eval / cos(L) by F(cos L) / F(cos V), with F the (separately
+    pinned) conductor reflectance; a Fresnel at H.V would leave f unchanged."""
+    L = O.lib()
+    n, k = (0.2, 1.1, 1.6), (3.9, 2.6, 2.3)
+    m = _mat(7, diffuse=(1.0, 1.0, 1.0), radiance=n, radianceFactor=k, anisoAlpha=(0.3, 0.3))
+    a, b = (C.c_float * 3)(), (C.c_float * 3)()
+    rng = np.random.default_rng(5)
+    for _ in range(50):
+        t1, t2 = rng.uniform(0.1, 1.3, 2)
+        p1, p2 = rng.uniform(0, 2 * math.pi, 2)
+        Lv = np.array([math.sin(t1) * math.cos(p1), math.sin(t1) * math.sin(p1), math.cos(t1)], np.float32)
+        Vv = np.array([math.sin(t2) * math.cos(p2), math.sin(t2) * math.sin(p2), math.cos(t2)], np.float32)
+        L.orc_bsdf_eval(C.byref(m), O.f3(Lv), O.f3(Vv), 2, a)
+        L.orc_bsdf_eval(C.byref(m), O.f3(Vv), O.f3(Lv), 2, b)
+        for c in range(3):
+            fl = L.orc_conductor_reflectance(n[c], k[c], float(Lv[2]))
+            fv = L.orc_conductor_reflectance(n[c], k[c], float(Vv[2]))
+            np.testing.assert_allclose((a[c] / Lv[2]) / (b[c] / Vv[2]), fl / fv, rtol=2e-4)
