@@ -84,9 +84,10 @@ struct Wavefront {
                               // shadow tris, ext rays, shadow rays (kNumCnt64 in all)
 };
 
-// A work unit: frames [f0, f0 + nf) of the linear pixel run [pix0, pix0 + np)
-// of one shot.  Its paths are base + fi * np + k (frame-major), and its
-// accumulate threads accBase + k.  The units of one batch cover disjoint
+// A work unit: frames [f0, f0 + nf) of the pixel run [pix0, pix0 + np) of
+// one shot (positions in the film's tile order).  Its paths are
+// base + k * nf + fi (pixel-major: a wave's paths are a few pixels' frames,
+// whose camera rays nearly coincide), and its accumulate threads accBase + k.  The units of one batch cover disjoint
 // (shot, pixel) sets, so each pixel's frames are folded by one thread in
 // frame order.
 struct WorkUnit {

@@ -537,11 +537,11 @@ __global__ __launch_bounds__(256) void kRaygen(BatchDesc b, Wavefront w) {
   if (p >= b.n) return;
   const WorkUnit un = b.units[findUnit<false>(b.units, b.nUnits, p)];
   uint32_t rel = p - un.base;
-  uint32_t fslot = rel / un.np;
+  uint32_t fslot = rel % un.nf;  // pixel-major: the unit's frames of one pixel are adjacent
   int32_t frame = un.f0 + int32_t(fslot);
   uint32_t shot = un.shot;
   const ShotParams& sp = b.shotTab[shot];
-  const uint32_t pix = runPixel(un.pix0 + rel % un.np, sp.width, sp.height);
+  const uint32_t pix = runPixel(un.pix0 + rel / un.nf, sp.width, sp.height);
   uint32_t x = pix % sp.width, y = pix / sp.width;
   uint32_t seed = xxhash32Seed(x, y, uint32_t(frame));
   float jx, jy;
@@ -2686,7 +2686,7 @@ __global__ __launch_bounds__(256) void kAccumulate(BatchDesc b, Wavefront w) {
   float4 mean = img0[pix];
   float W = img8[pix].x;
   for (uint32_t f = 0; f < un.nf; ++f) {
-    uint32_t p = un.base + f * un.np + k;
+    uint32_t p = un.base + k * un.nf + f;
     float4 r = w.rad[p];
     float fw = r.w;
     f3 L = f3{clampf(r.x, 0.f, 10.f), clampf(r.y, 0.f, 10.f), clampf(r.z, 0.f, 10.f)};
