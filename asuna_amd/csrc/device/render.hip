@@ -2672,12 +2672,38 @@ __global__ void kStatRays(Wavefront w, int cur) {
 }
 
 // Ordered fold of the batch's frames into the output images
-// (raytrace.projective.rgen:146-178).
-__global__ __launch_bounds__(256) void kAccumulate(BatchDesc b, Wavefront w) {
-  uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= b.nAcc) return;
-  const WorkUnit un = b.units[findUnit<true>(b.units, b.nUnits, t)];
-  uint32_t k = t - un.accBase;
+// (raytrace.projective.rgen:146-178), one thread per pixel, one wave per
+// block.  A pixel's frames are adjacent in the wavefront (pixel-major
+// paths), so when the wave's 64 pixels belong to one work unit their frames
+// are loaded in coalesced runs into LDS, kAccChunk frames at a time (rows
+// padded by one entry against bank conflicts), and each lane folds its own
+// row in frame order; otherwise each lane reads its frames directly.
+constexpr uint32_t kAccChunk = 16;
+__device__ __forceinline__ void accFold(const WorkUnit& un, uint32_t f, float4 r, float4& mean, float& W) {
+  float fw = r.w;
+  f3 L = f3{clampf(r.x, 0.f, 10.f), clampf(r.y, 0.f, 10.f), clampf(r.z, 0.f, 10.f)};
+  f3 rws = fw * L;
+  if (un.f0 + int32_t(f) == 0) {
+    f3 m = rws / fw;
+    mean = f4(m, 1.f);
+    W = fw;
+  } else {
+    f3 oldRws = xyz(mean) * W;
+    float nW = W + fw;
+    f3 m = (oldRws + rws) / nW;
+    mean = f4(m, 1.f);
+    W = nW;
+  }
+}
+__global__ __launch_bounds__(64) void kAccumulate(BatchDesc b, Wavefront w) {
+  __shared__ float4 sRad[64 * (kAccChunk + 1)];
+  const uint32_t t = blockIdx.x * 64 + threadIdx.x, lane = threadIdx.x;
+  const bool valid = t < b.nAcc;
+  const uint32_t ui = valid ? findUnit<true>(b.units, b.nUnits, t) : 0xffffffffu;
+  const bool together = __ballot(valid && ui == __shfl(ui, 0)) == ~0ull;  // wave-uniform
+  if (!valid) return;  // only in the last wave, which then is not `together`
+  const WorkUnit un = b.units[ui];
+  const uint32_t k = t - un.accBase;
   const ShotParams& sp = b.shotTab[un.shot];
   const uint32_t pix = runPixel(un.pix0 + k, sp.width, sp.height);
   size_t plane = size_t(sp.width) * sp.height;
@@ -2685,23 +2711,21 @@ __global__ __launch_bounds__(256) void kAccumulate(BatchDesc b, Wavefront w) {
   float4* img8 = sp.images + 8 * plane;
   float4 mean = img0[pix];
   float W = img8[pix].x;
-  for (uint32_t f = 0; f < un.nf; ++f) {
-    uint32_t p = un.base + k * un.nf + f;
-    float4 r = w.rad[p];
-    float fw = r.w;
-    f3 L = f3{clampf(r.x, 0.f, 10.f), clampf(r.y, 0.f, 10.f), clampf(r.z, 0.f, 10.f)};
-    f3 rws = fw * L;
-    if (un.f0 + int32_t(f) == 0) {
-      f3 m = rws / fw;
-      mean = f4(m, 1.f);
-      W = fw;
-    } else {
-      f3 oldRws = xyz(mean) * W;
-      float nW = W + fw;
-      f3 m = (oldRws + rws) / nW;
-      mean = f4(m, 1.f);
-      W = nW;
+  if (together) {
+    const uint32_t k0 = k - lane;  // the wave's first pixel of the unit
+    const float4* src = w.rad + un.base + size_t(k0) * un.nf;
+    for (uint32_t c0 = 0; c0 < un.nf; c0 += kAccChunk) {
+      const uint32_t cnt = min(kAccChunk, un.nf - c0);
+      __syncthreads();
+      for (uint32_t e = lane; e < 64 * cnt; e += 64) {
+        const uint32_t px = e / cnt, fr = e - px * cnt;
+        sRad[px * (cnt + 1) + fr] = src[size_t(px) * un.nf + c0 + fr];
+      }
+      __syncthreads();
+      for (uint32_t fr = 0; fr < cnt; ++fr) accFold(un, c0 + fr, sRad[lane * (cnt + 1) + fr], mean, W);
     }
+  } else {
+    for (uint32_t f = 0; f < un.nf; ++f) accFold(un, f, w.rad[un.base + k * un.nf + f], mean, W);
   }
   img0[pix] = mean;
   img8[pix] = make_float4(W, W, W, W);
@@ -2812,7 +2836,7 @@ void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t scheme, uint
   else shadowFor<false>(sc, w, scheme, grid, s);
 }
 void launchAccumulate(const BatchDesc& b, const Wavefront& w, hipStream_t s) {
-  kAccumulate<<<(b.nAcc + 255) / 256, 256, 0, s>>>(b, w);
+  kAccumulate<<<(b.nAcc + 63) / 64, 64, 0, s>>>(b, w);
 }
 void launchSkyPrepare(DevSky* sky, hipStream_t s) { kSkyPrepare<<<1, 64, 0, s>>>(sky); }
 
