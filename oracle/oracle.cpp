@@ -8,6 +8,12 @@
 // the reference does, so that the FP32 instantiation reproduces every
 // rounding of the same operation sequence.
 //
+// Parity unpinned (no closed form or known answer checks them; they are
+// checked by review against the shaders and by GPU-equals-oracle tests
+// only): the rough-plastic NEE with its permuted eval/pdf arguments
+// (brdf_rough_plastic.rchit), and kang18's object-space normal and tangent
+// maps (brdf_kang18.rchit).
+//
 // Nothing in the product links or calls this file.
 #include "oracle.h"
 

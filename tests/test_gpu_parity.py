@@ -241,6 +241,39 @@ def test_rough_plastic_and_emissive_scene(renderer, tmp_path):
     np.testing.assert_allclose(mg, mo, rtol=5e-3)
 
 
+def test_normal_mapped_scene_matches_oracle(renderer, tmp_path):
+    """Normal maps (brdf_lambertian.rchit:83-90 and the same block in the
+    other plugins): a checker of two tilted tangent-space normals on a
+    lambertian, a rough plastic and a pbr surface of the box scene."""
+    d = tmp_path / "nmap"
+    d.mkdir()
+    sp = A.generate_scene(0, str(d))
+    tex = np.ones((16, 16, 4), np.float32)
+    tex[..., :3] = (0.5, 0.5, 1.0)
+    tex[::2, ::2, :3] = (0.8125, 0.5, 0.875)
+    tex[1::2, 1::2, :3] = (0.5, 0.25, 0.875)
+    A.write_image(str(d / "bumps.exr"), tex)
+    import json
+    js = json.load(open(sp))
+    js["textures"] = [{"name": "bumps", "path": "bumps.exr"}]
+    mats = js["materials"]
+    mats[0] = {"name": mats[0]["name"], "type": "brdf_lambertian", "diffuse_reflectance": [0.6, 0.6, 0.5],
+               "normal_texture": "bumps"}
+    mats[1] = {"name": mats[1]["name"], "type": "brdf_rough_plastic", "diffuse_reflectance": [0.5, 0.3, 0.2],
+               "alpha": [0.2, 0.2], "normal_texture": "bumps"}
+    mats[2] = {"name": mats[2]["name"], "type": "brdf_pbr_metalness_roughness", "diffuse_reflectance": [0.7, 0.5, 0.3],
+               "metalness": 0.5, "roughness": 0.4, "normal_texture": "bumps"}
+    json.dump(js, open(sp, "w"))
+    s, _ = _load(renderer, sp)
+    spp = 16
+    st, ost, g, o, sq = _radiance_parity(renderer, s, spp)
+    frac_bad = _three_sigma(g[0], o[0], sq, spp)
+    mg, mo = g[0][..., :3].mean((0, 1)), o[0][..., :3].mean((0, 1))
+    log(f"normal-mapped radiance: outside 3 sigma {frac_bad:.5f}, means {mg} vs {mo}")
+    assert frac_bad <= 0.01
+    np.testing.assert_allclose(mg, mo, rtol=5e-3)
+
+
 def test_more_plugins_scene_matches_oracle(renderer, tmp_path):
     """Mirror, conductor (Au), rough conductor (Cu, anisotropic), smooth
     plastic, phong and kang18 surfaces in the box, lit by its rect light plus

@@ -532,6 +532,48 @@ def test_e2e_delta_light_direct_illumination(tmp_path, light):
             assert abs(got - want) / want < 0.01, (x, y, got, want)
 
 
+def _textured_plane(tmp_path, name, diffuse_tex=None, normal_tex=None, albedo=0.5):
+    """The lambertian plane under a distant light along its normal (+y),
+    optionally with constant diffuse / normal textures (EXR, half-exact)."""
+    import json
+    sp = _plane_and_rect_light_scene(tmp_path / name, radiance=(3.0, 3.0, 3.0), albedo=albedo, light="distant")
+    js = json.load(open(sp))
+    js["lights"][0]["direction"] = [0.0, 1.0, 0.0]
+    js["textures"] = []
+    for key, rgb in (("diffuse_texture", diffuse_tex), ("normal_texture", normal_tex)):
+        if rgb is None:
+            continue
+        tex = np.ones((4, 4, 4), np.float32)
+        tex[..., :3] = rgb
+        A.write_image(str(tmp_path / name / f"{key}.exr"), tex)
+        js["textures"].append({"name": key, "path": f"{key}.exr"})
+        js["materials"][0][key] = key
+    json.dump(js, open(sp, "w"))
+    s = A.HostScene(sp)
+    cam, pc = s.shot(0)
+    img, _ = O.OracleScene(s.desc).render(cam, pc, 64, 48, 0, 4, crop=(26, 18, 34, 26))
+    return img[0, 18:26, 26:34, :3]
+
+
+def test_textures_replace_the_constant_and_normal_maps_tilt_the_shading_normal(tmp_path):
+    """brdf_lambertian.rchit:81-90: a diffuse texture replaces
+    diffuse_reflectance (a constant 0.5 texture over reflectance 0.2 renders
+    as albedo 0.5); a normal map's texel c gives the tangent-space normal
+    2c - 1, normalised: the flat texel (0.5, 0.5, 1) changes nothing, and a
+    constant tilted one scales the distant light's cosine (light along the
+    geometric normal, black background) to n_z / |n|."""
+    plain = _textured_plane(tmp_path, "plain")
+    assert plain.min() > 0.1
+    np.testing.assert_allclose(plain, 0.5 / math.pi * 3.0, rtol=2e-3)  # the light pdf's EPS guard: 0.1 %
+    np.testing.assert_allclose(_textured_plane(tmp_path, "dtex", diffuse_tex=(0.5, 0.5, 0.5), albedo=0.2), plain,
+                               rtol=1e-6)
+    np.testing.assert_allclose(_textured_plane(tmp_path, "flat", normal_tex=(0.5, 0.5, 1.0)), plain, rtol=1e-6)
+    c = np.array([0.8125, 0.5, 0.875])
+    n = 2 * c - 1
+    tilted = _textured_plane(tmp_path, "tilt", normal_tex=tuple(c))
+    np.testing.assert_allclose(tilted / plain, n[2] / np.linalg.norm(n), rtol=1e-5)
+
+
 def test_accumulation_weighted_mean_recurrence(tmp_path):
     """A film that sees only a constant background: every sample is the
     background colour, so the running weighted mean must return it exactly
