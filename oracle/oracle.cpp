@@ -10,9 +10,10 @@
 //
 // Parity unpinned (no closed form or known answer checks them; they are
 // checked by review against the shaders and by GPU-equals-oracle tests
-// only): the rough-plastic NEE with its permuted eval/pdf arguments
-// (brdf_rough_plastic.rchit), and kang18's object-space normal and tangent
-// maps (brdf_kang18.rchit).
+// only): the argument permutation of the rough-plastic NEE
+// (brdf_rough_plastic.rchit:234-240; the NEE itself is checked where the
+// permutation is invisible, alpha = 1/ior^2), and kang18's object-space
+// normal and tangent maps (brdf_kang18.rchit).
 //
 // Nothing in the product links or calls this file.
 #include "oracle.h"

@@ -574,6 +574,44 @@ def test_textures_replace_the_constant_and_normal_maps_tilt_the_shading_normal(t
     np.testing.assert_allclose(tilted / plain, n[2] / np.linalg.norm(n), rtol=1e-5)
 
 
+def test_rough_plastic_light_sample_reduces_to_the_bsdf_when_the_permutation_is_invisible(tmp_path):
+    """brdf_rough_plastic.rchit:234-240 passes (invEta2, ax, ay) to eval in
+    the order (ax, ay, invEta2).  With alpha = 1/ior^2 on both axes the three
+    are equal, so a rough-plastic plane under a distant light (black
+    background, depth 2: only the light sample contributes) must show the
+    plain BSDF value times the radiance: the filter-weighted mean of
+    eval(L, V) over the pixel, with the 0.1 % EPS guard of the light pdf."""
+    import json
+    sp = _plane_and_rect_light_scene(tmp_path / "rp", radiance=(3.0, 3.0, 3.0), light="distant")
+    js = json.load(open(sp))
+    js["materials"][0] = {"name": "m", "type": "brdf_rough_plastic", "diffuse_reflectance": [0.5, 0.4, 0.3],
+                          "ior": 2.0, "alpha": [0.25, 0.25]}
+    json.dump(js, open(sp, "w"))
+    s = A.HostScene(sp)
+    cam, pc = s.shot(0)
+    m = [s.desc.materials[i] for i in range(s.desc.num_materials)
+         if abs(s.desc.materials[i].ior - 2.0) < 1e-6 and s.desc.materials[i].anisoAlpha[0] > 0][0]
+    assert m.anisoAlpha[0] == m.anisoAlpha[1] == 1.0 / (m.ior * m.ior)
+    crop = (28, 20, 31, 23)
+    img, _ = O.OracleScene(s.desc).render(cam, pc, 64, 48, 0, 64, crop=crop, precision=1)
+    ld = np.array([0.3, 1.0, 0.2]) / np.linalg.norm([0.3, 1.0, 0.2])
+    local = lambda v: O.f3((v[0], v[2], v[1]))  # plane normal +y -> local +z (isotropic lobe)
+    g = (np.arange(8) + 0.5) / 8
+    out = (C.c_float * 3)()
+    for y in range(crop[1], crop[3]):
+        for x in range(crop[0], crop[2]):
+            num, den = np.zeros(3), 0.0
+            for jx in g:
+                for jy in g:
+                    o, d = _ray(cam, x + jx, y + jy)
+                    O.lib().orc_bsdf_eval(C.byref(m), local(ld), local(-d / np.linalg.norm(d)), 2, out)
+                    wgt = ((math.exp(-2 * (jx - .5) ** 2) - math.exp(-8)) *
+                           (math.exp(-2 * (jy - .5) ** 2) - math.exp(-8)))
+                    num += wgt * np.array(list(out)) * 3.0 / 1.001
+                    den += wgt
+            np.testing.assert_allclose(img[0, y, x, :3], num / den, rtol=1e-2, err_msg=str((x, y)))
+
+
 def test_accumulation_weighted_mean_recurrence(tmp_path):
     """A film that sees only a constant background: every sample is the
     background colour, so the running weighted mean must return it exactly
