@@ -1501,6 +1501,17 @@ template <class R> void chsPbr(const RCtx& c, Payload<R>& pl, HitState<R>& st) {
   pl.throughput = pl.throughput * (bw / (bpdf + K<R>::EPS));
 }
 
+// The light-sample terms of brdf_rough_plastic.rchit:229-241: eval and pdf
+// are called with the shader's permuted arguments (eval binds
+// invEta2 <- ax, ax <- ay, ay <- invEta2; pdf binds ax <- eta, ay <- ax,
+// eta <- ay, against the signatures at rchit:82-83 and 113-114).
+template <class R>
+void rpLightTerms(V3<R> L, V3<R> V, V3<R> N, V3<R> X, V3<R> Y, V3<R> kd, R eta, R fdrInt, R invEta2, R ax, R ay,
+                  R ssw, uint32_t lightFlags, V3<R>& bw, R& bsdfPdf) {
+  bw = rpEval(L, V, N, X, Y, kd, V3<R>(1), eta, fdrInt, /*invEta2<-*/ ax, /*ax<-*/ ay, /*ay<-*/ invEta2, EArea);
+  bsdfPdf = rpPdf(L, V, N, X, Y, /*ax<-*/ eta, /*ay<-*/ ax, /*eta<-*/ ay, ssw, lightFlags);
+}
+
 template <class R> void chsRoughPlastic(const RCtx& c, Payload<R>& pl, HitState<R>& st) {
   const orc_scene& s = *c.s;
   const GpuPushConstantRaytrace& pc = *c.pc;
@@ -1529,10 +1540,9 @@ template <class R> void chsRoughPlastic(const RCtx& c, Payload<R>& pl, HitState<
     LRec<R> l;
     V3<R> radiance = sampleLights(c, pl, st.pos, st.ffN, visible, l);
     if (visible) {
-      // Argument permutation of brdf_rough_plastic.rchit:234-240.
-      V3<R> bw = rpEval(l.d, st.V, st.ffN, st.X, st.Y, kd, ks, eta, fdrInt, /*invEta2<-*/ ax,
-                        /*ax<-*/ ay, /*ay<-*/ invEta2, EArea);
-      R bsdfPdf = rpPdf(l.d, st.V, st.ffN, st.X, st.Y, /*ax<-*/ eta, /*ay<-*/ ax, /*eta<-*/ ay, ssw, l.flags);
+      V3<R> bw;
+      R bsdfPdf;
+      rpLightTerms(l.d, st.V, st.ffN, st.X, st.Y, kd, eta, fdrInt, invEta2, ax, ay, ssw, l.flags, bw, bsdfPdf);
       R mis = powerHeuristic(l.pdf, bsdfPdf);
       Ld = Ld + mis * bw * radiance * pl.throughput / (l.pdf + K<R>::EPS);
     }
@@ -2976,6 +2986,19 @@ float orc_bsdf_pdf(const GpuMaterial* m, const float Lp[3], const float Vp[3], u
 }
 
 float orc_conductor_reflectance(float eta, float k, float cosThetaI) { return conductorReflectance(eta, k, cosThetaI); }
+
+// The rough-plastic light-sample terms exactly as chsRoughPlastic forms them
+// (permuted eval and pdf), in the probe frame.
+void orc_rough_plastic_light_terms(const GpuMaterial* m, const float Lp[3], const float Vp[3], uint32_t lightFlags,
+                                   float bw[3], float* bsdfPdf) {
+  V3<float> N, X, Y, L = load3<float>(Lp), V = load3<float>(Vp), w;
+  localFrame(V, N, X, Y);
+  const float eta = m->ior, invEta2 = 1.f / (eta * eta);
+  const float ax = std::max(K<float>::EPS, m->anisoAlpha[0]), ay = std::max(K<float>::EPS, m->anisoAlpha[1]);
+  const V3<float> kd = load3<float>(m->diffuse);
+  rpLightTerms(L, V, N, X, Y, kd, eta, m->radiance[0], invEta2, ax, ay, luminance(kd), lightFlags, w, *bsdfPdf);
+  bw[0] = w.x, bw[1] = w.y, bw[2] = w.z;
+}
 
 // sampleOneLight (sample_light.glsl:10-82) for one light at shading point p.
 void orc_sample_light(const GpuLight* l, float r0, float r1, const float p[3], float rad[3], float d[3], float* dist,

@@ -98,6 +98,10 @@ float orc_compute_diffuse_fresnel(float ior, int sample_count);
 void orc_bsdf_eval(const GpuMaterial* m, const float L[3], const float V[3], uint32_t flags,
                    float out[3]);
 float orc_bsdf_pdf(const GpuMaterial* m, const float L[3], const float V[3], uint32_t flags);
+/* The rough-plastic light-sample terms (brdf_rough_plastic.rchit:229-241):
+ * eval and pdf with the shader's permuted arguments, as the render uses them. */
+void orc_rough_plastic_light_terms(const GpuMaterial* m, const float L[3], const float V[3], uint32_t light_flags,
+                                   float bw[3], float* bsdf_pdf);
 /* Shirley's conductor Fresnel (brdf_conductor.rchit:14-34). */
 float orc_conductor_reflectance(float eta, float k, float cos_theta_i);
 /* Sun & sky (utils/sun_and_sky.glsl): set the scene's record; evaluate the

@@ -59,6 +59,7 @@ def lib():
         "orc_bsdf_eval": (None, [P(A.GpuMaterial), F, F, C.c_uint32, F]),
         "orc_bsdf_pdf": (C.c_float, [P(A.GpuMaterial), F, F, C.c_uint32]),
         "orc_conductor_reflectance": (C.c_float, [C.c_float, C.c_float, C.c_float]),
+        "orc_rough_plastic_light_terms": (None, [P(A.GpuMaterial), F, F, C.c_uint32, F, F]),
         "orc_bsdf_sample": (None, [P(A.GpuMaterial), F, P(C.c_uint32), F, F, F, P(C.c_uint32)]),
         "orc_sample_envmap": (None, [vp, F, C.c_float, C.c_float, C.c_float, F, F, F]),
         "orc_pdf_envmap": (C.c_float, [vp, F, F]),
