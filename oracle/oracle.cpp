@@ -2342,30 +2342,18 @@ struct Counters {
 };
 
 // One sample-frame of one pixel: raytrace.projective.rgen:26-179.
+// Camera ray at raster position (px, py), raytrace.projective.rgen:30-87:
+// perspective (with the thin lens when aperture > 0, lens sample (l0, l1))
+// or OpenCV intrinsics.  The lens direction pFocus - pLens is not
+// normalised, as in the shader.
 template <class R>
-void renderPixel(const RCtx& c, uint32_t x, uint32_t y, int32_t frame, R& wsum, V3<R>& rws, V3<R>* aov,
-                 R& firstT, V3<R>& Lout, Counters& cnt) {
-  const GpuCamera& cam = *c.cam;
-  const GpuPushConstantRaytrace& pc = *c.pc;
-  Payload<R> pl;
-  pl.seed = xxhash32Seed(x, y, uint32_t(frame));
+void cameraRayAt(const GpuCamera& cam, R px, R py, R l0, R l1, V3<R>& ro, V3<R>& rd) {
   V3<R> origin = transformPoint(cam.cameraToWorld, V3<R>(0));
-  R pcx = R(x), pcy = R(y);
-  R jx, jy;
-  if (frame == 0) {
-    jx = jy = R(0.5);
-  } else {
-    jx = rnd<R>(pl.seed);
-    jy = rnd<R>(pl.seed);
-  }
-  R px = pcx + jx, py = pcy + jy;
-  V3<R> ro, rd;
   if (cam.type == CameraTypePerspective) {
     ro = origin;
     V3<R> pCamera = transformPoint(cam.rasterToCamera, V3<R>(px, py, R(0)));
     V3<R> r = makeNormal(pCamera);
     if (cam.aperture > 0.f) {
-      R l0 = rnd<R>(pl.seed), l1 = rnd<R>(pl.seed);
       R dx, dy;
       concentricSampleDisk(l0, l1, dx, dy);
       V3<R> pLens(R(cam.aperture) * dx, R(cam.aperture) * dy, R(0));
@@ -2381,6 +2369,38 @@ void renderPixel(const RCtx& c, uint32_t x, uint32_t y, int32_t frame, R& wsum, 
     R ry = (py - R(cam.fxfycxcy[3])) / R(cam.fxfycxcy[1]);
     rd = transformDirection(cam.cameraToWorld, V3<R>(rx, ry, R(1)));
   }
+}
+
+// The camera sample of pixel (x, y) in frame `frame` with the seed's draws in
+// the shader's order: jitter (rgen:41, none on frame 0), then the lens
+// (rgen:57, only for a perspective camera with aperture > 0).
+template <class R>
+void cameraSample(const GpuCamera& cam, uint32_t x, uint32_t y, int32_t frame, uint32_t& seed, R& jx, R& jy,
+                  V3<R>& ro, V3<R>& rd) {
+  if (frame == 0) {
+    jx = jy = R(0.5);
+  } else {
+    jx = rnd<R>(seed);
+    jy = rnd<R>(seed);
+  }
+  R l0 = 0, l1 = 0;
+  if (cam.type == CameraTypePerspective && cam.aperture > 0.f) {
+    l0 = rnd<R>(seed);
+    l1 = rnd<R>(seed);
+  }
+  cameraRayAt(cam, R(x) + jx, R(y) + jy, l0, l1, ro, rd);
+}
+
+template <class R>
+void renderPixel(const RCtx& c, uint32_t x, uint32_t y, int32_t frame, R& wsum, V3<R>& rws, V3<R>* aov,
+                 R& firstT, V3<R>& Lout, Counters& cnt) {
+  const GpuCamera& cam = *c.cam;
+  const GpuPushConstantRaytrace& pc = *c.pc;
+  Payload<R> pl;
+  pl.seed = xxhash32Seed(x, y, uint32_t(frame));
+  R jx, jy;
+  V3<R> ro, rd;
+  cameraSample(cam, x, y, frame, pl.seed, jx, jy, ro, rd);
   pl.ro = ro;
   pl.rd = rd;
   pl.stop = false;
@@ -2572,6 +2592,13 @@ void orc_camera_perspective(uint32_t w, uint32_t h, float fov, const float eye[3
   out->focalDistance = focal;
 }
 
+void orc_camera_ray_lens(const GpuCamera* cam, float px, float py, float l0, float l1, float o[3], float d[3]) {
+  V3<float> ro, rd;
+  cameraRayAt(*cam, px, py, l0, l1, ro, rd);
+  o[0] = ro.x, o[1] = ro.y, o[2] = ro.z;
+  d[0] = rd.x, d[1] = rd.y, d[2] = rd.z;
+}
+
 void orc_camera_ray(const GpuCamera* cam, float px, float py, float o[3], float d[3]) {
   V3<float> origin = transformPoint(cam->cameraToWorld, V3<float>(0));
   V3<float> pCamera = transformPoint(cam->rasterToCamera, V3<float>(px, py, 0.f));
@@ -2722,19 +2749,12 @@ int orc_trace_any(const orc_scene* s, const float o[3], const float d[3], float 
 int orc_first_hit(const orc_scene* s, const GpuCamera* cam, uint32_t w, uint32_t h, ash_hit* out, int threads) {
   parallelRows(0, h, threads, [&](uint32_t y) {
     for (uint32_t x = 0; x < w; ++x) {
-      float o[3], d[3];
-      if (cam->type == CameraTypePerspective) {
-        orc_camera_ray(cam, float(x) + 0.5f, float(y) + 0.5f, o, d);
-      } else {
-        V3<float> origin = transformPoint(cam->cameraToWorld, V3<float>(0));
-        float rx = (float(x) + 0.5f - cam->fxfycxcy[2]) / cam->fxfycxcy[0];
-        float ry = (float(y) + 0.5f - cam->fxfycxcy[3]) / cam->fxfycxcy[1];
-        V3<float> dir = transformDirection(cam->cameraToWorld, V3<float>(rx, ry, 1.f));
-        o[0] = origin.x, o[1] = origin.y, o[2] = origin.z;
-        d[0] = dir.x, d[1] = dir.y, d[2] = dir.z;
-      }
-      V3<float> dd = load3<float>(d);
-      HitR hr = traceClosest<float>(*s, load3<float>(o), dd, K<float>::MINIMUM, K<float>::INF, false);
+      // frame 0: pixel centre, and the lens draws of a thin-lens camera
+      uint32_t seed = xxhash32Seed(x, y, 0u);
+      float jx, jy;
+      V3<float> oo, dd;
+      cameraSample(*cam, x, y, 0, seed, jx, jy, oo, dd);
+      HitR hr = traceClosest<float>(*s, oo, dd, K<float>::MINIMUM, K<float>::INF, false);
       fillHit(s, hr, dd, &out[size_t(y) * w + x]);
     }
   });

@@ -47,6 +47,10 @@ void orc_camera_perspective(uint32_t w, uint32_t h, float fov_deg, const float e
                             GpuCamera* out);
 /* Primary ray of pixel (x,y) with jitter (jx,jy), no lens sample (rgen:30-90). */
 void orc_camera_ray(const GpuCamera* cam, float px, float py, float o[3], float d[3]);
+/* Camera ray of either camera type at raster (px,py); a perspective camera
+ * with aperture > 0 takes the lens sample (l0,l1) (rgen:30-87). */
+void orc_camera_ray_lens(const GpuCamera* cam, float px, float py, float l0, float l1, float o[3],
+                         float d[3]);
 
 /* Env tables (texture.cpp:144-226): marginal H x 2 (x,y of column 0),
  * conditional W*H x 2. */
@@ -67,7 +71,8 @@ int orc_trace_closest(const orc_scene* s, const float o[3], const float d[3], fl
 int orc_trace_any(const orc_scene* s, const float o[3], const float d[3], float tmin,
                   float tmax, int brute_force);
 
-/* Frame-0 primary hits of a w x h film (first-hit parity buffer). */
+/* Frame-0 primary hits of a w x h film (first-hit parity buffer): pixel
+ * centres, plus the frame-0 lens draws of a thin-lens camera. */
 int orc_first_hit(const orc_scene* s, const GpuCamera* cam, uint32_t w, uint32_t h,
                   ash_hit* out, int threads);
 

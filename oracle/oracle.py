@@ -39,6 +39,7 @@ def lib():
         "orc_camera_perspective": (None, [C.c_uint32, C.c_uint32, C.c_float, F, F, F, C.c_float, C.c_float, F,
                                           P(A.GpuCamera)]),
         "orc_camera_ray": (None, [P(A.GpuCamera), C.c_float, C.c_float, F, F]),
+        "orc_camera_ray_lens": (None, [P(A.GpuCamera), C.c_float, C.c_float, C.c_float, C.c_float, F, F]),
         "orc_env_tables": (None, [C.c_uint32, C.c_uint32, F, F, F]),
         "orc_texture_bilinear": (None, [C.c_uint32, C.c_uint32, F, C.c_float, C.c_float, F]),
         "orc_scene_create": (C.c_int, [P(A.SceneDesc), P(vp)]),

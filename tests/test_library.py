@@ -229,6 +229,39 @@ def test_tonemap_filmic_closed_form():
     np.testing.assert_allclose(y[0, :3], (v * (6.2 * v + 0.5)) / (v * (6.2 * v + 1.7) + 0.06), rtol=1e-6)
 
 
+def _tonemap(kind, x):
+    y = np.zeros((1, len(x), 4), np.float32)
+    y[0, :, :3] = np.asarray(x, np.float32)[:, None]
+    A.lib().ash_tonemap(kind, len(x), 1, A.fptr(y))
+    return y[0, :, 0].astype(np.float64)
+
+
+def test_tonemap_gamma_reinhard_aces_and_pbrt_closed_forms():
+    """post.idle.frag:75-96 with tonemapping.glsl:32-81:
+    gamma    (x / (1 + x/1.5))^(1/2.2): 0 at 0, tends to 1.5^(1/2.2) for large x;
+    reinhard Hejl & Burgess-Dawson's filmic fit, gamma built in: 0 up to the
+             0.004 toe, (v(6.2v + .5)) / (v(6.2v + 1.7) + .06), tends to 1;
+    aces     Narkowicz's fit (x(2.51x + .03)) / (x(2.43x + .59) + .14),
+             clamped, then the 1/2.2 gamma: 0 at 0, 1 once the fit passes 1;
+    pbrt     the sRGB transfer function: 12.92 x below 0.0031308, else
+             1.055 x^(1/2.4) - 0.055 (continuous there; 0.5 -> 0.735357,
+             1 -> 1)."""
+    g = _tonemap(1, [0.0, 1.5, 1e6])
+    np.testing.assert_allclose(g, [0.0, 0.75 ** (1 / 2.2), 1.5 ** (1 / 2.2)], rtol=1e-5)
+    r = _tonemap(2, [0.0, 0.004, 0.5, 1e4])
+    v = 0.496
+    assert r[0] == 0.0 and r[1] == 0.0 and abs(r[3] - 1.0) < 1e-4
+    np.testing.assert_allclose(r[2], v * (6.2 * v + 0.5) / (v * (6.2 * v + 1.7) + 0.06), rtol=1e-6)
+    a = _tonemap(3, [0.0, 1.0, 100.0])
+    fit1 = 2.54 / 3.16
+    np.testing.assert_allclose(a, [0.0, fit1 ** (1 / 2.2), 1.0], rtol=1e-6, atol=1e-7)
+    p = _tonemap(5, [0.001, 0.0031308, 0.5, 1.0])
+    np.testing.assert_allclose(p, [0.01292, 1.055 * 0.0031308 ** (1 / 2.4) - 0.055, 0.735357, 1.0], rtol=2e-6)
+    assert abs(12.92 * 0.0031308 - p[1]) < 1e-6  # the two pieces meet
+    n = _tonemap(0, [0.25, 7.0])
+    np.testing.assert_array_equal(n, [0.25, 7.0])
+
+
 def _pcg3d(v):
     v = [(x * 1664525 + 1013904223) & 0xFFFFFFFF for x in v]
     m = 0xFFFFFFFF
