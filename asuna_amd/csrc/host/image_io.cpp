@@ -2,7 +2,8 @@
 // specifications (Radiance RGBE, OpenEXR scanline files, PNG, NumPy .npy):
 //  - readImage  (src/core/texture.cpp:306-338): .hdr (RGBE -> float as
 //    mantissa * 2^(e-136)), .exr (HALF/FLOAT channels; NONE/RLE/ZIPS/ZIP),
-//    .png (8/16-bit; channels v/255 raised to `gamma`, alpha linear).
+//    .png (8/16-bit; channels v/255 raised to `gamma`, alpha linear),
+//    .jpg (jpeg.cpp; channels v/255 raised to `gamma`, alpha 1).
 //  - writeImage (src/core/texture.cpp:340-372): .exr as HALF R,G,B with ZIP
 //    compression (RgbaOutputFile WRITE_RGB), .hdr RGBE, .png 8-bit RGBA with
 //    v = int(clamp(x*255 + 0.5)) (gamma 1), .npy float32.
@@ -677,10 +678,7 @@ bool readImage(const std::string& path, float gamma, Image& out, std::string& er
   if (e == "hdr") ok = readHdr(b, out, err);
   else if (e == "exr") ok = readExr(b, out, err);
   else if (e == "png") ok = readPng(b, gamma, out, err);
-  else {
-    err = "Scene Error: jpg decoding is not available in this build (" + path + ")";
-    ok = false;
-  }
+  else ok = readJpeg(b, gamma, out, err);
   if (!ok) err = "Scene Error: failed to read image file " + path + ": " + err;
   return ok;
 }

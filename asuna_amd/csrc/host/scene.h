@@ -90,6 +90,8 @@ void buildEnvTables(uint32_t w, uint32_t h, const float* rgba, std::vector<float
                     std::vector<float>& cond);
 // Image IO (src/core/texture.cpp:16-75, 306-372).
 bool readImage(const std::string& path, float gamma, Image& out, std::string& err);
+// Baseline / progressive Huffman JPEG (jpeg.cpp); channels (v/255)^gamma, alpha 1.
+bool readJpeg(const std::vector<unsigned char>& bytes, float gamma, Image& out, std::string& err);
 bool writeImage(const std::string& path, uint32_t w, uint32_t h, const float* rgba, std::string& err);
 bool writeNpy(const std::string& path, const std::vector<unsigned long>& shape, const float* data,
               std::string& err);

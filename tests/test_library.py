@@ -325,3 +325,59 @@ def test_multishot_scene_has_64_shots_with_own_fov(multishot_scene):
     eyes = np.array([[c.cameraToWorld[12], c.cameraToWorld[13], c.cameraToWorld[14]] for c in cams])
     r = np.linalg.norm(eyes, axis=1)
     assert r.min() >= 5 - 1e-4 and r.max() <= 9 + 1e-4
+
+
+def test_jpeg_textures_decode_like_libjpeg(tmp_path):
+    """.jpg textures (texture.cpp:308-327): baseline and progressive files,
+    4:4:4 / 4:2:2 / 4:2:0 chroma, grey, restart markers and sizes that are
+    not whole MCUs decode to within 3/255 of libjpeg's islow decoder (via
+    Pillow) and 0.25/255 on average; channels are (v/255)^gamma, alpha 1."""
+    Image = pytest.importorskip("PIL.Image")
+    rng = np.random.default_rng(8)
+    cases = [dict(quality=90, subsampling=2), dict(quality=95, subsampling=0), dict(quality=80, subsampling=1),
+             dict(quality=90, subsampling=2, progressive=True), dict(quality=97, subsampling=0, progressive=True),
+             dict(quality=75, subsampling=2, restart_marker_blocks=3)]
+    for (w, h) in [(123, 77), (17, 9)]:
+        y, x = np.mgrid[0:h, 0:w]
+        base = np.stack([128 + 100 * np.sin(x / 7.0 + y / 11.0), 128 + 90 * np.cos(x / 5.0 - y / 9.0),
+                         (x * 3 + y * 2) % 256], -1) + rng.normal(0, 12, (h, w, 3))
+        rgb = np.clip(base, 0, 255).astype(np.uint8)
+        for mode, arr in (("RGB", rgb), ("L", rgb[..., 0])):
+            for kw in cases:
+                p = str(tmp_path / "t.jpg")
+                Image.fromarray(arr, mode).save(p, "JPEG", **kw)
+                ref = np.asarray(Image.open(p).convert("RGB")).astype(np.float64)
+                got = A.read_image(p)
+                assert got.shape == (h, w, 4) and np.all(got[..., 3] == 1.0)
+                d = np.abs(got[..., :3] * 255.0 - ref)
+                assert d.max() <= 3.0 and d.mean() <= 0.25, (w, h, mode, kw, d.max(), d.mean())
+    # gamma: the same 8-bit values raised to the texture's gamma
+    g = A.read_image(p, gamma=2.2)
+    lin = A.read_image(p)
+    np.testing.assert_allclose(g[..., :3], np.round(lin[..., :3] * 255) ** 2.2 / 255 ** 2.2, rtol=1e-5, atol=1e-7)
+    # a flat colour survives the round trip within one level
+    Image.fromarray(np.full((24, 40, 3), (200, 40, 90), np.uint8), "RGB").save(p, "JPEG", quality=100, subsampling=0)
+    np.testing.assert_allclose(A.read_image(p)[..., :3] * 255.0, np.broadcast_to([200, 40, 90], (24, 40, 3)), atol=1.0)
+
+
+def test_loader_reads_a_jpg_texture(tmp_path):
+    Image = pytest.importorskip("PIL.Image")
+    Image.fromarray(np.full((8, 8, 3), (255, 128, 0), np.uint8), "RGB").save(str(tmp_path / "t.jpg"), "JPEG",
+                                                                              quality=100, subsampling=0)
+    (tmp_path / "tri.obj").write_text("v 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\nvt 1 0\nvt 0 1\nvn 0 0 1\n"
+                                      "f 1/1/1 2/2/1 3/3/1\n")
+    js = {"state": {"path_tracing": {"spp": 1}}, "camera": {"type": "perspective", "film": {"resolution": [8, 8]}},
+          "textures": [{"name": "t", "path": "t.jpg"}],
+          "materials": [{"name": "m", "type": "brdf_lambertian", "diffuse_texture": "t"}],
+          "meshes": [{"name": "tri", "path": "tri.obj"}], "instances": [{"mesh": "tri", "material": "m"}],
+          "shots": [{"type": "lookat", "eye": [0, 0, 2], "lookat": [0, 0, 0], "up": [0, 1, 0]}]}
+    import json
+    (tmp_path / "s.json").write_text(json.dumps(js))
+    s = A.HostScene(str(tmp_path / "s.json"))
+    tid = [s.desc.materials[i].diffuseTextureId for i in range(s.desc.num_materials)
+           if s.desc.materials[i].diffuseTextureId >= 0]
+    assert len(tid) == 1
+    t = s.desc.textures[tid[0]]
+    assert (t.width, t.height) == (8, 8)
+    px = np.ctypeslib.as_array(t.rgba, (t.height, t.width, 4))
+    np.testing.assert_allclose(px[0, 0, :3], np.array([255, 128, 0]) / 255.0, atol=1.5 / 255)
