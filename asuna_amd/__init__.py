@@ -235,6 +235,8 @@ def lib():
         "ash_set_sun_and_sky": (C.c_int, [vp, P(GpuSunAndSky)]),
         "ash_render": (C.c_int, [vp, P(GpuCamera), P(GpuPushConstantRaytrace), C.c_int32, C.c_int32,
                                  P(RenderOpts), P(Stats)]),
+        "ash_render_noalloc": (C.c_int, [vp, P(GpuCamera), P(GpuPushConstantRaytrace), C.c_int32, C.c_int32,
+                                         C.c_uint32, C.c_uint32, vp, P(Stats)]),
         "ash_render_shots": (C.c_int, [vp, C.c_uint32, P(GpuCamera), P(GpuPushConstantRaytrace), C.c_int32,
                                        P(RenderOpts), P(Stats)]),
         "ash_autotune": (C.c_int, [vp, C.c_uint32, P(GpuCamera), P(GpuPushConstantRaytrace), C.c_int32,
@@ -460,6 +462,21 @@ class Renderer:
         self._check(lib().ash_render(self._c, C.byref(cam), C.byref(pc), first_frame, n_frames, C.byref(opts),
                                      C.byref(st)), "render")
         return st.as_dict()
+
+    def render_noalloc(self, cam, pc, first_frame, n_frames, width, height, images) -> dict:
+        """ash_render_noalloc into the device buffer `images` (an integer
+        device address of 9 planes of width x height RGBA32F)."""
+        st = Stats()
+        self._check(lib().ash_render_noalloc(self._c, C.byref(cam), C.byref(pc), first_frame, n_frames, width, height,
+                                             C.c_void_p(images), C.byref(st)), "render_noalloc")
+        return st.as_dict()
+
+    def output_device_ptr(self, image: int = 0) -> int:
+        """Device address of output image `image` of the film (planes are
+        contiguous, so image 0's address is the whole film's)."""
+        p = C.c_void_p()
+        self._check(lib().ash_output_device_ptr(self._c, image, C.byref(p)), "output_device_ptr")
+        return int(p.value)
 
     def render_shots(self, cams, pcs, n_frames, queue_paths=0, timing=False, blocks_per_cu=0, graphs=True,
                      count_traversal=False, lds_stack=0, traversal=0) -> dict:

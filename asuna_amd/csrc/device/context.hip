@@ -1,10 +1,12 @@
 // C ABI of the render library (include/asuna_hip.h): device context, scene
 // upload, BVH build, wavefront render loop, output readback.
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -117,6 +119,9 @@ struct ash_ctx {
   GraphEntry graphs[4];
   uint64_t graphClock = 0;
   bool graphsBroken = false;
+  // device address of this GPU's allocation-free render scratch (null when the lookup failed)
+  unsigned char* scratch = nullptr;
+  hipError_t scratchErr = hipSuccess;
 
   ~ash_ctx() {
     if (nodes) (void)hipFree(nodes);
@@ -287,8 +292,8 @@ static int checkPc(ash_ctx* c, const GpuPushConstantRaytrace* pc) {
 // extend, shade, shadow.  `shadeRays` sizes the shading grid.
 // `count` selects the counting kernels; with `st` set, every traversal and
 // shading launch is bracketed by HIP events and its time added to st.
-static int launchLoop(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid,
-                      uint32_t lds, uint32_t scheme, uint32_t shadeRays, bool count, ash_stats* st) {
+static int launchLoop(ash_ctx* c, const Wavefront& wf, const SceneView& sc, const BatchDesc& b, uint32_t iters,
+                      uint32_t grid, uint32_t lds, uint32_t scheme, uint32_t shadeRays, bool count, ash_stats* st) {
   hipStream_t s = c->stream;
   hipEvent_t ev[2] = {nullptr, nullptr};
   if (st) {
@@ -318,23 +323,23 @@ static int launchLoop(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint3
   int cur = 0;
   if (!canFuseTrace(sc, scheme)) {
     for (uint32_t it = 0; it < iters; ++it) {
-      launchResetCounters(c->wf, cur, s);
-      timed(tExt, [&] { launchExtend(sc, c->wf, cur, lds, scheme, grid, count, s); });
-      timed(tShade, [&] { launchShade(sc, b, c->wf, cur, shadeRays, s); });
-      timed(tShadow, [&] { launchShadow(sc, c->wf, scheme, grid, count, s); });
+      launchResetCounters(wf, cur, s);
+      timed(tExt, [&] { launchExtend(sc, wf, cur, lds, scheme, grid, count, s); });
+      timed(tShade, [&] { launchShade(sc, b, wf, cur, shadeRays, s); });
+      timed(tShadow, [&] { launchShadow(sc, wf, scheme, grid, count, s); });
       cur = 1 - cur;
     }
   } else {
-    launchResetCounters(c->wf, 0, s);
-    timed(tExt, [&] { launchExtend(sc, c->wf, 0, lds, scheme, grid, count, s); });
+    launchResetCounters(wf, 0, s);
+    timed(tExt, [&] { launchExtend(sc, wf, 0, lds, scheme, grid, count, s); });
     for (uint32_t it = 0; it < iters; ++it) {
-      if (it > 0) launchResetShade(c->wf, cur, s);
-      timed(tShade, [&] { launchShade(sc, b, c->wf, cur, shadeRays, s); });
+      if (it > 0) launchResetShade(wf, cur, s);
+      timed(tShade, [&] { launchShade(sc, b, wf, cur, shadeRays, s); });
       if (it + 1 < iters) {
-        launchResetExtend(c->wf, s);
-        timed(tTrace, [&] { launchTrace(sc, c->wf, 1 - cur, lds, grid, count, s); });
+        launchResetExtend(wf, s);
+        timed(tTrace, [&] { launchTrace(sc, wf, 1 - cur, lds, grid, count, s); });
       } else {
-        timed(tShadow, [&] { launchShadow(sc, c->wf, scheme, grid, count, s); });
+        timed(tShadow, [&] { launchShadow(sc, wf, scheme, grid, count, s); });
       }
       cur = 1 - cur;
     }
@@ -350,7 +355,7 @@ static int launchLoop(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint3
 // count exit), so one captured loop serves every batch size.
 static void launchBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, uint32_t iters, uint32_t grid,
                           uint32_t lds, uint32_t scheme) {
-  (void)launchLoop(c, sc, b, iters, grid, lds, scheme, c->wf.capacity, false, nullptr);
+  (void)launchLoop(c, c->wf, sc, b, iters, grid, lds, scheme, c->wf.capacity, false, nullptr);
 }
 
 // Replays the bounce loop of a batch from a captured HIP graph (captured on
@@ -429,7 +434,7 @@ static int runBatch(ash_ctx* c, const BatchDesc& b, int maxDepth, uint32_t grid,
   if (!eager && graphBounces(c, sc, b, iters, grid, lds, scheme)) {
     st->graph_launches++;
   } else {
-    int rc = launchLoop(c, sc, b, iters, grid, lds, scheme, b.n, count, timing ? st : nullptr);
+    int rc = launchLoop(c, c->wf, sc, b, iters, grid, lds, scheme, b.n, count, timing ? st : nullptr);
     if (rc) return fail(c, rc, "bounce loop: HIP event failure");
   }
   launchAccumulate(b, c->wf, s);
@@ -579,6 +584,88 @@ static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots
   return ASH_OK;
 }
 
+// ------------------------------------------------- allocation-free render
+// ash_render_noalloc keeps its wavefront (path state, queues, counters, the
+// batch's work unit and shot record, traversal stack spill) in one fixed
+// array per GPU instead of context buffers.  Host code never passes the
+// array's name to a kernel: it names a host-side stand-in.  The device
+// address comes from hipGetSymbolAddress, once per device with that device
+// current (ash_create), and only that address reaches kernels.
+constexpr size_t kNoAllocScratchBytes = size_t(96) << 20;
+constexpr uint32_t kNoAllocPaths = 1u << 18;  // paths in flight per batch
+constexpr uint32_t kNoAllocBlocksPerCU = 2;   // persistent traversal blocks per CU
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+
+constexpr int kMaxScratchDevices = 64;
+static std::mutex gScratchLookupMu;
+static unsigned char* gScratchAddr[kMaxScratchDevices];
+// Every call on one GPU shares its array; calls are serialised per device.
+static std::mutex gScratchUseMu[kMaxScratchDevices];
+
+static hipError_t scratchAddress(int device, unsigned char** out) {
+  *out = nullptr;
+  if (device < 0 || device >= kMaxScratchDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lk(gScratchLookupMu);
+  if (!gScratchAddr[device]) {
+    void* p = nullptr;
+    hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    gScratchAddr[device] = static_cast<unsigned char*>(p);
+  }
+  *out = gScratchAddr[device];
+  return hipSuccess;
+}
+
+__global__ void kNoAllocReset(uint32_t* ctr, unsigned long long* cnt64) {
+  for (uint32_t i = threadIdx.x; i < uint32_t(kNumCtr); i += blockDim.x) ctr[i] = 0;
+  if (threadIdx.x < uint32_t(kNumCnt64)) cnt64[threadIdx.x] = 0;
+}
+
+// Stores the batch's work unit and shot record (kernel arguments) into the
+// scratch, in stream order after the previous batch's readers.
+__global__ void kNoAllocPut(WorkUnit* unit, WorkUnit u, ShotParams* shot, ShotParams sp) {
+  if (threadIdx.x == 0) {
+    *unit = u;
+    *shot = sp;
+  }
+}
+
+// Carves the scratch: per-path arrays for kNoAllocPaths paths, counters,
+// one work unit, one shot record, then the traversal spill of `threads`
+// persistent threads with `cap` entries each.  Returns the bytes used.
+static size_t noAllocLayout(unsigned char* base, uint32_t threads, uint32_t cap, Wavefront& w, WorkUnit** unit,
+                            ShotParams** shot) {
+  size_t off = 0;
+  auto take = [&](size_t bytes) {
+    unsigned char* p = base + off;
+    off += (bytes + 255) & ~size_t(255);
+    return p;
+  };
+  const size_t n = kNoAllocPaths;
+  std::memset(&w, 0, sizeof w);
+  w.capacity = kNoAllocPaths;
+  w.ro = reinterpret_cast<float4*>(take(n * sizeof(float4)));
+  w.rd = reinterpret_cast<float4*>(take(n * sizeof(float4)));
+  w.thr = reinterpret_cast<float4*>(take(n * sizeof(float4)));
+  w.rad = reinterpret_cast<float4*>(take(n * sizeof(float4)));
+  w.ld = reinterpret_cast<float4*>(take(n * sizeof(float4)));
+  w.hitF = reinterpret_cast<float4*>(take(n * sizeof(float4)));
+  w.shO = reinterpret_cast<float4*>(take(n * sizeof(float4)));
+  w.shD = reinterpret_cast<float4*>(take(n * sizeof(float4)));
+  w.info = reinterpret_cast<uint4*>(take(n * sizeof(uint4)));
+  w.hitI = reinterpret_cast<int4*>(take(n * sizeof(int4)));
+  for (int q = 0; q < 3; ++q) w.queue[q] = reinterpret_cast<int*>(take(n * sizeof(int)));
+  w.ctr = reinterpret_cast<uint32_t*>(take(kNumCtr * sizeof(uint32_t)));
+  w.cnt64 = reinterpret_cast<unsigned long long*>(take(kNumCnt64 * sizeof(unsigned long long)));
+  *unit = reinterpret_cast<WorkUnit*>(take(sizeof(WorkUnit)));
+  *shot = reinterpret_cast<ShotParams*>(take(sizeof(ShotParams)));
+  w.spill = reinterpret_cast<int*>(take(size_t(threads) * cap * sizeof(int)));
+  w.spillT = reinterpret_cast<float*>(take(size_t(threads) * cap * sizeof(float)));
+  w.spillThreads = threads;
+  w.spillCap = cap;
+  return off;
+}
+
 extern "C" {
 
 int ash_create(int device, ash_ctx** out) {
@@ -595,6 +682,7 @@ int ash_create(int device, ash_ctx** out) {
   }
   hipDeviceProp_t prop;
   if (hipGetDeviceProperties(&prop, device) == hipSuccess) c->numCU = prop.multiProcessorCount;
+  c->scratchErr = scratchAddress(device, &c->scratch);
   *out = c;
   return ASH_OK;
 }
@@ -884,6 +972,83 @@ int ash_render(ash_ctx* c, const GpuCamera* cam, const GpuPushConstantRaytrace* 
   if (o && o->depthOutChannel >= int32_t(ASH_NUM_OUTPUT_IMAGES - 1)) return fail(c, ASH_E_INVALID, "depth channel out of range");
   ShotParams sp = makeShot(c, cam, pc, o, c->images.p, c->width, c->height);
   return renderCommon(c, 1, &sp, first, n, o, st);
+}
+
+int ash_render_noalloc(ash_ctx* c, const GpuCamera* cam, const GpuPushConstantRaytrace* pc, int32_t first, int32_t n,
+                       uint32_t w, uint32_t h, void* images, ash_stats* st) {
+  if (!c || !cam || !pc || !images || !w || !h) return ASH_E_INVALID;
+  if (!c->hasScene || !c->hasAccel) return fail(c, ASH_E_STATE, "render called before upload_scene/build_accel");
+  if (n <= 0 || first < 0 || int64_t(first) + n > int64_t(INT32_MAX)) return fail(c, ASH_E_INVALID, "bad frame range");
+  if (uint64_t(w) * h > (uint64_t(1) << 30)) return fail(c, ASH_E_INVALID, "film above 2^30 pixels");
+  if (!c->scratch)
+    return fail(c, ASH_E_HIP, "allocation-free scratch unavailable on device %d: %s", c->device,
+                hipGetErrorString(c->scratchErr));
+  if (int rc = checkPc(c, pc)) return rc;
+  const uint32_t grid = uint32_t(std::max(1, c->numCU)) * kNoAllocBlocksPerCU;
+  const uint32_t cap =
+      std::max<uint32_t>(uint32_t(kSpill), c->stackNeed > uint32_t(kStackLdsMin) ? c->stackNeed - kStackLdsMin : 0);
+  Wavefront wf;
+  WorkUnit* unitDev = nullptr;
+  ShotParams* shotDev = nullptr;
+  const size_t need = noAllocLayout(c->scratch, grid * kTraceBlock, cap, wf, &unitDev, &shotDev);
+  if (need > kNoAllocScratchBytes)
+    return fail(c, ASH_E_INVALID, "allocation-free render needs %zu bytes of scratch (%u traversal threads x %u stack "
+                "entries), above its %zu-byte limit", need, grid * kTraceBlock, cap, kNoAllocScratchBytes);
+  std::lock_guard<std::mutex> use(gScratchUseMu[c->device]);
+  HIPC(c, hipSetDevice(c->device));
+  hipStream_t s = c->stream;
+  const ShotParams sp = makeShot(c, cam, pc, nullptr, static_cast<float4*>(images), w, h);
+  const SceneView sc = viewOf(c);
+  const uint32_t P = w * h, iters = uint32_t(std::max(pc->maxPathDepth, 1));
+  const int32_t end = first + n;
+  auto t0 = std::chrono::steady_clock::now();
+  kNoAllocReset<<<1, 256, 0, s>>>(wf.ctr, wf.cnt64);
+  uint64_t batches = 0;
+  int rc = ASH_OK;
+  // The batches of a single-shot render (planBatches with one shot): whole
+  // frames grouped while a frame fits the queue, else runs of pixels.
+  auto batch = [&](uint32_t pix0, uint32_t np, int32_t f0, uint32_t nf) {
+    if (rc != ASH_OK) return;
+    WorkUnit u;
+    std::memset(&u, 0, sizeof u);
+    u.shot = 0, u.pix0 = pix0, u.np = np, u.f0 = f0, u.nf = nf;
+    kNoAllocPut<<<1, 64, 0, s>>>(unitDev, u, shotDev, sp);
+    BatchDesc b;
+    std::memset(&b, 0, sizeof b);
+    b.n = np * nf;
+    b.nAcc = np;
+    b.nUnits = 1;
+    b.shots = 1;
+    b.units = unitDev;
+    b.shotTab = shotDev;
+    b.one = sp;
+    launchRaygen(b, wf, s);
+    rc = launchLoop(c, wf, sc, b, iters, grid, uint32_t(kStackLds), uint32_t(kSchemeXcdSlices), b.n, false, nullptr);
+    launchAccumulate(b, wf, s);
+    ++batches;
+  };
+  if (P <= kNoAllocPaths) {
+    const int32_t F = int32_t(std::min<uint32_t>(uint32_t(n), kNoAllocPaths / P));
+    for (int32_t f0 = first; f0 < end; f0 += F) batch(0, P, f0, uint32_t(std::min(F, end - f0)));
+  } else {
+    for (int32_t f = first; f < end; ++f)
+      for (uint32_t p0 = 0; p0 < P; p0 += kNoAllocPaths) batch(p0, std::min(kNoAllocPaths, P - p0), f, 1);
+  }
+  if (rc) return fail(c, rc, "allocation-free render: launch failure");
+  HIPC(c, hipGetLastError());
+  HIPC(c, hipStreamSynchronize(s));
+  if (st) {
+    unsigned long long cnt[kNumCnt64];
+    HIPC(c, hipMemcpy(cnt, wf.cnt64, sizeof cnt, hipMemcpyDeviceToHost));
+    std::memset(st, 0, sizeof *st);
+    st->render_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    st->paths = uint64_t(P) * uint64_t(n);
+    st->batches = batches;
+    st->extension_rays = cnt[4];
+    st->shadow_rays = cnt[5];
+    st->dielectric_shadow_rays = cnt[6];
+  }
+  return ASH_OK;
 }
 
 int ash_render_shots(ash_ctx* c, uint32_t nShots, const GpuCamera* cams, const GpuPushConstantRaytrace* pcs,

@@ -357,6 +357,26 @@ int ash_render(ash_ctx* ctx, const GpuCamera* cam, const GpuPushConstantRaytrace
                int32_t first_frame, int32_t n_frames, const ash_render_opts* opts,
                ash_stats* stats);
 
+/* Allocation-free variant of ash_render.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  It renders frames [first_frame, first_frame + n_frames) of
+ * one shot into the caller's device buffer `images`: ASH_NUM_OUTPUT_IMAGES
+ * planes of width*height RGBA32F, laid out like ash_render's film and
+ * accumulated the same way (frame 0 overwrites; later frames fold into the
+ * running mean, so a caller continuing a render passes the same buffer).
+ * The wavefront lives in one fixed 96 MiB device array per GPU, held
+ * until the process ends; at most 2^18 paths are in flight per batch and
+ * the traversal stacks of 2 persistent blocks per CU must fit beside them,
+ * else ASH_E_INVALID.  No hipMalloc / hipFree is called.  Every call on a
+ * GPU shares that GPU's array and keeps nothing from the previous call;
+ * calls on one GPU are serialised (a second caller waits).  Results are
+ * bit-identical to ash_render's.  Returns after the render completes;
+ * stats (optional) carry counts and host-timed render_ms. */
+int ash_render_noalloc(ash_ctx* ctx, const GpuCamera* cam, const GpuPushConstantRaytrace* pc,
+                       int32_t first_frame, int32_t n_frames, uint32_t width, uint32_t height,
+                       void* images, ash_stats* stats);
+
 /* Batched multi-shot render: n_shots shots of one scene, each with its own
  * camera and push constants, all frames [0, n_frames) in one wavefront.
  * Output images of shot s are kept on the device; read them with
