@@ -173,3 +173,42 @@ def test_film_path_equals_host_path(spheres_small):
     assert (film[..., 3] == 1).all()
     d.close()
     r.close()
+
+
+def test_film_1080p_workload(tmp_path):
+    """Inference at the workload's size: the 1920x1080 procedural scene
+    (5.24M triangles) rendered at 4 spp and denoised in HBM.  The film path
+    equals the host path on the whole padded 1088x1920 frame bit for bit,
+    and on four 128x128 crops of the same input planes the network's
+    prediction is within the forward bar of the float64 oracle."""
+    s = A.HostScene(A.generate_scene(1, str(tmp_path / "c2")))
+    st = s.state
+    cam, pc = s.shot(0)
+    w, h = s.film
+    assert (w, h) == (1920, 1080)
+    r = A.Renderer(0)
+    r.upload(s)
+    r.build()
+    r.set_film(w, h)
+    r.render(cam, pc, 0, 4, depth_channel=st.depth_out_channel)
+    planes = np.concatenate([r.read(0)[..., :3], r.read(pc.diffuseOutChannel + 1)[..., :3],
+                             r.read(pc.normalOutChannel + 1)[..., :3], r.read(st.depth_out_channel + 1)[..., :1]], -1)
+    d = A.Denoiser(0, seed=13)
+    d.denoise_film(r, pc.diffuseOutChannel, pc.normalOutChannel, st.depth_out_channel, 7)  # warm
+    ms = d.denoise_film(r, pc.diffuseOutChannel, pc.normalOutChannel, st.depth_out_channel, 7)
+    film = r.read(7)
+    r.close()
+    x = np.zeros((1, 1088, 1920, 10), np.float32)
+    x[0, :h, :w] = planes
+    host = d.forward(x)[0, :h, :w]
+    log(f"1080p film denoise: {ms:.2f} ms ({181_000 * 1920 * 1088 / (ms * 1e-3) / 1e12:.1f} TFLOP/s)")
+    np.testing.assert_array_equal(film[..., :3], np.maximum(host, 0.0))
+    theta = d.get_params()
+    crops = np.stack([planes[y:y + 128, xo:xo + 128] for xo, y in ((0, 0), (896, 464), (1792, 952), (416, 208))])
+    out = d.forward(crops)
+    ref = U.forward(bf16(theta).astype(np.float64), _features64(crops))
+    got = np.log1p(out.astype(np.float64))
+    err = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+    log(f"1080p crops forward: relative L2 error {err:.4f}")
+    assert err <= 0.03
+    d.close()

@@ -175,8 +175,10 @@ def _three_sigma(g, o, sq, spp, mask=None):
 
 
 def test_radiance_box_matches_oracle(renderer, box_scene):
+    """The 512x512 box workload at its stated 64 spp, depth 8."""
     s, _ = _load(renderer, box_scene)
-    spp = 16
+    spp = 64
+    assert s.shot(0)[1].maxPathDepth == 8
     st, ost, g, o, sq = _radiance_parity(renderer, s, spp)
     frac_bad = _three_sigma(g[0], o[0], sq, spp)
     mg, mo = g[0][..., :3].mean((0, 1)), o[0][..., :3].mean((0, 1))
