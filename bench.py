@@ -17,12 +17,19 @@ Prints one JSON line:
   roofline     HBM roofline of the dominant traversal kernel: on the default
                scheme the fused trace launch (shadow rays of one bounce and
                extension rays of the next), otherwise the closest-hit kernel.
-               Algorithmic bytes per launch = node visits x node bytes (64 B
-               for the quantized 4-wide node, 112 B for the float 4-wide
-               node, 64 B binary) + triangle tests x 48 B + ray I/O (68 B per
-               extension ray, 48 B per shadow ray), counted in a calibration
-               render of the same batches; over the kernel's average launch
-               time (HIP events on the render stream)
+               achieved = the HBM bytes per launch that rocprofv3 counts for
+               that kernel in a counter pass of this run (one child render
+               under --pmc, L2-to-memory read and write requests priced by
+               request size) over the kernel's average launch time (HIP events
+               on the render stream), frac = achieved / 8 TB/s; beside it the
+               VALU issue and lane utilisation of the same dispatches.  The
+               algorithmic bytes per launch (node visits x node bytes: 64 B for
+               the quantized 4-wide node, 112 B for the float 4-wide node, 64 B
+               binary; + triangle tests x 48 B + ray I/O, 68 B per extension
+               ray, 48 B per shadow ray; counted in a calibration render of the
+               same batches) are reported as algorithmic_bytes: the L2 and the
+               MALL serve most of them, so over the launch time they exceed the
+               HBM peak
   cpu_baseline the CPU oracle (the same estimator, FP32, own SAH BVH) on the
                host cores over a bounded sample of the same workload
 """
@@ -47,10 +54,127 @@ TRI_BYTES, RAY_IO_BYTES = 48, 68
 L2_BYTES = 4 << 20  # L2 of one XCD
 # shadow ray I/O: origin+tmax, direction+path id, and the pending NEE radiance
 SHADOW_IO_BYTES = 48
+# counters of the in-run pass: L2-to-memory (EA) read and write requests by
+# size, VALU issue and lane activity, GPU cycles (within one pass's limits:
+# 4 TCC, 4 SQ, 1 GRBM)
+PMC_COUNTERS = ["TCC_EA0_RDREQ_sum", "TCC_EA0_RDREQ_32B_sum", "TCC_EA0_WRREQ_sum", "TCC_EA0_WRREQ_64B_sum",
+                "SQ_INSTS_VALU", "SQ_ACTIVE_INST_VALU", "SQ_THREAD_CYCLES_VALU", "SQ_WAVES", "GRBM_GUI_ACTIVE"]
+COPY_PROBE_BYTES = 256 << 20
+XCDS, SIMDS_PER_CU = 8, 4
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return None
+
+
+def copy_probe():
+    """A plain device-to-device copy of COPY_PROBE_BYTES through the HIP
+    runtime (the runtime's copy kernel), before anything else runs on the
+    device: the counter pass checks its request counts against the bytes."""
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    src, dst = ctypes.c_void_p(), ctypes.c_void_p()
+    assert hip.hipMalloc(ctypes.byref(src), ctypes.c_size_t(COPY_PROBE_BYTES)) == 0
+    assert hip.hipMalloc(ctypes.byref(dst), ctypes.c_size_t(COPY_PROBE_BYTES)) == 0
+    assert hip.hipMemset(src, 1, ctypes.c_size_t(COPY_PROBE_BYTES)) == 0
+    assert hip.hipMemcpy(dst, src, ctypes.c_size_t(COPY_PROBE_BYTES), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(src)
+    hip.hipFree(dst)
+    log(f"[bench] copy probe: {COPY_PROBE_BYTES} bytes read + written")
+
+
+def counter_pass(args, qp, bpc, lds, trav, kernel_prefix):
+    """Runs one step of the same render (same batches and launch shape) in a
+    child process under rocprofv3 --pmc and returns the per-dispatch HBM
+    bytes and VALU activity of the kernel whose name contains
+    `kernel_prefix`.  Read requests are priced at 128 B (32 B for RDREQ_32B),
+    writes at 64 B for WRREQ_64B and 32 B otherwise; the child first copies
+    COPY_PROBE_BYTES, whose counted bytes must equal what it moved.  Returns
+    None (with a log line) when rocprofv3 is missing or the pass fails."""
+    import csv
+    import glob
+    import shutil
+    import subprocess
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        log("[bench] counter pass skipped: rocprofv3 not found")
+        return None
+    d = tempfile.mkdtemp(prefix="asuna_pmc_")
+    child = [sys.executable, os.path.abspath(__file__), "--config", args.config, "--steps", "1", "--warmup", "0",
+             "--no-cpu-baseline", "--no-pmc", "--no-autotune", "--copy-probe", "--queue-paths", str(qp),
+             "--blocks-per-cu", str(bpc), "--lds-stack", str(lds), "--traversal", str(trav),
+             "--branching", str(args.branching), "--quality", str(args.quality),
+             "--node-format", str(args.node_format), "--scale", str(args.scale)]
+    if args.scene_dir:
+        child += ["--scene-dir", args.scene_dir]
+    if args.no_graphs:
+        child.append("--no-graphs")
+    cmd = ["timeout", "-s", "KILL", "180", exe, "--pmc", *PMC_COUNTERS, "--output-format", "csv", "-d", d,
+           "-o", "pmc", "--", *child]
+    t0 = time.time()
+    try:
+        res = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.PIPE, text=True, timeout=240)
+    except (OSError, subprocess.TimeoutExpired) as e:
+        log(f"[bench] counter pass failed: {e}")
+        return None
+    files = glob.glob(os.path.join(d, "**", "*counter_collection.csv"), recursive=True)
+    if res.returncode != 0 or not files:
+        log(f"[bench] counter pass failed (rc {res.returncode}): {res.stderr[-400:]}")
+        return None
+    per = {}
+    with open(files[0]) as f:
+        for row in csv.DictReader(f):
+            key = int(row["Dispatch_Id"])
+            e = per.setdefault(key, {"name": row["Kernel_Name"]})
+            e[row["Counter_Name"]] = float(row["Counter_Value"])
+
+    def nbytes(e):
+        rd32 = e.get("TCC_EA0_RDREQ_32B_sum", 0.0)
+        rd = (e.get("TCC_EA0_RDREQ_sum", 0.0) - rd32) * 128 + rd32 * 32
+        w64 = e.get("TCC_EA0_WRREQ_64B_sum", 0.0)
+        wr = w64 * 64 + (e.get("TCC_EA0_WRREQ_sum", 0.0) - w64) * 32
+        return rd, wr
+
+    order = sorted(per)
+    copies = [per[k] for k in order if "copyBuffer" in per[k]["name"]]
+    copy_ratio = None
+    if copies:
+        rd, wr = nbytes(copies[0])  # the probe is the process's first copy
+        copy_ratio = [round(rd / COPY_PROBE_BYTES, 4), round(wr / COPY_PROBE_BYTES, 4)]
+    ks = [per[k] for k in order if kernel_prefix in per[k]["name"]]
+    if not ks:
+        log(f"[bench] counter pass: no {kernel_prefix} dispatch found")
+        return None
+    rds, wrs = zip(*[nbytes(e) for e in ks])
+    n = len(ks)
+    import ctypes
+    hip = ctypes.CDLL("libamdhip64.so")
+    ncu = ctypes.c_int()
+    hip.hipDeviceGetAttribute(ctypes.byref(ncu), 63, 0)  # hipDeviceAttributeMultiprocessorCount
+    simds = max(1, ncu.value) * SIMDS_PER_CU
+    insts = sum(e.get("SQ_INSTS_VALU", 0.0) for e in ks)
+    active = sum(e.get("SQ_ACTIVE_INST_VALU", 0.0) for e in ks)
+    lanes = sum(e.get("SQ_THREAD_CYCLES_VALU", 0.0) for e in ks)
+    cycles = sum(e.get("GRBM_GUI_ACTIVE", 0.0) for e in ks) / XCDS  # GRBM counts every XCD's cycles
+    out = {"read_bytes": sum(rds) / n, "write_bytes": sum(wrs) / n, "dispatches": n, "copy_ratio": copy_ratio,
+           # a wave64 VALU instruction occupies a 16-lane SIMD for 4 cycles
+           "valu_issue": round(insts * 4 / (simds * cycles), 4) if cycles else None,
+           "lane_util": round(lanes / (64 * active), 4) if active else None}
+    out["hbm_bytes"] = out["read_bytes"] + out["write_bytes"]
+    log(f"[bench] counter pass ({time.time() - t0:.1f}s, {n} dispatches): {out}")
+    return out
 
 
 def main():
@@ -91,7 +215,9 @@ def main():
                     help="ray fetch of the traversal kernels when not autotuned: 0 per-XCD queue slices, "
                          "1 queue order, 2 per-lane refill")
     ap.add_argument("--copy-probe", action="store_true",
-                    help="run a 256 MiB device-to-device copy first (calibrates rocprofv3 FETCH/WRITE_SIZE)")
+                    help="run a 256 MiB device-to-device copy first (checks the request sizes of the counter pass)")
+    ap.add_argument("--no-pmc", action="store_true",
+                    help="skip the rocprofv3 counter pass that measures the traversal kernel's HBM traffic")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
@@ -106,13 +232,7 @@ def main():
         return run_denoiser(A, args)
 
     if args.copy_probe:
-        import torch  # plumbing only: a plain copy kernel of known size
-        src = torch.ones(64 << 20, dtype=torch.float32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"[bench] copy probe: {src.numel() * 4} bytes read + written")
-        del src, dst
+        copy_probe()
 
     cfg_id = {"c1": 0, "c2": 1, "c3": 2}[args.config]
     sdir = args.scene_dir or os.path.join(tempfile.gettempdir(), f"asuna_bench_{args.config}_s{args.scale}")
@@ -243,35 +363,38 @@ def main():
         bytes_per_launch = rays_per_launch * bytes_per_ray
         kernel, tkey = "kExtend (closest-hit traversal)", "extend"
         per_launch = {"rays_per_launch": round(rays_per_launch)}
-    achieved = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
-    traffic = None
-    tfile = os.path.join(HERE, "profiles", f"traffic_{tkey}_{args.config}_{tree}.json")
-    if os.path.exists(tfile) and args.scale == 0:
-        with open(tfile) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-                "kernel": kernel, "avg_launch_ms": round(ms_per_launch, 4),
-                "bytes_per_launch": round(bytes_per_launch), "node_bytes": node_bytes,
-                "bytes_per_ray": round(bytes_per_ray, 1), "nodes_per_ray": round(nodes_per_ray, 3),
-                "tris_per_ray": round(tris_per_ray, 3), **per_launch}
-    if traffic:
-        # HBM bytes rocprofv3 counted per launch (profiles/), over the same launch time
-        roofline["hbm_gbs"] = round(traffic / (ms_per_launch * 1e-3) / 1e9, 1)
-        roofline["hbm_frac"] = round(roofline["hbm_gbs"] / HBM_PEAK_GBS, 4)
-        if achieved > HBM_PEAK_GBS:
-            roofline["note"] = ("frac above 1: the L2 and MALL serve part of the algorithmic bytes; hbm_gbs is the "
-                                "HBM traffic rocprofv3 counted for this kernel over the same launch time")
+    algo_gbs = bytes_per_launch / (ms_per_launch * 1e-3) / 1e9
+    pmc = None
+    if not args.no_pmc:
+        pmc = counter_pass(args, qp, bpc, lds, trav, "kTraceDyn<false" if fused else "kExtend")
+    roofline = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                "traffic": None, "kernel": kernel, "avg_launch_ms": round(ms_per_launch, 4),
+                "algorithmic_bytes": round(bytes_per_launch), "algorithmic_gbs": round(algo_gbs, 1),
+                "node_bytes": node_bytes, "bytes_per_ray": round(bytes_per_ray, 1),
+                "nodes_per_ray": round(nodes_per_ray, 3), "tris_per_ray": round(tris_per_ray, 3), **per_launch}
+    if pmc:
+        achieved = pmc["hbm_bytes"] / (ms_per_launch * 1e-3) / 1e9
+        roofline.update({"achieved": round(achieved, 1), "frac": round(achieved / HBM_PEAK_GBS, 4),
+                         "traffic": round(pmc["hbm_bytes"]), "traffic_read": round(pmc["read_bytes"]),
+                         "traffic_write": round(pmc["write_bytes"]),
+                         "traffic_source": "rocprofv3 --pmc pass of this run: TCC_EA0_RDREQ x 128 B "
+                                           "(- RDREQ_32B x 96 B) + TCC_EA0_WRREQ_64B x 64 B + the other WRREQ x 32 B, "
+                                           "per dispatch of the kernel",
+                         "copy_probe_ratio": pmc["copy_ratio"],
+                         "valu_issue_frac": pmc["valu_issue"], "valu_lane_util": pmc["lane_util"],
+                         "pmc_dispatches": pmc["dispatches"],
+                         "note": "the kernel sits nearer its VALU issue roof (valu_issue_frac) than the HBM roof "
+                                 "(frac); algorithmic_gbs above the HBM peak is cache service"})
     tree_bytes = binfo["num_wide_nodes"] * 128 + binfo["num_triangles"] * TRI_BYTES if binfo["branching"] == 4 \
         else binfo["num_nodes"] * 64 + binfo["num_triangles"] * TRI_BYTES
     if tree_bytes < L2_BYTES:
         roofline["note"] = (f"tree and triangles ({tree_bytes} B) fit in one XCD's 4 MiB L2: the algorithmic bytes "
-                            f"are served from cache, so frac can exceed 1 and does not measure HBM use")
+                            f"are served from cache and the kernel is bound by issue, not HBM")
 
     cpu = None
     if not args.no_cpu_baseline:
         from oracle import oracle as O
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or os.cpu_count() or 1
         osc = O.OracleScene(scene.desc)
         nf = args.cpu_frames
         t0 = time.perf_counter()
@@ -280,6 +403,7 @@ def main():
         cpu = {"value": round(ost["paths"] / ct / 1e6, 4), "unit": "Mpaths/s", "cores": threads, "kind": "port",
                "sample": f"frames 0..{nf - 1} of shot 0 at {w}x{h} ({ost['paths']} paths, {ct:.1f} s), "
                          f"CPU oracle FP32 with its own binned-SAH BVH, {threads} threads",
+               "nproc": os.cpu_count(), "cpu_model": cpu_model(),
                "mrays_per_s": round((ost["extension_rays"] + ost["shadow_rays"]) / ct / 1e6, 3)}
         log(f"[bench] cpu baseline {cpu}")
 
