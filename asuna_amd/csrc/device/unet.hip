@@ -8,8 +8,9 @@
 //   backward-x   dX[p][ci] = sum_tap sum_co dY[p - d(tap)][co] W[tap][co][ci]
 //                GEMM rows = pixels, cols = cin, K = 9 x cout
 //   backward-w   dW[tap][co][ci] = sum_p dY[p][co] X[p + d(tap)][ci]
-//                GEMM rows = cout, cols = cin, K = pixels (split over waves,
-//                FP32 atomics into dW)
+//                GEMM rows = cout, cols = cin, K = pixels (split over waves;
+//                each split writes its partial sums, which a second kernel
+//                adds in split order, so gradients are reproducible)
 // Activations are NHWC BF16 with the channel count padded to a multiple of
 // 32 (padding channels hold zeros and their weights stay zero), so an A or B
 // fragment row is one 16-byte load.  Decoder convolutions read their input
@@ -68,6 +69,7 @@ struct ConvArgs {
   float* upTmp;        // full-resolution gradient of an upsampled s0
   float* dW;           // [9][Cout][Cin]
   float* dB;           // [Cout]
+  float* wPart;        // backward-w partial sums [splits][9][Cout][Cin]
   uint32_t relu;
   uint32_t splits;     // backward-w pixel splits
 };
@@ -370,25 +372,43 @@ __global__ __launch_bounds__(256) void kConvBwdW(ConvArgs a) {
     for (int i = 0; i < 2; ++i)
       for (int j = 0; j < 2; ++j) acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A[i], B[j], acc[i][j], 0, 0, 0);
   }
-  // D: col = cin (l % 16), rows = cout 4q .. 4q+3
-  float* dw = a.dW + size_t(tap) * a.Cout * a.Cin;
+  // D: col = cin (l % 16), rows = cout 4q .. 4q+3; one slot per (split, weight)
+  float* dw = a.wPart + (size_t(split) * 9 + size_t(tap)) * a.Cout * a.Cin;
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j)
       for (int r = 0; r < 4; ++r) {
         const uint32_t co = cob * 32 + 16 * i + 4 * q + r, ci = cib * 32 + 16 * j + r16;
-        atomicAdd(dw + size_t(co) * a.Cin + ci, acc[i][j][r]);
+        dw[size_t(co) * a.Cin + ci] = acc[i][j][r];
       }
 }
 
-__global__ void kBiasGrad(const float* dY, uint32_t P, uint32_t C, float* dB) {
-  const uint32_t c = threadIdx.x % C, lanesPerC = blockDim.x / C;
-  const uint32_t g = threadIdx.x / C;
-  if (g >= lanesPerC) return;
-  const uint32_t chunk = 256;
-  const uint32_t p0 = blockIdx.x * chunk;
+// dW[i] += the splits' partial sums, added in split order.
+__global__ void kReduceSplits(const float* part, uint32_t splits, size_t n, float* dW) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
   float s = 0.f;
-  for (uint32_t p = p0 + g; p < min(P, p0 + chunk); p += lanesPerC) s += dY[size_t(p) * C + c];
-  atomicAdd(dB + c, s);
+  for (uint32_t k = 0; k < splits; ++k) s += part[size_t(k) * n + i];
+  dW[i] += s;
+}
+
+// Bias gradient in two fixed-order passes: block b sums dY over its 256
+// pixels for every channel (thread c, pixels in order), then one thread per
+// channel adds the block sums in block order.
+constexpr uint32_t kBiasChunk = 256;
+__global__ void kBiasPart(const float* dY, uint32_t P, uint32_t C, float* part) {
+  const uint32_t c = threadIdx.x;
+  if (c >= C) return;
+  const uint32_t p0 = blockIdx.x * kBiasChunk, p1 = min(P, p0 + kBiasChunk);
+  float s = 0.f;
+  for (uint32_t p = p0; p < p1; ++p) s += dY[size_t(p) * C + c];
+  part[size_t(blockIdx.x) * C + c] = s;
+}
+__global__ void kBiasReduce(const float* part, uint32_t nb, uint32_t C, float* dB) {
+  const uint32_t c = threadIdx.x;
+  if (c >= C) return;
+  float s = 0.f;
+  for (uint32_t b = 0; b < nb; ++b) s += part[size_t(b) * C + c];
+  dB[c] += s;
 }
 
 // --------------------------------------------------------- elementwise
@@ -591,6 +611,7 @@ struct ash_denoiser {
   Act x, out;
   Act enc[kLevels][2], pooled[kLevels], dec[kLevels][2];
   Buf<float> upTmp, outF, lossPart, raw, target, rgb;
+  Buf<float> wPart, bPart;  // backward-w and bias-gradient partial sums
   bool wDirty = true;
 };
 
@@ -705,6 +726,8 @@ static int packWeights(ash_denoiser* d) {
   return ASH_OK;
 }
 
+static uint32_t splitsFor(size_t P) { return uint32_t(std::max<size_t>(1, std::min<size_t>(256, P / 2048))); }
+
 static int allocAct(ash_denoiser* d, Act& t, uint32_t N, uint32_t H, uint32_t W, uint32_t C) {
   t.N = N, t.H = H, t.W = W, t.C = C;
   DH(d, t.a.reserve(t.count()));
@@ -814,8 +837,6 @@ static int forwardPass(ash_denoiser* d) {
   return ASH_OK;
 }
 
-static uint32_t splitsFor(size_t P) { return uint32_t(std::max<size_t>(1, std::min<size_t>(256, P / 2048))); }
-
 // Gradient of the loss w.r.t. all parameters into d->grad (after forwardPass
 // and the loss kernel wrote the output gradient into d->out.g).
 static int backwardPass(ash_denoiser* d) {
@@ -832,6 +853,16 @@ static int backwardPass(ash_denoiser* d) {
     }
   }
   auto sched = schedule(d);
+  // partial-sum space of the largest layer (splits x weights) and of the
+  // finest level's bias blocks; kept across calls
+  size_t wp = 0;
+  for (const Step& st : sched) {
+    const Layer& l = d->L[st.layer];
+    const Act& o = st.out ? *st.out : d->out;
+    wp = std::max(wp, size_t(9) * l.CoS * (l.CaS + l.CbS) * splitsFor(size_t(N) * o.H * o.W));
+  }
+  DH(d, d->wPart.reserve(wp));
+  DH(d, d->bPart.reserve(size_t(blocks(size_t(N) * d->H * d->W, kBiasChunk)) * 128));
   for (int k = int(sched.size()) - 1; k >= 0; --k) {
     const Step& st = sched[k];
     const Layer& l = d->L[st.layer];
@@ -843,9 +874,16 @@ static int backwardPass(ash_denoiser* d) {
     if (a.Cin != a.s0.C + a.s1.C || a.Cout != o.C) return dfail(d, ASH_E_INVALID, "layer %d: channel layout mismatch", st.layer);
     a.dY = o.g.p;
     a.splits = splitsFor(P);
+    a.wPart = d->wPart.p;
+    const size_t nW = size_t(9) * a.Cout * a.Cin;
+    if (nW * a.splits > d->wPart.n || a.Cout > 128)
+      return dfail(d, ASH_E_INVALID, "layer %d: gradient partial buffer too small", st.layer);
     size_t tasks = size_t(9) * (a.Cout / 32) * (a.Cin / 32) * a.splits;
     kConvBwdW<<<blocks(tasks, 4), 256, 0, d->stream>>>(a);
-    kBiasGrad<<<blocks(P, 256), 256, 0, d->stream>>>(a.dY, uint32_t(P), a.Cout, a.dB);
+    kReduceSplits<<<blocks(nW), 256, 0, d->stream>>>(d->wPart.p, a.splits, nW, a.dW);
+    const unsigned nb = blocks(P, kBiasChunk);
+    kBiasPart<<<nb, 128, 0, d->stream>>>(a.dY, uint32_t(P), a.Cout, d->bPart.p);
+    kBiasReduce<<<1, 128, 0, d->stream>>>(d->bPart.p, nb, a.Cout, a.dB);
     if (st.s0 == &d->x) continue;  // no gradient for the input features
     if (st.up) DH(d, hipMemsetAsync(d->upTmp.p, 0, P * st.s0->C * sizeof(float), d->stream));
     kConvBwdX<<<blocks(P, 64), 256, 0, d->stream>>>(a);

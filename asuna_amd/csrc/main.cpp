@@ -15,14 +15,16 @@ int main(int argc, char** argv) {
   ash_offline_opts o;
   std::memset(&o, 0, sizeof o);
   o.render.depthOutChannel = -1;
-  bool offline = false, override = false;
+  bool offline = false, override = false, missing = false;
   int spp = 0;
-  for (int i = 1; i < argc; ++i) {
+  for (int i = 1; i < argc && !missing; ++i) {
     std::string a = argv[i];
+    // the value after a flag; a flag at the end of the line is reported below
     auto val = [&](const char* name) -> const char* {
       if (i + 1 >= argc) {
         std::fprintf(stderr, "missing value for %s\n", name);
-        std::exit(2);
+        missing = true;
+        return "";
       }
       return argv[++i];
     };
@@ -38,6 +40,7 @@ int main(int argc, char** argv) {
     else if (a == "--spp_override") override = true;
     else if (a == "--quiet") o.quiet = 1;
   }
+  if (missing) return 2;
   if (!offline) {
     std::fprintf(stderr, "asuna: only --offline rendering is available (the interactive window is not part of this build)\n");
     return 2;
