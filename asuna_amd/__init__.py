@@ -479,11 +479,11 @@ class Renderer:
         return int(p.value)
 
     def render_shots(self, cams, pcs, n_frames, queue_paths=0, timing=False, blocks_per_cu=0, graphs=True,
-                     count_traversal=False, lds_stack=0, traversal=0) -> dict:
+                     count_traversal=False, lds_stack=0, traversal=0, depth_channel=-1) -> dict:
         n = len(cams)
         ca = (GpuCamera * n)(*cams)
         pa = (GpuPushConstantRaytrace * n)(*pcs)
-        opts = RenderOpts.make(queue_paths, blocks_per_cu, timing, -1, 0, count_traversal, graphs, lds_stack,
+        opts = RenderOpts.make(queue_paths, blocks_per_cu, timing, depth_channel, 0, count_traversal, graphs, lds_stack,
                                traversal)
         st = Stats()
         self._check(lib().ash_render_shots(self._c, n, ca, pa, n_frames, C.byref(opts), C.byref(st)),

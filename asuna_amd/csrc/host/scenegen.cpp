@@ -393,6 +393,117 @@ int genSpheres(uint64_t master, int grid, int level, int envW, uint32_t W, uint3
   return writeText(scenePath, js) ? ASH_OK : ASH_E_IO;
 }
 
+// Random training scene (the denoiser's data set): 1-20 fBm-displaced
+// icospheres of 320 to 5120 triangles with random size, position and
+// material from the plugin set (lambertian, pbr metalness-roughness, rough
+// plastic, dielectric) on a lambertian ground quad, 1-4 rect area lights
+// facing down with random size and radiance, and an env map with a random
+// sky gradient and sun (or a constant colour).  Every draw comes from the
+// scene's own seed, so the scene set is reproducible.
+int genTraining(uint64_t master, const std::string& dir, std::string& scenePath) {
+  Rng r = objectRng(master, 3, 0);
+  const int nObj = 1 + int(r.uni() * 20.0);
+  std::string mats, meshes, insts;
+  for (int id = 0; id < nObj; ++id) {
+    Rng o = objectRng(master, 3, 1 + id);
+    const int level = 2 + int(o.uni() * 3.0);  // 2..4
+    Mesh m = icosphere(level);
+    Noise nz{uint32_t(o.next())};
+    V3 off{o.uni() * 100.0, o.uni() * 100.0, o.uni() * 100.0};
+    const double amp = o.uf(0.02f, 0.25f);
+    for (auto& p : m.v) p = p * (1.0 + amp * nz.fbm(p * 2.0 + off, 6));
+    smoothNormals(m);
+    std::string name = "obj" + std::to_string(id);
+    if (!writeObj(dir + "/" + name + ".obj", m)) return ASH_E_IO;
+    const double kind = o.uni();
+    const float a = o.uf(0.05f, 0.95f), b = o.uf(0.05f, 0.95f), c = o.uf(0.05f, 0.95f);
+    char mb[512];
+    if (kind < 0.3) {
+      std::snprintf(mb, sizeof mb, "{\"name\": \"m_%s\", \"type\": \"brdf_lambertian\", \"diffuse_reflectance\": %s}",
+                    name.c_str(), fmt3(a, b, c).c_str());
+    } else if (kind < 0.65) {
+      std::snprintf(mb, sizeof mb,
+                    "{\"name\": \"m_%s\", \"type\": \"brdf_pbr_metalness_roughness\", \"diffuse_reflectance\": %s, "
+                    "\"roughness\": %.9g, \"metalness\": %.9g}",
+                    name.c_str(), fmt3(a, b, c).c_str(), o.uf(0.05f, 1.0f), o.uni() < 0.5 ? 1.0 : 0.0);
+    } else if (kind < 0.9) {
+      std::snprintf(mb, sizeof mb,
+                    "{\"name\": \"m_%s\", \"type\": \"brdf_rough_plastic\", \"diffuse_reflectance\": %s, "
+                    "\"alpha\": [%.9g, %.9g], \"ior\": %.9g}",
+                    name.c_str(), fmt3(a, b, c).c_str(), o.uf(0.02f, 0.6f), o.uf(0.02f, 0.6f), o.uf(1.3f, 1.8f));
+    } else {
+      std::snprintf(mb, sizeof mb, "{\"name\": \"m_%s\", \"type\": \"bsdf_dielectric\", \"ior\": %.9g}", name.c_str(),
+                    o.uf(1.3f, 1.8f));
+    }
+    mats += std::string(mats.empty() ? "" : ",\n") + "    " + mb;
+    meshes += std::string(meshes.empty() ? "" : ",\n") + "    {\"name\": \"" + name + "\", \"path\": \"" + name + ".obj\"}";
+    const double rad = o.uf(0.15f, 0.7f);
+    const double cx = o.uf(-2.0f, 2.0f), cz = o.uf(-2.0f, 2.0f), cy = rad * (1.0 + amp) + o.uf(0.0f, 0.8f);
+    insts += std::string(insts.empty() ? "" : ",\n") + "    {\"mesh\": \"" + name + "\", \"material\": \"m_" + name +
+             "\", \"toworld\": [{\"type\": \"scale\", \"value\": " + fmt3(rad, rad, rad) +
+             "}, {\"type\": \"translate\", \"value\": " + fmt3(cx, cy, cz) + "}]}";
+  }
+  Mesh ground = quad({-8, 0, 8}, {16, 0, 0}, {0, 0, -16}, {0, 1, 0});
+  if (!writeObj(dir + "/ground.obj", ground)) return ASH_E_IO;
+  {
+    const float g = r.uf(0.15f, 0.8f);
+    mats += ",\n    {\"name\": \"m_ground\", \"type\": \"brdf_lambertian\", \"diffuse_reflectance\": " +
+            fmt3(g * r.uf(0.8f, 1.0f), g * r.uf(0.8f, 1.0f), g * r.uf(0.8f, 1.0f)) + "}";
+  }
+  meshes += ",\n    {\"name\": \"ground\", \"path\": \"ground.obj\"}";
+  insts += ",\n    {\"mesh\": \"ground\", \"material\": \"m_ground\"}";
+  // rect lights facing down: cross(v1 - p, v2 - p) points -y
+  std::string lights;
+  const int nLights = 1 + int(r.uni() * 4.0);
+  for (int l = 0; l < nLights; ++l) {
+    Rng q = objectRng(master, 3, 1000 + l);
+    const double sx = q.uf(0.2f, 1.2f), sz = q.uf(0.2f, 1.2f), y = q.uf(2.0f, 4.0f);
+    const double px = q.uf(-2.5f, 2.5f) - sx * 0.5, pz = q.uf(-2.5f, 2.5f) - sz * 0.5;
+    const float e = q.uf(2.0f, 25.0f);
+    lights += std::string(lights.empty() ? "" : ",\n") + "    {\"type\": \"rect\", \"radiance\": " +
+              fmt3(e * q.uf(0.7f, 1.0f), e * q.uf(0.7f, 1.0f), e * q.uf(0.7f, 1.0f)) + ", \"position\": " +
+              fmt3(px, y, pz) + ", \"v1\": " + fmt3(px + sx, y, pz) + ", \"v2\": " + fmt3(px, y, pz + sz) + "}";
+  }
+  // env map 256 x 128
+  const int envW = 256, envH = 128;
+  std::vector<float> env(size_t(envW) * envH * 4);
+  const bool constant = r.uni() < 0.25;
+  const float scale = r.uf(0.05f, 1.5f);
+  const float cr = r.uf(0.3f, 1.0f), cg = r.uf(0.3f, 1.0f), cb = r.uf(0.3f, 1.0f);
+  const double el = r.uf(5.0f, 70.0f) * M_PI / 180, az = r.uf(0.0f, 360.0f) * M_PI / 180;
+  V3 sun = nrm(V3{std::cos(el) * std::cos(az), std::sin(el), std::cos(el) * std::sin(az)});
+  const float sunL = r.uni() < 0.5 ? r.uf(50.0f, 2000.0f) : 0.0f;
+  for (int j = 0; j < envH; ++j)
+    for (int i = 0; i < envW; ++i) {
+      double theta = (j + 0.5) / envH * M_PI, phi = (i + 0.5) / envW * 2 * M_PI - M_PI;
+      V3 d{std::sin(theta) * std::cos(phi), std::cos(theta), std::sin(theta) * std::sin(phi)};
+      float* px = &env[(size_t(j) * envW + i) * 4];
+      if (constant) {
+        px[0] = cr * scale, px[1] = cg * scale, px[2] = cb * scale;
+      } else {
+        skyColor(d, sun, px);
+        px[0] *= cr * scale, px[1] *= cg * scale, px[2] *= cb * scale;
+        if (dot(d, sun) > std::cos(2.0 * M_PI / 180.0)) px[0] = px[1] = px[2] = sunL;
+      }
+      px[3] = 1.f;
+    }
+  std::string err;
+  if (!ash::writeImage(dir + "/sky.exr", uint32_t(envW), uint32_t(envH), env.data(), err)) return ASH_E_IO;
+  std::string js = "{\n";
+  js += "  \"state\": {\"path_tracing\": {\"spp\": 4, \"max_path_depth\": 6, "
+        "\"multi_channel\": [\"diffuse\", \"normal\", \"depth\"]}, "
+        "\"post_processing\": {\"tone_mapping\": \"filmic\"}, \"output_hdr\": true},\n";
+  js += "  \"camera\": {\"type\": \"perspective\", \"film\": {\"resolution\": [256, 256]}, \"fov\": 50.0},\n";
+  js += "  \"materials\": [\n" + mats + "\n  ],\n";
+  js += "  \"lights\": [\n" + lights + "\n  ],\n";
+  js += "  \"envmap\": {\"path\": \"sky.exr\"},\n";
+  js += "  \"meshes\": [\n" + meshes + "\n  ],\n";
+  js += "  \"instances\": [\n" + insts + "\n  ],\n";
+  js += "  \"shots\": [\n    {\"type\": \"lookat\", \"eye\": [0, 2.5, 6], \"lookat\": [0, 0.4, 0], \"up\": [0, 1, 0]}\n  ]\n}\n";
+  scenePath = dir + "/training.json";
+  return writeText(scenePath, js) ? ASH_OK : ASH_E_IO;
+}
+
 }  // namespace
 
 extern "C" int ash_generate_scene(int32_t config, uint64_t seed, int32_t scale, const char* dir, char* pathOut,
@@ -405,6 +516,8 @@ extern "C" int ash_generate_scene(int32_t config, uint64_t seed, int32_t scale, 
     rc = genBox(master, config, dir, path);
   } else if (config == 1) {
     rc = scale > 0 ? genSpheres(master, 2, 3, 256, 320, 180, dir, path) : genSpheres(master, 8, 6, 2048, 1920, 1080, dir, path);
+  } else if (config == 3) {
+    rc = genTraining(master, dir, path);
   } else if (config == 4) {
     // resident serving scene of about `scale` triangles: the (grid, level)
     // whose grid^2 * 20 * 4^level is closest to it, grid in 2..8, level 2..6

@@ -191,8 +191,8 @@ def main():
     ap.add_argument("--config", choices=["c2", "c1", "c3", "c4", "c4-infer", "c5"], default="c2",
                     help="c4-infer: only the denoiser's 1080p inference on the procedural scene's 4-spp film")
     ap.add_argument("--train-iters", type=int, default=4000, help="c4: denoiser training steps")
-    ap.add_argument("--train-views", type=int, default=64, help="c4: training views per scene (six scenes)")
-    ap.add_argument("--jobs", type=int, default=600, help="c5: jobs per arrival rate")
+    ap.add_argument("--train-scenes", type=int, default=200, help="c4: seeded random training scenes (10 views each)")
+    ap.add_argument("--jobs", type=int, default=2000, help="c5: jobs per arrival rate")
     ap.add_argument("--rates", default="10,50,200", help="c5: Poisson arrival rates (jobs/s)")
     ap.add_argument("--serve-policy", type=int, default=0,
                     help="c5: server batch fill order (0 oldest first across scenes, 1 round robin, 2 oldest first)")
@@ -233,15 +233,14 @@ def main():
 
     import asuna_amd as A
 
+    if args.copy_probe:
+        copy_probe()
     if args.config == "c5":
         return run_serving(A, args)
     if args.config == "c4":
         return run_denoiser(A, args)
     if args.config == "c4-infer":
         return run_denoiser_inference(A, args)
-
-    if args.copy_probe:
-        copy_probe()
 
     cfg_id = {"c1": 0, "c2": 1, "c3": 2}[args.config]
     sdir = args.scene_dir or os.path.join(tempfile.gettempdir(), f"asuna_bench_{args.config}_s{args.scale}")
@@ -443,63 +442,88 @@ def main():
 
 
 def run_denoiser(A, args):
-    """Learned denoiser: trained on the GPU on renders of seeded procedural
-    scenes (1, 2, 4 and 8-spp radiance + albedo + normal + depth -> 1024-spp
-    radiance, 128x128 views), then run on the 1080p procedural scene.
-    Reports training throughput, 1080p inference time and relMSE against the
-    FP64 oracle at 4096 spp (16 fixed 32x32 crops) for plain renders at 1 to
-    256 spp and denoised renders at 1 to 16 spp, and the equal-relMSE map:
-    for each plain spp, the fewest denoised spp that reach its relMSE."""
+    """Learned denoiser on its training workload: 200 seeded random scenes
+    (1-20 displaced meshes with materials from the plugin set, 1-4 rect
+    lights, a random sky or constant env), 10 random 256x256 views each
+    (2,000 crops), every view rendered at 4 spp (and 1, 16 and 64 spp)
+    with its albedo / normal / depth planes, stored as FP16, against a
+    4096-spp target.  The U-Net is trained on the GPU (Adam, batches of 8
+    random 128x128 windows, half of them at the 4-spp input, the rest at
+    1, 16 or 64 spp), then run on the 1080p procedural scene.  Reports
+    training throughput, 1080p inference time and relMSE against the FP64
+    oracle at 4096 spp (16 fixed 32x32 crops) for plain and denoised renders
+    at 1 to 256 spp, and the equal-relMSE map: for each plain spp, the
+    fewest denoised spp that reach its relMSE."""
+    from concurrent.futures import ThreadPoolExecutor
     rng = np.random.default_rng(0xA5A52026)
     t0 = time.time()
-    scenes = []
-    for k in range(6):
-        d = os.path.join(tempfile.gettempdir(), f"asuna_den_{k}")
-        p = os.path.join(d, "spheres.json")
+    nsc, views, res, spp_ref = args.train_scenes, 10, 256, 4096
+    in_spp = (4, 1, 16, 64)
+
+    def gen(k):
+        d = os.path.join(tempfile.gettempdir(), f"asuna_train_{k}")
+        p = os.path.join(d, "training.json")
         if not os.path.exists(p):
-            p = A.generate_scene(4, d, scale=int([20_000, 80_000, 300_000][k % 3]), seed=100 + k)
-        scenes.append(A.HostScene(p))
-    res, spp_ref = 128, 1024
-    xs, ys = [], []
-    for s in scenes:
-        r = A.Renderer(0)
-        r.upload(s)
-        r.build()
+            p = A.generate_scene(3, d, seed=0xA5A52026 + k)
+        return p
+
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 1)) as ex:
+        paths = list(ex.map(gen, range(nsc)))
+    log(f"[den] {nsc} training scenes generated in {time.time() - t0:.1f}s")
+    X = np.zeros((len(in_spp), nsc * views, res, res, 10), np.float16)
+    Y = np.zeros((nsc * views, res, res, 3), np.float32)
+    t1 = time.time()
+    r = A.Renderer(0)
+    for k, p in enumerate(paths):
+        s = A.HostScene(p)
         st = s.state
         _, pc = s.shot(0)
-        for _ in range(args.train_views):
-            eye = rng.normal(size=3)
-            eye = eye / np.linalg.norm(eye) * rng.uniform(2.0, 6.0)
-            eye[1] = abs(eye[1]) + 0.3
-            cam = A.camera_lookat(res, res, float(rng.uniform(30, 75)), tuple(eye), tuple(rng.uniform(-1, 1, 3) * [1, 0.3, 1]))
-            for spp_in in (1, 2, 4, 8):
-                r.set_film(res, res)
-                r.render(cam, pc, 0, spp_in, depth_channel=st.depth_out_channel)
-                xs.append(np.concatenate([r.read(0)[..., :3], r.read(pc.diffuseOutChannel + 1)[..., :3],
-                                          r.read(pc.normalOutChannel + 1)[..., :3],
-                                          r.read(st.depth_out_channel + 1)[..., :1]], -1))
-            r.set_film(res, res)
-            r.render(cam, pc, 0, spp_ref)
-            ys += [r.read(0)[..., :3]] * 4
-        r.close()
-    X, Y = np.stack(xs), np.stack(ys)
-    log(f"[den] {len(X)} training views rendered in {time.time() - t0:.1f}s")
+        r.upload(s)
+        r.build()
+        r.set_film(res, res)
+        cams = []
+        for _ in range(views):
+            ang = rng.uniform(0, 2 * np.pi)
+            dist = rng.uniform(3.0, 7.0)
+            eye = (dist * np.cos(ang), rng.uniform(0.5, 4.0), dist * np.sin(ang))
+            look = (rng.uniform(-1.5, 1.5), rng.uniform(0.0, 0.8), rng.uniform(-1.5, 1.5))
+            cams.append(A.camera_lookat(res, res, float(rng.uniform(30, 65)), eye, look))
+        pcs = [pc] * views
+        for i, spp in enumerate(in_spp):
+            r.render_shots(cams, pcs, spp, depth_channel=st.depth_out_channel)
+            for v in range(views):
+                X[i, k * views + v] = np.concatenate(
+                    [r.read_shot(v, 0)[..., :3], r.read_shot(v, pc.diffuseOutChannel + 1)[..., :3],
+                     r.read_shot(v, pc.normalOutChannel + 1)[..., :3],
+                     r.read_shot(v, st.depth_out_channel + 1)[..., :1]], -1)
+        r.render_shots(cams, pcs, spp_ref)
+        for v in range(views):
+            Y[k * views + v] = r.read_shot(v, 0)[..., :3]
+        s.close()
+        if (k + 1) % 20 == 0:
+            log(f"[den] rendered {k + 1}/{nsc} scenes ({time.time() - t1:.0f}s)")
+    r.close()
+    t_data = time.time() - t1
+    n = len(Y)
     den = A.Denoiser(0, seed=1)
-    nb, crop = 8, 96
-    t0 = time.perf_counter()
+    nb, crop = 8, 128
+    t2 = time.perf_counter()
     losses = []
     for it in range(args.train_iters):
-        idx = rng.choice(len(X), nb, replace=False)
+        idx = rng.choice(n, nb, replace=False)
+        lvl = np.where(rng.random(nb) < 0.5, 0, rng.integers(1, len(in_spp), nb))
         oy, ox = rng.integers(0, res - crop + 1, 2)
-        xb, yb = X[idx, oy:oy + crop, ox:ox + crop], Y[idx, oy:oy + crop, ox:ox + crop]
+        xb = np.stack([X[l, i, oy:oy + crop, ox:ox + crop] for l, i in zip(lvl, idx)]).astype(np.float32)
+        yb = Y[idx, oy:oy + crop, ox:ox + crop]
         if it % 2:
             xb, yb = xb[:, :, ::-1], yb[:, :, ::-1]
         lr = 2e-3 if it < 0.7 * args.train_iters else 4e-4
-        losses.append(den.train_step(xb, yb, lr=lr))
-        if it % 250 == 0:
+        losses.append(den.train_step(np.ascontiguousarray(xb), np.ascontiguousarray(yb), lr=lr))
+        if it % 500 == 0:
             log(f"[den] step {it}: loss {np.mean(losses[-50:]):.4f}")
-    t_train = time.perf_counter() - t0
-    # inference on the 1080p procedural scene
+    t_train = time.perf_counter() - t2
+    # held-out check on the last scene's views is not separate: the data set
+    # is the training set; the judge of the network is the 1080p scene below
     sdir = os.path.join(tempfile.gettempdir(), "asuna_bench_c2_s0")
     p = os.path.join(sdir, "spheres.json")
     if not os.path.exists(p):
@@ -519,12 +543,11 @@ def run_denoiser(A, args):
         return float(np.mean((x - g["ref"]) ** 2 / (g["ref"] ** 2 + 1e-2)))
 
     out = {}
-    for spp in (1, 2, 4, 8, 16, 64, 256):
+    for spp in (1, 4, 16, 64, 256):
         r.set_film(w, h)
         st_r = r.render(cam, pc, 0, spp, depth_channel=st.depth_out_channel)
-        if spp in (1, 4, 16, 64, 256):
-            out[f"plain_{spp}spp"] = {"relmse": round(relmse(r.read(0)), 5), "render_ms": round(st_r["render_ms"], 2)}
-        if spp <= 16:
+        out[f"plain_{spp}spp"] = {"relmse": round(relmse(r.read(0)), 5), "render_ms": round(st_r["render_ms"], 2)}
+        if spp <= 64:
             den.denoise_film(r, pc.diffuseOutChannel, pc.normalOutChannel, st.depth_out_channel, 7)  # warm
             ms = min(den.denoise_film(r, pc.diffuseOutChannel, pc.normalOutChannel, st.depth_out_channel, 7)
                      for _ in range(3))
@@ -533,18 +556,18 @@ def run_denoiser(A, args):
     # equal relMSE: for each plain spp, the fewest denoised spp whose relMSE is no higher
     matched = {}
     for sp in (1, 4, 16, 64, 256):
-        ok = [d for d in (1, 2, 4, 8, 16) if out[f"denoised_{d}spp"]["relmse"] <= out[f"plain_{sp}spp"]["relmse"]]
+        ok = [d for d in (1, 4, 16, 64) if out[f"denoised_{d}spp"]["relmse"] <= out[f"plain_{sp}spp"]["relmse"]]
         if ok:
             d = min(ok)
             matched[str(sp)] = {"denoised_spp": d, "plain_ms": out[f"plain_{sp}spp"]["render_ms"],
                                 "denoised_ms": out[f"denoised_{d}spp"]["total_ms"]}
     flop = UNET_FLOP_PER_PIXEL * 1920 * 1088
     res_line = {"metric": "denoiser: relMSE vs FP64 4096-spp oracle and 1080p inference time",
-                "config": {"workload": f"U-Net 32/48/64/96/128, BF16 MFMA; trained on {6 * args.train_views} views "
-                                       "of 6 seeded procedural scenes (1/2/4/8 spp -> 1024 spp, 128x128); inference "
-                                       "on the 1920x1080 procedural scene (5.24M triangles)",
+                "config": {"workload": f"U-Net 32/48/64/96/128, BF16 MFMA; trained on {n} 256x256 crops of {nsc} "
+                                       "seeded random scenes (4-spp inputs, plus 1/16/64 spp, FP16; 4096-spp "
+                                       "targets); inference on the 1920x1080 procedural scene (5.24M triangles)",
                            "train_iters": args.train_iters, "batch": f"{nb}x{crop}x{crop}"},
-                "train_step_ms": round(t_train / args.train_iters * 1e3, 2),
+                "data_s": round(t_data, 1), "train_step_ms": round(t_train / args.train_iters * 1e3, 2),
                 "train_loss_first_last": [round(float(np.mean(losses[:50])), 4), round(float(np.mean(losses[-50:])), 4)],
                 "inference_tflops": round(flop / (out["denoised_4spp"]["denoise_ms"] * 1e-3) / 1e12, 1),
                 "results": out, "equal_relmse": matched}
@@ -594,9 +617,9 @@ def run_serving(A, args):
     triangles, two seeds each), Poisson arrivals of jobs with resolution from
     {256^2, 512^2, 1024^2, 1920x1080} (weights .4/.3/.2/.1) and spp from
     {1, 4, 16, 64}; scene uniform.  The server co-batches the jobs of a
-    scene and overlaps scenes; the baseline renders the same jobs one at a
-    time in arrival order (latency = FIFO queueing with the measured render
-    times).  Default: both columns do the same work (half of the jobs
+    scene and overlaps scenes; the baseline serves the same arrival stream
+    one job at a time: a worker renders each job alone, in arrival order, as
+    it arrives (latency = completion - submission, both measured).  Default: both columns do the same work (half of the jobs
     denoised).  --matched: equal relMSE -- the server renders every job at
     the denoised spp that c4 measured to match the job's plain spp and
     denoises it; the baseline renders the plain spp."""
@@ -695,6 +718,8 @@ def run_serving(A, args):
                 lat[k] = st["done_ms"] - st["submit_ms"]
                 paths[0] += st["paths"]
                 srv.release(ids[k])
+                if (k + 1) % 200 == 0:
+                    log(f"[serve] rate {lam}/s server: {k + 1} jobs done")
 
         th, tc = threading.Thread(target=feeder), threading.Thread(target=collector)
         th.start()
@@ -705,21 +730,46 @@ def run_serving(A, args):
         span = (time.perf_counter() - base)
         served = {"jobs_per_s": round(len(jobs) / span, 2), "p50_ms": round(float(np.percentile(lat, 50)), 1),
                   "p99_ms": round(float(np.percentile(lat, 99)), 1), "mpaths_per_s": round(paths / span / 1e6, 1)}
-        # baseline: the same jobs rendered one at a time (FIFO)
-        service = []
-        for sid, cam, pc, w, h, _, _, spp, dn in jobs:
-            r = rs[sid]
-            dch = scenes[sid].state.depth_out_channel
-            ts = time.perf_counter()
-            r.set_film(w, h)
-            r.render(cam, pc, 0, spp, depth_channel=dch)
-            if dn:
-                den.denoise_film(r, pc.diffuseOutChannel, pc.normalOutChannel, dch, 7)
-            service.append(time.perf_counter() - ts)
-        finish, blat = 0.0, []
-        for a, sv in zip(arrivals, service):
-            finish = max(finish, a) + sv
-            blat.append((finish - a) * 1e3)
+        # baseline: the same arrival stream served one job at a time -- a
+        # worker takes the jobs in arrival order as they arrive and renders
+        # (and, when flagged, denoises) each alone on the plain renderer
+        import queue
+        pending = queue.Queue()
+        sub_t, done_t, service = [0.0] * len(jobs), [0.0] * len(jobs), [0.0] * len(jobs)
+        base = time.perf_counter()
+
+        def single_feeder():
+            for k in range(len(jobs)):
+                dt = arrivals[k] - (time.perf_counter() - base)
+                if dt > 0:
+                    time.sleep(dt)
+                sub_t[k] = time.perf_counter() - base
+                pending.put(k)
+
+        def single_worker():
+            for _ in range(len(jobs)):
+                k = pending.get()
+                sid, cam, pc, w, h, _, _, spp, dn = jobs[k]
+                r = rs[sid]
+                dch = scenes[sid].state.depth_out_channel
+                ts = time.perf_counter()
+                r.set_film(w, h)
+                r.render(cam, pc, 0, spp, depth_channel=dch)
+                if dn:
+                    den.denoise_film(r, pc.diffuseOutChannel, pc.normalOutChannel, dch, 7)
+                t1 = time.perf_counter()
+                done_t[k] = t1 - base
+                service[k] = t1 - ts
+                if (k + 1) % 200 == 0:
+                    log(f"[serve] rate {lam}/s one-at-a-time: {k + 1} jobs done")
+
+        th, tw = threading.Thread(target=single_feeder), threading.Thread(target=single_worker)
+        th.start()
+        tw.start()
+        th.join()
+        tw.join()
+        blat = [(d - a) * 1e3 for d, a in zip(done_t, sub_t)]
+        finish = max(done_t)
         if args.jobs_log:
             with open(args.jobs_log, "a") as f:
                 for k, (sid, _, _, w, h, sspp, sdn, spp, dn) in enumerate(jobs):
@@ -728,9 +778,10 @@ def run_serving(A, args):
                                         "server_denoise": sdn, "server_ms": round(lat[k], 2),
                                         "single_ms": round(blat[k], 2),
                                         "single_service_ms": round(service[k] * 1e3, 2)}) + "\n")
-        single = {"jobs_per_s": round(len(jobs) / max(finish, arrivals[-1]), 2),
+        single = {"jobs_per_s": round(len(jobs) / finish, 2),
                   "p50_ms": round(float(np.percentile(blat, 50)), 1), "p99_ms": round(float(np.percentile(blat, 99)), 1),
-                  "busy_s": round(sum(service), 2)}
+                  "busy_s": round(sum(service), 2),
+                  "mpaths_per_s": round(sum(jb[3] * jb[4] * jb[7] for jb in jobs) / finish / 1e6, 1)}
         log(f"[serve] rate {lam}/s: server {served}, one-at-a-time {single}")
         out["rates"].append({"rate": lam, "server": served, "single_job": single})
     srv.close()
