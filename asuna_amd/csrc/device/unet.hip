@@ -239,10 +239,102 @@ __global__ __launch_bounds__(256) void kConvFwdTile(ConvArgs a) {
   }
 }
 
+// K-chunked tiled forward convolution for the large levels: a block owns
+// an 8-row x 32-column output tile and walks the input channels in chunks
+// of 32.  Per chunk it stages in LDS, with coalesced 16-byte loads, the
+// chunk of its (8 + 2) x (32 + 2) input tile with the halo (zeros outside
+// the image), and the chunk's weights of all cout for TG taps at a time
+// (all 9 for two output tiles, 3 for four, keeping a block under 64 KB);
+// each wave computes 2 rows x 32 columns (4 groups of 16 pixels) for all
+// NT output tiles with every MFMA operand read from LDS.  Row stride in
+// LDS: 40 BF16 (80 bytes), which spreads a fragment's 16 lanes over the
+// banks.
+constexpr int kChunkRows = 8, kChunkCols = 32, kChunkStride = 40;
+template <int NT> constexpr int chunkTaps() { return NT <= 2 ? 9 : 3; }
+template <int NT>
+__global__ __launch_bounds__(256) void kConvFwdChunk(ConvArgs a) {
+  extern __shared__ __align__(16) bf16 lds[];
+  constexpr uint32_t tw = kChunkCols + 2, tpix = (kChunkRows + 2) * tw;
+  constexpr uint32_t Cout = 16 * NT, TG = chunkTaps<NT>();
+  bf16* tile = lds;                       // tpix x kChunkStride
+  bf16* wts = lds + tpix * kChunkStride;  // TG x Cout x kChunkStride
+  const uint32_t tilesX = (a.W + kChunkCols - 1) / kChunkCols, tilesY = (a.H + kChunkRows - 1) / kChunkRows;
+  const uint32_t tx0 = (blockIdx.x % tilesX) * kChunkCols, ty0 = ((blockIdx.x / tilesX) % tilesY) * kChunkRows;
+  const uint32_t n = blockIdx.x / (tilesX * tilesY);
+  const uint32_t lane = threadIdx.x & 63, q = lane >> 4, r16 = lane & 15, wv = threadIdx.x >> 6;
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int g = 0; g < 4; ++g)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[g][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+  for (uint32_t c0 = 0; c0 < a.Cin; c0 += 32) {
+    if (c0) __syncthreads();  // the previous chunk's fragments are consumed
+    for (uint32_t k = threadIdx.x; k < tpix * 4; k += 256) {
+      const uint32_t c8 = k & 3, pix = k >> 2;
+      const int yy = int(ty0) - 1 + int(pix / tw), xx = int(tx0) - 1 + int(pix % tw);
+      *reinterpret_cast<bf16x8*>(tile + pix * kChunkStride + 8 * c8) = loadIn(a, n, yy, xx, c0 + 8 * c8);
+    }
+    for (uint32_t t0 = 0; t0 < 9; t0 += TG) {
+      if (t0) __syncthreads();  // the previous tap group's fragments are consumed
+      for (uint32_t k = threadIdx.x; k < TG * Cout * 4; k += 256) {
+        const uint32_t c8 = k & 3, row = k >> 2;  // row = (tap - t0) * Cout + co
+        *reinterpret_cast<bf16x8*>(wts + row * kChunkStride + 8 * c8) =
+            *reinterpret_cast<const bf16x8*>(a.Wf + (size_t(t0) * Cout + row) * a.Cin + c0 + 8 * c8);
+      }
+      __syncthreads();
+#pragma unroll 3
+      for (uint32_t tl = 0; tl < TG; ++tl) {
+        const uint32_t tap = t0 + tl, dy = tap / 3, dx = tap % 3;  // tile coordinates are offset by the halo
+        bf16x8 B[NT];
+#pragma unroll
+        for (int t = 0; t < NT; ++t)
+          B[t] = *reinterpret_cast<const bf16x8*>(wts + (tl * Cout + 16 * t + r16) * kChunkStride + 8 * q);
+#pragma unroll
+        for (int g = 0; g < 4; ++g) {
+          const uint32_t row = 2 * wv + uint32_t(g) / 2, col = (uint32_t(g) % 2) * 16 + r16;
+          const bf16x8 A =
+              *reinterpret_cast<const bf16x8*>(tile + ((row + dy) * tw + col + dx) * kChunkStride + 8 * q);
+#pragma unroll
+          for (int t = 0; t < NT; ++t) acc[g][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[t], acc[g][t], 0, 0, 0);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    const uint32_t y = ty0 + 2 * wv + uint32_t(g) / 2;
+    if (y >= a.H) continue;
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const uint32_t co = 16 * t + r16;
+      const float b = a.bias[co];
+      for (int r = 0; r < 4; ++r) {
+        const uint32_t x = tx0 + (uint32_t(g) % 2) * 16 + 4 * q + r;
+        if (x >= a.W) continue;
+        const size_t pp = (size_t(n) * a.H + y) * a.W + x;
+        float v = acc[g][t][r] + b;
+        if (a.relu) v = fmaxf(v, 0.f);
+        if (a.Y) a.Y[pp * Cout + co] = bf16(v);
+        if (a.Yf) a.Yf[pp * Cout + co] = v;
+      }
+    }
+  }
+}
+template <int NT> constexpr size_t chunkLds() {
+  return (size_t(kChunkRows + 2) * (kChunkCols + 2) + size_t(chunkTaps<NT>()) * 16 * NT) * kChunkStride * sizeof(bf16);
+}
+static_assert(chunkLds<2>() <= 65536 && chunkLds<4>() <= 65536, "chunked convolution LDS above 64 KB");
+
 template <int NT>
 static void convFwdFor(const ConvArgs& a, size_t P, hipStream_t s) {
+  if (P >= (size_t(1) << 16) && NT <= 4) {
+    // large levels: the K-chunked LDS kernel (input tile and weights of each 32-channel chunk in LDS)
+    const uint32_t tiles = a.N * ((a.H + kChunkRows - 1) / kChunkRows) * ((a.W + kChunkCols - 1) / kChunkCols);
+    kConvFwdChunk<NT><<<tiles, 256, chunkLds<NT>(), s>>>(a);
+    return;
+  }
   if (P >= (size_t(1) << 18) && a.Cin <= 128) {
-    // large levels: the LDS-tiled kernel (32-column tiles up to 64 input channels, else 16)
+    // large levels with wide outputs: the LDS-tiled kernel (32-column tiles up to 64 input channels, else 16)
     const bool wide = a.Cin <= 64;
     const uint32_t TC = wide ? 32 : 16;
     const uint32_t tiles = a.N * ((a.H + kTileRows - 1) / kTileRows) * ((a.W + TC - 1) / TC);
