@@ -316,6 +316,36 @@ def test_scene_generator_is_deterministic(tmp_path):
         assert filecmp.cmp(os.path.join(os.path.dirname(a), f), os.path.join(os.path.dirname(b), f), shallow=False), f
 
 
+def test_training_scenes_follow_the_denoiser_data_spec(tmp_path):
+    """Generator config 3 (the denoiser's training set): per seed 1-20
+    displaced meshes with materials from the plugin set, a ground, 1-4 rect
+    lights facing down and an env map; the same seed writes the same files,
+    different seeds different scenes."""
+    import json
+    kinds, counts = set(), []
+    for k in range(12):
+        p = A.generate_scene(3, str(tmp_path / f"s{k}"), seed=1000 + k)
+        js = json.load(open(p))
+        n = len(js["instances"]) - 1  # the ground is the last instance
+        assert 1 <= n <= 20 and js["instances"][-1]["mesh"] == "ground"
+        assert 1 <= len(js["lights"]) <= 4
+        for l in js["lights"]:
+            assert l["type"] == "rect"
+            u = np.subtract(l["v1"], l["position"])
+            v = np.subtract(l["v2"], l["position"])
+            assert np.cross(u, v)[1] < 0  # emits downwards
+        assert js["envmap"]["path"] == "sky.exr"
+        kinds |= {m["type"] for m in js["materials"][:-1]}
+        counts.append(n)
+        s = A.HostScene(p)
+        assert s.film == (256, 256) and s.state.num_triangles > 0
+    assert kinds <= {"brdf_lambertian", "brdf_pbr_metalness_roughness", "brdf_rough_plastic", "bsdf_dielectric"}
+    assert len(kinds) >= 3 and len(set(counts)) > 3
+    a = A.generate_scene(3, str(tmp_path / "again"), seed=1000)
+    for f in os.listdir(os.path.dirname(a)):
+        assert filecmp.cmp(os.path.join(os.path.dirname(a), f), os.path.join(str(tmp_path / "s0"), f), shallow=False), f
+
+
 def test_multishot_scene_has_64_shots_with_own_fov(multishot_scene):
     s = A.HostScene(multishot_scene)
     assert s.state.num_shots == 64
