@@ -4,7 +4,7 @@
 // One flat BVH over world-space triangles of all instances:
 //   1. world triangles + boxes + centroids (one thread per triangle)
 //   2. centroid bounds (block reduction)
-//   3. 63-bit Morton keys (21 bits/axis), radix sort (hipCUB)
+//   3. 63-bit Morton keys (21 bits/axis), stable LSD radix sort (below)
 //   4. binary radix tree, Karras 2012 "Maximizing Parallelism in the
 //      Construction of BVHs, Octrees, and k-d Trees" (equal keys ordered by
 //      index)
@@ -14,8 +14,7 @@
 //   6. compaction of the surviving internal nodes (prefix sum) and emission
 //      of 64-byte BVH2 nodes that hold both child boxes; triangles are
 //      reordered into leaf order.
-#include <hipcub/hipcub.hpp>
-
+#include <algorithm>
 #include <cfloat>
 #include <cstdio>
 #include <vector>
@@ -663,6 +662,154 @@ template <class T> struct DevBuf {
   }
 };
 
+// ------------------------------------------------------- scan and sort
+// Exclusive prefix sum of uint32 values in three passes, deterministic:
+// per-2048-element tile sums, one block scanning the tile sums, then each
+// tile's scan plus its offset.  Work per thread: 8 consecutive elements.
+constexpr uint32_t kScanTile = 2048;
+__device__ __forceinline__ uint32_t blockExclusive(uint32_t v, uint32_t* sh, uint32_t& total) {
+  // exclusive scan of one value per thread over a 256-thread block
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  uint32_t x = v;
+  for (uint32_t d = 1; d < 64; d <<= 1) {
+    const uint32_t y = __shfl_up(x, d);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) sh[w] = x;
+  __syncthreads();
+  uint32_t base = 0;
+  for (uint32_t k = 0; k < w; ++k) base += sh[k];
+  total = sh[0] + sh[1] + sh[2] + sh[3];
+  __syncthreads();
+  return base + x - v;
+}
+__global__ __launch_bounds__(256) void kScanTileSums(const uint32_t* in, uint32_t n, uint32_t* tileSums) {
+  __shared__ uint32_t sh[4];
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 8;
+  uint32_t s = 0;
+  for (uint32_t k = 0; k < 8; ++k)
+    if (base + k < n) s += in[base + k];
+  uint32_t total;
+  (void)blockExclusive(s, sh, total);
+  if (threadIdx.x == 0) tileSums[blockIdx.x] = total;
+}
+__global__ __launch_bounds__(256) void kScanOfSums(uint32_t* sums, uint32_t nt) {
+  // one block: each thread scans a contiguous run of the tile sums
+  __shared__ uint32_t sh[4];
+  const uint32_t per = (nt + 255) / 256, b0 = threadIdx.x * per, b1 = min(nt, b0 + per);
+  uint32_t s = 0;
+  for (uint32_t k = b0; k < b1; ++k) s += sums[k];
+  uint32_t total, run = blockExclusive(s, sh, total);
+  for (uint32_t k = b0; k < b1; ++k) {
+    const uint32_t v = sums[k];
+    sums[k] = run;
+    run += v;
+  }
+}
+__global__ __launch_bounds__(256) void kScanApply(const uint32_t* in, uint32_t n, const uint32_t* tileOff,
+                                                  uint32_t* out) {
+  __shared__ uint32_t sh[4];
+  const uint32_t base = blockIdx.x * kScanTile + threadIdx.x * 8;
+  uint32_t v[8], s = 0;
+  for (uint32_t k = 0; k < 8; ++k) {
+    v[k] = base + k < n ? in[base + k] : 0u;
+    s += v[k];
+  }
+  uint32_t total, run = blockExclusive(s, sh, total) + tileOff[blockIdx.x];
+  for (uint32_t k = 0; k < 8; ++k) {
+    if (base + k < n) out[base + k] = run;
+    run += v[k];
+  }
+}
+// Scratch words the scan of n values needs (the tile sums).
+static size_t scanScratch(uint32_t n) { return std::max<size_t>(1, (size_t(n) + kScanTile - 1) / kScanTile); }
+static void exclusiveScan(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* scratch, hipStream_t s) {
+  if (!n) return;
+  const uint32_t nt = uint32_t(scanScratch(n));
+  kScanTileSums<<<nt, 256, 0, s>>>(in, n, scratch);
+  kScanOfSums<<<1, 256, 0, s>>>(scratch, nt);
+  kScanApply<<<nt, 256, 0, s>>>(in, n, scratch, out);
+}
+
+// Stable LSD radix sort of (64-bit key, 32-bit value) pairs, 8-bit digits.
+// Per pass: each 2048-key tile counts its digits (written digit-major, so
+// that one exclusive scan of the counts gives every (digit, tile) its
+// output base), then scatters its keys in their input order: the tile is
+// walked in 8 rows of 256 keys; within a row a lane's rank among the lanes
+// of its wave with the same digit comes from 8 ballots, and the waves'
+// per-digit counts (in LDS) order the waves, so equal digits keep their
+// input order and the sort is stable.
+constexpr uint32_t kSortTile = 2048, kRadix = 256;
+__global__ __launch_bounds__(256) void kRadixCount(const uint64_t* keys, uint32_t n, uint32_t shift,
+                                                   uint32_t nTiles, uint32_t* counts) {
+  __shared__ uint32_t h[kRadix];
+  h[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t base = blockIdx.x * kSortTile;
+  for (uint32_t k = threadIdx.x; k < kSortTile; k += 256)
+    if (base + k < n) atomicAdd(&h[uint32_t(keys[base + k] >> shift) & (kRadix - 1)], 1u);
+  __syncthreads();
+  counts[size_t(threadIdx.x) * nTiles + blockIdx.x] = h[threadIdx.x];
+}
+__global__ __launch_bounds__(256) void kRadixScatter(const uint64_t* keys, const uint32_t* vals, uint32_t n,
+                                                     uint32_t shift, uint32_t nTiles, const uint32_t* offsets,
+                                                     uint64_t* keysOut, uint32_t* valsOut) {
+  __shared__ uint32_t run[kRadix];      // keys of each digit already placed by this tile
+  __shared__ uint32_t wcnt[4][kRadix];  // per-wave digit counts of the current row
+  const uint32_t t = threadIdx.x, lane = t & 63, w = t >> 6;
+  run[t] = offsets[size_t(t) * nTiles + blockIdx.x];
+  for (uint32_t k = 0; k < 4; ++k) wcnt[k][t] = 0;
+  __syncthreads();
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const uint32_t base = blockIdx.x * kSortTile;
+  for (uint32_t row = 0; row < kSortTile / 256; ++row) {
+    const uint32_t i = base + row * 256 + t;
+    const bool in = i < n;
+    uint64_t key = 0;
+    uint32_t val = 0, d = 0;
+    if (in) {
+      key = keys[i];
+      val = vals[i];
+      d = uint32_t(key >> shift) & (kRadix - 1);
+    }
+    uint64_t match = __ballot(in);
+    for (uint32_t b = 0; b < 8; ++b) {
+      const uint64_t m = __ballot(in && ((d >> b) & 1u));
+      match &= ((d >> b) & 1u) ? m : ~m;
+    }
+    const uint32_t rank = uint32_t(__popcll(match & below));
+    if (in && rank == 0) wcnt[w][d] = uint32_t(__popcll(match));
+    __syncthreads();
+    if (in) {
+      uint32_t pos = run[d] + rank;
+      for (uint32_t k = 0; k < w; ++k) pos += wcnt[k][d];
+      keysOut[pos] = key;
+      valsOut[pos] = val;
+    }
+    __syncthreads();
+    run[t] += wcnt[0][t] + wcnt[1][t] + wcnt[2][t] + wcnt[3][t];
+    for (uint32_t k = 0; k < 4; ++k) wcnt[k][t] = 0;
+    __syncthreads();
+  }
+}
+
+// Sorts n (key, value) pairs by the low `bits` bits of the key, stably, in
+// passes of 8 bits; each pass swaps the (keys, keysTmp) and (vals, valsTmp)
+// pointers, so the result ends in keys / vals.  counts and offsets hold
+// 256 x tiles words, scanScr scanScratch(256 x tiles) words.
+static void radixSortPairs(uint64_t*& keys, uint64_t*& keysTmp, uint32_t*& vals, uint32_t*& valsTmp, uint32_t n,
+                           uint32_t bits, uint32_t* counts, uint32_t* offsets, uint32_t* scanScr, hipStream_t s) {
+  const uint32_t nTiles = (n + kSortTile - 1) / kSortTile;
+  for (uint32_t shift = 0; shift < bits; shift += 8) {
+    kRadixCount<<<nTiles, 256, 0, s>>>(keys, n, shift, nTiles, counts);
+    exclusiveScan(counts, offsets, kRadix * nTiles, scanScr, s);
+    kRadixScatter<<<nTiles, 256, 0, s>>>(keys, vals, n, shift, nTiles, offsets, keysTmp, valsTmp);
+    std::swap(keys, keysTmp);
+    std::swap(vals, valsTmp);
+  }
+}
+
+
 }  // namespace
 
 int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_t errLen) {
@@ -688,7 +835,7 @@ int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_
   DevBuf<int> level;
   DevBuf<float> cost;
   DevBuf<uint8_t> collapsed;
-  DevBuf<char> tmp;
+  DevBuf<uint32_t> counts, countOffs, scanScr;
   size_t nn = n ? n : 1;
   HIPB(hipMalloc(&wtri.p, nn * 3 * sizeof(float4)));
   HIPB(hipMalloc(&boxLo.p, nn * sizeof(float4)));
@@ -707,10 +854,14 @@ int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_
     kWorldTris<<<grid, kBlock, 0, s>>>(a, wtri.p, boxLo.p, boxHi.p);
     kCentroidBounds<<<nPart, kBlock, 0, s>>>(boxLo.p, boxHi.p, n, partLo.p, partHi.p);
     kMorton<<<grid, kBlock, 0, s>>>(boxLo.p, boxHi.p, n, partLo.p, partHi.p, nPart, keys.p, ids.p);
-    size_t tb = 0;
-    HIPB(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, keys.p, keys2.p, ids.p, ids2.p, int(n), 0, 63, s));
-    HIPB(hipMalloc(&tmp.p, tb));
-    HIPB(hipcub::DeviceRadixSort::SortPairs(tmp.p, tb, keys.p, keys2.p, ids.p, ids2.p, int(n), 0, 63, s));
+    const uint32_t nTiles = (n + kSortTile - 1) / kSortTile, nCounts = kRadix * nTiles;
+    HIPB(hipMalloc(&counts.p, size_t(nCounts) * sizeof(uint32_t)));
+    HIPB(hipMalloc(&countOffs.p, size_t(nCounts) * sizeof(uint32_t)));
+    HIPB(hipMalloc(&scanScr.p, std::max(scanScratch(nCounts), scanScratch(n)) * sizeof(uint32_t)));
+    radixSortPairs(keys.p, keys2.p, ids.p, ids2.p, n, 63, counts.p, countOffs.p, scanScr.p, s);
+    std::swap(keys.p, keys2.p);  // the sorted pairs live in keys2 / ids2 from here on
+    std::swap(ids.p, ids2.p);
+    HIPB(hipGetLastError());
   }
   const uint32_t* sorted = ids2.p;
   uint32_t nInternal = n > 1 ? n - 1 : 0;
@@ -758,17 +909,13 @@ int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_
       int rootParent = -1;
       HIPB(hipMemcpyAsync(parent.p, &rootParent, sizeof(int), hipMemcpyHostToDevice, s));
       kPlocInit<<<grid, kBlock, 0, s>>>(n, ids2.p, boxLo.p, boxHi.p, cNode[0].p, cLo[0].p, cHi[0].p);
-      size_t stb = 0;
-      HIPB(hipcub::DeviceScan::ExclusiveSum(nullptr, stb, valid.p, offs.p, int(n), s));
-      DevBuf<char> stmp;
-      HIPB(hipMalloc(&stmp.p, stb));
       uint32_t m = n, iters = 0;
       while (m > 1) {
         unsigned gm = (m + kBlock - 1) / kBlock;
         kPlocNearest<<<gm, kBlock, 0, s>>>(m, cLo[0].p, cHi[0].p, nnb.p);
         kPlocMerge<<<gm, kBlock, 0, s>>>(m, n, nnb.p, cNode[0].p, cLo[0].p, cHi[0].p, cNode[1].p, cLo[1].p, cHi[1].p,
                                           valid.p, left.p, right.p, parent.p, leafParent.p, nodeCnt.p, idCounter.p);
-        HIPB(hipcub::DeviceScan::ExclusiveSum(stmp.p, stb, valid.p, offs.p, int(m), s));
+        exclusiveScan(valid.p, offs.p, m, scanScr.p, s);
         kPlocCompact<<<gm, kBlock, 0, s>>>(m, valid.p, offs.p, cNode[1].p, cLo[1].p, cHi[1].p, cNode[0].p, cLo[0].p,
                                             cHi[0].p);
         uint32_t lastOff = 0, lastValid = 0;
@@ -811,11 +958,7 @@ int buildBvh(const BuildArgs& a, hipStream_t s, BuildResult& r, char* err, size_
       kRefitLevel<<<gi, kBlock, 0, s>>>(int(nInternal), lv, level.p, sorted, boxLo.p, boxHi.p, left.p, right.p,
                                         range.p, nodeLo.p, nodeHi.p, cost.p, collapsed.p, a.leafMax, a.quality);
     kEmitFlags<<<gi, kBlock, 0, s>>>(int(nInternal), parent.p, collapsed.p, emitted.p, maxDepth.p);
-    size_t tb = 0;
-    HIPB(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, emitted.p, newIdx.p, int(nInternal), s));
-    DevBuf<char> tmp2;
-    HIPB(hipMalloc(&tmp2.p, tb));
-    HIPB(hipcub::DeviceScan::ExclusiveSum(tmp2.p, tb, emitted.p, newIdx.p, int(nInternal), s));
+    exclusiveScan(emitted.p, newIdx.p, nInternal, scanScr.p, s);
     uint32_t lastIdx = 0, lastEm = 0;
     uint8_t rootCollapsed = 0;
     float rootCost = 0.f;
