@@ -131,14 +131,15 @@ class Stats(C.Structure):
 
 class BuildOpts(C.Structure):
     _fields_ = [("leaf_max_tris", C.c_uint32), ("quality", C.c_uint32), ("branching", C.c_uint32),
-                ("node_format", C.c_uint32)]
+                ("node_format", C.c_uint32), ("two_level", C.c_uint32)]
 
 
 class BuildInfo(C.Structure):
     _fields_ = [("num_triangles", C.c_uint64), ("num_nodes", C.c_uint64), ("max_depth", C.c_uint32),
                 ("node_bytes", C.c_uint32), ("build_ms", C.c_double), ("sah_cost", C.c_double),
                 ("device_bytes", C.c_uint64), ("num_wide_nodes", C.c_uint64), ("wide_depth", C.c_uint32),
-                ("branching", C.c_uint32)]
+                ("branching", C.c_uint32), ("blas_count", C.c_uint32), ("pad", C.c_uint32),
+                ("blas_triangles", C.c_uint64)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_ if k != "pad"}
@@ -439,9 +440,9 @@ class Renderer:
     def upload_desc(self, desc: SceneDesc):
         self._check(lib().ash_upload_scene(self._c, C.byref(desc)), "upload_scene")
 
-    def build(self, leaf_max_tris=4, quality=2, branching=4, node_format=0) -> dict:
+    def build(self, leaf_max_tris=4, quality=2, branching=4, node_format=0, two_level=False) -> dict:
         info = BuildInfo()
-        opts = BuildOpts(leaf_max_tris, quality, branching, node_format)
+        opts = BuildOpts(leaf_max_tris, quality, branching, node_format, int(bool(two_level)))
         self._check(lib().ash_build_accel(self._c, C.byref(opts), C.byref(info)),
                     "build_accel")
         return info.as_dict()

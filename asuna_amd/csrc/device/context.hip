@@ -119,11 +119,25 @@ struct ash_ctx {
   GraphEntry graphs[4];
   uint64_t graphClock = 0;
   bool graphsBroken = false;
+  // two-level acceleration: BLAS arrays owned here, TLAS and tables
+  std::vector<DevInstance> hostInst;  // host copy of the instance records
+  bool twoLevel = false;
+  std::vector<float4*> blasArrays;
+  DBuf<float4> tlas;
+  DBuf<uint32_t> tlasInst, instBlas;
+  DBuf<DevBlas> blasTab;
   // device address of this GPU's allocation-free render scratch (null when the lookup failed)
   unsigned char* scratch = nullptr;
   hipError_t scratchErr = hipSuccess;
 
+  void releaseTwoLevel() {
+    for (float4* p : blasArrays) (void)hipFree(p);
+    blasArrays.clear();
+    tlas.release(), tlasInst.release(), instBlas.release(), blasTab.release();
+    twoLevel = false;
+  }
   ~ash_ctx() {
+    releaseTwoLevel();
     if (nodes) (void)hipFree(nodes);
     if (nodes4) (void)hipFree(nodes4);
     if (nodesQ) (void)hipFree(nodesQ);
@@ -213,6 +227,11 @@ static SceneView viewOf(const ash_ctx* c) {
   v.marg = c->marg.p;
   v.cond = c->cond.p;
   v.sky = c->sky.p;
+  v.twoLevel = c->twoLevel ? 1u : 0u;
+  v.tlas = c->tlas.p;
+  v.tlasInst = c->tlasInst.p;
+  v.blas = c->blasTab.p;
+  v.instBlas = c->instBlas.p;
   return v;
 }
 
@@ -584,6 +603,218 @@ static int renderCommon(ash_ctx* c, uint32_t nShots, const ShotParams* hostShots
   return ASH_OK;
 }
 
+// ------------------------------------------------- two-level acceleration
+// TLAS over the instances' world boxes: a binary tree built on the host by
+// median splits of the box centres along their widest axis (the instance
+// count is small beside the triangle count), in the BVH2 node layout the
+// binary walk reads: child boxes in the parent, leaf = one instance.
+struct HostBox {
+  double lo[3], hi[3];
+};
+static float refBits(int32_t r) {
+  float f;
+  std::memcpy(&f, &r, 4);
+  return f;
+}
+static int32_t bitsRef(float f) {
+  int32_t r;
+  std::memcpy(&r, &f, 4);
+  return r;
+}
+static void tlasNode(std::vector<float4>& nodes, size_t at, const HostBox& L, int32_t rl, const HostBox& R, int32_t rr) {
+  auto f = [](double v, bool up) {
+    float x = float(v);
+    if (up ? double(x) < v : double(x) > v) x = std::nextafter(x, up ? INFINITY : -INFINITY);
+    return x;
+  };
+  nodes[at + 0] = make_float4(f(L.lo[0], false), f(L.hi[0], true), f(L.lo[1], false), f(L.hi[1], true));
+  nodes[at + 1] = make_float4(f(R.lo[0], false), f(R.hi[0], true), f(R.lo[1], false), f(R.hi[1], true));
+  nodes[at + 2] = make_float4(f(L.lo[2], false), f(L.hi[2], true), f(R.lo[2], false), f(R.hi[2], true));
+  nodes[at + 3] = make_float4(refBits(rl), refBits(rr), 0.f, 0.f);
+}
+static int32_t tlasBuild(std::vector<float4>& nodes, std::vector<uint32_t>& order, const std::vector<HostBox>& box,
+                         std::vector<uint32_t>& items, size_t b, size_t e, HostBox& out, uint32_t depth,
+                         uint32_t& maxDepth) {
+  maxDepth = std::max(maxDepth, depth);
+  if (e - b == 1) {
+    out = box[items[b]];
+    order.push_back(items[b]);
+    return encodeLeaf(uint32_t(order.size() - 1), 1);
+  }
+  const size_t at = nodes.size();
+  nodes.resize(at + 4);
+  double clo[3] = {INFINITY, INFINITY, INFINITY}, chi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (size_t k = b; k < e; ++k)
+    for (int a = 0; a < 3; ++a) {
+      const double cc = 0.5 * (box[items[k]].lo[a] + box[items[k]].hi[a]);
+      clo[a] = std::min(clo[a], cc), chi[a] = std::max(chi[a], cc);
+    }
+  int ax = 0;
+  for (int a = 1; a < 3; ++a)
+    if (chi[a] - clo[a] > chi[ax] - clo[ax]) ax = a;
+  const size_t m = b + (e - b) / 2;
+  std::nth_element(items.begin() + b, items.begin() + m, items.begin() + e, [&](uint32_t x, uint32_t y) {
+    const double cx = box[x].lo[ax] + box[x].hi[ax], cy = box[y].lo[ax] + box[y].hi[ax];
+    return cx < cy || (cx == cy && x < y);
+  });
+  HostBox L, R;
+  const int32_t rl = tlasBuild(nodes, order, box, items, b, m, L, depth + 1, maxDepth);
+  const int32_t rr = tlasBuild(nodes, order, box, items, m, e, R, depth + 1, maxDepth);
+  tlasNode(nodes, at, L, rl, R, rr);
+  for (int a = 0; a < 3; ++a) out.lo[a] = std::min(L.lo[a], R.lo[a]), out.hi[a] = std::max(L.hi[a], R.hi[a]);
+  return int32_t(at / 4);
+}
+
+// One object-space BLAS per distinct mesh (instances share a mesh when
+// they share its vertex and index ranges), each built by buildBvh over a
+// single identity instance, then the TLAS.
+static int buildTwoLevel(ash_ctx* c, const BuildArgs& tmpl, ash_build_info* info) {
+  hipStream_t s = c->stream;
+  std::vector<uint32_t> instBlas(c->numInst, 0xffffffffu);
+  std::vector<DevInstance> meshInst;  // the identity instance of each distinct mesh
+  for (uint32_t i = 0; i < c->numInst; ++i) {
+    const DevInstance& in = c->hostInst[i];
+    if (!in.ntri) continue;
+    uint32_t k = 0;
+    while (k < meshInst.size() && !(meshInst[k].vbase == in.vbase && meshInst[k].ibase == in.ibase &&
+                                     meshInst[k].ntri == in.ntri))
+      ++k;
+    if (k == meshInst.size()) {
+      DevInstance id;
+      std::memset(&id, 0, sizeof id);
+      id.o2w[0] = id.o2w[5] = id.o2w[10] = 1.f;
+      id.w2o[0] = id.w2o[5] = id.w2o[10] = 1.f;
+      id.vbase = in.vbase, id.ibase = in.ibase, id.ntri = in.ntri, id.light = -1;
+      meshInst.push_back(id);
+    }
+    instBlas[i] = k;
+  }
+  const uint32_t nb = uint32_t(meshInst.size());
+  std::vector<DevBlas> tab(nb);
+  std::vector<HostBox> meshBox(nb);
+  uint64_t shadeTotal = 0, nodes4 = 0, nodes2 = 0, tris = 0;
+  uint32_t depth4 = 0;
+  double ms = 0;
+  for (uint32_t k = 0; k < nb; ++k) tab[k].shadeBase = uint32_t(shadeTotal), shadeTotal += meshInst[k].ntri;
+  float4* shade = nullptr;
+  HIPC(c, hipMalloc(&shade, std::max<uint64_t>(shadeTotal, 1) * 6 * sizeof(float4)));
+  c->shadeTris = shade;
+  DBuf<DevInstance> one;
+  DBuf<uint32_t> offs;
+  MALLOC(c, one, 1);
+  MALLOC(c, offs, 2);
+  for (uint32_t k = 0; k < nb; ++k) {
+    const uint32_t o2[2] = {0, meshInst[k].ntri};
+    HIPC(c, hipMemcpyAsync(one.p, &meshInst[k], sizeof(DevInstance), hipMemcpyHostToDevice, s));
+    HIPC(c, hipMemcpyAsync(offs.p, o2, sizeof o2, hipMemcpyHostToDevice, s));
+    BuildArgs a = tmpl;
+    a.inst = one.p, a.numInst = 1, a.instTriOffset = offs.p, a.numTris = meshInst[k].ntri;
+    a.branching = 4, a.quantize = 0;
+    BuildResult r;
+    char err[512] = {0};
+    int rc = buildBvh(a, s, r, err, sizeof err);  // synchronises the stream before returning
+    float4 root[4];
+    if (rc == ASH_OK && hipMemcpy(root, r.nodes, sizeof root, hipMemcpyDeviceToHost) != hipSuccess) rc = ASH_E_HIP;
+    if (rc == ASH_OK && hipMemcpy(shade + size_t(tab[k].shadeBase) * 6, r.shade,
+                                  size_t(meshInst[k].ntri) * 6 * sizeof(float4), hipMemcpyDeviceToDevice) != hipSuccess)
+      rc = ASH_E_HIP;
+    if (r.nodes) (void)hipFree(r.nodes);
+    if (r.nodesQ) (void)hipFree(r.nodesQ);
+    if (r.shade) (void)hipFree(r.shade);
+    if (r.nodes4) c->blasArrays.push_back(r.nodes4);
+    if (r.tris) c->blasArrays.push_back(r.tris);
+    if (rc) return fail(c, rc, "BLAS %u: %s", k, err[0] ? err : "copy failed");
+    tab[k].nodes4 = r.nodes4, tab[k].tris = r.tris, tab[k].ntri = meshInst[k].ntri;
+    // the root's box: union of its non-empty children (binary node layout)
+    HostBox b{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+    const float cl[2][6] = {{root[0].x, root[0].y, root[0].z, root[0].w, root[2].x, root[2].y},
+                            {root[1].x, root[1].y, root[1].z, root[1].w, root[2].z, root[2].w}};
+    const int32_t refs[2] = {bitsRef(root[3].x), bitsRef(root[3].y)};
+    for (int ch = 0; ch < 2; ++ch) {
+      if (refs[ch] == kEmptyRef) continue;
+      for (int a = 0; a < 3; ++a)
+        b.lo[a] = std::min(b.lo[a], double(cl[ch][2 * a])), b.hi[a] = std::max(b.hi[a], double(cl[ch][2 * a + 1]));
+    }
+    meshBox[k] = b;
+    nodes4 += r.numNodes4, nodes2 += r.numNodes, tris += meshInst[k].ntri;
+    depth4 = std::max(depth4, r.depth4);
+    ms += r.ms;
+  }
+  // instance world boxes: the 8 corners of the mesh box through o2w, in
+  // double, padded outwards by 1e-5 of their size and magnitude
+  std::vector<HostBox> ibox(c->numInst);
+  std::vector<uint32_t> items;
+  for (uint32_t i = 0; i < c->numInst; ++i) {
+    if (instBlas[i] == 0xffffffffu) continue;
+    const HostBox& mb = meshBox[instBlas[i]];
+    if (!(mb.lo[0] <= mb.hi[0])) continue;
+    const float* m = c->hostInst[i].o2w;
+    HostBox w{{INFINITY, INFINITY, INFINITY}, {-INFINITY, -INFINITY, -INFINITY}};
+    for (int corner = 0; corner < 8; ++corner) {
+      const double p[3] = {corner & 1 ? mb.hi[0] : mb.lo[0], corner & 2 ? mb.hi[1] : mb.lo[1],
+                           corner & 4 ? mb.hi[2] : mb.lo[2]};
+      for (int a = 0; a < 3; ++a) {
+        const double v = double(m[4 * a]) * p[0] + double(m[4 * a + 1]) * p[1] + double(m[4 * a + 2]) * p[2] +
+                         double(m[4 * a + 3]);
+        w.lo[a] = std::min(w.lo[a], v), w.hi[a] = std::max(w.hi[a], v);
+      }
+    }
+    double mag = 0, ext = 0;
+    for (int a = 0; a < 3; ++a)
+      mag = std::max({mag, std::fabs(w.lo[a]), std::fabs(w.hi[a])}), ext = std::max(ext, w.hi[a] - w.lo[a]);
+    const double pad = 1e-5 * (mag + ext) + 1e-30;
+    for (int a = 0; a < 3; ++a) w.lo[a] -= pad, w.hi[a] += pad;
+    ibox[i] = w;
+    items.push_back(i);
+  }
+  std::vector<float4> tnodes;
+  std::vector<uint32_t> order;
+  uint32_t tdepth = 1;
+  const HostBox far{{3.0e38, 3.0e38, 3.0e38}, {3.0e38, 3.0e38, 3.0e38}};
+  if (items.size() <= 1) {
+    tnodes.resize(4);
+    if (items.empty()) {
+      tlasNode(tnodes, 0, far, kEmptyRef, far, kEmptyRef);
+    } else {
+      order.push_back(items[0]);
+      tlasNode(tnodes, 0, ibox[items[0]], encodeLeaf(0, 1), far, kEmptyRef);
+    }
+  } else {
+    HostBox all;
+    tdepth = 0;
+    (void)tlasBuild(tnodes, order, ibox, items, 0, items.size(), all, 0, tdepth);
+  }
+  MALLOC(c, c->tlas, tnodes.size());
+  MALLOC(c, c->tlasInst, std::max<size_t>(order.size(), 1));
+  MALLOC(c, c->instBlas, std::max<size_t>(instBlas.size(), 1));
+  MALLOC(c, c->blasTab, std::max<size_t>(tab.size(), 1));
+  HIPC(c, hipMemcpy(c->tlas.p, tnodes.data(), tnodes.size() * sizeof(float4), hipMemcpyHostToDevice));
+  if (!order.empty()) HIPC(c, hipMemcpy(c->tlasInst.p, order.data(), order.size() * 4, hipMemcpyHostToDevice));
+  if (!instBlas.empty()) HIPC(c, hipMemcpy(c->instBlas.p, instBlas.data(), instBlas.size() * 4, hipMemcpyHostToDevice));
+  if (!tab.empty()) HIPC(c, hipMemcpy(c->blasTab.p, tab.data(), tab.size() * sizeof(DevBlas), hipMemcpyHostToDevice));
+  c->twoLevel = true;
+  c->nodeCount = nodes4;
+  c->maxDepth = tdepth + depth4;
+  c->stackNeed = tdepth + 1 + 3 * depth4;  // one TLAS entry per level, three per 4-wide BLAS level
+  if (info) {
+    std::memset(info, 0, sizeof *info);
+    info->num_triangles = c->numTris;
+    info->num_nodes = nodes4 + tnodes.size() / 4;
+    info->max_depth = c->maxDepth;
+    info->node_bytes = 128;
+    info->build_ms = ms;
+    info->sah_cost = 0;
+    info->device_bytes = nodes4 * 128 + tris * (48 + 96) + tnodes.size() * 16 + order.size() * 4 +
+                         instBlas.size() * 4 + tab.size() * sizeof(DevBlas) + c->verts.n * 16 + c->idx.n * 4;
+    info->num_wide_nodes = nodes4;
+    info->wide_depth = depth4;
+    info->branching = 4;
+    info->blas_count = nb;
+    info->blas_triangles = tris;
+  }
+  return ASH_OK;
+}
+
 // ------------------------------------------------- allocation-free render
 // ash_render_noalloc keeps its wavefront (path state, queues, counters, the
 // batch's work unit and shot record, traversal stack spill) in one fixed
@@ -811,6 +1042,7 @@ int ash_upload_scene(ash_ctx* c, const ash_scene_desc* d) {
   HIPC(c, hipMemcpyAsync(c->idx.p, hi.data(), hi.size() * 4, hipMemcpyHostToDevice, s));
   HIPC(c, hipMemcpyAsync(c->inst.p, hinst.data(), hinst.size() * sizeof(DevInstance), hipMemcpyHostToDevice, s));
   HIPC(c, hipMemcpyAsync(c->instTriOffset.p, off.data(), off.size() * 4, hipMemcpyHostToDevice, s));
+  c->hostInst = hinst;
   std::vector<DevMaterial> hmats(d->num_materials);
   for (uint32_t k = 0; k < d->num_materials; ++k) {
     std::memset(&hmats[k], 0, sizeof(DevMaterial));
@@ -881,6 +1113,7 @@ int ash_build_accel(ash_ctx* c, const ash_build_opts* o, ash_build_info* info) {
   if (c->nodesQ) (void)hipFree(c->nodesQ), c->nodesQ = nullptr;
   if (c->tris) (void)hipFree(c->tris), c->tris = nullptr;
   if (c->shadeTris) (void)hipFree(c->shadeTris), c->shadeTris = nullptr;
+  c->releaseTwoLevel();
   c->hasAccel = false;
   BuildArgs a;
   a.inst = c->inst.p;
@@ -898,6 +1131,17 @@ int ash_build_accel(ash_ctx* c, const ash_build_opts* o, ash_build_info* info) {
   uint32_t fmt = o ? o->node_format : 0;
   if (fmt > 2) return fail(c, ASH_E_INVALID, "node_format must be 0, 1 or 2");
   a.quantize = a.branching == 4 && fmt == 2;
+  if (o && o->two_level) {
+    if (a.branching != 4 || fmt == 2) return fail(c, ASH_E_INVALID, "two_level needs the float 4-wide tree");
+    const int rc = buildTwoLevel(c, a, info);
+    if (rc) {
+      c->releaseTwoLevel();
+      if (c->shadeTris) (void)hipFree(c->shadeTris), c->shadeTris = nullptr;
+      return rc;
+    }
+    c->hasAccel = true;
+    return ASH_OK;
+  }
   BuildResult r;
   char err[512] = {0};
   int rc = buildBvh(a, c->stream, r, err, sizeof err);
@@ -932,6 +1176,8 @@ int ash_build_accel(ash_ctx* c, const ash_build_opts* o, ash_build_info* info) {
     info->num_wide_nodes = r.numNodes4;
     info->wide_depth = r.depth4;
     info->branching = a.branching;
+    info->blas_count = 0;
+    info->blas_triangles = 0;
   }
   return ASH_OK;
 }

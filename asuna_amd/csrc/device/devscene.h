@@ -7,7 +7,7 @@
 //                                           node: lo.x[4] hi.x[4] lo.y[4] hi.y[4] lo.z[4] hi.z[4]
 //                                           child refs[4], unused[4]
 //   tris       float4[3*numTris]           world-space v0,v1,v2 in leaf order (48 B/tri);
-//                                           .w of v0/v1/v2 = instance, primitive, global id
+//                                           .w of v0/v1/v2 = instance, shade word, global id
 //   materials  GpuMaterial[], lights GpuLight[], textures (RGBA32F), env + tables
 #pragma once
 
@@ -65,6 +65,18 @@ __host__ __device__ inline void decodeLeaf(int32_t ref, uint32_t& first, uint32_
 }
 // A child slot that holds nothing: far-away degenerate box and this ref.
 constexpr int32_t kEmptyRef = int32_t(0x7fffffff);
+
+// Two-level acceleration (instancing without duplication): one object-space
+// BVH (BLAS) per distinct mesh, shared by all its instances, under a binary
+// tree over the instances' world boxes (TLAS).  A BLAS triangle record keeps
+// the mesh-local primitive in v2.w; the instance, its shade word and its
+// global triangle ids come from the instance walked.
+struct DevBlas {
+  const float4* nodes4;  // 4-wide nodes, object space
+  const float4* tris;    // 3 float4 per triangle in the BLAS's leaf order
+  uint32_t shadeBase;    // first shading record of the BLAS in the shared array
+  uint32_t ntri;
+};
 
 struct BuildArgs {
   const DevInstance* inst;

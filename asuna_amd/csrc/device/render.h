@@ -54,6 +54,13 @@ struct SceneView {
   const float2* marg;  // H entries (column 0 of the marginal table)
   const float2* cond;  // W*H entries
   const DevSky* sky;   // sun & sky (inUse 0: env map / background)
+  // two-level acceleration (twoLevel = 1): TLAS binary nodes over instances
+  // (leaf = one instance: its index in tlasInst), per-instance BLAS slot
+  uint32_t twoLevel;
+  const float4* tlas;
+  const uint32_t* tlasInst;
+  const DevBlas* blas;
+  const uint32_t* instBlas;
 };
 
 // Wavefront state for up to `capacity` paths (path numbering: WorkUnit).

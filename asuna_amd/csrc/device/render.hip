@@ -476,6 +476,200 @@ __device__ __forceinline__ int popClosest(Stack& st, int best, float bt) {
   return kDoneRef;
 }
 
+// ------------------------------------------------ two-level traversal
+// The TLAS is a binary tree over the instances' world boxes, walked with
+// the world ray.  At an instance leaf the ray enters the instance's BLAS in
+// object space (origin and direction through the world-to-object
+// transform; the direction is not normalised, so t is the same parameter
+// in both spaces) and the walk stays there until the stack falls back to
+// the depth it had on entry.  Culling (entry distance against the closest
+// hit) and the tie rule (the smaller global triangle id at equal t) are
+// those of the flat walks.
+
+__device__ __forceinline__ void toObject(const DevInstance& in, const TRay& rw, TRay& ro) {
+  const float* m = in.w2o;
+  const f3 o = rw.o, d = rw.d;
+  setupRay(ro,
+           f3{m[0] * o.x + m[1] * o.y + m[2] * o.z + m[3], m[4] * o.x + m[5] * o.y + m[6] * o.z + m[7],
+              m[8] * o.x + m[9] * o.y + m[10] * o.z + m[11]},
+           f3{m[0] * d.x + m[1] * d.y + m[2] * d.z, m[4] * d.x + m[5] * d.y + m[6] * d.z,
+              m[8] * d.x + m[9] * d.y + m[10] * d.z});
+}
+
+// A BLAS vertex in world space, rounded as the flat build rounds it
+// (((m0*x + m1*y) + m2*z) + m3 per row, no contraction).
+__device__ __forceinline__ float4 toWorld(const DevInstance& in, float4 p) {
+  const float* m = in.o2w;
+  return make_float4(m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3], m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7],
+                     m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11], 0.f);
+}
+
+// One TLAS node step: pushes the farther of two entered children; returns
+// the next node, or kDoneRef when neither child is entered.
+__device__ __forceinline__ int tlasStep(const SceneView& sc, const TRay& rw, int node, float tmin, float lim,
+                                        Stack& st) {
+  const float4* np = sc.tlas + size_t(node) * 4;
+  const float4 n0 = np[0], n1 = np[1], n2 = np[2], n3 = np[3];
+  float t0, t1;
+  const bool h0 = slab(n0.x, n0.y, n0.z, n0.w, n2.x, n2.y, rw, tmin, lim, t0);
+  const bool h1 = slab(n1.x, n1.y, n1.z, n1.w, n2.z, n2.w, rw, tmin, lim, t1);
+  int c0 = __float_as_int(n3.x), c1 = __float_as_int(n3.y);
+  if (h0 && h1) {
+    if (t1 < t0) {
+      const int c = c0;
+      c0 = c1, c1 = c;
+      const float t = t0;
+      t0 = t1, t1 = t;
+    }
+    st.push(c1, t1);
+    return c0;
+  }
+  return h0 ? c0 : (h1 ? c1 : kDoneRef);
+}
+
+// Closest hit; returns the BLAS leaf position of the hit triangle (instance
+// in bInst) or -1.
+template <bool COUNT>
+__device__ int traceClosest2L(const SceneView& sc, const TRay& rw, float tmin, float tmax, Stack& st, float& bt,
+                              float& bu, float& bv, int& bInst, uint32_t& nNodes, uint32_t& nTris) {
+  int best = -1, bestGid = 0x7fffffff;
+  bInst = -1;
+  bt = tmax;
+  st.sp = 0;
+  int node = 0, cur = -1, base = 0;
+  uint32_t triBase = 0;
+  const DevBlas* bl = nullptr;
+  TRay ro;
+  for (;;) {
+    if (node == kDoneRef) {
+      // falls through to the pop below
+    } else if (cur < 0) {
+      if (node >= 0) {
+        if (node != kEmptyRef) {
+          if (COUNT) ++nNodes;
+          node = tlasStep(sc, rw, node, tmin, best >= 0 ? bt : tmax, st);
+          if (node != kDoneRef) continue;
+        }
+      } else {
+        uint32_t first, cnt;
+        decodeLeaf(node, first, cnt);
+        const uint32_t i = sc.tlasInst[first];
+        const DevInstance& in = sc.inst[i];
+        toObject(in, rw, ro);
+        bl = &sc.blas[sc.instBlas[i]];
+        cur = int(i);
+        triBase = in.triBase;
+        base = st.sp;
+        node = 0;
+        continue;
+      }
+    } else if (node >= 0) {
+      if (node != kEmptyRef) {
+        if (COUNT) ++nNodes;
+        Kids k;
+        testNode4(bl->nodes4 + size_t(node) * 8, ro, tmin, best >= 0 ? bt : tmax, k);
+        if (k.n > 0) {
+          if (k.n > 3) st.push(k.c[3], k.t[3]);
+          if (k.n > 2) st.push(k.c[2], k.t[2]);
+          if (k.n > 1) st.push(k.c[1], k.t[1]);
+          node = k.c[0];
+          continue;
+        }
+      }
+    } else {
+      uint32_t first, cnt;
+      decodeLeaf(node, first, cnt);
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        const float4* tp = bl->tris + size_t(k) * 3;
+        const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+        if (COUNT) ++nTris;
+        float t, u, v;
+        if (hitTri(ro, p0, p1, p2, tmin, best >= 0 ? bt : tmax, t, u, v)) {
+          const int gid = int(triBase) + __float_as_int(p2.w);
+          if (best < 0 || t < bt || (t == bt && gid < bestGid)) {
+            bt = t, bu = u, bv = v;
+            best = int(k);
+            bestGid = gid;
+            bInst = cur;
+          }
+        }
+      }
+    }
+    bool more = false;
+    for (;;) {
+      if (cur >= 0 && st.sp == base) cur = -1;  // this instance's BLAS is done
+      if (st.sp == 0) break;
+      float tn;
+      const int c = st.pop(tn);
+      if (best < 0 || tn <= bt * 1.0000004f) {
+        node = c;
+        more = true;
+        break;
+      }
+    }
+    if (!more) break;
+  }
+  return best;
+}
+
+template <bool COUNT>
+__device__ bool traceAny2L(const SceneView& sc, const TRay& rw, float tmin, float tmax, Stack& st, uint32_t& nNodes,
+                           uint32_t& nTris) {
+  st.sp = 0;
+  int node = 0, cur = -1, base = 0;
+  const DevBlas* bl = nullptr;
+  TRay ro;
+  for (;;) {
+    if (node == kDoneRef) {
+    } else if (cur < 0) {
+      if (node >= 0) {
+        if (node != kEmptyRef) {
+          if (COUNT) ++nNodes;
+          node = tlasStep(sc, rw, node, tmin, tmax, st);
+          if (node != kDoneRef) continue;
+        }
+      } else {
+        uint32_t first, cnt;
+        decodeLeaf(node, first, cnt);
+        const uint32_t i = sc.tlasInst[first];
+        toObject(sc.inst[i], rw, ro);
+        bl = &sc.blas[sc.instBlas[i]];
+        cur = int(i);
+        base = st.sp;
+        node = 0;
+        continue;
+      }
+    } else if (node >= 0) {
+      if (node != kEmptyRef) {
+        if (COUNT) ++nNodes;
+        Kids k;
+        testNode4(bl->nodes4 + size_t(node) * 8, ro, tmin, tmax, k);
+        if (k.n > 0) {
+          if (k.n > 3) st.push(k.c[3], k.t[3]);
+          if (k.n > 2) st.push(k.c[2], k.t[2]);
+          if (k.n > 1) st.push(k.c[1], k.t[1]);
+          node = k.c[0];
+          continue;
+        }
+      }
+    } else {
+      uint32_t first, cnt;
+      decodeLeaf(node, first, cnt);
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        const float4* tp = bl->tris + size_t(k) * 3;
+        float t, u, v;
+        if (COUNT) ++nTris;
+        if (hitTri(ro, tp[0], tp[1], tp[2], tmin, tmax, t, u, v)) return true;
+      }
+    }
+    if (cur >= 0 && st.sp == base) cur = -1;
+    if (st.sp == 0) break;
+    float tn;
+    node = st.pop(tn);
+  }
+  return false;
+}
+
 // -------------------------------------------------------------- raygen
 // raytrace.projective.rgen:26-90 for one (shot, frame, pixel).
 __device__ __forceinline__ void cameraRay(const GpuCamera& cam, float px, float py, uint32_t& seed, f3& ro,
@@ -967,6 +1161,86 @@ __device__ __forceinline__ void shadowDynLoop(const SceneView& sc, const Wavefro
         w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
       }
     }
+  }
+}
+
+// Two-level closest-hit and shadow kernels (wave fetch in queue order).
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kExtend2L(SceneView sc, Wavefront w, int cur) {
+  __shared__ int lds[kStackLds * kTraceBlock];
+  __shared__ float ldsT[kStackLds * kTraceBlock];
+  Stack st;
+  initStack<true, kStackLds>(st, lds, ldsT, w);
+  const uint32_t count = w.ctr[cur];
+  const int* q = w.queue[cur];
+  const int lane = __lane_id();
+  uint32_t nNodes = 0, nTris = 0;
+  ChunkFetch f;
+  chunkInit<false>(f, &w.ctr[3], count);
+  uint32_t start;
+  while (chunkNext<false>(f, lane, start)) {
+    const uint32_t i = start + uint32_t(lane);
+    if (i < f.hi) {
+      const int pid = q[i];
+      TRay r;
+      setupRay(r, xyz(w.ro[pid]), xyz(w.rd[pid]));
+      float t = 0.f, u = 0.f, v = 0.f;
+      int inst = -1;
+      const int pos = traceClosest2L<COUNT>(sc, r, kMinimum, kInf, st, t, u, v, inst, nNodes, nTris);
+      if (pos >= 0) {
+        const DevBlas& bl = sc.blas[sc.instBlas[inst]];
+        const DevInstance& in = sc.inst[inst];
+        const float4* tp = bl.tris + size_t(pos) * 3;
+        const int prim = __float_as_int(tp[2].w);
+        // t and barycentrics of the hit triangle evaluated again in world
+        // space, on the vertices the flat build would store: the object-space
+        // test picks the triangle, this gives it the flat path's numbers
+        // (the object-space ray differs from the world ray by rounding, which
+        // moves barycentrics by about 1e-4 on small triangles)
+        float t2, u2, v2;
+        if (hitTri(r, toWorld(in, tp[0]), toWorld(in, tp[1]), toWorld(in, tp[2]), kMinimum, kInf, t2, u2, v2))
+          t = t2, u = u2, v = v2;
+        w.hitF[pid] = make_float4(t, u, v, 0.f);
+        w.hitI[pid] = make_int4(inst, int(in.shade), int(in.triBase) + prim, int(bl.shadeBase) + pos);
+      } else {
+        w.hitI[pid] = make_int4(-1, -1, -1, -1);
+      }
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&w.cnt64[0], (unsigned long long)nNodes);
+    atomicAdd(&w.cnt64[1], (unsigned long long)nTris);
+  }
+}
+
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kShadow2L(SceneView sc, Wavefront w) {
+  __shared__ int lds[kStackLds * kTraceBlock];
+  __shared__ float ldsT[kStackLds * kTraceBlock];
+  Stack st;
+  initStack<true, kStackLds>(st, lds, ldsT, w);
+  const uint32_t count = w.ctr[2];
+  const int lane = __lane_id();
+  uint32_t nNodes = 0, nTris = 0;
+  ChunkFetch f;
+  chunkInit<false>(f, &w.ctr[4], count);
+  uint32_t start;
+  while (chunkNext<false>(f, lane, start)) {
+    const uint32_t i = start + uint32_t(lane);
+    if (i < f.hi) {
+      const float4 so = w.shO[i], sd = w.shD[i];
+      TRay r;
+      setupRay(r, xyz(so), xyz(sd));
+      const int pid = __float_as_int(sd.w);
+      if (!traceAny2L<COUNT>(sc, r, 0.0f, so.w, st, nNodes, nTris)) {
+        const float4 L = w.rad[pid], d = w.ld[pid];
+        w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
+      }
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&w.cnt64[2], (unsigned long long)nNodes);
+    atomicAdd(&w.cnt64[3], (unsigned long long)nTris);
   }
 }
 
@@ -2800,6 +3074,11 @@ static void extendFor(const SceneView& sc, const Wavefront& w, int cur, bool sha
 }
 void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t scheme,
                   uint32_t grid, bool count, hipStream_t s) {
+  if (sc.twoLevel) {
+    if (count) kExtend2L<true><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kExtend2L<false><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    return;
+  }
   const bool shallow = ldsStack == uint32_t(kStackLdsMin);
   if (count) extendFor<true>(sc, w, cur, shallow, scheme, grid, s);
   else extendFor<false>(sc, w, cur, shallow, scheme, grid, s);
@@ -2832,6 +3111,11 @@ static void shadowFor(const SceneView& sc, const Wavefront& w, uint32_t scheme, 
 }
 void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t scheme, uint32_t grid, bool count,
                   hipStream_t s) {
+  if (sc.twoLevel) {
+    if (count) kShadow2L<true><<<grid, kTraceBlock, 0, s>>>(sc, w);
+    else kShadow2L<false><<<grid, kTraceBlock, 0, s>>>(sc, w);
+    return;
+  }
   if (count) shadowFor<true>(sc, w, scheme, grid, s);
   else shadowFor<false>(sc, w, scheme, grid, s);
 }
@@ -2844,7 +3128,7 @@ void launchResetCounters(const Wavefront& w, int cur, hipStream_t s) { kResetCou
 void launchResetShade(const Wavefront& w, int cur, hipStream_t s) { kResetCounters<false, true><<<1, 64, 0, s>>>(w, cur); }
 void launchResetExtend(const Wavefront& w, hipStream_t s) { kResetCounters<true, false><<<1, 64, 0, s>>>(w, 0); }
 bool canFuseTrace(const SceneView& sc, uint32_t scheme) {
-  return scheme == kSchemeXcdSlices && sc.nodes4 != nullptr && sc.nodesQ == nullptr;
+  return scheme == kSchemeXcdSlices && sc.nodes4 != nullptr && sc.nodesQ == nullptr && !sc.twoLevel;
 }
 void launchTrace(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t grid, bool count,
                  hipStream_t s) {

@@ -275,6 +275,10 @@ typedef struct ash_build_opts {
   uint32_t node_format;   /* 4-wide nodes the wave-fetch kernels walk: 0 = default (float),
                              1 = 128-byte float boxes, 2 = 64-byte quantized boxes (8-bit planes on a
                              per-node power-of-two grid, rounded outwards: the same closest hits) */
+  uint32_t two_level;     /* 1 = instancing without duplication: one object-space 4-wide BVH per distinct
+                             mesh (instances sharing vertex and index ranges share it) under a binary tree
+                             over the instances' world boxes (needs branching 4 and float nodes); the
+                             flat default is one world-space tree over every instance's triangles */
 } ash_build_opts;
 
 typedef struct ash_build_info {
@@ -288,6 +292,8 @@ typedef struct ash_build_info {
   uint64_t num_wide_nodes; /* 4-wide nodes (0 when branching = 2) */
   uint32_t wide_depth;     /* levels of the 4-wide tree */
   uint32_t branching;      /* tree the kernels traverse */
+  uint32_t blas_count;     /* two-level: object-space trees built (0 when flat) */
+  uint64_t blas_triangles; /* two-level: triangles stored in them (distinct meshes, not instances) */
 } ash_build_info;
 
 /* Render options beyond the push constants. */
