@@ -118,7 +118,7 @@ struct BatchDesc {
   ShotParams one;             // kernel-argument copy of the only shot (scalar loads)
 };
 
-constexpr int kStackLds = 16;    // per-thread traversal stack entries in LDS (default)
+constexpr int kStackLds = 12;    // per-thread traversal stack entries in LDS (default)
 constexpr int kStackLdsMin = 8;  // smallest LDS share (the spill covers the rest)
 constexpr int kSpill = 48;       // minimum further entries spilled to global memory
 constexpr int kTraceBlock = 256;
@@ -136,7 +136,7 @@ enum TraceScheme : uint32_t {
 
 // Host-side launchers (render.hip).
 void launchRaygen(const BatchDesc& b, const Wavefront& w, hipStream_t s);
-// ldsStack: traversal stack entries per thread kept in LDS (8 or 16) by the
+// ldsStack: traversal stack entries per thread kept in LDS (8 or 12) by the
 // 4-wide closest-hit kernel
 void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t scheme,
                   uint32_t grid, bool count, hipStream_t s);

@@ -216,7 +216,7 @@ def main():
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graphs", action="store_true")
     ap.add_argument("--blocks-per-cu", type=int, default=0, help="persistent traversal blocks per CU when not autotuned")
-    ap.add_argument("--lds-stack", type=int, choices=[0, 8, 16], default=0,
+    ap.add_argument("--lds-stack", type=int, choices=[0, 8, 12], default=0,
                     help="closest-hit stack entries in LDS when not autotuned")
     ap.add_argument("--traversal", type=int, choices=[0, 1, 2], default=0,
                     help="ray fetch of the traversal kernels when not autotuned: 0 per-XCD queue slices, "

@@ -306,7 +306,7 @@ typedef struct ash_render_opts {
   uint32_t max_bounce_iters; /* safety cap on wavefront iterations (0 = auto) */
   uint32_t count_traversal;  /* 1 = count node visits / triangle tests (slower) */
   uint32_t no_graphs;        /* 1 = launch the bounce loop eagerly instead of replaying a HIP graph */
-  uint32_t lds_stack;        /* closest-hit stack entries per thread held in LDS: 8 or 16 (0 = 16) */
+  uint32_t lds_stack;        /* closest-hit stack entries per thread held in LDS: 8 or 12 (0 = 12) */
   uint32_t traversal;        /* ray fetch of the traversal kernels: 0 = per-XCD slices of the queue
                                 (default; closest hit on float 4-wide nodes with dynamic per-lane fetch
                                 and speculative while-while rounds), 1 = wave fetch in queue order,
@@ -398,7 +398,7 @@ int ash_read_shot_output(ash_ctx* ctx, uint32_t shot, int32_t image, float* host
  * sweeps the wavefront batch size queue_paths over 2^18, 2^19, ... up to
  * the first size at which the probe fits in one batch (at most 2^26), the
  * persistent traversal grid blocks_per_cu over {4, 8, 16} and the LDS share
- * of the closest-hit stack over {8, 16} with the traversal fetch scheme of
+ * of the closest-hit stack over {8, 12} with the traversal fetch scheme of
  * `inout`, then tries the other fetch schemes at the best of those, and
  * writes the fastest setting into `inout`.  The probe renders overwrite the
  * shot outputs. */

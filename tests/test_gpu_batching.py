@@ -114,7 +114,7 @@ def test_traversal_fetch_schemes_render_identically(renderer, spheres_small, bra
     w, h = s.film
     imgs, stats = [], []
     for trav in (0, 1, 2):
-        for lds in (8, 16):
+        for lds in (8, 12):
             renderer.set_film(w, h)
             st = renderer.render(cam, pc, 0, 3, traversal=trav, lds_stack=lds, count_traversal=True,
                                  blocks_per_cu=4)
@@ -144,7 +144,7 @@ def test_autotune_picks_a_candidate_and_keeps_results(renderer, multishot_scene)
     top = next(k for k in range(18, 27) if (1 << k) >= w * h * 8 or k == 26)
     assert rep["candidates"] == 6 * (top - 17) + 2
     assert opts.queue_paths in [1 << k for k in range(18, top + 1)] and opts.blocks_per_cu in (4, 8, 16)
-    assert opts.lds_stack in (8, 16) and rep["lds_stack"] == opts.lds_stack
+    assert opts.lds_stack in (8, 12) and rep["lds_stack"] == opts.lds_stack
     assert opts.traversal in (0, 1, 2) and rep["traversal"] == opts.traversal
     assert 0 < rep["best_ms"] <= rep["worst_ms"]
     renderer.render_shots(list(cams), list(pcs), 2, queue_paths=opts.queue_paths, blocks_per_cu=opts.blocks_per_cu,
