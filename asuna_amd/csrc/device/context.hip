@@ -86,7 +86,8 @@ struct ash_ctx {
   std::vector<DBuf<float4>> texData;
   DBuf<DevImage> texTab;
   DBuf<float4> env;
-  DBuf<float2> marg, cond;
+  DBuf<float2> marg;
+  DBuf<float> condX;  // inverse-CDF channel of the conditional table (its pdf channel is the env texels' alpha)
   uint32_t envW = 1, envH = 1;
   bool hasEnv = false;
   DBuf<DevSky> sky;
@@ -147,7 +148,7 @@ struct ash_ctx {
     for (auto* b : {&verts, &env, &images, &shotImages, &wRo, &wRd, &wThr, &wRad, &wLd, &wHitF, &wShO, &wShD})
       b->release();
     inst.release(), idx.release(), instTriOffset.release(), mats.release(), lights.release(), texTab.release();
-    marg.release(), cond.release(), shotTab.release(), wInfo.release(), wHitI.release();
+    marg.release(), condX.release(), shotTab.release(), wInfo.release(), wHitI.release();
     wQ0.release(), wQ1.release(), wQ2.release(), wSpill.release(), wSpillT.release(), wCtr.release(), wCnt64.release();
     unitBuf.release(), sky.release();
     for (auto& g : graphs) releaseGraph(g);
@@ -225,7 +226,7 @@ static SceneView viewOf(const ash_ctx* c) {
   v.env.w = c->envW;
   v.env.h = c->envH;
   v.marg = c->marg.p;
-  v.cond = c->cond.p;
+  v.condX = c->condX.p;
   v.sky = c->sky.p;
   v.twoLevel = c->twoLevel ? 1u : 0u;
   v.tlas = c->tlas.p;
@@ -1074,22 +1075,29 @@ int ash_upload_scene(ash_ctx* c, const ash_scene_desc* d) {
     if (!c->envW || !c->envH) return fail(c, ASH_E_INVALID, "env map has zero size");
     ash::buildEnvTables(c->envW, c->envH, d->envmap.rgba, marg, cond);
     size_t npx = size_t(c->envW) * c->envH;
+    std::vector<float4> px(npx);
+    std::vector<float> cx(npx);
+    for (size_t k = 0; k < npx; ++k) {
+      px[k] = make_float4(d->envmap.rgba[4 * k], d->envmap.rgba[4 * k + 1], d->envmap.rgba[4 * k + 2], cond[2 * k + 1]);
+      cx[k] = cond[2 * k];
+    }
     MALLOC(c, c->env, npx);
     MALLOC(c, c->marg, c->envH);
-    MALLOC(c, c->cond, npx);
-    HIPC(c, hipMemcpyAsync(c->env.p, d->envmap.rgba, npx * sizeof(float4), hipMemcpyHostToDevice, s));
-    HIPC(c, hipMemcpyAsync(c->marg.p, marg.data(), marg.size() * 4, hipMemcpyHostToDevice, s));
-    HIPC(c, hipMemcpyAsync(c->cond.p, cond.data(), cond.size() * 4, hipMemcpyHostToDevice, s));
+    MALLOC(c, c->condX, npx);
+    HIPC(c, hipMemcpy(c->env.p, px.data(), npx * sizeof(float4), hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->marg.p, marg.data(), marg.size() * 4, hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->condX.p, cx.data(), npx * 4, hipMemcpyHostToDevice));
   } else {
     c->envW = c->envH = 1;
     float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
     float2 z2 = make_float2(0.f, 0.f);
+    float z1 = 0.f;
     MALLOC(c, c->env, 1);
     MALLOC(c, c->marg, 1);
-    MALLOC(c, c->cond, 1);
-    HIPC(c, hipMemcpyAsync(c->env.p, &z, sizeof z, hipMemcpyHostToDevice, s));
-    HIPC(c, hipMemcpyAsync(c->marg.p, &z2, sizeof z2, hipMemcpyHostToDevice, s));
-    HIPC(c, hipMemcpyAsync(c->cond.p, &z2, sizeof z2, hipMemcpyHostToDevice, s));
+    MALLOC(c, c->condX, 1);
+    HIPC(c, hipMemcpy(c->env.p, &z, sizeof z, hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->marg.p, &z2, sizeof z2, hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->condX.p, &z1, sizeof z1, hipMemcpyHostToDevice));
   }
   {  // every scene starts from the default sun & sky (in_use 0)
     GpuSunAndSky ss;
@@ -1459,7 +1467,13 @@ int ash_env_tables(ash_ctx* c, float* m, float* cd) {
   if (!c->hasEnv) return fail(c, ASH_E_STATE, "no env map");
   HIPC(c, hipSetDevice(c->device));
   HIPC(c, hipMemcpy(m, c->marg.p, size_t(c->envH) * sizeof(float2), hipMemcpyDeviceToHost));
-  HIPC(c, hipMemcpy(cd, c->cond.p, size_t(c->envW) * c->envH * sizeof(float2), hipMemcpyDeviceToHost));
+  // the conditional table as (inverse CDF, pdf) pairs, from its two device planes
+  const size_t npx = size_t(c->envW) * c->envH;
+  std::vector<float4> px(npx);
+  std::vector<float> cx(npx);
+  HIPC(c, hipMemcpy(px.data(), c->env.p, npx * sizeof(float4), hipMemcpyDeviceToHost));
+  HIPC(c, hipMemcpy(cx.data(), c->condX.p, npx * sizeof(float), hipMemcpyDeviceToHost));
+  for (size_t k = 0; k < npx; ++k) cd[2 * k] = cx[k], cd[2 * k + 1] = px[k].w;
   return ASH_OK;
 }
 

@@ -224,7 +224,13 @@ __device__ __forceinline__ float2 texMarginal(const float2* m, uint32_t W, uint3
   return make_float2(w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x,
                      w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y);
 }
-__device__ __forceinline__ float2 texCond(const float2* c, uint32_t W, uint32_t H, float u, float v) {
+// The conditional table's two channels are stored apart: its inverse-CDF
+// channel as a float plane, its pdf channel in the env texels' alpha, so
+// that the pdf and the radiance of one direction come from the same four
+// texel loads (the two textures have the same size, hence the same
+// footprint and weights).  Each channel is filtered exactly as a
+// two-channel lookup filters it.
+__device__ __forceinline__ float texCondX(const float* c, uint32_t W, uint32_t H, float u, float v) {
   if (!(u == u)) u = 0.f;
   if (!(v == v)) v = 0.f;
   float x = u * float(W) - 0.5f, y = v * float(H) - 0.5f;
@@ -233,11 +239,10 @@ __device__ __forceinline__ float2 texCond(const float2* c, uint32_t W, uint32_t 
   int ix = int(fx), iy = int(fy);
   int x0 = wrapi(ix, int(W)), y0 = wrapi(iy, int(H));
   int x1 = wrapi(ix + 1, int(W)), y1 = wrapi(iy + 1, int(H));
-  float2 t00 = c[size_t(y0) * W + x0], t10 = c[size_t(y0) * W + x1];
-  float2 t01 = c[size_t(y1) * W + x0], t11 = c[size_t(y1) * W + x1];
+  float t00 = c[size_t(y0) * W + x0], t10 = c[size_t(y0) * W + x1];
+  float t01 = c[size_t(y1) * W + x0], t11 = c[size_t(y1) * W + x1];
   float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
-  return make_float2(w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x,
-                     w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y);
+  return w00 * t00 + w10 * t10 + w01 * t01 + w11 * t11;
 }
 
 // ---------------------------------------------------------- microfacet

@@ -50,9 +50,9 @@ struct SceneView {
   const DevImage* textures;
   uint32_t numTextures;
   uint32_t hasEnv;
-  DevImage env;
+  DevImage env;        // W*H texels: (r, g, b, conditional pdf); the image's alpha is unused
   const float2* marg;  // H entries (column 0 of the marginal table)
-  const float2* cond;  // W*H entries
+  const float* condX;  // W*H entries: the conditional table's inverse-CDF channel
   const DevSky* sky;   // sun & sky (inUse 0: env map / background)
   // two-level acceleration (twoLevel = 1): TLAS binary nodes over instances
   // (leaf = one instance: its index in tlasInst), per-instance BLAS slot

@@ -1568,15 +1568,15 @@ __device__ __forceinline__ f3 sampleEnv(const SceneView& sc, float r1, float r2,
                                         float resy, float intensity, f3& L, float& pdf) {
   uint32_t W = sc.env.w, H = sc.env.h;
   float uvy = texMarginal(sc.marg, W, H, r1).x;
-  float uvx = texCond(sc.cond, W, H, r2, uvy).x;
-  pdf = texCond(sc.cond, W, H, uvx, uvy).y * texMarginal(sc.marg, W, H, uvy).y;
+  float uvx = texCondX(sc.condX, W, H, r2, uvy);
+  const float4 e = tex2D(sc.env, uvx, uvy);  // radiance and conditional pdf of the direction
+  pdf = e.w * texMarginal(sc.marg, W, H, uvy).y;
   float phi = uvx * kTwoPi;
   float theta = uvy * kPi;
   if (sinf(theta) == 0.0f) pdf = 0.0f;
   pdf = (pdf * resx * resy) / (kTwoPi * kPi * sinf(theta));
   L = f3{-sinf(theta) * cosf(phi), cosf(theta), -sinf(theta) * sinf(phi)};
   L = transformDirection(envT, L);
-  float4 e = tex2D(sc.env, uvx, uvy);
   return intensity * xyz(e);
 }
 __device__ __forceinline__ void envUV(const float* envT, f3 L, float& u, float& v, float& theta) {
@@ -1585,20 +1585,17 @@ __device__ __forceinline__ void envUV(const float* envT, f3 L, float& u, float& 
   u = (kPi + atan2f(L.z, L.x)) * kInv2Pi;
   v = theta * kInvPi;
 }
-// pdfEnvmap, sample_light.glsl:84-95
-__device__ __forceinline__ float pdfEnv(const SceneView& sc, const float* envT, float resx, float resy, f3 L) {
+// evalEnvmap, sample_light.glsl:97-104, and pdfEnvmap, sample_light.glsl:84-95,
+// of one direction from one texel lookup (the miss shader needs both).
+__device__ __forceinline__ f3 evalEnvPdf(const SceneView& sc, const float* envT, float intensity, float resx,
+                                         float resy, f3 L, float& pdfOut) {
   float u, v, theta;
   envUV(envT, L, u, v, theta);
-  float pdf = texCond(sc.cond, sc.env.w, sc.env.h, u, v).y * texMarginal(sc.marg, sc.env.w, sc.env.h, v).y;
+  const float4 e = tex2D(sc.env, u, v);
+  float pdf = e.w * texMarginal(sc.marg, sc.env.w, sc.env.h, v).y;
   float s = sinf(theta);
-  if (s == 0.0f) return 0.0f;
-  return (pdf * resx * resy) / (kTwoPi * kPi * s);
-}
-// evalEnvmap, sample_light.glsl:97-104
-__device__ __forceinline__ f3 evalEnv(const SceneView& sc, const float* envT, float intensity, f3 L) {
-  float u, v, theta;
-  envUV(envT, L, u, v, theta);
-  return intensity * xyz(tex2D(sc.env, u, v));
+  pdfOut = s == 0.0f ? 0.0f : (pdf * resx * resy) / (kTwoPi * kPi * s);
+  return intensity * xyz(e);
 }
 
 // Sun & sky, utils/sun_and_sky.glsl:20-533 (NVIDIA's MDL sun & sky:
@@ -2779,16 +2776,14 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
       // raytrace.default.rmiss:24-55
       c.stop = true;
       const bool sky = sc.sky->inUse == 1;
-      f3 env = sky ? skyEval(*sc.sky, c.rd)
-                   : (sc.hasEnv && pc.hasEnvMap == 1 ? evalEnv(sc, sp.cam.envTransform, pc.envMapIntensity, c.rd)
-                                                     : ld3(pc.bgColor));
+      const bool envMap = !sky && sc.hasEnv && pc.hasEnvMap == 1;
+      float envPdf = kInv4Pi;
+      f3 env = sky      ? skyEval(*sc.sky, c.rd)
+               : envMap ? evalEnvPdf(sc, sp.cam.envTransform, pc.envMapIntensity, pc.envMapResolution[0],
+                                     pc.envMapResolution[1], c.rd, envPdf)
+                        : ld3(pc.bgColor);
       float mis = 1.0f;
-      if (c.depth != 1 && isNonSpecular(c.bflags)) {
-        float envPdf = (!sky && sc.hasEnv && pc.hasEnvMap == 1)
-                           ? pdfEnv(sc, sp.cam.envTransform, pc.envMapResolution[0], pc.envMapResolution[1], c.rd)
-                           : kInv4Pi;
-        mis = powerHeuristic(c.bpdf, envPdf);
-      }
+      if (c.depth != 1 && isNonSpecular(c.bflags)) mis = powerHeuristic(c.bpdf, envPdf);
       c.L = c.L + c.T * env * mis;
     } else {
       float4 hf = w.hitF[pid];
