@@ -1074,19 +1074,22 @@ int ash_upload_scene(ash_ctx* c, const ash_scene_desc* d) {
     c->envH = d->envmap.height;
     if (!c->envW || !c->envH) return fail(c, ASH_E_INVALID, "env map has zero size");
     ash::buildEnvTables(c->envW, c->envH, d->envmap.rgba, marg, cond);
-    size_t npx = size_t(c->envW) * c->envH;
-    std::vector<float4> px(npx);
-    std::vector<float> cx(npx);
-    for (size_t k = 0; k < npx; ++k) {
-      px[k] = make_float4(d->envmap.rgba[4 * k], d->envmap.rgba[4 * k + 1], d->envmap.rgba[4 * k + 2], cond[2 * k + 1]);
-      cx[k] = cond[2 * k];
-    }
-    MALLOC(c, c->env, npx);
-    MALLOC(c, c->marg, c->envH);
-    MALLOC(c, c->condX, npx);
-    HIPC(c, hipMemcpy(c->env.p, px.data(), npx * sizeof(float4), hipMemcpyHostToDevice));
+    const uint32_t W = c->envW, H = c->envH;
+    std::vector<float4> px(envTexels(W, H), make_float4(0.f, 0.f, 0.f, 0.f));
+    std::vector<float> cx(condTexels(W, H), 0.f);
+    for (uint32_t y = 0; y < H; ++y)
+      for (uint32_t x = 0; x < W; ++x) {
+        const size_t k = size_t(y) * W + x;
+        px[envTexel(W, x, y)] =
+            make_float4(d->envmap.rgba[4 * k], d->envmap.rgba[4 * k + 1], d->envmap.rgba[4 * k + 2], cond[2 * k + 1]);
+        cx[condTexel(W, x, y)] = cond[2 * k];
+      }
+    MALLOC(c, c->env, px.size());
+    MALLOC(c, c->marg, H);
+    MALLOC(c, c->condX, cx.size());
+    HIPC(c, hipMemcpy(c->env.p, px.data(), px.size() * sizeof(float4), hipMemcpyHostToDevice));
     HIPC(c, hipMemcpy(c->marg.p, marg.data(), marg.size() * 4, hipMemcpyHostToDevice));
-    HIPC(c, hipMemcpy(c->condX.p, cx.data(), npx * 4, hipMemcpyHostToDevice));
+    HIPC(c, hipMemcpy(c->condX.p, cx.data(), cx.size() * 4, hipMemcpyHostToDevice));
   } else {
     c->envW = c->envH = 1;
     float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -1468,12 +1471,17 @@ int ash_env_tables(ash_ctx* c, float* m, float* cd) {
   HIPC(c, hipSetDevice(c->device));
   HIPC(c, hipMemcpy(m, c->marg.p, size_t(c->envH) * sizeof(float2), hipMemcpyDeviceToHost));
   // the conditional table as (inverse CDF, pdf) pairs, from its two device planes
-  const size_t npx = size_t(c->envW) * c->envH;
-  std::vector<float4> px(npx);
-  std::vector<float> cx(npx);
-  HIPC(c, hipMemcpy(px.data(), c->env.p, npx * sizeof(float4), hipMemcpyDeviceToHost));
-  HIPC(c, hipMemcpy(cx.data(), c->condX.p, npx * sizeof(float), hipMemcpyDeviceToHost));
-  for (size_t k = 0; k < npx; ++k) cd[2 * k] = cx[k], cd[2 * k + 1] = px[k].w;
+  const uint32_t W = c->envW, H = c->envH;
+  std::vector<float4> px(c->env.n);
+  std::vector<float> cx(c->condX.n);
+  HIPC(c, hipMemcpy(px.data(), c->env.p, px.size() * sizeof(float4), hipMemcpyDeviceToHost));
+  HIPC(c, hipMemcpy(cx.data(), c->condX.p, cx.size() * sizeof(float), hipMemcpyDeviceToHost));
+  for (uint32_t y = 0; y < H; ++y)
+    for (uint32_t x = 0; x < W; ++x) {
+      const size_t k = size_t(y) * W + x;
+      cd[2 * k] = cx[condTexel(W, x, y)];
+      cd[2 * k + 1] = px[envTexel(W, x, y)].w;
+    }
   return ASH_OK;
 }
 

@@ -224,12 +224,26 @@ __device__ __forceinline__ float2 texMarginal(const float2* m, uint32_t W, uint3
   return make_float2(w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x,
                      w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y);
 }
-// The conditional table's two channels are stored apart: its inverse-CDF
-// channel as a float plane, its pdf channel in the env texels' alpha, so
-// that the pdf and the radiance of one direction come from the same four
-// texel loads (the two textures have the same size, hence the same
-// footprint and weights).  Each channel is filtered exactly as a
-// two-channel lookup filters it.
+// The env map's importance tables.  The conditional table's two channels
+// are stored apart: its inverse-CDF channel as a float plane, its pdf
+// channel in the env texels' alpha, so that the pdf and the radiance of one
+// direction come from the same four texel loads (the two textures have the
+// same size, hence the same footprint and weights).  Both planes are laid
+// out in tiles of one 128-byte line (env: 4x2 texels of 16 B; inverse CDF:
+// 8x4 texels of 4 B), so that a bilinear footprint touches about 1.9 and
+// 1.4 lines instead of 2.25 and 2.06 in row-major order.  Each channel is
+// filtered exactly as a two-channel row-major lookup filters it.
+__host__ __device__ inline uint32_t envTilesX(uint32_t W) { return (W + 3) / 4; }
+__host__ __device__ inline uint32_t condTilesX(uint32_t W) { return (W + 7) / 8; }
+__host__ __device__ inline size_t envTexel(uint32_t W, uint32_t x, uint32_t y) {
+  return ((size_t(y >> 1) * envTilesX(W) + (x >> 2)) << 3) | ((y & 1u) << 2) | (x & 3u);
+}
+__host__ __device__ inline size_t condTexel(uint32_t W, uint32_t x, uint32_t y) {
+  return ((size_t(y >> 2) * condTilesX(W) + (x >> 3)) << 5) | ((y & 3u) << 3) | (x & 7u);
+}
+__host__ __device__ inline size_t envTexels(uint32_t W, uint32_t H) { return size_t(envTilesX(W)) * 8 * ((H + 1) / 2); }
+__host__ __device__ inline size_t condTexels(uint32_t W, uint32_t H) { return size_t(condTilesX(W)) * 32 * ((H + 3) / 4); }
+
 __device__ __forceinline__ float texCondX(const float* c, uint32_t W, uint32_t H, float u, float v) {
   if (!(u == u)) u = 0.f;
   if (!(v == v)) v = 0.f;
@@ -237,12 +251,30 @@ __device__ __forceinline__ float texCondX(const float* c, uint32_t W, uint32_t H
   float fx = floorf(x), fy = floorf(y);
   float a = x - fx, b = y - fy;
   int ix = int(fx), iy = int(fy);
-  int x0 = wrapi(ix, int(W)), y0 = wrapi(iy, int(H));
-  int x1 = wrapi(ix + 1, int(W)), y1 = wrapi(iy + 1, int(H));
-  float t00 = c[size_t(y0) * W + x0], t10 = c[size_t(y0) * W + x1];
-  float t01 = c[size_t(y1) * W + x0], t11 = c[size_t(y1) * W + x1];
+  uint32_t x0 = wrapi(ix, int(W)), y0 = wrapi(iy, int(H));
+  uint32_t x1 = wrapi(ix + 1, int(W)), y1 = wrapi(iy + 1, int(H));
+  float t00 = c[condTexel(W, x0, y0)], t10 = c[condTexel(W, x1, y0)];
+  float t01 = c[condTexel(W, x0, y1)], t11 = c[condTexel(W, x1, y1)];
   float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
   return w00 * t00 + w10 * t10 + w01 * t01 + w11 * t11;
+}
+// tex2D on the tiled env plane
+__device__ __forceinline__ float4 texEnv(const DevImage& im, float u, float v) {
+  if (!(u == u)) u = 0.f;
+  if (!(v == v)) v = 0.f;
+  float x = u * float(im.w) - 0.5f, y = v * float(im.h) - 0.5f;
+  float fx = floorf(x), fy = floorf(y);
+  float a = x - fx, b = y - fy;
+  int ix = int(fx), iy = int(fy);
+  uint32_t x0 = wrapi(ix, int(im.w)), y0 = wrapi(iy, int(im.h));
+  uint32_t x1 = wrapi(ix + 1, int(im.w)), y1 = wrapi(iy + 1, int(im.h));
+  float4 t00 = im.px[envTexel(im.w, x0, y0)], t10 = im.px[envTexel(im.w, x1, y0)];
+  float4 t01 = im.px[envTexel(im.w, x0, y1)], t11 = im.px[envTexel(im.w, x1, y1)];
+  float w00 = (1.0f - a) * (1.0f - b), w10 = a * (1.0f - b), w01 = (1.0f - a) * b, w11 = a * b;
+  return make_float4(w00 * t00.x + w10 * t10.x + w01 * t01.x + w11 * t11.x,
+                     w00 * t00.y + w10 * t10.y + w01 * t01.y + w11 * t11.y,
+                     w00 * t00.z + w10 * t10.z + w01 * t01.z + w11 * t11.z,
+                     w00 * t00.w + w10 * t10.w + w01 * t01.w + w11 * t11.w);
 }
 
 // ---------------------------------------------------------- microfacet

@@ -50,9 +50,9 @@ struct SceneView {
   const DevImage* textures;
   uint32_t numTextures;
   uint32_t hasEnv;
-  DevImage env;        // W*H texels: (r, g, b, conditional pdf); the image's alpha is unused
+  DevImage env;        // W*H texels: (r, g, b, conditional pdf) in 4x2-texel tiles; the image's alpha is unused
   const float2* marg;  // H entries (column 0 of the marginal table)
-  const float* condX;  // W*H entries: the conditional table's inverse-CDF channel
+  const float* condX;  // W*H entries: the conditional table's inverse-CDF channel, in 8x4-texel tiles
   const DevSky* sky;   // sun & sky (inUse 0: env map / background)
   // two-level acceleration (twoLevel = 1): TLAS binary nodes over instances
   // (leaf = one instance: its index in tlasInst), per-instance BLAS slot

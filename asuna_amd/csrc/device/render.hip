@@ -1569,7 +1569,7 @@ __device__ __forceinline__ f3 sampleEnv(const SceneView& sc, float r1, float r2,
   uint32_t W = sc.env.w, H = sc.env.h;
   float uvy = texMarginal(sc.marg, W, H, r1).x;
   float uvx = texCondX(sc.condX, W, H, r2, uvy);
-  const float4 e = tex2D(sc.env, uvx, uvy);  // radiance and conditional pdf of the direction
+  const float4 e = texEnv(sc.env, uvx, uvy);  // radiance and conditional pdf of the direction
   pdf = e.w * texMarginal(sc.marg, W, H, uvy).y;
   float phi = uvx * kTwoPi;
   float theta = uvy * kPi;
@@ -1591,7 +1591,7 @@ __device__ __forceinline__ f3 evalEnvPdf(const SceneView& sc, const float* envT,
                                          float resy, f3 L, float& pdfOut) {
   float u, v, theta;
   envUV(envT, L, u, v, theta);
-  const float4 e = tex2D(sc.env, u, v);
+  const float4 e = texEnv(sc.env, u, v);
   float pdf = e.w * texMarginal(sc.marg, sc.env.w, sc.env.h, v).y;
   float s = sinf(theta);
   pdfOut = s == 0.0f ? 0.0f : (pdf * resx * resy) / (kTwoPi * kPi * s);
@@ -1868,6 +1868,13 @@ struct PathCtx {
   f3 Ld, shO, shD;
   float shDist;
   bool fromDielectric;
+  // An env-map light sample taken ahead of the hit state, so that its
+  // texel loads overlap those of the shading record: valid when the light
+  // selection draw is reached with seed preSeed (checked where it is used).
+  bool preEnv;
+  uint32_t preSeed;
+  f3 preL, preRad;
+  float prePdf;
 };
 
 // sampleLights, rchit_layouts.glsl:118-167
@@ -1885,6 +1892,7 @@ __device__ f3 sampleLights(PathCtx& c, f3 pos, f3 nrm, bool& visible, LSample& l
   l.d = l.n = mk1(0.f);
   l.dist = l.pdf = 0.f;
   l.flags = 0;
+  const uint32_t selSeed = c.seed;
   float sel = rnd(c.seed);
   if (sel < envSel) {
     l.flags = kLightArea;
@@ -1897,6 +1905,10 @@ __device__ f3 sampleLights(PathCtx& c, f3 pos, f3 nrm, bool& visible, LSample& l
       Ldir = uniformSphere(r0, r1);
       pdf = kInv4Pi;
       rad = skyEval(*c.sc->sky, Ldir);
+    } else if (c.preEnv && c.preSeed == selSeed) {  // the same draws: the sample taken ahead
+      rad = c.preRad;
+      Ldir = c.preL;
+      pdf = c.prePdf;
     } else {
       rad = sampleEnv(*c.sc, r0, r1, c.sp->cam.envTransform, pc.envMapResolution[0], pc.envMapResolution[1],
                       pc.envMapIntensity, Ldir, pdf);
@@ -2768,6 +2780,7 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
     c.stop = false;
     c.shadow = false;
     c.fromDielectric = false;
+    c.preEnv = false;
     c.Ld = mk1(0.f);
     int4 hi = w.hitI[pid];
     uint32_t pix = info.z;
@@ -2786,6 +2799,25 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
       if (c.depth != 1 && isNonSpecular(c.bflags)) mis = powerHeuristic(c.bpdf, envPdf);
       c.L = c.L + c.T * env * mis;
     } else {
+      // Surfaces draw their light sample after a fixed number of draws: one
+      // (the opacity draw) for pbr, disney and kang18, none for the others.
+      // With an env map, the env branch of that sample depends on nothing
+      // but those draws, so it is taken here; sampleLights uses it only if
+      // its selection draw starts from the same seed.
+      const uint32_t sw = uint32_t(hi.y);
+      if (!(sw & kShadeLight) && sc.hasEnv && pc.hasEnvMap == 1 && sc.sky->inUse != 1) {
+        const uint32_t t = (sw >> kShadeTypeShift) & kShadeTypeMask;
+        uint32_t s = c.seed;
+        if (t == MaterialTypeBrdfPbrMetalnessRoughness || t == MaterialTypeBrdfDisney || t == MaterialTypeBrdfKang18)
+          (void)rnd(s);
+        c.preSeed = s;
+        if (rnd(s) < (pc.numLights > 0 ? 0.5f : 1.0f)) {
+          const float r0 = rnd(s), r1 = rnd(s);
+          c.preRad = sampleEnv(sc, r0, r1, sp.cam.envTransform, pc.envMapResolution[0], pc.envMapResolution[1],
+                               pc.envMapIntensity, c.preL, c.prePdf);
+          c.preEnv = true;
+        }
+      }
       float4 hf = w.hitF[pid];
       Hit h;
       hitState(sc, hi.x, uint32_t(hi.y), hi.w, hf.y, hf.z, c.rd, pc.useFaceNormal, h);
