@@ -460,7 +460,7 @@ __device__ bool traceAny4(const SceneView& sc, const TRay& r, float tmin, float 
 // Pop for the closest-hit walks: the next stacked entry whose entry
 // distance is not beyond the current hit (kDoneRef when none is left).
 constexpr int32_t kDoneRef = -0x7fffffff - 1;  // walk finished (leaf refs stay above: first < 2^28)
-constexpr int kShadowRefill = 32;  // idle lanes of a wave that trigger a shadow-ray refill
+constexpr int kShadowRefill = 48;  // idle lanes of a wave that trigger a shadow-ray refill
 // A round's node descent ends once fewer than kLeafBreak of the wave's lanes
 // still descending hold no postponed leaf: the last few would otherwise keep
 // the whole wave stepping nodes for them (8 to 12 measured best on the 1080p
@@ -3079,7 +3079,7 @@ static void extendWave(const SceneView& sc, const Wavefront& w, int cur, bool sh
 // 4-wide nodes with dynamic fetch (kExtendDyn), quantized nodes with wave
 // fetch; queue order uses wave fetch; the per-lane refill scheme walks the
 // 128-byte nodes (kept beside the quantized ones).
-constexpr int kDynRefill = 32;
+constexpr int kDynRefill = 48;
 template <bool COUNT>
 static void extendFor(const SceneView& sc, const Wavefront& w, int cur, bool shallow, uint32_t scheme, uint32_t grid,
                       hipStream_t s) {
