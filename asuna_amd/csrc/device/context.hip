@@ -109,10 +109,10 @@ struct ash_ctx {
   uint32_t shotCount = 0, shotW = 0, shotH = 0;
   // wavefront
   Wavefront wf{};
-  DBuf<float4> wRo, wRd, wThr, wRad, wLd, wHitF, wShO, wShD;
+  DBuf<float4> wRayO0, wRayO1, wRayD0, wRayD1, wThr, wRad, wLd, wHitF, wShO, wShD;
   DBuf<uint4> wInfo;
   DBuf<int4> wHitI;
-  DBuf<int> wQ0, wQ1, wQ2, wSpill;
+  DBuf<int> wSide, wSpill;
   DBuf<float> wSpillT;
   DBuf<uint32_t> wCtr;
   DBuf<unsigned long long> wCnt64;
@@ -145,11 +145,12 @@ struct ash_ctx {
     if (tris) (void)hipFree(tris);
     if (shadeTris) (void)hipFree(shadeTris);
     for (auto& t : texData) t.release();
-    for (auto* b : {&verts, &env, &images, &shotImages, &wRo, &wRd, &wThr, &wRad, &wLd, &wHitF, &wShO, &wShD})
+    for (auto* b : {&verts, &env, &images, &shotImages, &wRayO0, &wRayO1, &wRayD0, &wRayD1, &wThr, &wRad, &wLd, &wHitF,
+                    &wShO, &wShD})
       b->release();
     inst.release(), idx.release(), instTriOffset.release(), mats.release(), lights.release(), texTab.release();
     marg.release(), condX.release(), shotTab.release(), wInfo.release(), wHitI.release();
-    wQ0.release(), wQ1.release(), wQ2.release(), wSpill.release(), wSpillT.release(), wCtr.release(), wCnt64.release();
+    wSide.release(), wSpill.release(), wSpillT.release(), wCtr.release(), wCnt64.release();
     unitBuf.release(), sky.release();
     for (auto& g : graphs) releaseGraph(g);
     if (stream) (void)hipStreamDestroy(stream);
@@ -237,9 +238,11 @@ static SceneView viewOf(const ash_ctx* c) {
 }
 
 static int ensureWavefront(ash_ctx* c, uint32_t cap) {
-  if (c->wf.capacity >= cap && c->wRo.p) return ASH_OK;
-  MALLOC(c, c->wRo, cap);
-  MALLOC(c, c->wRd, cap);
+  if (c->wf.capacity >= cap && c->wRayO0.p) return ASH_OK;
+  MALLOC(c, c->wRayO0, cap);
+  MALLOC(c, c->wRayO1, cap);
+  MALLOC(c, c->wRayD0, cap);
+  MALLOC(c, c->wRayD1, cap);
   MALLOC(c, c->wThr, cap);
   MALLOC(c, c->wRad, cap);
   MALLOC(c, c->wLd, cap);
@@ -248,24 +251,22 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   MALLOC(c, c->wShD, cap);
   MALLOC(c, c->wInfo, cap);
   MALLOC(c, c->wHitI, cap);
-  MALLOC(c, c->wQ0, cap);
-  MALLOC(c, c->wQ1, cap);
-  MALLOC(c, c->wQ2, cap);
+  MALLOC(c, c->wSide, cap);
   MALLOC(c, c->wCtr, kNumCtr);
   MALLOC(c, c->wCnt64, kNumCnt64);
   Wavefront& w = c->wf;
   w.capacity = cap;
-  w.ro = c->wRo.p;
-  w.rd = c->wRd.p;
+  w.rayO[0] = c->wRayO0.p;
+  w.rayO[1] = c->wRayO1.p;
+  w.rayD[0] = c->wRayD0.p;
+  w.rayD[1] = c->wRayD1.p;
   w.thr = c->wThr.p;
   w.rad = c->wRad.p;
   w.ld = c->wLd.p;
   w.info = c->wInfo.p;
   w.hitF = c->wHitF.p;
   w.hitI = c->wHitI.p;
-  w.queue[0] = c->wQ0.p;
-  w.queue[1] = c->wQ1.p;
-  w.queue[2] = c->wQ2.p;
+  w.side = c->wSide.p;
   w.shO = c->wShO.p;
   w.shD = c->wShD.p;
   w.ctr = c->wCtr.p;
@@ -823,7 +824,7 @@ static int buildTwoLevel(ash_ctx* c, const BuildArgs& tmpl, ash_build_info* info
 // array's name to a kernel: it names a host-side stand-in.  The device
 // address comes from hipGetSymbolAddress, once per device with that device
 // current (ash_create), and only that address reaches kernels.
-constexpr size_t kNoAllocScratchBytes = size_t(96) << 20;
+constexpr size_t kNoAllocScratchBytes = size_t(128) << 20;
 constexpr uint32_t kNoAllocPaths = 1u << 18;  // paths in flight per batch
 constexpr uint32_t kNoAllocBlocksPerCU = 2;   // persistent traversal blocks per CU
 static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
@@ -876,8 +877,10 @@ static size_t noAllocLayout(unsigned char* base, uint32_t threads, uint32_t cap,
   const size_t n = kNoAllocPaths;
   std::memset(&w, 0, sizeof w);
   w.capacity = kNoAllocPaths;
-  w.ro = reinterpret_cast<float4*>(take(n * sizeof(float4)));
-  w.rd = reinterpret_cast<float4*>(take(n * sizeof(float4)));
+  for (int q = 0; q < 2; ++q) {
+    w.rayO[q] = reinterpret_cast<float4*>(take(n * sizeof(float4)));
+    w.rayD[q] = reinterpret_cast<float4*>(take(n * sizeof(float4)));
+  }
   w.thr = reinterpret_cast<float4*>(take(n * sizeof(float4)));
   w.rad = reinterpret_cast<float4*>(take(n * sizeof(float4)));
   w.ld = reinterpret_cast<float4*>(take(n * sizeof(float4)));
@@ -886,7 +889,7 @@ static size_t noAllocLayout(unsigned char* base, uint32_t threads, uint32_t cap,
   w.shD = reinterpret_cast<float4*>(take(n * sizeof(float4)));
   w.info = reinterpret_cast<uint4*>(take(n * sizeof(uint4)));
   w.hitI = reinterpret_cast<int4*>(take(n * sizeof(int4)));
-  for (int q = 0; q < 3; ++q) w.queue[q] = reinterpret_cast<int*>(take(n * sizeof(int)));
+  w.side = reinterpret_cast<int*>(take(n * sizeof(int)));
   w.ctr = reinterpret_cast<uint32_t*>(take(kNumCtr * sizeof(uint32_t)));
   w.cnt64 = reinterpret_cast<unsigned long long*>(take(kNumCnt64 * sizeof(unsigned long long)));
   *unit = reinterpret_cast<WorkUnit*>(take(sizeof(WorkUnit)));

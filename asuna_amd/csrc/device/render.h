@@ -64,17 +64,21 @@ struct SceneView {
 };
 
 // Wavefront state for up to `capacity` paths (path numbering: WorkUnit).
+// Per-path state is indexed by path id; the rays of a bounce and their hits
+// live in the slot order of the extension queue that holds them, so that
+// traversal and shading read and write them in runs (the queue after
+// raygen is the identity: slot = path id).
 struct Wavefront {
   uint32_t capacity;
-  float4* ro;    // ray origin of the current segment
-  float4* rd;    // ray direction
+  float4* rayO[2];  // extension queues (alternating): ray origin.xyz per slot
+  float4* rayD[2];  // ray direction.xyz, w = path id
   float4* thr;   // throughput.xyz, w = pdf of the last BSDF sample
   float4* rad;   // radiance.xyz, w = Gaussian filter weight
   float4* ld;    // pending NEE radiance of the current bounce
   uint4* info;   // x seed, y depth | bsdfFlags << 16, z pixel, w shot | (frame == 0) << 16
-  float4* hitF;  // t, u, v
-  int4* hitI;    // instance (-1 miss), primitive, global triangle id, leaf position
-  int* queue[3];   // [0], [1] extension queues (alternating), [2] second-group shading queue
+  float4* hitF;  // per slot of the queue being traced: t, u, v
+  int4* hitI;    // instance (-1 miss), shade word, global triangle id, leaf position
+  int* side;     // second-group shading queue: slots of the current extension queue
   float4* shO;  // shadow rays: origin.xyz, tmax
   float4* shD;  // direction.xyz, path id
   uint32_t* ctr;  // [0],[1] queue counts, [2] shadow count, [3] extend fetch,

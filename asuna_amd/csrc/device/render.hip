@@ -752,12 +752,11 @@ __global__ __launch_bounds__(256) void kRaygen(BatchDesc b, Wavefront w) {
   const float expXY = 0.00033546262f;
   float ox = jx - 0.5f, oy = jy - 0.5f;
   float fw = fmaxf(0.0f, expf(-2.0f * ox * ox) - expXY) * fmaxf(0.0f, expf(-2.0f * oy * oy) - expXY);
-  w.ro[p] = f4(ro, 0.f);
-  w.rd[p] = f4(rd, 0.f);
+  w.rayO[0][p] = f4(ro, 0.f);
+  w.rayD[0][p] = f4(rd, __int_as_float(int(p)));
   w.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
   w.rad[p] = make_float4(0.f, 0.f, 0.f, fw);
   w.info[p] = make_uint4(seed, 1u, pix, shot | (frame == 0 ? 0x10000u : 0u));
-  w.queue[0][p] = int(p);
   if (frame == 0) {
     size_t plane = size_t(sp.width) * sp.height;
     for (uint32_t c = 0; c < sp.pc.nMultiChannel; ++c) sp.images[(c + 1) * plane + pix] = make_float4(0.f, 0.f, 0.f, 1.f);
@@ -834,7 +833,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w
   Stack st;
   initStack<TREE != 0, NL>(st, lds, ldsT, w);
   const uint32_t count = w.ctr[cur];
-  const int* q = w.queue[cur];
+  const float4 *qo = w.rayO[cur], *qd = w.rayD[cur];
   int lane = __lane_id();
   uint32_t nNodes = 0, nTris = 0;
   ChunkFetch f;
@@ -843,18 +842,17 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend(SceneView sc, Wavefront w
   while (chunkNext<XCD>(f, lane, start)) {
     uint32_t i = start + uint32_t(lane);
     if (i < f.hi) {
-      int pid = q[i];
       TRay r;
-      setupRay(r, xyz(w.ro[pid]), xyz(w.rd[pid]));
+      setupRay(r, xyz(qo[i]), xyz(qd[i]));
       float t = 0.f, u = 0.f, v = 0.f;
       int pos = TREE ? traceClosest4<COUNT, TREE == 2>(sc, r, kMinimum, kInf, st, t, u, v, nNodes, nTris)
                      : traceClosest<COUNT>(sc, r, kMinimum, kInf, st, t, u, v, nNodes, nTris);
       if (pos >= 0) {
         const float4* tp = sc.tris + size_t(pos) * 3;
-        w.hitF[pid] = make_float4(t, u, v, 0.f);
-        w.hitI[pid] = make_int4(__float_as_int(tp[0].w), __float_as_int(tp[1].w), __float_as_int(tp[2].w), pos);
+        w.hitF[i] = make_float4(t, u, v, 0.f);
+        w.hitI[i] = make_int4(__float_as_int(tp[0].w), __float_as_int(tp[1].w), __float_as_int(tp[2].w), pos);
       } else {
-        w.hitI[pid] = make_int4(-1, -1, -1, -1);
+        w.hitI[i] = make_int4(-1, -1, -1, -1);
       }
     }
   }
@@ -974,11 +972,11 @@ template <bool COUNT, int REFILL>
 __device__ __forceinline__ void extendDynLoop(const SceneView& sc, const Wavefront& w, int cur, Stack& st, int lane,
                                               uint32_t& nNodes, uint32_t& nTris) {
   const uint32_t count = w.ctr[cur];
-  const int* q = w.queue[cur];
+  const float4 *qo = w.rayO[cur], *qd = w.rayD[cur];
   ChunkFetch f;
   chunkInit<true>(f, &w.ctr[kXcdCtr], count);
   bool active = false, drained = false;
-  int pid = -1;
+  uint32_t slot = 0;
   TRay r;
   DynState s;
   for (;;) {
@@ -986,8 +984,8 @@ __device__ __forceinline__ void extendDynLoop(const SceneView& sc, const Wavefro
     if (!drained && nIdle >= uint32_t(REFILL)) {
       uint32_t i = 0;
       if (fetchIdle(f, !active, lane, drained, i)) {
-        pid = q[i];
-        setupRay(r, xyz(w.ro[pid]), xyz(w.rd[pid]));
+        slot = i;
+        setupRay(r, xyz(qo[i]), xyz(qd[i]));
         s.node = 0;
         s.post = kDoneRef;
         s.best = -1;
@@ -1006,10 +1004,10 @@ __device__ __forceinline__ void extendDynLoop(const SceneView& sc, const Wavefro
       active = false;
       if (s.best >= 0) {
         const float4* tp = sc.tris + size_t(s.best) * 3;
-        w.hitF[pid] = make_float4(s.bt, s.bu, s.bv, 0.f);
-        w.hitI[pid] = make_int4(__float_as_int(tp[0].w), __float_as_int(tp[1].w), __float_as_int(tp[2].w), s.best);
+        w.hitF[slot] = make_float4(s.bt, s.bu, s.bv, 0.f);
+        w.hitI[slot] = make_int4(__float_as_int(tp[0].w), __float_as_int(tp[1].w), __float_as_int(tp[2].w), s.best);
       } else {
-        w.hitI[pid] = make_int4(-1, -1, -1, -1);
+        w.hitI[slot] = make_int4(-1, -1, -1, -1);
       }
     }
   }
@@ -1172,7 +1170,7 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend2L(SceneView sc, Wavefront
   Stack st;
   initStack<true, kStackLds>(st, lds, ldsT, w);
   const uint32_t count = w.ctr[cur];
-  const int* q = w.queue[cur];
+  const float4 *qo = w.rayO[cur], *qd = w.rayD[cur];
   const int lane = __lane_id();
   uint32_t nNodes = 0, nTris = 0;
   ChunkFetch f;
@@ -1181,9 +1179,8 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend2L(SceneView sc, Wavefront
   while (chunkNext<false>(f, lane, start)) {
     const uint32_t i = start + uint32_t(lane);
     if (i < f.hi) {
-      const int pid = q[i];
       TRay r;
-      setupRay(r, xyz(w.ro[pid]), xyz(w.rd[pid]));
+      setupRay(r, xyz(qo[i]), xyz(qd[i]));
       float t = 0.f, u = 0.f, v = 0.f;
       int inst = -1;
       const int pos = traceClosest2L<COUNT>(sc, r, kMinimum, kInf, st, t, u, v, inst, nNodes, nTris);
@@ -1200,10 +1197,10 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend2L(SceneView sc, Wavefront
         float t2, u2, v2;
         if (hitTri(r, toWorld(in, tp[0]), toWorld(in, tp[1]), toWorld(in, tp[2]), kMinimum, kInf, t2, u2, v2))
           t = t2, u = u2, v = v2;
-        w.hitF[pid] = make_float4(t, u, v, 0.f);
-        w.hitI[pid] = make_int4(inst, int(in.shade), int(in.triBase) + prim, int(bl.shadeBase) + pos);
+        w.hitF[i] = make_float4(t, u, v, 0.f);
+        w.hitI[i] = make_int4(inst, int(in.shade), int(in.triBase) + prim, int(bl.shadeBase) + pos);
       } else {
-        w.hitI[pid] = make_int4(-1, -1, -1, -1);
+        w.hitI[i] = make_int4(-1, -1, -1, -1);
       }
     }
   }
@@ -1433,17 +1430,17 @@ __global__ __launch_bounds__(kTraceBlock) void kExtendWide(SceneView sc, Wavefro
   Stack st;
   initStack<true, NL>(st, lds, ldsT, w);
   const uint32_t count = w.ctr[cur];
-  const int* q = w.queue[cur];
+  const float4 *qo = w.rayO[cur], *qd = w.rayD[cur];
   uint32_t nNodes = 0, nTris = 0;
   Trav s;
-  int pid = -1;
+  uint32_t slot = 0;
   bool active = false, drained = false;
   for (;;) {
     bool got;
     uint32_t i = refill(&w.ctr[3], count, active, drained, got);
     if (got) {
-      pid = q[i];
-      startRay(s, st, xyz(w.ro[pid]), xyz(w.rd[pid]), kInf);
+      slot = i;
+      startRay(s, st, xyz(qo[i]), xyz(qd[i]), kInf);
       active = true;
     }
     if (__ballot(active) == 0) {
@@ -1454,10 +1451,10 @@ __global__ __launch_bounds__(kTraceBlock) void kExtendWide(SceneView sc, Wavefro
       active = false;
       if (s.best >= 0) {
         const float4* tp = sc.tris + size_t(s.best) * 3;
-        w.hitF[pid] = make_float4(s.bt, s.bu, s.bv, 0.f);
-        w.hitI[pid] = make_int4(__float_as_int(tp[0].w), __float_as_int(tp[1].w), __float_as_int(tp[2].w), s.best);
+        w.hitF[slot] = make_float4(s.bt, s.bu, s.bv, 0.f);
+        w.hitI[slot] = make_int4(__float_as_int(tp[0].w), __float_as_int(tp[1].w), __float_as_int(tp[2].w), s.best);
       } else {
-        w.hitI[pid] = make_int4(-1, -1, -1, -1);
+        w.hitI[slot] = make_int4(-1, -1, -1, -1);
       }
     }
   }
@@ -2752,8 +2749,9 @@ __device__ __forceinline__ void blockAppend(BlockAppend& s, uint32_t* ctrA, bool
 
 template <bool ONE, bool GROUP2, int MATS, bool BLOCK>
 __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur,
-                                         int pid, float4* sMat, BlockAppend* sApp) {
-  const bool active = pid >= 0;
+                                         int slot, float4* sMat, BlockAppend* sApp) {
+  const bool active = slot >= 0;
+  int pid = -1;
   PathCtx c;
   c.stop = true;
   c.shadow = false;
@@ -2762,6 +2760,8 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
   float4 rad = make_float4(0.f, 0.f, 0.f, 0.f);
   bool cont = false;
   if (active) {
+    const float4 d4 = w.rayD[cur][slot];
+    pid = __float_as_int(d4.w);
     info = w.info[pid];
     uint32_t shot = info.w & 0xffffu;
     const ShotParams& sp = ONE ? b.one : b.shotTab[shot];
@@ -2773,7 +2773,7 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
     c.seed = info.x;
     c.depth = info.y & 0xffffu;
     c.bflags = info.y >> 16;
-    c.rd = xyz(w.rd[pid]);  // the origin is read only for light hits (MIS distance)
+    c.rd = xyz(d4);  // the origin is read only for light hits (MIS distance)
     c.T = xyz(thr);
     c.bpdf = thr.w;
     c.L = xyz(rad);
@@ -2782,7 +2782,7 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
     c.fromDielectric = false;
     c.preEnv = false;
     c.Ld = mk1(0.f);
-    int4 hi = w.hitI[pid];
+    int4 hi = w.hitI[slot];
     uint32_t pix = info.z;
     bool frame0 = (info.w >> 16) & 1u;
     if (hi.x < 0) {
@@ -2818,13 +2818,13 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
           c.preEnv = true;
         }
       }
-      float4 hf = w.hitF[pid];
+      float4 hf = w.hitF[slot];
       Hit h;
       hitState(sc, hi.x, uint32_t(hi.y), hi.w, hf.y, hf.z, c.rd, pc.useFaceNormal, h);
       bool aov = frame0 && c.depth == 1;
       if (aov && sp.depthChannel >= 0) writeAov(c, sp.depthChannel, mk1(hf.x), pix);
       if (h.lightId >= 0) {
-        c.ro = xyz(w.ro[pid]);
+        c.ro = xyz(w.rayO[cur][slot]);
         hitLight(c, h.lightId, h.pos);
       } else {
         if (MATS == kMatStaged) {
@@ -2873,18 +2873,19 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
     w.shD[si] = f4(c.shD, __int_as_float(pid));
     w.ld[pid] = f4(c.Ld, 0.f);
   }
-  if (ni >= 0) w.queue[1 - cur][ni] = pid;
+  if (ni >= 0) {  // the continuing ray, in its slot of the next queue
+    w.rayO[1 - cur][ni] = f4(c.ro, 0.f);
+    w.rayD[1 - cur][ni] = f4(c.rd, __int_as_float(pid));
+  }
   if (!active) return;
   w.rad[pid] = f4(c.L, rad.w);
   if (!cont) return;  // a finished path is read again only for its radiance
   w.info[pid] = make_uint4(c.seed, c.depth | (c.bflags << 16), info.z, info.w);
   w.thr[pid] = f4(c.T, c.bpdf);
-  w.ro[pid] = f4(c.ro, 0.f);
-  w.rd[pid] = f4(c.rd, 0.f);
 }
 
 // Shades the queue; paths whose surface belongs to the second plugin group
-// are moved to the side queue (queue[2], count ctr[8]) for kShadeGroup2.
+// are moved to the side queue (their slots, count ctr[8]) for kShadeGroup2.
 // Each block first sorts its paths by what they hit (miss, emitter, material
 // type; counting sort in LDS), so that its waves run one plugin branch each
 // instead of every branch their paths need.
@@ -2894,7 +2895,7 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
 template <bool ONE, bool TABLE>
 __global__ __launch_bounds__(kShadeBlock) void kShade(SceneView sc, BatchDesc b, Wavefront w, int cur) {
   __shared__ float4 sMat[TABLE ? kMatTableMax * 9 : kShadeBlock * 9];
-  __shared__ int sPid[kShadeBlock];
+  __shared__ int sSlot[kShadeBlock];
   __shared__ uint32_t sCnt[kShadeKeys];
   __shared__ BlockAppend sApp;
   const uint32_t count = w.ctr[cur];
@@ -2905,18 +2906,17 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(SceneView sc, BatchDesc b,
   }
   uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
   const bool valid = i < count;
-  const int pid = valid ? w.queue[cur][i] : -1;
   bool side = false;
   uint32_t key = kShadeKeys - 1;
   if (valid) {
-    int2 hi = *reinterpret_cast<const int2*>(&w.hitI[pid]);  // the hit carries its instance's shade word
+    int2 hi = *reinterpret_cast<const int2*>(&w.hitI[i]);  // the hit carries its instance's shade word
     const uint32_t sw = uint32_t(hi.y);
     side = hi.x >= 0 && (sw & kShadeSide);
     key = hi.x < 0 ? 0u : (sw & kShadeLight) ? 1u : 2u + ((sw >> kShadeTypeShift) & kShadeTypeMask);
     if (side || key >= uint32_t(kShadeKeys - 1)) key = kShadeKeys - 1;
   }
   int si = waveAppend(&w.ctr[8], side);
-  if (side) w.queue[2][si] = pid;
+  if (side) w.side[si] = int(i);
   if (threadIdx.x < kShadeKeys) sCnt[threadIdx.x] = 0;
   __syncthreads();
   const uint32_t rank = key < uint32_t(kShadeKeys - 1) ? atomicAdd(&sCnt[key], 1u) : 0u;
@@ -2931,10 +2931,10 @@ __global__ __launch_bounds__(kShadeBlock) void kShade(SceneView sc, BatchDesc b,
     sCnt[kShadeKeys - 1] = run;
   }
   __syncthreads();
-  if (key < uint32_t(kShadeKeys - 1)) sPid[sCnt[key] + rank] = pid;
+  if (key < uint32_t(kShadeKeys - 1)) sSlot[sCnt[key] + rank] = int(i);
   const uint32_t total = sCnt[kShadeKeys - 1];
   __syncthreads();
-  const int mine = threadIdx.x < total ? sPid[threadIdx.x] : -1;
+  const int mine = threadIdx.x < total ? sSlot[threadIdx.x] : -1;
   if (TABLE) shadeOne<ONE, false, kMatTable, true>(sc, b, w, cur, mine, sMat, &sApp);
   else shadeOne<ONE, false, kMatStaged, true>(sc, b, w, cur, mine, sMat + threadIdx.x * 9, &sApp);
 }
@@ -2944,7 +2944,7 @@ template <bool ONE>
 __global__ __launch_bounds__(256) void kShadeGroup2(SceneView sc, BatchDesc b, Wavefront w, int cur) {
   const uint32_t count = w.ctr[8];
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < count; i += gridDim.x * blockDim.x)
-    shadeOne<ONE, true, kMatGlobal, false>(sc, b, w, cur, w.queue[2][i], nullptr, nullptr);
+    shadeOne<ONE, true, kMatGlobal, false>(sc, b, w, cur, w.side[i], nullptr, nullptr);
 }
 
 // EXT: the extension fetch counters; SHADE: the counts shading appends to
@@ -3036,7 +3036,7 @@ __global__ __launch_bounds__(256) void kFirstHitState(SceneView sc, const ShotPa
                                                       uint32_t P) {
   uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p >= P) return;
-  int4 hi = w.hitI[p];
+  int4 hi = w.hitI[p];  // slot p of queue 0 after raygen is path p
   ash_hit o;
   if (hi.x < 0) {
     o.t = kInf;
@@ -3047,7 +3047,7 @@ __global__ __launch_bounds__(256) void kFirstHitState(SceneView sc, const ShotPa
   } else {
     float4 hf = w.hitF[p];
     Hit h;
-    hitState(sc, hi.x, uint32_t(hi.y), hi.w, hf.y, hf.z, xyz(w.rd[p]), 0u, h);
+    hitState(sc, hi.x, uint32_t(hi.y), hi.w, hf.y, hf.z, xyz(w.rayD[0][p]), 0u, h);
     o.t = hf.x;
     o.u = hf.y;
     o.v = hf.z;

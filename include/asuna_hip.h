@@ -371,7 +371,7 @@ int ash_render(ash_ctx* ctx, const GpuCamera* cam, const GpuPushConstantRaytrace
  * planes of width*height RGBA32F, laid out like ash_render's film and
  * accumulated the same way (frame 0 overwrites; later frames fold into the
  * running mean, so a caller continuing a render passes the same buffer).
- * The wavefront lives in one fixed 96 MiB device array per GPU, held
+ * The wavefront lives in one fixed 128 MiB device array per GPU, held
  * until the process ends; at most 2^18 paths are in flight per batch and
  * the traversal stacks of 2 persistent blocks per CU must fit beside them,
  * else ASH_E_INVALID.  No hipMalloc / hipFree is called.  Every call on a
