@@ -109,8 +109,8 @@ struct ash_ctx {
   uint32_t shotCount = 0, shotW = 0, shotH = 0;
   // wavefront
   Wavefront wf{};
-  DBuf<float4> wRayO0, wRayO1, wRayD0, wRayD1, wThr, wRad, wLd, wHitF, wShO, wShD;
-  DBuf<uint4> wInfo;
+  DBuf<float4> wRayO0, wRayO1, wRayD0, wRayD1, wThr0, wThr1, wRad, wLd, wHitF, wShO, wShD;
+  DBuf<uint4> wInfo0, wInfo1;
   DBuf<int4> wHitI;
   DBuf<int> wSide, wSpill;
   DBuf<float> wSpillT;
@@ -145,11 +145,11 @@ struct ash_ctx {
     if (tris) (void)hipFree(tris);
     if (shadeTris) (void)hipFree(shadeTris);
     for (auto& t : texData) t.release();
-    for (auto* b : {&verts, &env, &images, &shotImages, &wRayO0, &wRayO1, &wRayD0, &wRayD1, &wThr, &wRad, &wLd, &wHitF,
-                    &wShO, &wShD})
+    for (auto* b : {&verts, &env, &images, &shotImages, &wRayO0, &wRayO1, &wRayD0, &wRayD1, &wThr0, &wThr1, &wRad, &wLd,
+                    &wHitF, &wShO, &wShD})
       b->release();
     inst.release(), idx.release(), instTriOffset.release(), mats.release(), lights.release(), texTab.release();
-    marg.release(), condX.release(), shotTab.release(), wInfo.release(), wHitI.release();
+    marg.release(), condX.release(), shotTab.release(), wInfo0.release(), wInfo1.release(), wHitI.release();
     wSide.release(), wSpill.release(), wSpillT.release(), wCtr.release(), wCnt64.release();
     unitBuf.release(), sky.release();
     for (auto& g : graphs) releaseGraph(g);
@@ -243,13 +243,15 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   MALLOC(c, c->wRayO1, cap);
   MALLOC(c, c->wRayD0, cap);
   MALLOC(c, c->wRayD1, cap);
-  MALLOC(c, c->wThr, cap);
+  MALLOC(c, c->wThr0, cap);
+  MALLOC(c, c->wThr1, cap);
   MALLOC(c, c->wRad, cap);
   MALLOC(c, c->wLd, cap);
   MALLOC(c, c->wHitF, cap);
   MALLOC(c, c->wShO, cap);
   MALLOC(c, c->wShD, cap);
-  MALLOC(c, c->wInfo, cap);
+  MALLOC(c, c->wInfo0, cap);
+  MALLOC(c, c->wInfo1, cap);
   MALLOC(c, c->wHitI, cap);
   MALLOC(c, c->wSide, cap);
   MALLOC(c, c->wCtr, kNumCtr);
@@ -260,10 +262,12 @@ static int ensureWavefront(ash_ctx* c, uint32_t cap) {
   w.rayO[1] = c->wRayO1.p;
   w.rayD[0] = c->wRayD0.p;
   w.rayD[1] = c->wRayD1.p;
-  w.thr = c->wThr.p;
+  w.thr[0] = c->wThr0.p;
+  w.thr[1] = c->wThr1.p;
   w.rad = c->wRad.p;
   w.ld = c->wLd.p;
-  w.info = c->wInfo.p;
+  w.info[0] = c->wInfo0.p;
+  w.info[1] = c->wInfo1.p;
   w.hitF = c->wHitF.p;
   w.hitI = c->wHitI.p;
   w.side = c->wSide.p;
@@ -881,13 +885,13 @@ static size_t noAllocLayout(unsigned char* base, uint32_t threads, uint32_t cap,
     w.rayO[q] = reinterpret_cast<float4*>(take(n * sizeof(float4)));
     w.rayD[q] = reinterpret_cast<float4*>(take(n * sizeof(float4)));
   }
-  w.thr = reinterpret_cast<float4*>(take(n * sizeof(float4)));
+  for (int q = 0; q < 2; ++q) w.thr[q] = reinterpret_cast<float4*>(take(n * sizeof(float4)));
   w.rad = reinterpret_cast<float4*>(take(n * sizeof(float4)));
   w.ld = reinterpret_cast<float4*>(take(n * sizeof(float4)));
   w.hitF = reinterpret_cast<float4*>(take(n * sizeof(float4)));
   w.shO = reinterpret_cast<float4*>(take(n * sizeof(float4)));
   w.shD = reinterpret_cast<float4*>(take(n * sizeof(float4)));
-  w.info = reinterpret_cast<uint4*>(take(n * sizeof(uint4)));
+  for (int q = 0; q < 2; ++q) w.info[q] = reinterpret_cast<uint4*>(take(n * sizeof(uint4)));
   w.hitI = reinterpret_cast<int4*>(take(n * sizeof(int4)));
   w.side = reinterpret_cast<int*>(take(n * sizeof(int)));
   w.ctr = reinterpret_cast<uint32_t*>(take(kNumCtr * sizeof(uint32_t)));

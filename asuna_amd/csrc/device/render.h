@@ -64,18 +64,19 @@ struct SceneView {
 };
 
 // Wavefront state for up to `capacity` paths (path numbering: WorkUnit).
-// Per-path state is indexed by path id; the rays of a bounce and their hits
-// live in the slot order of the extension queue that holds them, so that
+// A path's radiance is indexed by path id (the accumulation reads it
+// there); its ray, throughput and RNG / pixel record, and the hit of its
+// ray, live in the slot order of the extension queue that holds it, so that
 // traversal and shading read and write them in runs (the queue after
 // raygen is the identity: slot = path id).
 struct Wavefront {
   uint32_t capacity;
   float4* rayO[2];  // extension queues (alternating): ray origin.xyz per slot
   float4* rayD[2];  // ray direction.xyz, w = path id
-  float4* thr;   // throughput.xyz, w = pdf of the last BSDF sample
-  float4* rad;   // radiance.xyz, w = Gaussian filter weight
-  float4* ld;    // pending NEE radiance of the current bounce
-  uint4* info;   // x seed, y depth | bsdfFlags << 16, z pixel, w shot | (frame == 0) << 16
+  float4* thr[2];   // throughput.xyz, w = pdf of the last BSDF sample
+  uint4* info[2];   // x seed, y depth | bsdfFlags << 16, z pixel, w shot | (frame == 0) << 16
+  float4* rad;   // by path id: radiance.xyz, w = Gaussian filter weight
+  float4* ld;    // pending NEE radiance of the current bounce, per shadow-ray slot
   float4* hitF;  // per slot of the queue being traced: t, u, v
   int4* hitI;    // instance (-1 miss), shade word, global triangle id, leaf position
   int* side;     // second-group shading queue: slots of the current extension queue

@@ -754,9 +754,9 @@ __global__ __launch_bounds__(256) void kRaygen(BatchDesc b, Wavefront w) {
   float fw = fmaxf(0.0f, expf(-2.0f * ox * ox) - expXY) * fmaxf(0.0f, expf(-2.0f * oy * oy) - expXY);
   w.rayO[0][p] = f4(ro, 0.f);
   w.rayD[0][p] = f4(rd, __int_as_float(int(p)));
-  w.thr[p] = make_float4(1.f, 1.f, 1.f, 0.f);
+  w.thr[0][p] = make_float4(1.f, 1.f, 1.f, 0.f);
   w.rad[p] = make_float4(0.f, 0.f, 0.f, fw);
-  w.info[p] = make_uint4(seed, 1u, pix, shot | (frame == 0 ? 0x10000u : 0u));
+  w.info[0][p] = make_uint4(seed, 1u, pix, shot | (frame == 0 ? 0x10000u : 0u));
   if (frame == 0) {
     size_t plane = size_t(sp.width) * sp.height;
     for (uint32_t c = 0; c < sp.pc.nMultiChannel; ++c) sp.images[(c + 1) * plane + pix] = make_float4(0.f, 0.f, 0.f, 1.f);
@@ -1048,7 +1048,7 @@ __device__ __forceinline__ void shadowLoop(const SceneView& sc, const Wavefront&
       bool occluded = TREE ? traceAny4<COUNT, TREE == 2>(sc, r, 0.0f, so.w, st, nNodes, nTris)
                            : traceAny<COUNT>(sc, r, 0.0f, so.w, st, nNodes, nTris);
       if (!occluded) {
-        float4 L = w.rad[pid], d = w.ld[pid];
+        float4 L = w.rad[pid], d = w.ld[i];
         w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
       }
     }
@@ -1129,6 +1129,7 @@ __device__ __forceinline__ void shadowDynLoop(const SceneView& sc, const Wavefro
   chunkInit<true>(f, &w.ctr[kXcdCtr + kXcds * 32], w.ctr[2]);
   bool active = false, drained = false;
   int pid = -1;
+  uint32_t slot = 0;
   float tmax = 0.0f;
   TRay r;
   AnyState s;
@@ -1141,6 +1142,7 @@ __device__ __forceinline__ void shadowDynLoop(const SceneView& sc, const Wavefro
         setupRay(r, xyz(so), xyz(sd));
         tmax = so.w;
         pid = __float_as_int(sd.w);
+        slot = i;
         s.node = 0;
         s.post = kDoneRef;
         s.hit = false;
@@ -1155,7 +1157,7 @@ __device__ __forceinline__ void shadowDynLoop(const SceneView& sc, const Wavefro
     if (active && anyRound<COUNT>(sc, r, tmax, s, st, nNodes, nTris)) {
       active = false;
       if (!s.hit) {
-        float4 L = w.rad[pid], d = w.ld[pid];
+        float4 L = w.rad[pid], d = w.ld[slot];
         w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
       }
     }
@@ -1230,7 +1232,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow2L(SceneView sc, Wavefront
       setupRay(r, xyz(so), xyz(sd));
       const int pid = __float_as_int(sd.w);
       if (!traceAny2L<COUNT>(sc, r, 0.0f, so.w, st, nNodes, nTris)) {
-        const float4 L = w.rad[pid], d = w.ld[pid];
+        const float4 L = w.rad[pid], d = w.ld[i];
         w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
       }
     }
@@ -1473,6 +1475,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadowWide(SceneView sc, Wavefro
   uint32_t nNodes = 0, nTris = 0;
   Trav s;
   int pid = -1;
+  uint32_t slot = 0;
   bool active = false, drained = false;
   for (;;) {
     bool got;
@@ -1480,6 +1483,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadowWide(SceneView sc, Wavefro
     if (got) {
       float4 so = w.shO[i], sd = w.shD[i];
       pid = __float_as_int(sd.w);
+      slot = i;
       startRay(s, st, xyz(so), xyz(sd), so.w);
       active = true;
     }
@@ -1492,7 +1496,7 @@ __global__ __launch_bounds__(kTraceBlock) void kShadowWide(SceneView sc, Wavefro
       if (r) {
         active = false;
         if (r == 1) {
-          float4 L = w.rad[pid], d = w.ld[pid];
+          float4 L = w.rad[pid], d = w.ld[slot];
           w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
         }
       }
@@ -2762,11 +2766,11 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
   if (active) {
     const float4 d4 = w.rayD[cur][slot];
     pid = __float_as_int(d4.w);
-    info = w.info[pid];
+    info = w.info[cur][slot];
     uint32_t shot = info.w & 0xffffu;
     const ShotParams& sp = ONE ? b.one : b.shotTab[shot];
     const GpuPushConstantRaytrace& pc = sp.pc;
-    float4 thr = w.thr[pid];
+    float4 thr = w.thr[cur][slot];
     rad = w.rad[pid];
     c.sc = &sc;
     c.sp = &sp;
@@ -2871,17 +2875,15 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
   if (si >= 0) {
     w.shO[si] = f4(c.shO, c.shDist - 2.0f * kEps);
     w.shD[si] = f4(c.shD, __int_as_float(pid));
-    w.ld[pid] = f4(c.Ld, 0.f);
+    w.ld[si] = f4(c.Ld, 0.f);
   }
-  if (ni >= 0) {  // the continuing ray, in its slot of the next queue
+  if (ni >= 0) {  // the continuing path, in its slot of the next queue
     w.rayO[1 - cur][ni] = f4(c.ro, 0.f);
     w.rayD[1 - cur][ni] = f4(c.rd, __int_as_float(pid));
+    w.thr[1 - cur][ni] = f4(c.T, c.bpdf);
+    w.info[1 - cur][ni] = make_uint4(c.seed, c.depth | (c.bflags << 16), info.z, info.w);
   }
-  if (!active) return;
-  w.rad[pid] = f4(c.L, rad.w);
-  if (!cont) return;  // a finished path is read again only for its radiance
-  w.info[pid] = make_uint4(c.seed, c.depth | (c.bflags << 16), info.z, info.w);
-  w.thr[pid] = f4(c.T, c.bpdf);
+  if (active) w.rad[pid] = f4(c.L, rad.w);
 }
 
 // Shades the queue; paths whose surface belongs to the second plugin group
