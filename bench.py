@@ -204,7 +204,7 @@ def main():
                          "job at the mapped spp (denoised unless p), the one-at-a-time baseline renders the plain spp")
     ap.add_argument("--scale", type=int, default=0, help="shrink the procedural scene (tests)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-frames", type=int, default=12)
+    ap.add_argument("--cpu-frames", type=int, default=16)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--queue-paths", type=int, default=0)
     ap.add_argument("--scene-dir", default="")
