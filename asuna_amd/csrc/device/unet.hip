@@ -300,22 +300,55 @@ __global__ __launch_bounds__(256) void kConvFwdChunk(ConvArgs a) {
       }
     }
   }
+  if (a.Y) {
+    // BF16 output through LDS: each lane holds 4 pixels of one channel per
+    // fragment; the block's 256 x Cout tile is written to LDS pixel-major and
+    // stored as 16-byte runs of 8 channels (whole tile rows are contiguous
+    // in NHWC memory)
+    constexpr uint32_t OS = Cout + 8;  // bf16 per pixel in LDS (the pad spreads banks)
+    bf16* ot = lds;
+    __syncthreads();  // every wave is done with the tile and the weights
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    const uint32_t y = ty0 + 2 * wv + uint32_t(g) / 2;
-    if (y >= a.H) continue;
+    for (int g = 0; g < 4; ++g)
 #pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      const uint32_t co = 16 * t + r16;
-      const float b = a.bias[co];
-      for (int r = 0; r < 4; ++r) {
-        const uint32_t x = tx0 + (uint32_t(g) % 2) * 16 + 4 * q + r;
-        if (x >= a.W) continue;
-        const size_t pp = (size_t(n) * a.H + y) * a.W + x;
-        float v = acc[g][t][r] + b;
-        if (a.relu) v = fmaxf(v, 0.f);
-        if (a.Y) a.Y[pp * Cout + co] = bf16(v);
-        if (a.Yf) a.Yf[pp * Cout + co] = v;
+      for (int t = 0; t < NT; ++t) {
+        const uint32_t co = 16 * t + r16;
+        const float b = a.bias[co];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t px = (2 * wv + uint32_t(g) / 2) * kChunkCols + (uint32_t(g) % 2) * 16 + 4 * q + r;
+          float v = acc[g][t][r] + b;
+          if (a.relu) v = fmaxf(v, 0.f);
+          ot[px * OS + co] = bf16(v);
+        }
+      }
+    __syncthreads();
+    constexpr uint32_t C8 = Cout / 8;
+    for (uint32_t k = threadIdx.x; k < kChunkRows * kChunkCols * C8; k += 256) {
+      const uint32_t px = k / C8, c8 = k % C8;
+      const uint32_t y = ty0 + px / kChunkCols, x = tx0 + px % kChunkCols;
+      if (y < a.H && x < a.W)
+        *reinterpret_cast<bf16x8*>(a.Y + ((size_t(n) * a.H + y) * a.W + x) * Cout + 8 * c8) =
+            *reinterpret_cast<const bf16x8*>(ot + px * OS + 8 * c8);
+    }
+  }
+  if (a.Yf) {
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+      const uint32_t y = ty0 + 2 * wv + uint32_t(g) / 2;
+      if (y >= a.H) continue;
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const uint32_t co = 16 * t + r16;
+        const float b = a.bias[co];
+        for (int r = 0; r < 4; ++r) {
+          const uint32_t x = tx0 + (uint32_t(g) % 2) * 16 + 4 * q + r;
+          if (x >= a.W) continue;
+          const size_t pp = (size_t(n) * a.H + y) * a.W + x;
+          float v = acc[g][t][r] + b;
+          if (a.relu) v = fmaxf(v, 0.f);
+          a.Yf[pp * Cout + co] = v;
+        }
       }
     }
   }
@@ -324,6 +357,9 @@ template <int NT> constexpr size_t chunkLds() {
   return (size_t(kChunkRows + 2) * (kChunkCols + 2) + size_t(chunkTaps<NT>()) * 16 * NT) * kChunkStride * sizeof(bf16);
 }
 static_assert(chunkLds<2>() <= 65536 && chunkLds<4>() <= 65536, "chunked convolution LDS above 64 KB");
+static_assert(chunkLds<2>() >= size_t(kChunkRows) * kChunkCols * (32 + 8) * sizeof(bf16) &&
+                  chunkLds<4>() >= size_t(kChunkRows) * kChunkCols * (64 + 8) * sizeof(bf16),
+              "chunked convolution LDS below its output tile");
 
 template <int NT>
 static void convFwdFor(const ConvArgs& a, size_t P, hipStream_t s) {
