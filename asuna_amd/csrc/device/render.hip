@@ -1164,17 +1164,14 @@ __device__ __forceinline__ void shadowDynLoop(const SceneView& sc, const Wavefro
   }
 }
 
-// Two-level closest-hit and shadow kernels (wave fetch in queue order).
+// Two-level closest-hit and shadow walks (wave fetch in queue order), as
+// separate kernels and fused like kTraceDyn (a wave that runs out of
+// closest-hit rays takes shadow rays).
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kExtend2L(SceneView sc, Wavefront w, int cur) {
-  __shared__ int lds[kStackLds * kTraceBlock];
-  __shared__ float ldsT[kStackLds * kTraceBlock];
-  Stack st;
-  initStack<true, kStackLds>(st, lds, ldsT, w);
+__device__ __forceinline__ void extend2LLoop(const SceneView& sc, const Wavefront& w, int cur, Stack& st, int lane,
+                                             uint32_t& nNodes, uint32_t& nTris) {
   const uint32_t count = w.ctr[cur];
   const float4 *qo = w.rayO[cur], *qd = w.rayD[cur];
-  const int lane = __lane_id();
-  uint32_t nNodes = 0, nTris = 0;
   ChunkFetch f;
   chunkInit<false>(f, &w.ctr[3], count);
   uint32_t start;
@@ -1206,21 +1203,11 @@ __global__ __launch_bounds__(kTraceBlock) void kExtend2L(SceneView sc, Wavefront
       }
     }
   }
-  if (COUNT) {
-    atomicAdd(&w.cnt64[0], (unsigned long long)nNodes);
-    atomicAdd(&w.cnt64[1], (unsigned long long)nTris);
-  }
 }
-
 template <bool COUNT>
-__global__ __launch_bounds__(kTraceBlock) void kShadow2L(SceneView sc, Wavefront w) {
-  __shared__ int lds[kStackLds * kTraceBlock];
-  __shared__ float ldsT[kStackLds * kTraceBlock];
-  Stack st;
-  initStack<true, kStackLds>(st, lds, ldsT, w);
+__device__ __forceinline__ void shadow2LLoop(const SceneView& sc, const Wavefront& w, Stack& st, int lane,
+                                             uint32_t& nNodes, uint32_t& nTris) {
   const uint32_t count = w.ctr[2];
-  const int lane = __lane_id();
-  uint32_t nNodes = 0, nTris = 0;
   ChunkFetch f;
   chunkInit<false>(f, &w.ctr[4], count);
   uint32_t start;
@@ -1237,9 +1224,53 @@ __global__ __launch_bounds__(kTraceBlock) void kShadow2L(SceneView sc, Wavefront
       }
     }
   }
+}
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kExtend2L(SceneView sc, Wavefront w, int cur) {
+  __shared__ int lds[kStackLds * kTraceBlock];
+  __shared__ float ldsT[kStackLds * kTraceBlock];
+  Stack st;
+  initStack<true, kStackLds>(st, lds, ldsT, w);
+  uint32_t nNodes = 0, nTris = 0;
+  extend2LLoop<COUNT>(sc, w, cur, st, __lane_id(), nNodes, nTris);
+  if (COUNT) {
+    atomicAdd(&w.cnt64[0], (unsigned long long)nNodes);
+    atomicAdd(&w.cnt64[1], (unsigned long long)nTris);
+  }
+}
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kShadow2L(SceneView sc, Wavefront w) {
+  __shared__ int lds[kStackLds * kTraceBlock];
+  __shared__ float ldsT[kStackLds * kTraceBlock];
+  Stack st;
+  initStack<true, kStackLds>(st, lds, ldsT, w);
+  uint32_t nNodes = 0, nTris = 0;
+  shadow2LLoop<COUNT>(sc, w, st, __lane_id(), nNodes, nTris);
   if (COUNT) {
     atomicAdd(&w.cnt64[2], (unsigned long long)nNodes);
     atomicAdd(&w.cnt64[3], (unsigned long long)nTris);
+  }
+}
+// COUNT: as kTraceDyn (totals and the fused launches' own counters).
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) void kTrace2L(SceneView sc, Wavefront w, int cur) {
+  __shared__ int lds[kStackLds * kTraceBlock];
+  __shared__ float ldsT[kStackLds * kTraceBlock];
+  if (COUNT && blockIdx.x == 0 && threadIdx.x == 0) {
+    atomicAdd(&w.cnt64[11], (unsigned long long)w.ctr[cur]);
+    atomicAdd(&w.cnt64[12], (unsigned long long)w.ctr[2]);
+  }
+  Stack st;
+  initStack<true, kStackLds>(st, lds, ldsT, w);
+  uint32_t eN = 0, eT = 0, sN = 0, sT = 0;
+  extend2LLoop<COUNT>(sc, w, cur, st, __lane_id(), eN, eT);
+  shadow2LLoop<COUNT>(sc, w, st, __lane_id(), sN, sT);
+  if (COUNT) {
+    const unsigned long long v[4] = {eN, eT, sN, sT};
+    for (int k = 0; k < 4; ++k) {
+      atomicAdd(&w.cnt64[k], v[k]);
+      atomicAdd(&w.cnt64[7 + k], v[k]);
+    }
   }
 }
 
@@ -3157,10 +3188,15 @@ void launchResetCounters(const Wavefront& w, int cur, hipStream_t s) { kResetCou
 void launchResetShade(const Wavefront& w, int cur, hipStream_t s) { kResetCounters<false, true><<<1, 64, 0, s>>>(w, cur); }
 void launchResetExtend(const Wavefront& w, hipStream_t s) { kResetCounters<true, false><<<1, 64, 0, s>>>(w, 0); }
 bool canFuseTrace(const SceneView& sc, uint32_t scheme) {
-  return scheme == kSchemeXcdSlices && sc.nodes4 != nullptr && sc.nodesQ == nullptr && !sc.twoLevel;
+  return sc.twoLevel || (scheme == kSchemeXcdSlices && sc.nodes4 != nullptr && sc.nodesQ == nullptr);
 }
 void launchTrace(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t grid, bool count,
                  hipStream_t s) {
+  if (sc.twoLevel) {
+    if (count) kTrace2L<true><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    else kTrace2L<false><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
+    return;
+  }
   const bool shallow = ldsStack == uint32_t(kStackLdsMin);
   if (count) {
     if (shallow) kTraceDyn<true, kStackLdsMin, kDynRefill><<<grid, kTraceBlock, 0, s>>>(sc, w, cur);
