@@ -705,8 +705,13 @@ static int buildTwoLevel(ash_ctx* c, const BuildArgs& tmpl, ash_build_info* info
   float4* shade = nullptr;
   HIPC(c, hipMalloc(&shade, std::max<uint64_t>(shadeTotal, 1) * 6 * sizeof(float4)));
   c->shadeTris = shade;
-  DBuf<DevInstance> one;
-  DBuf<uint32_t> offs;
+  struct Staging {  // the identity instance and its triangle-count prefix, freed on every return
+    DBuf<DevInstance> one;
+    DBuf<uint32_t> offs;
+    ~Staging() { one.release(), offs.release(); }
+  } tmp;
+  DBuf<DevInstance>& one = tmp.one;
+  DBuf<uint32_t>& offs = tmp.offs;
   MALLOC(c, one, 1);
   MALLOC(c, offs, 2);
   for (uint32_t k = 0; k < nb; ++k) {
