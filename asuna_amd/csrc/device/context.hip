@@ -349,14 +349,17 @@ static int launchLoop(ash_ctx* c, const Wavefront& wf, const SceneView& sc, cons
   if (!canFuseTrace(sc, scheme)) {
     for (uint32_t it = 0; it < iters; ++it) {
       launchResetCounters(wf, cur, s);
-      timed(tExt, [&] { launchExtend(sc, wf, cur, lds, scheme, grid, count, s); });
+      timed(tExt, [&] {
+        if (it == 0) launchExtendCamera(sc, wf, lds, scheme, grid, count, s);
+        else launchExtend(sc, wf, cur, lds, scheme, grid, count, s);
+      });
       timed(tShade, [&] { launchShade(sc, b, wf, cur, shadeRays, s); });
       timed(tShadow, [&] { launchShadow(sc, wf, scheme, grid, count, s); });
       cur = 1 - cur;
     }
   } else {
     launchResetCounters(wf, 0, s);
-    timed(tExt, [&] { launchExtend(sc, wf, 0, lds, scheme, grid, count, s); });
+    timed(tExt, [&] { launchExtendCamera(sc, wf, lds, scheme, grid, count, s); });
     for (uint32_t it = 0; it < iters; ++it) {
       if (it > 0) launchResetShade(wf, cur, s);
       timed(tShade, [&] { launchShade(sc, b, wf, cur, shadeRays, s); });
@@ -1468,7 +1471,7 @@ int ash_first_hit(ash_ctx* c, const GpuCamera* cam, ash_hit* out) {
   SceneView sc = viewOf(c);
   launchRaygen(b, c->wf, c->stream);
   launchResetCounters(c->wf, 0, c->stream);
-  launchExtend(sc, c->wf, 0, kStackLds, kSchemeXcdSlices, grid, false, c->stream);
+  launchExtendCamera(sc, c->wf, kStackLds, kSchemeXcdSlices, grid, false, c->stream);
   launchFirstHit(sc, c->shotTab.p, c->wf, dout.p, P, c->stream);
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(out, dout.p, P * sizeof(ash_hit), hipMemcpyDeviceToHost, c->stream));

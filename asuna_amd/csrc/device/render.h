@@ -145,6 +145,11 @@ void launchRaygen(const BatchDesc& b, const Wavefront& w, hipStream_t s);
 // 4-wide closest-hit kernel
 void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t ldsStack, uint32_t scheme,
                   uint32_t grid, bool count, hipStream_t s);
+// Closest hit of the camera rays (queue 0 after raygen): the wave-coherent
+// walk on the float 4-wide tree with XCD slices, else launchExtend.
+bool canPacketCamera(const SceneView& sc, uint32_t scheme);
+void launchExtendCamera(const SceneView& sc, const Wavefront& w, uint32_t ldsStack, uint32_t scheme, uint32_t grid,
+                        bool count, hipStream_t s);
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
                  hipStream_t s);
 void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t scheme, uint32_t grid, bool count,

@@ -67,14 +67,14 @@ __device__ __forceinline__ void setupRay(TRay& r, f3 o, f3 d) {
   r.Sz = 1.0f / dz;
 }
 
-// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013).
-__device__ __forceinline__ bool hitTri(const TRay& r, float4 p0, float4 p1, float4 p2, float tmin, float tmax,
-                                       float& t, float& u, float& v) {
-  f3 A = f3{p0.x, p0.y, p0.z} - r.o, B = f3{p1.x, p1.y, p1.z} - r.o, C = f3{p2.x, p2.y, p2.z} - r.o;
-  float Akz = comp(A, r.kz), Bkz = comp(B, r.kz), Ckz = comp(C, r.kz);
-  float Ax = comp(A, r.kx) - r.Sx * Akz, Ay = comp(A, r.ky) - r.Sy * Akz;
-  float Bx = comp(B, r.kx) - r.Sx * Bkz, By = comp(B, r.ky) - r.Sy * Bkz;
-  float Cx = comp(C, r.kx) - r.Sx * Ckz, Cy = comp(C, r.ky) - r.Sy * Ckz;
+// Watertight ray/triangle test (Woop, Benthin, Wald, JCGT 2013) on the
+// vertices relative to the ray origin, permuted to (kx, ky, kz).
+__device__ __forceinline__ bool hitTriCore(const TRay& r, float Akx, float Aky, float Akz, float Bkx, float Bky,
+                                           float Bkz, float Ckx, float Cky, float Ckz, float tmin, float tmax,
+                                           float& t, float& u, float& v) {
+  float Ax = Akx - r.Sx * Akz, Ay = Aky - r.Sy * Akz;
+  float Bx = Bkx - r.Sx * Bkz, By = Bky - r.Sy * Bkz;
+  float Cx = Ckx - r.Sx * Ckz, Cy = Cky - r.Sy * Ckz;
   float U = Cx * By - Cy * Bx;
   float V = Ax * Cy - Ay * Cx;
   float W = Bx * Ay - By * Ax;
@@ -95,6 +95,12 @@ __device__ __forceinline__ bool hitTri(const TRay& r, float4 p0, float4 p1, floa
   u = V * rcp;
   v = W * rcp;
   return true;
+}
+__device__ __forceinline__ bool hitTri(const TRay& r, float4 p0, float4 p1, float4 p2, float tmin, float tmax,
+                                       float& t, float& u, float& v) {
+  f3 A = f3{p0.x, p0.y, p0.z} - r.o, B = f3{p1.x, p1.y, p1.z} - r.o, C = f3{p2.x, p2.y, p2.z} - r.o;
+  return hitTriCore(r, comp(A, r.kx), comp(A, r.ky), comp(A, r.kz), comp(B, r.kx), comp(B, r.ky), comp(B, r.kz),
+                    comp(C, r.kx), comp(C, r.ky), comp(C, r.kz), tmin, tmax, t, u, v);
 }
 
 // Conservative slab test (far bound scaled by 1 + 2*gamma_3, Ize 2013
@@ -1024,6 +1030,309 @@ __global__ __launch_bounds__(kTraceBlock) void kExtendDyn(SceneView sc, Wavefron
   if (COUNT) {
     atomicAdd(&w.cnt64[0], (unsigned long long)nNodes);
     atomicAdd(&w.cnt64[1], (unsigned long long)nTris);
+  }
+}
+
+// ------------------------------------------------- coherent (camera) walk
+// Closest hit of the camera-ray queue, one walk per wave.  A wave's 64
+// rays are the frames of a few neighbouring pixels and nearly coincide, so
+// the wave visits nodes together: each lane tests its own ray against the
+// node's four boxes, a child is taken when any lane's ray enters it, and the
+// wave stack keeps the node ref once plus every lane's entry distance, so a
+// popped entry is visited only when some lane's ray still needs it (the
+// per-ray walk's pop test, lane by lane).  Children are visited in the
+// first lane's order.  Every lane tests every node and triangle the wave
+// visits, with its own culling bound, so it visits a superset of what its
+// own walk would; with the tie rule of traceClosest4 (equal t: smaller
+// global id) the closest hit does not depend on the visit order and is the
+// per-ray walk's.  Stack: kPacketLds entries per wave in LDS, then the
+// per-thread global spill (entries of the wave's ref written by every lane),
+// as deep as the per-ray stacks (three entries per 4-wide level).
+//
+// When every ray of the wave has the same direction signs and the same
+// dominant axis (UNI, nearly always for camera rays), node planes and
+// triangle vertices come through the scalar cache at wave-uniform offsets:
+// the near and far plane quads of each axis, and the vertex components in
+// the rays' (kx, ky, kz) order, so that neither the slab test's plane choice
+// nor the triangle test's axis permutation costs a per-lane select.  The
+// arithmetic is that of slab and hitTri, so the results are bit-identical.
+// Other waves run the same walk with slab and hitTri as they are.
+constexpr int kPacketLds = 16;
+
+typedef float v16f __attribute__((ext_vector_type(16)));
+typedef float v8f __attribute__((ext_vector_type(8)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+// Scalar loads at wave-uniform addresses; each block waits for its data,
+// and its outputs are early-clobber (no output shares a register with the
+// address or offsets that later loads of the block read).
+// 48-byte triangle record.
+__device__ __forceinline__ void loadTriS(const float4* tp, float4& p0, float4& p1, float4& p2) {
+  v8f a;
+  v4f b;
+  asm volatile("s_load_dwordx8 %0, %2, 0x0\n\ts_load_dwordx4 %1, %2, 0x20\n\ts_waitcnt lgkmcnt(0)"
+               : "=&s"(a), "=&s"(b)
+               : "s"(tp)
+               : "memory");
+  p0 = make_float4(a[0], a[1], a[2], a[3]);
+  p1 = make_float4(a[4], a[5], a[6], a[7]);
+  p2 = make_float4(b[0], b[1], b[2], b[3]);
+}
+
+// Wave-uniform traits of the rays of a UNI packet: byte offsets, within a
+// node, of the near and far plane quads of each axis (by the direction's
+// sign: lo planes at 0 / 32 / 64, hi planes 16 bytes further), and of the
+// kx, ky, kz components within a vertex.
+struct PacketUni {
+  uint32_t offN[3], offF[3];
+  uint32_t offK[3];
+  float okx, oky, okz;  // per lane: the origin's kx, ky, kz components
+};
+// The near and far plane quads of a node, and its child refs.
+__device__ __forceinline__ void loadNodeNF(const float4* np, const PacketUni& q, v4f& nx, v4f& fx, v4f& ny, v4f& fy,
+                                           v4f& nz, v4f& fz, v4f& rf) {
+  asm volatile(
+      "s_load_dwordx4 %0, %7, %8\n\t"
+      "s_load_dwordx4 %1, %7, %9\n\t"
+      "s_load_dwordx4 %2, %7, %10\n\t"
+      "s_load_dwordx4 %3, %7, %11\n\t"
+      "s_load_dwordx4 %4, %7, %12\n\t"
+      "s_load_dwordx4 %5, %7, %13\n\t"
+      "s_load_dwordx4 %6, %7, 0x60\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&s"(nx), "=&s"(fx), "=&s"(ny), "=&s"(fy), "=&s"(nz), "=&s"(fz), "=&s"(rf)
+      : "s"(np), "s"(q.offN[0]), "s"(q.offF[0]), "s"(q.offN[1]), "s"(q.offF[1]), "s"(q.offN[2]), "s"(q.offF[2])
+      : "memory");
+}
+// A triangle's vertex components in (kx, ky, kz) order, and its global id.
+__device__ __forceinline__ void loadTriK(const float4* tp, const PacketUni& q, float (&a)[9], int& gid) {
+  const float4 *b1 = tp + 1, *b2 = tp + 2;
+  asm volatile(
+      "s_load_dword %0, %10, %13\n\t"
+      "s_load_dword %1, %10, %14\n\t"
+      "s_load_dword %2, %10, %15\n\t"
+      "s_load_dword %3, %11, %13\n\t"
+      "s_load_dword %4, %11, %14\n\t"
+      "s_load_dword %5, %11, %15\n\t"
+      "s_load_dword %6, %12, %13\n\t"
+      "s_load_dword %7, %12, %14\n\t"
+      "s_load_dword %8, %12, %15\n\t"
+      "s_load_dword %9, %12, 0xc\n\t"
+      "s_waitcnt lgkmcnt(0)"
+      : "=&s"(a[0]), "=&s"(a[1]), "=&s"(a[2]), "=&s"(a[3]), "=&s"(a[4]), "=&s"(a[5]), "=&s"(a[6]), "=&s"(a[7]),
+        "=&s"(a[8]), "=&s"(gid)
+      : "s"(tp), "s"(b1), "s"(b2), "s"(q.offK[0]), "s"(q.offK[1]), "s"(q.offK[2])
+      : "memory");
+}
+
+__device__ __forceinline__ void pswap(uint32_t (&key)[4], int (&ref)[4], float (&t)[4], int a, int b) {
+  const bool sw = key[b] < key[a];  // wave-uniform
+  const uint32_t ka = key[a], kb = key[b];
+  const int ra = ref[a], rb = ref[b];
+  const float ta = t[a], tb = t[b];
+  key[a] = sw ? kb : ka;
+  key[b] = sw ? ka : kb;
+  ref[a] = sw ? rb : ra;
+  ref[b] = sw ? ra : rb;
+  t[a] = sw ? tb : ta;
+  t[b] = sw ? ta : tb;
+}
+
+// slab() on planes already ordered near / far for the ray.
+__device__ __forceinline__ bool slabNF(float nxp, float fxp, float nyp, float fyp, float nzp, float fzp,
+                                       const TRay& r, float tmin, float tmax, float& tn) {
+  const float nx = (nxp - r.o.x) * r.inv.x, fx = (fxp - r.o.x) * r.inv.x;
+  const float ny = (nyp - r.o.y) * r.inv.y, fy = (fyp - r.o.y) * r.inv.y;
+  const float nz = (nzp - r.o.z) * r.inv.z, fz = (fzp - r.o.z) * r.inv.z;
+  tn = fmaxf(fmaxf(nx, ny), fmaxf(nz, tmin));
+  const float tf = fminf(fminf(fx, fy), fminf(fz, tmax)) * 1.0000004f;
+  return tn <= tf;
+}
+
+template <bool COUNT, bool UNI>
+__device__ __forceinline__ void packetWalk(const SceneView& sc, const Wavefront& w, const TRay& r, const PacketUni& q,
+                                           bool valid, int lane, uint32_t gt, int cap, int* sRef, float* sT,
+                                           int& best, float& bt, float& bu, float& bv, uint32_t& nN, uint32_t& nT) {
+  const float kMiss = 3.0e38f;
+  float lim = valid ? kInf : -1.0f;  // culling bound: kInf, then the closest hit's t (-1: no ray)
+  int bestGid = 0x7fffffff;
+  best = -1;
+  bt = kInf;
+  bu = bv = 0.f;
+  int node = 0, sp = 0;
+  for (;;) {
+    if (node >= 0) {
+      if (COUNT && valid) ++nN;
+      float tc[4];
+      int rc[4];
+      float t;
+      if (UNI) {
+        v4f nx, fx, ny, fy, nz, fz, rf;
+        loadNodeNF(sc.nodes4 + size_t(node) * 8, q, nx, fx, ny, fy, nz, fz, rf);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+          tc[c] = slabNF(nx[c], fx[c], ny[c], fy[c], nz[c], fz[c], r, kMinimum, lim, t) ? t : kMiss;
+          rc[c] = __float_as_int(rf[c]);
+        }
+      } else {
+        const float4* nd = sc.nodes4 + size_t(node) * 8;
+        const float4 lx = nd[0], hx = nd[1], ly = nd[2], hy = nd[3], lz = nd[4], hz = nd[5], rf = nd[6];
+        tc[0] = slab(lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, r, kMinimum, lim, t) ? t : kMiss;
+        tc[1] = slab(lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, r, kMinimum, lim, t) ? t : kMiss;
+        tc[2] = slab(lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, r, kMinimum, lim, t) ? t : kMiss;
+        tc[3] = slab(lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, r, kMinimum, lim, t) ? t : kMiss;
+        rc[0] = __float_as_int(rf.x), rc[1] = __float_as_int(rf.y), rc[2] = __float_as_int(rf.z),
+        rc[3] = __float_as_int(rf.w);
+      }
+      uint32_t key[4];
+      int n = 0;
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const bool any = __ballot(tc[c] < kMiss) != 0 && rc[c] != kEmptyRef;
+        n += any;
+        // entry distance of the first lane (positive floats order as their bits); unused last
+        key[c] = any ? __builtin_amdgcn_readfirstlane(__float_as_uint(tc[c])) : 0xffffffffu;
+      }
+      pswap(key, rc, tc, 0, 1);
+      pswap(key, rc, tc, 2, 3);
+      pswap(key, rc, tc, 0, 2);
+      pswap(key, rc, tc, 1, 3);
+      pswap(key, rc, tc, 1, 2);
+      if (n > 0) {
+#pragma unroll
+        for (int j = 3; j >= 1; --j) {
+          if (j < n) {
+            if (sp < kPacketLds) {
+              if (lane == 0) sRef[sp] = rc[j];
+              sT[sp * 64 + lane] = tc[j];
+            } else if (sp < cap) {
+              const size_t o = size_t(sp - kPacketLds) * w.spillThreads + gt;
+              w.spill[o] = rc[j];
+              w.spillT[o] = tc[j];
+            }
+            ++sp;
+          }
+        }
+        node = rc[0];
+        continue;
+      }
+    } else {
+      uint32_t first, cnt;
+      decodeLeaf(node, first, cnt);
+      for (uint32_t k = first; k < first + cnt; ++k) {
+        if (COUNT && valid) ++nT;
+        float t, u, v;
+        bool h;
+        int gid;
+        if (UNI) {
+          float a[9];
+          loadTriK(sc.tris + size_t(k) * 3, q, a, gid);
+          h = hitTriCore(r, a[0] - q.okx, a[1] - q.oky, a[2] - q.okz, a[3] - q.okx, a[4] - q.oky, a[5] - q.okz,
+                         a[6] - q.okx, a[7] - q.oky, a[8] - q.okz, kMinimum, lim, t, u, v);
+        } else {
+          float4 p0, p1, p2;
+          loadTriS(sc.tris + size_t(k) * 3, p0, p1, p2);
+          h = hitTri(r, p0, p1, p2, kMinimum, lim, t, u, v);
+          gid = __float_as_int(p2.w);
+        }
+        if (h) {
+          if (best < 0 || t < bt || (t == bt && gid < bestGid)) {
+            bt = t;
+            bu = u;
+            bv = v;
+            best = int(k);
+            bestGid = gid;
+            lim = t;
+          }
+        }
+      }
+    }
+    // pop: the next entry some lane's ray still enters before its closest hit
+    node = kDoneRef;
+    while (sp > 0) {
+      --sp;
+      int ref = kEmptyRef;
+      float tl = kMiss;
+      if (sp < kPacketLds) {
+        ref = sRef[sp];
+        tl = sT[sp * 64 + lane];
+      } else if (sp < cap) {
+        const size_t o = size_t(sp - kPacketLds) * w.spillThreads + gt;
+        ref = w.spill[o];
+        tl = w.spillT[o];
+      }
+      ref = __builtin_amdgcn_readfirstlane(ref);
+      if (ref != kEmptyRef && __ballot(tl < kMiss && tl <= lim * 1.0000004f) != 0) {
+        node = ref;
+        break;
+      }
+    }
+    if (node == kDoneRef) break;
+  }
+}
+
+// 8 waves per SIMD (the scalar registers the walk holds would allow 7; the
+// spills that fit it in 78 cost less than the lost wave: 3.13 against 3.35
+// ms for the 1080p camera rays).
+template <bool COUNT>
+__global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void kExtendPacket(SceneView sc, Wavefront w, int cur) {
+  __shared__ int sRef[kTraceBlock / 64][kPacketLds];
+  __shared__ float sT[kTraceBlock / 64][kPacketLds * 64];
+  const int lane = __lane_id(), wv = int(threadIdx.x) >> 6;
+  const uint32_t gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int cap = kPacketLds + int(w.spillCap);
+  const uint32_t count = w.ctr[cur];
+  const float4 *qo = w.rayO[cur], *qd = w.rayD[cur];
+  uint32_t nN = 0, nT = 0;
+  ChunkFetch f;
+  chunkInit<true>(f, &w.ctr[kXcdCtr], count);
+  uint32_t start;
+  while (chunkNext<true>(f, lane, start)) {
+    const uint32_t i = start + uint32_t(lane);
+    const bool valid = i < f.hi;  // lane 0 always holds a ray
+    TRay r;
+    setupRay(r, valid ? xyz(qo[i]) : f3{0.f, 0.f, 0.f}, valid ? xyz(qd[i]) : f3{1.f, 1.f, 1.f});
+    // UNI when the wave's rays share their direction signs and axis order
+    // (the rays of invalid lanes take the wave's traits: their bound -1
+    // rejects every box and triangle)
+    const bool sx = __builtin_amdgcn_readfirstlane(int(r.inv.x < 0.0f)) != 0;
+    const bool sy = __builtin_amdgcn_readfirstlane(int(r.inv.y < 0.0f)) != 0;
+    const bool sz = __builtin_amdgcn_readfirstlane(int(r.inv.z < 0.0f)) != 0;
+    const int kx = __builtin_amdgcn_readfirstlane(r.kx), ky = __builtin_amdgcn_readfirstlane(r.ky),
+              kz = __builtin_amdgcn_readfirstlane(r.kz);
+    const bool same = !valid || ((r.inv.x < 0.0f) == sx && (r.inv.y < 0.0f) == sy && (r.inv.z < 0.0f) == sz &&
+                                 r.kx == kx && r.ky == ky && r.kz == kz);
+    PacketUni q;
+    q.offN[0] = sx ? 16u : 0u;
+    q.offF[0] = sx ? 0u : 16u;
+    q.offN[1] = sy ? 48u : 32u;
+    q.offF[1] = sy ? 32u : 48u;
+    q.offN[2] = sz ? 80u : 64u;
+    q.offF[2] = sz ? 64u : 80u;
+    q.offK[0] = 4u * uint32_t(kx);
+    q.offK[1] = 4u * uint32_t(ky);
+    q.offK[2] = 4u * uint32_t(kz);
+    q.okx = comp(r.o, kx);
+    q.oky = comp(r.o, ky);
+    q.okz = comp(r.o, kz);
+    int best;
+    float bt, bu, bv;
+    if (__ballot(!same) == 0)
+      packetWalk<COUNT, true>(sc, w, r, q, valid, lane, gt, cap, sRef[wv], sT[wv], best, bt, bu, bv, nN, nT);
+    else
+      packetWalk<COUNT, false>(sc, w, r, q, valid, lane, gt, cap, sRef[wv], sT[wv], best, bt, bu, bv, nN, nT);
+    if (valid) {
+      if (best >= 0) {
+        const float4* tp = sc.tris + size_t(best) * 3;
+        w.hitF[i] = make_float4(bt, bu, bv, 0.f);
+        w.hitI[i] = make_int4(__float_as_int(tp[0].w), __float_as_int(tp[1].w), __float_as_int(tp[2].w), best);
+      } else {
+        w.hitI[i] = make_int4(-1, -1, -1, -1);
+      }
+    }
+  }
+  if (COUNT) {
+    atomicAdd(&w.cnt64[0], (unsigned long long)nN);
+    atomicAdd(&w.cnt64[1], (unsigned long long)nT);
   }
 }
 
@@ -3142,6 +3451,18 @@ void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t lds
   const bool shallow = ldsStack == uint32_t(kStackLdsMin);
   if (count) extendFor<true>(sc, w, cur, shallow, scheme, grid, s);
   else extendFor<false>(sc, w, cur, shallow, scheme, grid, s);
+}
+bool canPacketCamera(const SceneView& sc, uint32_t scheme) {
+  return !sc.twoLevel && scheme == kSchemeXcdSlices && sc.nodes4 != nullptr && sc.nodesQ == nullptr;
+}
+void launchExtendCamera(const SceneView& sc, const Wavefront& w, uint32_t ldsStack, uint32_t scheme, uint32_t grid,
+                        bool count, hipStream_t s) {
+  if (!canPacketCamera(sc, scheme)) {
+    launchExtend(sc, w, 0, ldsStack, scheme, grid, count, s);
+    return;
+  }
+  if (count) kExtendPacket<true><<<grid, kTraceBlock, 0, s>>>(sc, w, 0);
+  else kExtendPacket<false><<<grid, kTraceBlock, 0, s>>>(sc, w, 0);
 }
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
                  hipStream_t s) {
