@@ -367,8 +367,151 @@ static_assert(chunkLds<2>() >= size_t(kChunkRows) * kChunkCols * (32 + 8) * size
                   chunkLds<4>() >= size_t(kChunkRows) * kChunkCols * (64 + 8) * sizeof(bf16),
               "chunked convolution LDS below its output tile");
 
+// Persistent forward convolution for 32 -> 32-channel layers on large
+// levels.  The weights of all 9 taps stay in LDS for the block's lifetime;
+// the block walks output tiles of 8 rows x 32 columns (tile = blockIdx.x +
+// k * gridDim.x), and while its waves run the MFMAs of one tile, each
+// thread's loads of the next tile's input (with the halo) are in flight into
+// registers, then written to the one LDS tile buffer once the current tile's
+// fragments and output are consumed.  Per output tile and fragment the taps
+// are accumulated in the order kConvFwdChunk uses, so the output is the
+// same bit for bit.
+constexpr uint32_t kP32Tile = (kChunkRows + 2) * (kChunkCols + 2);     // input pixels per tile
+constexpr uint32_t kP32Loads = (kP32Tile * 4 + 255) / 256;             // 16-byte loads per thread
+constexpr size_t kP32Lds = (size_t(9) * 32 + kP32Tile) * kChunkStride * sizeof(bf16);
+static_assert(kP32Lds <= 65536, "persistent convolution LDS above 64 KB");
+static_assert(size_t(kP32Tile) * kChunkStride >= size_t(kChunkRows) * kChunkCols * (32 + 8),
+              "persistent convolution tile buffer below its output tile");
+constexpr uint32_t kP32BlocksPerCU = 3;  // 50 KB of LDS each
+
+__global__ __launch_bounds__(256) void kConvFwd32(ConvArgs a, uint32_t tiles) {
+  extern __shared__ __align__(16) bf16 lds[];
+  constexpr uint32_t tw = kChunkCols + 2;
+  bf16* wts = lds;                             // 9 x 32 x kChunkStride: [tap][co][ci]
+  bf16* tile = lds + 9 * 32 * kChunkStride;    // kP32Tile x kChunkStride
+  const uint32_t tilesX = (a.W + kChunkCols - 1) / kChunkCols, tilesY = (a.H + kChunkRows - 1) / kChunkRows;
+  const uint32_t lane = threadIdx.x & 63, q = lane >> 4, r16 = lane & 15, wv = threadIdx.x >> 6;
+  for (uint32_t k = threadIdx.x; k < 9 * 32 * 4; k += 256) {
+    const uint32_t c8 = k & 3, row = k >> 2;  // row = tap * 32 + co
+    *reinterpret_cast<bf16x8*>(wts + row * kChunkStride + 8 * c8) =
+        *reinterpret_cast<const bf16x8*>(a.Wf + size_t(row) * 32 + 8 * c8);
+  }
+  bf16x8 pre[kP32Loads];
+  auto fetch = [&](uint32_t t) {
+    const uint32_t tx0 = (t % tilesX) * kChunkCols, ty0 = ((t / tilesX) % tilesY) * kChunkRows;
+    const uint32_t n = t / (tilesX * tilesY);
+#pragma unroll
+    for (uint32_t j = 0; j < kP32Loads; ++j) {
+      const uint32_t k = threadIdx.x + 256 * j;
+      if (k < kP32Tile * 4) {
+        const uint32_t c8 = k & 3, pix = k >> 2;
+        const int yy = int(ty0) - 1 + int(pix / tw), xx = int(tx0) - 1 + int(pix % tw);
+        pre[j] = loadIn(a, n, yy, xx, 8 * c8);
+      }
+    }
+  };
+  auto put = [&]() {
+#pragma unroll
+    for (uint32_t j = 0; j < kP32Loads; ++j) {
+      const uint32_t k = threadIdx.x + 256 * j;
+      if (k < kP32Tile * 4) *reinterpret_cast<bf16x8*>(tile + (k >> 2) * kChunkStride + 8 * (k & 3)) = pre[j];
+    }
+  };
+  uint32_t t = blockIdx.x;
+  if (t < tiles) {
+    fetch(t);
+    put();
+  }
+  __syncthreads();  // the weights and the first tile are in LDS
+  for (; t < tiles; t += gridDim.x) {
+    const uint32_t next = t + gridDim.x;
+    if (next < tiles) fetch(next);  // in flight during this tile's MFMAs
+    const uint32_t tx0 = (t % tilesX) * kChunkCols, ty0 = ((t / tilesX) % tilesY) * kChunkRows;
+    const uint32_t n = t / (tilesX * tilesY);
+    f32x4 acc[4][2];
+#pragma unroll
+    for (int g = 0; g < 4; ++g) acc[g][0] = acc[g][1] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll 3
+    for (uint32_t tap = 0; tap < 9; ++tap) {
+      const uint32_t dy = tap / 3, dx = tap % 3;  // tile coordinates are offset by the halo
+      bf16x8 B[2];
+#pragma unroll
+      for (int c = 0; c < 2; ++c)
+        B[c] = *reinterpret_cast<const bf16x8*>(wts + (tap * 32 + 16 * c + r16) * kChunkStride + 8 * q);
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const uint32_t row = 2 * wv + uint32_t(g) / 2, col = (uint32_t(g) % 2) * 16 + r16;
+        const bf16x8 A = *reinterpret_cast<const bf16x8*>(tile + ((row + dy) * tw + col + dx) * kChunkStride + 8 * q);
+#pragma unroll
+        for (int c = 0; c < 2; ++c) acc[g][c] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(A, B[c], acc[g][c], 0, 0, 0);
+      }
+    }
+    __syncthreads();  // every wave is done with the tile's fragments
+    if (a.Y) {
+      // BF16 output through the tile buffer, stored as 16-byte runs (as kConvFwdChunk)
+      constexpr uint32_t OS = 32 + 8;
+#pragma unroll
+      for (int g = 0; g < 4; ++g)
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const uint32_t co = 16 * c + r16;
+          const float b = a.bias[co];
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            const uint32_t px = (2 * wv + uint32_t(g) / 2) * kChunkCols + (uint32_t(g) % 2) * 16 + 4 * q + r;
+            float v = acc[g][c][r] + b;
+            if (a.relu) v = fmaxf(v, 0.f);
+            tile[px * OS + co] = bf16(v);
+          }
+        }
+      __syncthreads();
+      for (uint32_t k = threadIdx.x; k < kChunkRows * kChunkCols * 4; k += 256) {
+        const uint32_t px = k >> 2, c8 = k & 3;
+        const uint32_t y = ty0 + px / kChunkCols, x = tx0 + px % kChunkCols;
+        if (y < a.H && x < a.W)
+          *reinterpret_cast<bf16x8*>(a.Y + ((size_t(n) * a.H + y) * a.W + x) * 32 + 8 * c8) =
+              *reinterpret_cast<const bf16x8*>(tile + px * OS + 8 * c8);
+      }
+      __syncthreads();  // the output tile is read out
+    }
+    if (a.Yf) {
+#pragma unroll
+      for (int g = 0; g < 4; ++g) {
+        const uint32_t y = ty0 + 2 * wv + uint32_t(g) / 2;
+        if (y >= a.H) continue;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+          const uint32_t co = 16 * c + r16;
+          const float b = a.bias[co];
+          for (int r = 0; r < 4; ++r) {
+            const uint32_t x = tx0 + (uint32_t(g) % 2) * 16 + 4 * q + r;
+            if (x >= a.W) continue;
+            const size_t pp = (size_t(n) * a.H + y) * a.W + x;
+            float v = acc[g][c][r] + b;
+            if (a.relu) v = fmaxf(v, 0.f);
+            a.Yf[pp * 32 + co] = v;
+          }
+        }
+      }
+    }
+    if (next < tiles) put();
+    __syncthreads();  // the next tile is in LDS
+  }
+}
+
 template <int NT>
 static void convFwdFor(const ConvArgs& a, size_t P, hipStream_t s) {
+  if (P >= (size_t(1) << 16) && NT == 2 && a.Cin == 32) {
+    // large levels, 32 -> 32 channels: the persistent kernel with all taps' weights in LDS
+    const uint32_t tiles = a.N * ((a.H + kChunkRows - 1) / kChunkRows) * ((a.W + kChunkCols - 1) / kChunkCols);
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        cus <= 0)
+      cus = 256;
+    const uint32_t grid = std::min<uint32_t>(tiles, kP32BlocksPerCU * uint32_t(cus));
+    kConvFwd32<<<grid, 256, kP32Lds, s>>>(a, tiles);
+    return;
+  }
   if (P >= (size_t(1) << 16) && NT <= 4) {
     // large levels: the K-chunked LDS kernel (input tile and weights of each 32-channel chunk in LDS)
     const uint32_t tiles = a.N * ((a.H + kChunkRows - 1) / kChunkRows) * ((a.W + kChunkCols - 1) / kChunkCols);
