@@ -8,12 +8,12 @@
 // the reference does, so that the FP32 instantiation reproduces every
 // rounding of the same operation sequence.
 //
-// Parity unpinned (no closed form or known answer checks them; they are
-// checked by review against the shaders and by GPU-equals-oracle tests
-// only): the argument permutation of the rough-plastic NEE
-// (brdf_rough_plastic.rchit:234-240; the NEE itself is checked where the
-// permutation is invisible, alpha = 1/ior^2), and kang18's object-space
-// normal and tangent maps (brdf_kang18.rchit).
+// Every part is pinned by a closed form, a known answer or an invariant in
+// tests/test_oracle.py, among them the argument permutation of the
+// rough-plastic NEE (brdf_rough_plastic.rchit:234-240: eval at V = L = N
+// with distinct ax, ay and 1/ior^2) and kang18's object-space normal and
+// tangent maps (brdf_kang18.rchit: maps encoding the plane's own object
+// frame give the world frame of the transformed geometry).
 //
 // Nothing in the product links or calls this file.
 #include "oracle.h"

@@ -574,6 +574,76 @@ def test_textures_replace_the_constant_and_normal_maps_tilt_the_shading_normal(t
     np.testing.assert_allclose(tilted / plain, n[2] / np.linalg.norm(n), rtol=1e-5)
 
 
+def _kang18_plane_aovs(tmp_path, name, toworld):
+    """A kang18 plane (object normal +y, object tangent +x) with constant
+    object-space normal and tangent maps that encode exactly those two
+    directions, under the instance transform `toworld`; returns the
+    oracle's normal and tangent AOVs over a crop, the instance's
+    object-to-world matrix and the plane's object-space corners."""
+    import json
+    d = tmp_path / name
+    d.mkdir(parents=True, exist_ok=True)
+    corners = [(-5, 0, 5), (5, 0, 5), (5, 0, -5), (-5, 0, -5)]
+    (d / "plane.obj").write_text("".join("v %d %d %d\n" % c for c in corners) + "vn 0 1 0\n"
+                                 "f 1//1 2//1 3//1\nf 1//1 3//1 4//1\n")
+    for key, rgb in (("nrm", (0.5, 1.0, 0.5)), ("tan", (1.0, 0.5, 0.5))):  # (v + 1) / 2 of +y and +x
+        tex = np.ones((4, 4, 4), np.float32)
+        tex[..., :3] = rgb
+        A.write_image(str(d / f"{key}.exr"), tex)
+    js = {"state": {"path_tracing": {"spp": 1, "max_path_depth": 1, "multi_channel": ["normal", "tangent"]}},
+          "camera": {"type": "perspective", "film": {"resolution": [64, 48]}, "fov": 50},
+          "textures": [{"name": "nrm", "path": "nrm.exr"}, {"name": "tan", "path": "tan.exr"}],
+          "materials": [{"name": "m", "type": "brdf_kang18", "diffuse_reflectance": [0.5, 0.5, 0.5],
+                         "specular_reflectance": [0.2, 0.2, 0.2], "alpha": [0.3, 0.2],
+                         "normal_texture": "nrm", "tangent_texture": "tan"}],
+          "lights": [{"type": "distant", "radiance": [1.0, 1.0, 1.0], "direction": [0.0, 1.0, 0.0]}],
+          "meshes": [{"name": "plane", "path": "plane.obj"}],
+          "instances": [{"mesh": "plane", "material": "m", "toworld": toworld}],
+          "shots": [{"type": "lookat", "eye": [0, 2.5, 3.5], "lookat": [0, 0, 0], "up": [0, 1, 0]}]}
+    json.dump(js, open(d / "scene.json", "w"))
+    s = A.HostScene(str(d / "scene.json"))
+    cam, pc = s.shot(0)
+    assert pc.normalOutChannel == 0 and pc.tangentOutChannel == 1
+    crop = (24, 16, 40, 32)
+    img, _ = O.OracleScene(s.desc).render(cam, pc, 64, 48, 0, 1, crop=crop)
+    m = np.array(s.desc.instances[0].transform, np.float64).reshape(4, 4).T  # column-major storage
+    sl = (slice(crop[1], crop[3]), slice(crop[0], crop[2]))
+    return img[1][sl][..., :3], img[2][sl][..., :3], m, np.array(corners, np.float64)
+
+
+def test_kang18_object_space_maps_give_the_world_frame(tmp_path):
+    """brdf_kang18.rchit: the normal and tangent maps hold object-space
+    directions, taken to world space as (2c - 1) * gl_WorldToObjectEXT.
+    A normal map that encodes the plane's own object normal must give its
+    world geometric normal -- the cross product of the transformed edges,
+    which does not use the world-to-object matrix -- under a rotation with
+    non-uniform scale (a transposed or object-to-world transform would
+    tilt it).  Under rotation with uniform scale, a tangent map that
+    encodes the object x axis must give the direction of the transformed
+    x edge as the frame's tangent, orthogonal to the normal."""
+    def world(m, p):
+        return (m @ np.append(p, 1.0))[:3]
+
+    cases = (("aniso", [{"type": "scale", "value": [1.7, 0.6, 1.3]}, {"type": "rotate", "value": [25, -35, 10]}]),
+             ("iso", [{"type": "scale", "value": [1.4, 1.4, 1.4]}, {"type": "rotate", "value": [20, 40, -15]}]))
+    for name, toworld in cases:
+        nrm, tan, m, c = _kang18_plane_aovs(tmp_path, name, toworld)
+        w = [world(m, p) for p in c]
+        g = np.cross(w[1] - w[0], w[2] - w[0])
+        g /= np.linalg.norm(g)
+        hit = np.abs(nrm).sum(-1) > 0
+        assert hit.all(), name
+        # the AOV holds the forward-facing normal: +-g, the same on every pixel of the crop
+        sign = np.sign(nrm.reshape(-1, 3) @ g)
+        assert np.all(sign == sign[0]), name
+        np.testing.assert_allclose(nrm.reshape(-1, 3), np.tile(sign[0] * g, (hit.size, 1)), atol=2e-6, err_msg=name)
+        if name == "iso":
+            e = w[1] - w[0]
+            e /= np.linalg.norm(e)
+            assert abs(e @ g) < 1e-6
+            np.testing.assert_allclose(tan.reshape(-1, 3), np.tile(e, (hit.size, 1)), atol=2e-6, err_msg=name)
+
+
 def test_rough_plastic_light_sample_reduces_to_the_bsdf_when_the_permutation_is_invisible(tmp_path):
     """brdf_rough_plastic.rchit:234-240 passes (invEta2, ax, ay) to eval in
     the order (ax, ay, invEta2).  With alpha = 1/ior^2 on both axes the three
