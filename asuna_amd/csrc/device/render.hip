@@ -1473,18 +1473,19 @@ __device__ __forceinline__ void shadowDynLoop(const SceneView& sc, const Wavefro
   }
 }
 
-// Two-level closest-hit and shadow walks (wave fetch in queue order), as
-// separate kernels and fused like kTraceDyn (a wave that runs out of
-// closest-hit rays takes shadow rays).
+// Two-level closest-hit and shadow walks (wave fetch from the per-XCD
+// queue slices, as the flat walks fetch), as separate kernels and fused
+// like kTraceDyn (a wave that runs out of closest-hit rays takes shadow
+// rays).
 template <bool COUNT>
 __device__ __forceinline__ void extend2LLoop(const SceneView& sc, const Wavefront& w, int cur, Stack& st, int lane,
                                              uint32_t& nNodes, uint32_t& nTris) {
   const uint32_t count = w.ctr[cur];
   const float4 *qo = w.rayO[cur], *qd = w.rayD[cur];
   ChunkFetch f;
-  chunkInit<false>(f, &w.ctr[3], count);
+  chunkInit<true>(f, &w.ctr[kXcdCtr], count);
   uint32_t start;
-  while (chunkNext<false>(f, lane, start)) {
+  while (chunkNext<true>(f, lane, start)) {
     const uint32_t i = start + uint32_t(lane);
     if (i < f.hi) {
       TRay r;
@@ -1518,9 +1519,9 @@ __device__ __forceinline__ void shadow2LLoop(const SceneView& sc, const Wavefron
                                              uint32_t& nNodes, uint32_t& nTris) {
   const uint32_t count = w.ctr[2];
   ChunkFetch f;
-  chunkInit<false>(f, &w.ctr[4], count);
+  chunkInit<true>(f, &w.ctr[kXcdCtr + kXcds * 32], count);
   uint32_t start;
-  while (chunkNext<false>(f, lane, start)) {
+  while (chunkNext<true>(f, lane, start)) {
     const uint32_t i = start + uint32_t(lane);
     if (i < f.hi) {
       const float4 so = w.shO[i], sd = w.shD[i];
