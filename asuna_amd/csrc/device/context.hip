@@ -413,6 +413,12 @@ static bool graphBounces(ash_ctx* c, const SceneView& sc, const BatchDesc& b, ui
   GraphEntry* slot = &c->graphs[0];
   for (auto& g : c->graphs)
     if (!g.exec || g.lastUse < slot->lastUse) slot = &g;
+  // the evicted graph may still be executing on the stream (launched for an
+  // earlier batch): it is destroyed only once the stream has drained
+  if (slot->exec && hipStreamSynchronize(c->stream) != hipSuccess) {
+    c->graphsBroken = true;
+    return false;
+  }
   releaseGraph(*slot);
   if (hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
     c->graphsBroken = true;
