@@ -533,91 +533,6 @@ __device__ __forceinline__ int tlasStep(const SceneView& sc, const TRay& rw, int
   return h0 ? c0 : (h1 ? c1 : kDoneRef);
 }
 
-// Closest hit; returns the BLAS leaf position of the hit triangle (instance
-// in bInst) or -1.
-template <bool COUNT>
-__device__ int traceClosest2L(const SceneView& sc, const TRay& rw, float tmin, float tmax, Stack& st, float& bt,
-                              float& bu, float& bv, int& bInst, uint32_t& nNodes, uint32_t& nTris) {
-  int best = -1, bestGid = 0x7fffffff;
-  bInst = -1;
-  bt = tmax;
-  st.sp = 0;
-  int node = 0, cur = -1, base = 0;
-  uint32_t triBase = 0;
-  const DevBlas* bl = nullptr;
-  TRay ro;
-  for (;;) {
-    if (node == kDoneRef) {
-      // falls through to the pop below
-    } else if (cur < 0) {
-      if (node >= 0) {
-        if (node != kEmptyRef) {
-          if (COUNT) ++nNodes;
-          node = tlasStep(sc, rw, node, tmin, best >= 0 ? bt : tmax, st);
-          if (node != kDoneRef) continue;
-        }
-      } else {
-        uint32_t first, cnt;
-        decodeLeaf(node, first, cnt);
-        const uint32_t i = sc.tlasInst[first];
-        const DevInstance& in = sc.inst[i];
-        toObject(in, rw, ro);
-        bl = &sc.blas[sc.instBlas[i]];
-        cur = int(i);
-        triBase = in.triBase;
-        base = st.sp;
-        node = 0;
-        continue;
-      }
-    } else if (node >= 0) {
-      if (node != kEmptyRef) {
-        if (COUNT) ++nNodes;
-        Kids k;
-        testNode4(bl->nodes4 + size_t(node) * 8, ro, tmin, best >= 0 ? bt : tmax, k);
-        if (k.n > 0) {
-          if (k.n > 3) st.push(k.c[3], k.t[3]);
-          if (k.n > 2) st.push(k.c[2], k.t[2]);
-          if (k.n > 1) st.push(k.c[1], k.t[1]);
-          node = k.c[0];
-          continue;
-        }
-      }
-    } else {
-      uint32_t first, cnt;
-      decodeLeaf(node, first, cnt);
-      for (uint32_t k = first; k < first + cnt; ++k) {
-        const float4* tp = bl->tris + size_t(k) * 3;
-        const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
-        if (COUNT) ++nTris;
-        float t, u, v;
-        if (hitTri(ro, p0, p1, p2, tmin, best >= 0 ? bt : tmax, t, u, v)) {
-          const int gid = int(triBase) + __float_as_int(p2.w);
-          if (best < 0 || t < bt || (t == bt && gid < bestGid)) {
-            bt = t, bu = u, bv = v;
-            best = int(k);
-            bestGid = gid;
-            bInst = cur;
-          }
-        }
-      }
-    }
-    bool more = false;
-    for (;;) {
-      if (cur >= 0 && st.sp == base) cur = -1;  // this instance's BLAS is done
-      if (st.sp == 0) break;
-      float tn;
-      const int c = st.pop(tn);
-      if (best < 0 || tn <= bt * 1.0000004f) {
-        node = c;
-        more = true;
-        break;
-      }
-    }
-    if (!more) break;
-  }
-  return best;
-}
-
 template <bool COUNT>
 __device__ bool traceAny2L(const SceneView& sc, const TRay& rw, float tmin, float tmax, Stack& st, uint32_t& nNodes,
                            uint32_t& nTris) {
@@ -1473,10 +1388,128 @@ __device__ __forceinline__ void shadowDynLoop(const SceneView& sc, const Wavefro
   }
 }
 
-// Two-level closest-hit and shadow walks (wave fetch from the per-XCD
-// queue slices, as the flat walks fetch), as separate kernels and fused
-// like kTraceDyn (a wave that runs out of closest-hit rays takes shadow
-// rays).
+// Two-level closest-hit and shadow walks (rays from the per-XCD queue
+// slices, as the flat walks take them), as separate kernels and fused like
+// kTraceDyn (a wave that runs out of closest-hit rays takes shadow rays).
+//
+// The closest-hit walk is resumable, for per-lane refill: its state stays in
+// registers between rounds, and each step is one iteration of the walk (a
+// TLAS node, the entry into a BLAS with the ray taken to object space, a
+// BLAS node or a BLAS leaf, then the pop when the step does not descend).
+// It leaves a BLAS when the stack falls back to the depth it had on entry.
+// Visit order, culling and tie rule (equal t: smaller global id) are those
+// of the flat walk, so the closest hit does not depend on how a lane's steps
+// are grouped into rounds.
+struct Walk2L {
+  int node, cur, base, best, bestGid, bInst;
+  float bt, bu, bv;
+  uint32_t triBase, blas;
+};
+constexpr int kSteps2L = 8;    // steps per round between refills
+constexpr int kRefill2L = 48;  // idle lanes of a wave that trigger a refill
+
+// One step; true when the walk has finished.
+template <bool COUNT>
+__device__ __forceinline__ bool step2L(const SceneView& sc, const TRay& rw, TRay& ro, Walk2L& s, Stack& st,
+                                       uint32_t& nN, uint32_t& nT) {
+  const float tmin = kMinimum, tmax = kInf;
+  if (s.node == kDoneRef) {
+    // falls through to the pop below
+  } else if (s.cur < 0) {
+    if (s.node >= 0) {
+      if (s.node != kEmptyRef) {
+        if (COUNT) ++nN;
+        s.node = tlasStep(sc, rw, s.node, tmin, s.best >= 0 ? s.bt : tmax, st);
+        if (s.node != kDoneRef) return false;
+      }
+    } else {
+      uint32_t first, cnt;
+      decodeLeaf(s.node, first, cnt);
+      const uint32_t i = sc.tlasInst[first];
+      const DevInstance& in = sc.inst[i];
+      toObject(in, rw, ro);
+      s.blas = sc.instBlas[i];
+      s.cur = int(i);
+      s.triBase = in.triBase;
+      s.base = st.sp;
+      s.node = 0;
+      return false;
+    }
+  } else if (s.node >= 0) {
+    if (s.node != kEmptyRef) {
+      if (COUNT) ++nN;
+      Kids k;
+      testNode4(sc.blas[s.blas].nodes4 + size_t(s.node) * 8, ro, tmin, s.best >= 0 ? s.bt : tmax, k);
+      if (k.n > 0) {
+        if (k.n > 3) st.push(k.c[3], k.t[3]);
+        if (k.n > 2) st.push(k.c[2], k.t[2]);
+        if (k.n > 1) st.push(k.c[1], k.t[1]);
+        s.node = k.c[0];
+        return false;
+      }
+    }
+  } else {
+    uint32_t first, cnt;
+    decodeLeaf(s.node, first, cnt);
+    const float4* tris = sc.blas[s.blas].tris;
+    for (uint32_t k = first; k < first + cnt; ++k) {
+      const float4* tp = tris + size_t(k) * 3;
+      const float4 p0 = tp[0], p1 = tp[1], p2 = tp[2];
+      if (COUNT) ++nT;
+      float t, u, v;
+      if (hitTri(ro, p0, p1, p2, tmin, s.best >= 0 ? s.bt : tmax, t, u, v)) {
+        const int gid = int(s.triBase) + __float_as_int(p2.w);
+        if (s.best < 0 || t < s.bt || (t == s.bt && gid < s.bestGid)) {
+          s.bt = t, s.bu = u, s.bv = v;
+          s.best = int(k);
+          s.bestGid = gid;
+          s.bInst = s.cur;
+        }
+      }
+    }
+  }
+  for (;;) {
+    if (s.cur >= 0 && st.sp == s.base) s.cur = -1;  // this instance's BLAS is done
+    if (st.sp == 0) {
+      s.node = kDoneRef;
+      return true;
+    }
+    float tn;
+    const int c = st.pop(tn);
+    if (s.best < 0 || tn <= s.bt * 1.0000004f) {
+      s.node = c;
+      return false;
+    }
+  }
+}
+
+// Records the closest hit of queue slot i.  t and barycentrics of the hit
+// triangle are evaluated again in world space, on the vertices the flat
+// build would store: the object-space test picks the triangle, this gives
+// it the flat path's numbers (the object-space ray differs from the world
+// ray by rounding, which moves barycentrics by about 1e-4 on small
+// triangles).
+__device__ __forceinline__ void write2LHit(const SceneView& sc, const Wavefront& w, uint32_t i, const TRay& rw, int inst,
+                                           int pos, float t, float u, float v) {
+  if (pos < 0) {
+    w.hitI[i] = make_int4(-1, -1, -1, -1);
+    return;
+  }
+  const DevBlas& bl = sc.blas[sc.instBlas[inst]];
+  const DevInstance& in = sc.inst[inst];
+  const float4* tp = bl.tris + size_t(pos) * 3;
+  const int prim = __float_as_int(tp[2].w);
+  float t2, u2, v2;
+  if (hitTri(rw, toWorld(in, tp[0]), toWorld(in, tp[1]), toWorld(in, tp[2]), kMinimum, kInf, t2, u2, v2))
+    t = t2, u = u2, v = v2;
+  w.hitF[i] = make_float4(t, u, v, 0.f);
+  w.hitI[i] = make_int4(inst, int(in.shade), int(in.triBase) + prim, int(bl.shadeBase) + pos);
+}
+
+// Persistent closest-hit loop over queue `cur` with per-lane refill (idle
+// lanes take new rays from the XCD slices once kRefill2L of them are idle),
+// as the flat walk's extendDynLoop; each active lane runs up to kSteps2L
+// steps per round.
 template <bool COUNT>
 __device__ __forceinline__ void extend2LLoop(const SceneView& sc, const Wavefront& w, int cur, Stack& st, int lane,
                                              uint32_t& nNodes, uint32_t& nTris) {
@@ -1484,32 +1517,41 @@ __device__ __forceinline__ void extend2LLoop(const SceneView& sc, const Wavefron
   const float4 *qo = w.rayO[cur], *qd = w.rayD[cur];
   ChunkFetch f;
   chunkInit<true>(f, &w.ctr[kXcdCtr], count);
-  uint32_t start;
-  while (chunkNext<true>(f, lane, start)) {
-    const uint32_t i = start + uint32_t(lane);
-    if (i < f.hi) {
-      TRay r;
-      setupRay(r, xyz(qo[i]), xyz(qd[i]));
-      float t = 0.f, u = 0.f, v = 0.f;
-      int inst = -1;
-      const int pos = traceClosest2L<COUNT>(sc, r, kMinimum, kInf, st, t, u, v, inst, nNodes, nTris);
-      if (pos >= 0) {
-        const DevBlas& bl = sc.blas[sc.instBlas[inst]];
-        const DevInstance& in = sc.inst[inst];
-        const float4* tp = bl.tris + size_t(pos) * 3;
-        const int prim = __float_as_int(tp[2].w);
-        // t and barycentrics of the hit triangle evaluated again in world
-        // space, on the vertices the flat build would store: the object-space
-        // test picks the triangle, this gives it the flat path's numbers
-        // (the object-space ray differs from the world ray by rounding, which
-        // moves barycentrics by about 1e-4 on small triangles)
-        float t2, u2, v2;
-        if (hitTri(r, toWorld(in, tp[0]), toWorld(in, tp[1]), toWorld(in, tp[2]), kMinimum, kInf, t2, u2, v2))
-          t = t2, u = u2, v = v2;
-        w.hitF[i] = make_float4(t, u, v, 0.f);
-        w.hitI[i] = make_int4(inst, int(in.shade), int(in.triBase) + prim, int(bl.shadeBase) + pos);
-      } else {
-        w.hitI[i] = make_int4(-1, -1, -1, -1);
+  bool active = false, drained = false;
+  uint32_t slot = 0;
+  TRay rw, ro;
+  Walk2L s;
+  for (;;) {
+    const uint32_t nIdle = uint32_t(__popcll(__ballot(!active)));
+    if (!drained && nIdle >= uint32_t(kRefill2L)) {
+      uint32_t i = 0;
+      if (fetchIdle(f, !active, lane, drained, i)) {
+        slot = i;
+        setupRay(rw, xyz(qo[i]), xyz(qd[i]));
+        s.node = 0;
+        s.cur = -1;
+        s.base = 0;
+        s.best = -1;
+        s.bestGid = 0x7fffffff;
+        s.bInst = -1;
+        s.bt = kInf;
+        s.bu = s.bv = 0.f;
+        s.triBase = 0;
+        s.blas = 0;
+        st.sp = 0;
+        active = true;
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (drained) break;
+      continue;
+    }
+    if (active) {
+      bool done = false;
+      for (int k = 0; k < kSteps2L && !done; ++k) done = step2L<COUNT>(sc, rw, ro, s, st, nNodes, nTris);
+      if (done) {
+        active = false;
+        write2LHit(sc, w, slot, rw, s.bInst, s.best, s.bt, s.bu, s.bv);
       }
     }
   }
