@@ -533,63 +533,6 @@ __device__ __forceinline__ int tlasStep(const SceneView& sc, const TRay& rw, int
   return h0 ? c0 : (h1 ? c1 : kDoneRef);
 }
 
-template <bool COUNT>
-__device__ bool traceAny2L(const SceneView& sc, const TRay& rw, float tmin, float tmax, Stack& st, uint32_t& nNodes,
-                           uint32_t& nTris) {
-  st.sp = 0;
-  int node = 0, cur = -1, base = 0;
-  const DevBlas* bl = nullptr;
-  TRay ro;
-  for (;;) {
-    if (node == kDoneRef) {
-    } else if (cur < 0) {
-      if (node >= 0) {
-        if (node != kEmptyRef) {
-          if (COUNT) ++nNodes;
-          node = tlasStep(sc, rw, node, tmin, tmax, st);
-          if (node != kDoneRef) continue;
-        }
-      } else {
-        uint32_t first, cnt;
-        decodeLeaf(node, first, cnt);
-        const uint32_t i = sc.tlasInst[first];
-        toObject(sc.inst[i], rw, ro);
-        bl = &sc.blas[sc.instBlas[i]];
-        cur = int(i);
-        base = st.sp;
-        node = 0;
-        continue;
-      }
-    } else if (node >= 0) {
-      if (node != kEmptyRef) {
-        if (COUNT) ++nNodes;
-        Kids k;
-        testNode4(bl->nodes4 + size_t(node) * 8, ro, tmin, tmax, k);
-        if (k.n > 0) {
-          if (k.n > 3) st.push(k.c[3], k.t[3]);
-          if (k.n > 2) st.push(k.c[2], k.t[2]);
-          if (k.n > 1) st.push(k.c[1], k.t[1]);
-          node = k.c[0];
-          continue;
-        }
-      }
-    } else {
-      uint32_t first, cnt;
-      decodeLeaf(node, first, cnt);
-      for (uint32_t k = first; k < first + cnt; ++k) {
-        const float4* tp = bl->tris + size_t(k) * 3;
-        float t, u, v;
-        if (COUNT) ++nTris;
-        if (hitTri(ro, tp[0], tp[1], tp[2], tmin, tmax, t, u, v)) return true;
-      }
-    }
-    if (cur >= 0 && st.sp == base) cur = -1;
-    if (st.sp == 0) break;
-    float tn;
-    node = st.pop(tn);
-  }
-  return false;
-}
 
 // -------------------------------------------------------------- raygen
 // raytrace.projective.rgen:26-90 for one (shot, frame, pixel).
@@ -1556,23 +1499,112 @@ __device__ __forceinline__ void extend2LLoop(const SceneView& sc, const Wavefron
     }
   }
 }
+// The any-hit walk in the same resumable form; step result 0: walking,
+// 1: finished without a hit, 2: occluded.
+struct Any2L {
+  int node, cur, base;
+  uint32_t blas;
+};
+template <bool COUNT>
+__device__ __forceinline__ int stepAny2L(const SceneView& sc, const TRay& rw, TRay& ro, float tmax, Any2L& s,
+                                         Stack& st, uint32_t& nN, uint32_t& nT) {
+  const float tmin = 0.0f;
+  if (s.node == kDoneRef) {
+    // falls through to the pop below
+  } else if (s.cur < 0) {
+    if (s.node >= 0) {
+      if (s.node != kEmptyRef) {
+        if (COUNT) ++nN;
+        s.node = tlasStep(sc, rw, s.node, tmin, tmax, st);
+        if (s.node != kDoneRef) return 0;
+      }
+    } else {
+      uint32_t first, cnt;
+      decodeLeaf(s.node, first, cnt);
+      const uint32_t i = sc.tlasInst[first];
+      toObject(sc.inst[i], rw, ro);
+      s.blas = sc.instBlas[i];
+      s.cur = int(i);
+      s.base = st.sp;
+      s.node = 0;
+      return 0;
+    }
+  } else if (s.node >= 0) {
+    if (s.node != kEmptyRef) {
+      if (COUNT) ++nN;
+      Kids k;
+      testNode4(sc.blas[s.blas].nodes4 + size_t(s.node) * 8, ro, tmin, tmax, k);
+      if (k.n > 0) {
+        if (k.n > 3) st.push(k.c[3], k.t[3]);
+        if (k.n > 2) st.push(k.c[2], k.t[2]);
+        if (k.n > 1) st.push(k.c[1], k.t[1]);
+        s.node = k.c[0];
+        return 0;
+      }
+    }
+  } else {
+    uint32_t first, cnt;
+    decodeLeaf(s.node, first, cnt);
+    const float4* tris = sc.blas[s.blas].tris;
+    for (uint32_t k = first; k < first + cnt; ++k) {
+      const float4* tp = tris + size_t(k) * 3;
+      float t, u, v;
+      if (COUNT) ++nT;
+      if (hitTri(ro, tp[0], tp[1], tp[2], tmin, tmax, t, u, v)) return 2;
+    }
+  }
+  if (s.cur >= 0 && st.sp == s.base) s.cur = -1;  // this instance's BLAS is done
+  if (st.sp == 0) return 1;
+  float tn;
+  s.node = st.pop(tn);
+  return 0;
+}
+
+// Persistent any-hit loop over the shadow-ray queue with per-lane refill,
+// as extend2LLoop: an unoccluded ray adds its path's pending NEE radiance.
 template <bool COUNT>
 __device__ __forceinline__ void shadow2LLoop(const SceneView& sc, const Wavefront& w, Stack& st, int lane,
                                              uint32_t& nNodes, uint32_t& nTris) {
   const uint32_t count = w.ctr[2];
   ChunkFetch f;
   chunkInit<true>(f, &w.ctr[kXcdCtr + kXcds * 32], count);
-  uint32_t start;
-  while (chunkNext<true>(f, lane, start)) {
-    const uint32_t i = start + uint32_t(lane);
-    if (i < f.hi) {
-      const float4 so = w.shO[i], sd = w.shD[i];
-      TRay r;
-      setupRay(r, xyz(so), xyz(sd));
-      const int pid = __float_as_int(sd.w);
-      if (!traceAny2L<COUNT>(sc, r, 0.0f, so.w, st, nNodes, nTris)) {
-        const float4 L = w.rad[pid], d = w.ld[i];
-        w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
+  bool active = false, drained = false;
+  uint32_t slot = 0;
+  int pid = 0;
+  float tmax = 0.f;
+  TRay rw, ro;
+  Any2L s;
+  for (;;) {
+    const uint32_t nIdle = uint32_t(__popcll(__ballot(!active)));
+    if (!drained && nIdle >= uint32_t(kRefill2L)) {
+      uint32_t i = 0;
+      if (fetchIdle(f, !active, lane, drained, i)) {
+        slot = i;
+        const float4 so = w.shO[i], sd = w.shD[i];
+        setupRay(rw, xyz(so), xyz(sd));
+        tmax = so.w;
+        pid = __float_as_int(sd.w);
+        s.node = 0;
+        s.cur = -1;
+        s.base = 0;
+        s.blas = 0;
+        st.sp = 0;
+        active = true;
+      }
+    }
+    if (__ballot(active) == 0) {
+      if (drained) break;
+      continue;
+    }
+    if (active) {
+      int r = 0;
+      for (int k = 0; k < kSteps2L && r == 0; ++k) r = stepAny2L<COUNT>(sc, rw, ro, tmax, s, st, nNodes, nTris);
+      if (r != 0) {
+        active = false;
+        if (r == 1) {
+          const float4 L = w.rad[pid], d = w.ld[slot];
+          w.rad[pid] = make_float4(L.x + d.x, L.y + d.y, L.z + d.z, L.w);
+        }
       }
     }
   }
