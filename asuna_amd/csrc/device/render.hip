@@ -3,10 +3,11 @@
 // hardware traversal of the Vulkan pipeline).
 //
 // One render batch = shots x frames x pixels paths.  Per bounce:
-//   extend  : closest hit of every live path (persistent waves fetching 64
-//             rays at a time; 4-wide BVH traversal, nearest child first, with
-//             a per-thread stack of 16 entries in LDS spilling to global
-//             memory; the binary tree remains selectable)
+//   extend  : closest hit of every live path (persistent waves taking rays
+//             from per-XCD queue slices; 4-wide BVH traversal, nearest child
+//             first, with a per-thread stack of 12 entries in LDS spilling to
+//             global memory; the camera rays walk the tree once per wave; the
+//             binary tree remains selectable)
 //   shade   : miss / light hit / material closest-hit logic; next rays and
 //             shadow rays are appended to compacted queues with one atomic per
 //             wave (ballot + popcount)
