@@ -346,11 +346,12 @@ static int launchLoop(ash_ctx* c, const Wavefront& wf, const SceneView& sc, cons
   double* tShadow = st ? &st->shadow_ms : &scratch;
   double* tTrace = st ? &st->trace_ms : &scratch;
   int cur = 0;
+  const bool coarse = c->numTris <= kCoarseFetchTris;
   if (!canFuseTrace(sc, scheme)) {
     for (uint32_t it = 0; it < iters; ++it) {
       launchResetCounters(wf, cur, s);
       timed(tExt, [&] {
-        if (it == 0) launchExtendCamera(sc, wf, lds, scheme, grid, count, s);
+        if (it == 0) launchExtendCamera(sc, wf, lds, scheme, grid, count, coarse, s);
         else launchExtend(sc, wf, cur, lds, scheme, grid, count, s);
       });
       timed(tShade, [&] { launchShade(sc, b, wf, cur, shadeRays, s); });
@@ -359,7 +360,7 @@ static int launchLoop(ash_ctx* c, const Wavefront& wf, const SceneView& sc, cons
     }
   } else {
     launchResetCounters(wf, 0, s);
-    timed(tExt, [&] { launchExtendCamera(sc, wf, lds, scheme, grid, count, s); });
+    timed(tExt, [&] { launchExtendCamera(sc, wf, lds, scheme, grid, count, coarse, s); });
     for (uint32_t it = 0; it < iters; ++it) {
       if (it > 0) launchResetShade(wf, cur, s);
       timed(tShade, [&] { launchShade(sc, b, wf, cur, shadeRays, s); });
@@ -1477,7 +1478,8 @@ int ash_first_hit(ash_ctx* c, const GpuCamera* cam, ash_hit* out) {
   SceneView sc = viewOf(c);
   launchRaygen(b, c->wf, c->stream);
   launchResetCounters(c->wf, 0, c->stream);
-  launchExtendCamera(sc, c->wf, kStackLds, kSchemeXcdSlices, grid, false, c->stream);
+  launchExtendCamera(sc, c->wf, kStackLds, kSchemeXcdSlices, grid, false, c->numTris <= kCoarseFetchTris,
+                     c->stream);
   launchFirstHit(sc, c->shotTab.p, c->wf, dout.p, P, c->stream);
   HIPC(c, hipGetLastError());
   HIPC(c, hipMemcpyAsync(out, dout.p, P * sizeof(ash_hit), hipMemcpyDeviceToHost, c->stream));

@@ -147,9 +147,13 @@ void launchExtend(const SceneView& sc, const Wavefront& w, int cur, uint32_t lds
                   uint32_t grid, bool count, hipStream_t s);
 // Closest hit of the camera rays (queue 0 after raygen): the wave-coherent
 // walk on the float 4-wide tree with XCD slices, else launchExtend.
+// coarse: a wave fetches 256 rays per atomic instead of 64 (for trees of at
+// most kCoarseFetchTris triangles, whose cheap rays would otherwise wait on
+// the fetch atomics).
+constexpr uint32_t kCoarseFetchTris = 1u << 16;
 bool canPacketCamera(const SceneView& sc, uint32_t scheme);
 void launchExtendCamera(const SceneView& sc, const Wavefront& w, uint32_t ldsStack, uint32_t scheme, uint32_t grid,
-                        bool count, hipStream_t s);
+                        bool count, bool coarse, hipStream_t s);
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
                  hipStream_t s);
 void launchShadow(const SceneView& sc, const Wavefront& w, uint32_t scheme, uint32_t grid, bool count,

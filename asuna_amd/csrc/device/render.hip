@@ -673,13 +673,14 @@ __device__ __forceinline__ void chunkInit(ChunkFetch& f, uint32_t* ctr, uint32_t
     f.hi = count;
   }
 }
-// Next chunk start for the calling wave (all lanes); false when the queue is drained.
-template <bool XCD>
+// Next chunk start for the calling wave (all lanes), CH queue entries per
+// atomic fetch; false when the queue is drained.
+template <bool XCD, uint32_t CH = 64>
 __device__ __forceinline__ bool chunkNext(ChunkFetch& f, int lane, uint32_t& start) {
   for (;;) {
     uint32_t s = XCD ? (f.home + f.k) % kXcds : 0u;
     uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(f.ctr + (XCD ? s * 32u : 0u), 64u);
+    if (lane == 0) base = atomicAdd(f.ctr + (XCD ? s * 32u : 0u), CH);
     base = __shfl(base, 0);
     if (base < f.hi - f.lo) {
       start = f.lo + base;
@@ -1131,8 +1132,9 @@ __device__ __forceinline__ void packetWalk(const SceneView& sc, const Wavefront&
 
 // 8 waves per SIMD (the scalar registers the walk holds would allow 7; the
 // spills that fit it in 78 cost less than the lost wave: 3.13 against 3.35
-// ms for the 1080p camera rays).
-template <bool COUNT>
+// ms for the 1080p camera rays).  FETCH: rays per atomic fetch (a multiple
+// of 64), walked as consecutive 64-ray packets.
+template <bool COUNT, uint32_t FETCH>
 __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8, 8))) void kExtendPacket(SceneView sc, Wavefront w, int cur) {
   __shared__ int sRef[kTraceBlock / 64][kPacketLds];
   __shared__ float sT[kTraceBlock / 64][kPacketLds * 64];
@@ -1145,8 +1147,9 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8, 
   ChunkFetch f;
   chunkInit<true>(f, &w.ctr[kXcdCtr], count);
   uint32_t start;
-  while (chunkNext<true>(f, lane, start)) {
-    const uint32_t i = start + uint32_t(lane);
+  while (chunkNext<true, FETCH>(f, lane, start)) {
+   for (uint32_t sub = start; sub < start + FETCH && sub < f.hi; sub += 64) {
+    const uint32_t i = sub + uint32_t(lane);
     const bool valid = i < f.hi;  // lane 0 always holds a ray
     TRay r;
     setupRay(r, valid ? xyz(qo[i]) : f3{0.f, 0.f, 0.f}, valid ? xyz(qd[i]) : f3{1.f, 1.f, 1.f});
@@ -1188,6 +1191,7 @@ __global__ __launch_bounds__(kTraceBlock) __attribute__((amdgpu_waves_per_eu(8, 
         w.hitI[i] = make_int4(-1, -1, -1, -1);
       }
     }
+   }
   }
   if (COUNT) {
     atomicAdd(&w.cnt64[0], (unsigned long long)nN);
@@ -3536,13 +3540,18 @@ bool canPacketCamera(const SceneView& sc, uint32_t scheme) {
   return !sc.twoLevel && scheme == kSchemeXcdSlices && sc.nodes4 != nullptr && sc.nodesQ == nullptr;
 }
 void launchExtendCamera(const SceneView& sc, const Wavefront& w, uint32_t ldsStack, uint32_t scheme, uint32_t grid,
-                        bool count, hipStream_t s) {
+                        bool count, bool coarse, hipStream_t s) {
   if (!canPacketCamera(sc, scheme)) {
     launchExtend(sc, w, 0, ldsStack, scheme, grid, count, s);
     return;
   }
-  if (count) kExtendPacket<true><<<grid, kTraceBlock, 0, s>>>(sc, w, 0);
-  else kExtendPacket<false><<<grid, kTraceBlock, 0, s>>>(sc, w, 0);
+  if (coarse) {
+    if (count) kExtendPacket<true, 256><<<grid, kTraceBlock, 0, s>>>(sc, w, 0);
+    else kExtendPacket<false, 256><<<grid, kTraceBlock, 0, s>>>(sc, w, 0);
+  } else {
+    if (count) kExtendPacket<true, 64><<<grid, kTraceBlock, 0, s>>>(sc, w, 0);
+    else kExtendPacket<false, 64><<<grid, kTraceBlock, 0, s>>>(sc, w, 0);
+  }
 }
 void launchShade(const SceneView& sc, const BatchDesc& b, const Wavefront& w, int cur, uint32_t maxRays,
                  hipStream_t s) {
