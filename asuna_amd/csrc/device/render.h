@@ -73,7 +73,7 @@ struct Wavefront {
   uint32_t capacity;
   float4* rayO[2];  // extension queues (alternating): ray origin.xyz per slot
   float4* rayD[2];  // ray direction.xyz, w = path id
-  float4* thr[2];   // throughput.xyz, w = pdf of the last BSDF sample
+  float4* thr[2];   // throughput.xyz, w = pdf of the last BSDF sample (read only at depth > 1)
   uint4* info[2];   // x seed, y depth | bsdfFlags << 16, z pixel, w shot | (frame == 0) << 16
   float4* rad;   // by path id: radiance.xyz, w = Gaussian filter weight
   float4* ld;    // pending NEE radiance of the current bounce, per shadow-ray slot

@@ -619,7 +619,7 @@ __global__ __launch_bounds__(256) void kRaygen(BatchDesc b, Wavefront w) {
   float fw = fmaxf(0.0f, expf(-2.0f * ox * ox) - expXY) * fmaxf(0.0f, expf(-2.0f * oy * oy) - expXY);
   w.rayO[0][p] = f4(ro, 0.f);
   w.rayD[0][p] = f4(rd, __int_as_float(int(p)));
-  w.thr[0][p] = make_float4(1.f, 1.f, 1.f, 0.f);
+  // no throughput record: a depth-1 path's throughput is (1, 1, 1), pdf 0
   w.rad[p] = make_float4(0.f, 0.f, 0.f, fw);
   w.info[0][p] = make_uint4(seed, 1u, pix, shot | (frame == 0 ? 0x10000u : 0u));
   if (frame == 0) {
@@ -3190,7 +3190,10 @@ __device__ __forceinline__ void shadeOne(const SceneView& sc, const BatchDesc& b
     uint32_t shot = info.w & 0xffffu;
     const ShotParams& sp = ONE ? b.one : b.shotTab[shot];
     const GpuPushConstantRaytrace& pc = sp.pc;
-    float4 thr = w.thr[cur][slot];
+    // a depth-1 path (a camera path, or one that passed through a cut-out
+    // surface, which keeps its throughput) has throughput (1, 1, 1) and pdf
+    // 0; raygen stores no record for it
+    const float4 thr = (info.y & 0xffffu) == 1u ? make_float4(1.f, 1.f, 1.f, 0.f) : w.thr[cur][slot];
     rad = w.rad[pid];
     c.sc = &sc;
     c.sp = &sp;
